@@ -1,0 +1,83 @@
+"""Brute-force Fock-space operators (TEST INFRASTRUCTURE ONLY).
+
+Independent of the list-walking algorithm in xr_oracle.c: every operator is
+an explicit matrix over all 2^dim occupation bit strings, with the sign
+convention of general-XRCC/density_tensors.c:80-113 written as a bit
+formula, a_p|n> = (-1)^popcount(n >> (p+1)) |n - p>.  Used to pin the
+oracle's densities and, through the dimer FCI comparison, the whole chain
+build_density_tensors -> build_matrix_elements -> braket_loops -> fci.
+Only for dim <= 12.
+"""
+import itertools
+
+import numpy as np
+
+
+def popcount(x):
+    return bin(x).count("1")
+
+
+def annihilator(p, dim):
+    n = 1 << dim
+    M = np.zeros((n, n))
+    for s in range(n):
+        if (s >> p) & 1:
+            M[s & ~(1 << p), s] = -1.0 if popcount(s >> (p + 1)) & 1 else 1.0
+    return M
+
+
+def operators(dim):
+    a = [annihilator(p, dim) for p in range(dim)]
+    c = [x.T.copy() for x in a]
+    return c, a
+
+
+def config_bits(cfg):
+    b = 0
+    for p in cfg:
+        b |= 1 << int(p)
+    return b
+
+
+def embed(z, configs, dim):
+    """State coefficients (n_states x n_configs) -> Fock-space row vectors."""
+    out = np.zeros((z.shape[0], 1 << dim))
+    for k, cfg in enumerate(configs):
+        out[:, config_bits(cfg)] = z[:, k]
+    return out
+
+
+def density(ops, zbra_f, zket_f, dim, cache=None):
+    """<I| ops |J> for every index tuple, as (nI, nJ, dim, ..., dim)."""
+    c, a = cache if cache is not None else operators(dim)
+    k = len(ops)
+    out = np.zeros((zbra_f.shape[0], zket_f.shape[0]) + (dim,) * k)
+    for idx in itertools.product(range(dim), repeat=k):
+        M = np.eye(1 << dim)
+        for o, p in zip(ops, idx):
+            M = M @ (c[p] if o == "c" else a[p])
+        out[(slice(None), slice(None)) + idx] = zbra_f @ M @ zket_f.T
+    return out
+
+
+def hamiltonian(h, V, cache=None):
+    """Full Fock-space H = sum h_pq p+q + sum V_pqrs p+q+sr."""
+    dim = h.shape[0]
+    c, a = cache if cache is not None else operators(dim)
+    H = np.zeros((1 << dim, 1 << dim))
+    for p in range(dim):
+        for q in range(dim):
+            if h[p, q] != 0.0:
+                H += h[p, q] * (c[p] @ a[q])
+    cc = [[c[p] @ c[q] for q in range(dim)] for p in range(dim)]
+    aa = [[a[s] @ a[r] for r in range(dim)] for s in range(dim)]
+    for p, q, r, s in itertools.product(range(dim), repeat=4):
+        v = V[p, q, r, s]
+        if v != 0.0:
+            H += v * (cc[p][q] @ aa[s][r])
+    return H
+
+
+def number_operator(dim):
+    n = 1 << dim
+    return np.array([popcount(s) for s in range(n)], dtype=np.int64)
