@@ -1,0 +1,195 @@
+/*
+ * xr_hip.h -- C ABI of the MI355X (gfx950) XR hot-path library libxrhip.so.
+ *
+ * Plain C types only: int64_t / double / float / uint64_t and raw device
+ * pointers.  Every entry point returns an int status (XR_OK on success);
+ * xr_last_error() gives the message of the last failure on a context.
+ *
+ * Conventions shared by all entry points
+ *  - A fragment has n_orbs spatial orbitals, dim = 2*n_orbs spin orbitals
+ *    (alpha 0..n_orbs-1, then beta), dim <= 64.  The first n_core spatial
+ *    orbitals of each spin are frozen core.
+ *  - A charge block's determinants are the complete lexical enumeration of
+ *    its valence occupations (general-XRCC/atomic_states/mol_fci.py:99-127);
+ *    xr_configs_to_masks checks that.  State coefficients are row-major
+ *    (n_states x n_configs), the layout of build_density_tensors.py:77.
+ *  - Operator strings are ASCII strings over {'c','a'} (c = creation),
+ *    leftmost operator first, e.g. "ccaa" = p+ q+ r s.  Supported:
+ *    a c ca aa cc caa cca ccaa caaa ccca.
+ *  - Dense density layout: out[((I*nJ + J)*dim^k) + idx],
+ *    idx = ((p*dim + q)*dim + r)*dim + s  (general-XRCC/density_tensors.c:150).
+ *  - Outputs are overwritten (the reference accumulates into zeroed storage,
+ *    which gives the same result).
+ *  - All pointers named d_* are device pointers on the context's device; work
+ *    is queued on the context's stream and is asynchronous unless stated.
+ *  - A context serialises its calls; use one context per host thread.
+ */
+#ifndef XR_HIP_H
+#define XR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    XR_OK = 0,
+    XR_EINVAL = 1,               /* bad argument value                      */
+    XR_ESHAPE = 2,               /* inconsistent sizes                      */
+    XR_EINVAL_CONFIG_ORDER = 3,  /* configs not the complete lexical list   */
+    XR_EPAULI = 4,               /* a config repeats a spin orbital         */
+    XR_ENOMEM = 5,               /* device allocation failed                */
+    XR_EHIP = 6,                 /* HIP runtime error                       */
+    XR_EUNSUPPORTED = 7          /* operator string / size not supported    */
+};
+
+typedef struct xr_ctx xr_ctx;
+
+/* Context on `device`, queuing on `stream` (a hipStream_t; NULL = default).
+ * No reference counterpart: the reference is a set of stateless C functions
+ * compiled at import (general-XRCC/build_density_tensors.py:23-24). */
+int xr_create(int device, void *stream, xr_ctx **out);
+int xr_destroy(xr_ctx *ctx);
+int xr_set_stream(xr_ctx *ctx, void *stream);
+const char *xr_last_error(xr_ctx *ctx);
+int xr_synchronize(xr_ctx *ctx);
+
+/* Occupation lists (n_cfg x n_elec int64, ascending rows) -> 64-bit masks,
+ * checking Pauli, core occupancy and that row i has lexical rank i.
+ * Replaces the config marshalling of build_density_tensors.py:71-78, which
+ * does no checking (density_tensors.c:49-62 silently mis-ranks).
+ * Synchronous (returns the check result). */
+int xr_configs_to_masks(xr_ctx *ctx, const int64_t *d_cfg, int64_t n_cfg, int n_elec,
+                        int n_orbs, int n_core, uint64_t *d_masks);
+
+/* Lexical valence rank of each mask (find_config_index,
+ * general-XRCC/density_tensors.c:49-62). */
+int xr_config_rank(xr_ctx *ctx, const uint64_t *d_masks, int64_t n, int n_orbs, int n_core,
+                   int64_t *d_rank);
+
+/* Excitation lists of an operator string for ket configurations [q0, q1),
+ * in the reference loop order (ket config outermost, then the rightmost
+ * operator's index, ..., the leftmost operator's index innermost;
+ * general-XRCC/density_tensors.c:162-556).  Two passes:
+ *   xr_excitation_count: d_offsets[0..q1-q0] = exclusive prefix sum of the
+ *     surviving entries per ket config (d_offsets[q1-q0] = total; synchronous,
+ *     *total also returned);
+ *   xr_excitation_fill: entry e gets (Q, idx, P, sign) with P the bra rank.
+ * n_elec_ket: electrons in the ket space. */
+int xr_excitation_count(xr_ctx *ctx, const char *ops, int n_orbs, int n_core, int n_elec_ket,
+                        const uint64_t *d_ket_masks, int64_t q0, int64_t q1,
+                        int64_t *d_offsets, int64_t *total);
+int xr_excitation_fill(xr_ctx *ctx, const char *ops, int n_orbs, int n_core, int n_elec_ket,
+                       const uint64_t *d_ket_masks, int64_t q0, int64_t q1,
+                       const int64_t *d_offsets, int64_t *d_q, int64_t *d_idx, int64_t *d_p,
+                       int8_t *d_sign);
+
+/* Dense transition density <I| ops |J> for all state pairs (replaces
+ * a/c/ca/aa/cc/caa/cca/ccaa_tensor, general-XRCC/density_tensors.c:162-556,
+ * and caaa/ccca_tensor, Be-states/_attic_/deprecated/density_tensors.c:560-688,
+ * including its separate bra coefficient list).  The ket space has
+ * n_elec_ket electrons and nQ = C(dim-2n_core, n_elec_ket-2n_core)
+ * determinants; the bra space is implied by ops.  d_out: nI*nJ*dim^k doubles. */
+int xr_density(xr_ctx *ctx, const char *ops, int n_orbs, int n_core, int n_elec_ket,
+               int64_t nI, int64_t nP, const double *d_zbra,
+               int64_t nJ, int64_t nQ, const double *d_zket, double *d_out);
+
+/* Monomer two-electron energy matrix E[I,J] = sum_pqrs V[p,q,r,s] <I|p+q+sr|J>
+ * for one charge block, without forming the 4-index density (replaces the
+ * ccaa build plus per-(I,J) monomer_2e, build_density_tensors.py:122-133 and
+ * H_contractions.c:61-78).  d_V: dim^4 row-major.  d_out: nI x nJ. */
+int xr_monomer_2e(xr_ctx *ctx, int n_orbs, int n_core, int n_elec,
+                  int64_t nI, int64_t nP, const double *d_zbra,
+                  int64_t nJ, int64_t nQ, const double *d_zket,
+                  const double *d_V, double *d_out);
+
+/* Strided-batched FP64 GEMM on the matrix cores:
+ *   C_b = alpha * op(A_b) * op(B_b) + beta * C_b,  b < batch.
+ * transA = 0: A is M x K row-major (A[m*lda+k]); 1: A is K x M (A[k*lda+m]).
+ * transB = 0: B is K x N row-major (B[k*ldb+n]); 1: B is N x K (B[n*ldb+k]).
+ * C is M x N row-major.  Used by every contraction below. */
+int xr_gemm_f64(xr_ctx *ctx, int transA, int transB, int64_t M, int64_t N, int64_t K,
+                double alpha, const double *d_A, int64_t lda, int64_t strideA,
+                const double *d_B, int64_t ldb, int64_t strideB,
+                double beta, double *d_C, int64_t ldc, int64_t strideC, int64_t batch);
+
+/* Batched H_contractions.c families.  Each evaluates one reference scalar
+ * contraction for every pair (a, b) of state pairs at once:
+ *   out[a*nB + b] = f(R1[a], R2[b], integrals)
+ * with R1[a] the a-th density of fragment 1 (a = I1*nJ1 + J1, dense layout
+ * above) and R2[b] the b-th density of fragment 2.  out is overwritten. */
+
+/* sum h[p,q] R[a,p,q]   (monomer_1e / monomer_extPot, H_contractions.c:48,80);
+ * out has nA entries. */
+int xr_monomer_1e(xr_ctx *ctx, int64_t n, int64_t nA, const double *d_Rca, const double *d_h,
+                  double *d_out);
+
+/* sum V[p1,q1,r2,s2] Rcc1[a,p1,q1] Raa2[b,s2,r2]   (dimer_2min2pls, :95) */
+int xr_dimer_2min2pls(xr_ctx *ctx, int64_t n1, int64_t n2, int64_t nA, const double *d_Rcc1,
+                      int64_t nB, const double *d_Raa2, const double *d_V, double *d_out);
+
+/* sum h[p1,q2] Rc1[a,p1] Ra2[b,q2]   (dimer_1min1pls_1e, :116) */
+int xr_dimer_1min1pls_1e(xr_ctx *ctx, int64_t n1, int64_t n2, int64_t nA, const double *d_Rc1,
+                         int64_t nB, const double *d_Ra2, const double *d_h, double *d_out);
+
+/* 2 ( sum V1112[p1,q1,r1,s2] Rcca1[a,q1,p1,r1] Ra2[b,s2]
+ *   + sum V1222[p1,q2,r2,s2] Rc1[a,p1] Rcaa2[b,q2,s2,r2] )   (dimer_1min1pls_2e, :129) */
+int xr_dimer_1min1pls_2e(xr_ctx *ctx, int64_t n1, int64_t n2, int64_t nA, const double *d_Rc1,
+                         const double *d_Rcca1, int64_t nB, const double *d_Ra2,
+                         const double *d_Rcaa2, const double *d_V1112, const double *d_V1222,
+                         double *d_out);
+
+/* 4 sum V[p1,q2,r1,s2] Rca1[a,p1,r1] Rca2[b,q2,s2]   (dimer_ExEx, :163) */
+int xr_dimer_ExEx(xr_ctx *ctx, int64_t n1, int64_t n2, int64_t nA, const double *d_Rca1,
+                  int64_t nB, const double *d_Rca2, const double *d_V, double *d_out);
+
+/* Three-fragment families, out[(a*nB + b)*nC + c] for state pairs a, b, c of
+ * fragments 1, 2, 3 (H_contractions.c:184-252):
+ *   2min1pls1pls: 2 sum V[p1,q1,r2,s3] Rcc1[a,q1,p1] Ra2[b,r2] Ra3[c,s3]
+ *   2pls1min1min: 2 sum V[r2,s3,p1,q1] Raa1[a,q1,p1] Rc2[b,r2] Rc3[c,s3]
+ *   Ex1min1pls:   4 sum V[p1,r2,q1,s3] Rca1[a,p1,q1] Rc2[b,r2] Ra3[c,s3]   */
+int xr_trimer_2min1pls1pls(xr_ctx *ctx, int64_t n1, int64_t n2, int64_t n3, int64_t nA,
+                           const double *d_Rcc1, int64_t nB, const double *d_Ra2, int64_t nC,
+                           const double *d_Ra3, const double *d_V, double *d_out);
+int xr_trimer_2pls1min1min(xr_ctx *ctx, int64_t n1, int64_t n2, int64_t n3, int64_t nA,
+                           const double *d_Raa1, int64_t nB, const double *d_Rc2, int64_t nC,
+                           const double *d_Rc3, const double *d_V, double *d_out);
+int xr_trimer_Ex1min1pls(xr_ctx *ctx, int64_t n1, int64_t n2, int64_t n3, int64_t nA,
+                         const double *d_Rca1, int64_t nB, const double *d_Rc2, int64_t nC,
+                         const double *d_Ra3, const double *d_V, double *d_out);
+
+/* Index bookkeeping of the XR matrix (general-XRCC/test_H.py:63-108 and the
+ * element layout build_H.py returns into): adds a block of 1-3 fragment
+ * contractions into the tensor-product matrix,
+ *   out[row * ldo + col] += scale * block[sum_k e_k * stride[k]]
+ * where fragment k contributes the state pair e_k = I_k*nJ[k] + J_k,
+ * row = (.. (rows_k[I_k]) * d[k+1] + ..), col likewise with cols_k[J_k].
+ * diag_only[k] = 1 restricts fragment k to I_k == J_k (stride ignored), for
+ * terms that carry a Kronecker delta on a spectator fragment (the extPot and
+ * nuclear terms of build_H.py:85-87, 151).  d_block = NULL adds `scale`.
+ * Host arrays of length nfrag; d_rows / d_cols are host arrays of device
+ * pointers. */
+int xr_block_scatter(xr_ctx *ctx, int nfrag, const int64_t *nI, const int64_t *nJ,
+                     const int64_t *stride, const int *diag_only,
+                     const int64_t *const *d_rows, const int64_t *const *d_cols,
+                     const int64_t *d, const double *d_block, double scale, double *d_out,
+                     int64_t ldo);
+
+/* Tensor-product XR matrix from monomer, pair and trimer blocks for
+ * n_frag <= 8 fragments with spf states each (replaces braket_loops,
+ * general-XRCC/hamiltonian.py:21-83).  H1: host array of n_frag device
+ * pointers (spf x spf); H2: host array of n_frag*n_frag device pointers,
+ * H2[M*n_frag+N] for M < N (spf^2 x spf^2); H3: NULL or n_frag^3 pointers
+ * (NULL entries skipped).  d_par: NULL for the reference assembly, or
+ * n_frag*spf electron-count parities, which adds the sign of spectator
+ * fragments to one-electron transfers between non-adjacent fragments
+ * (the reference omits it).  d_out: spf^n_frag squared, overwritten. */
+int xr_braket_assemble(xr_ctx *ctx, int n_frag, int64_t spf, const double *const *H1,
+                       const double *const *H2, const double *const *H3, const int8_t *d_par,
+                       double *d_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

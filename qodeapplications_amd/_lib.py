@@ -1,0 +1,136 @@
+"""ctypes binding of libxrhip.so (the C ABI in include/xr_hip.h).
+
+The HIP library is the only compute path: if it is missing or no GPU is
+visible, every entry point raises instead of falling back to a CPU path.
+"""
+import ctypes
+import os
+import subprocess
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libxrhip.so")
+
+XR_OK, XR_EINVAL, XR_ESHAPE, XR_EINVAL_CONFIG_ORDER, XR_EPAULI, XR_ENOMEM, XR_EHIP, XR_EUNSUPPORTED = range(8)
+_CODE_NAMES = {1: "XR_EINVAL", 2: "XR_ESHAPE", 3: "XR_EINVAL_CONFIG_ORDER", 4: "XR_EPAULI",
+               5: "XR_ENOMEM", 6: "XR_EHIP", 7: "XR_EUNSUPPORTED"}
+
+# every symbol include/xr_hip.h declares, with its ctypes signature
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_L = ctypes.c_int64
+_D = ctypes.c_double
+_S = ctypes.c_char_p
+SIGNATURES = {
+    "xr_create": (_I, [_I, _P, ctypes.POINTER(_P)]),
+    "xr_destroy": (_I, [_P]),
+    "xr_set_stream": (_I, [_P, _P]),
+    "xr_last_error": (_S, [_P]),
+    "xr_synchronize": (_I, [_P]),
+    "xr_configs_to_masks": (_I, [_P, _P, _L, _I, _I, _I, _P]),
+    "xr_config_rank": (_I, [_P, _P, _L, _I, _I, _P]),
+    "xr_excitation_count": (_I, [_P, _S, _I, _I, _I, _P, _L, _L, _P, ctypes.POINTER(_L)]),
+    "xr_excitation_fill": (_I, [_P, _S, _I, _I, _I, _P, _L, _L, _P, _P, _P, _P, _P]),
+    "xr_density": (_I, [_P, _S, _I, _I, _I, _L, _L, _P, _L, _L, _P, _P]),
+    "xr_monomer_2e": (_I, [_P, _I, _I, _I, _L, _L, _P, _L, _L, _P, _P, _P]),
+    "xr_gemm_f64": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
+    "xr_monomer_1e": (_I, [_P, _L, _L, _P, _P, _P]),
+    "xr_dimer_2min2pls": (_I, [_P, _L, _L, _L, _P, _L, _P, _P, _P]),
+    "xr_dimer_1min1pls_1e": (_I, [_P, _L, _L, _L, _P, _L, _P, _P, _P]),
+    "xr_dimer_1min1pls_2e": (_I, [_P, _L, _L, _L, _P, _P, _L, _P, _P, _P, _P, _P]),
+    "xr_trimer_2min1pls1pls": (_I, [_P, _L, _L, _L, _L, _P, _L, _P, _L, _P, _P, _P]),
+    "xr_trimer_2pls1min1min": (_I, [_P, _L, _L, _L, _L, _P, _L, _P, _L, _P, _P, _P]),
+    "xr_trimer_Ex1min1pls": (_I, [_P, _L, _L, _L, _L, _P, _L, _P, _L, _P, _P, _P]),
+    "xr_block_scatter": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P, _L]),
+    "xr_braket_assemble": (_I, [_P, _I, _L, _P, _P, _P, _P, _P]),
+    "xr_dimer_ExEx": (_I, [_P, _L, _L, _L, _P, _L, _P, _P, _P]),
+}
+
+
+class XRError(RuntimeError):
+    """Raised when a libxrhip entry point returns a non-zero status."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"{_CODE_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def build(force=False):
+    """Compile libxrhip.so for gfx950 with the package Makefile."""
+    csrc = os.path.join(_HERE, "csrc")
+    if force:
+        subprocess.run(["make", "-s", "-C", csrc, "clean"], check=True)
+    subprocess.run(["make", "-s", "-C", csrc], check=True)
+
+
+def load():
+    """The loaded library (raises if it was not built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                                  "(the HIP library is required; there is no CPU fallback)")
+            L = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                f = getattr(L, name)
+                f.restype = res
+                f.argtypes = args
+            _lib = L
+    return _lib
+
+
+class Context:
+    """Owns one xr_ctx on a device, bound to torch's current stream there."""
+
+    def __init__(self, device=0):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("no GPU visible: libxrhip needs an MI355X (there is no CPU fallback)")
+        self.L = load()
+        self.device = device
+        with torch.cuda.device(device):
+            stream = torch.cuda.current_stream(device).cuda_stream
+        h = _P()
+        rc = self.L.xr_create(device, _P(stream), ctypes.byref(h))
+        if rc != XR_OK:
+            raise XRError(rc, "xr_create failed")
+        self.h = h
+
+    def check(self, rc):
+        if rc != XR_OK:
+            raise XRError(rc, self.L.xr_last_error(self.h).decode())
+
+    def call(self, name, *args):
+        self.check(getattr(self.L, name)(self.h, *args))
+
+    def sync_stream(self):
+        import torch
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        self.check(self.L.xr_set_stream(self.h, _P(stream)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.xr_destroy(self.h)
+            self.h = None
+
+
+_contexts = {}
+
+
+def context(device=0):
+    """Per-device singleton context, re-bound to torch's current stream."""
+    ctx = _contexts.get(device)
+    if ctx is None:
+        ctx = _contexts[device] = Context(device)
+    ctx.sync_stream()
+    return ctx
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (None for None)."""
+    return None if t is None else _P(t.data_ptr())

@@ -1,0 +1,118 @@
+// Shared definitions for the XR HIP library (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include <map>
+#include <tuple>
+#include <mutex>
+
+#include "../../include/xr_hip.h"
+
+#define XR_MAX_SO 64      // spin orbitals per fragment: one 64-bit occupation word
+#define XR_MAX_OPS 4
+
+// ---------------------------------------------------------------------------
+// binomial table in constant memory: C(n, k) for 0 <= n, k <= 64
+// ---------------------------------------------------------------------------
+extern __constant__ uint64_t c_binom[65 * 65];
+
+__device__ __forceinline__ uint64_t d_binom(int n, int k) {
+    if (k < 0 || n < 0 || k > n) return 0;
+    return c_binom[n * 65 + k];
+}
+
+__device__ __forceinline__ int d_popc(uint64_t x) { return __popcll(x); }
+
+// Parity of the occupied spin orbitals strictly above p: the sign that
+// inserting or removing p picks up in an ascending occupation list
+// (general-XRCC/density_tensors.c:80-113).
+__device__ __forceinline__ int d_sign_above(uint64_t occ, int p) {
+    uint64_t above = (p >= 63) ? 0ull : (occ >> (p + 1));
+    return (d_popc(above) & 1) ? -1 : 1;
+}
+
+// Lexical rank of a k-subset of {0..n-1} given as a bit mask:
+//   rank = C(n,k) - 1 - sum_i C(n-1-c_i, k-i),  c_0 < c_1 < ... the set bits.
+__device__ __forceinline__ int64_t d_lex_rank(uint64_t v, int n) {
+    int k = d_popc(v);
+    int64_t acc = 0;
+    int i = 0;
+    while (v) {
+        int c = __ffsll((unsigned long long)v) - 1;
+        v &= v - 1;
+        acc += (int64_t)d_binom(n - 1 - c, k - i);
+        i++;
+    }
+    return (int64_t)d_binom(n, k) - 1 - acc;
+}
+
+// Inverse of d_lex_rank: the k-subset of {0..n-1} with the given rank.
+__device__ __forceinline__ uint64_t d_lex_unrank(int64_t rank, int n, int k) {
+    int64_t s = (int64_t)d_binom(n, k) - 1 - rank;
+    uint64_t v = 0;
+    int d = n - 1;
+    for (int i = 0; i < k; i++) {
+        int m = k - i;
+        while (d >= 0 && (int64_t)d_binom(d, m) > s) d--;
+        // d_i = d  ->  c_i = n-1-d
+        s -= (int64_t)d_binom(d, m);
+        v |= 1ull << (n - 1 - d);
+        d--;
+    }
+    return v;
+}
+
+// Valence compression: drop the n_core lowest alpha and beta spin orbitals.
+__device__ __forceinline__ uint64_t d_to_valence(uint64_t occ, int n_orbs, int n_core) {
+    if (n_core == 0) return occ;
+    int nv = n_orbs - n_core;
+    uint64_t m = (nv >= 64) ? ~0ull : ((1ull << nv) - 1);
+    uint64_t a = (occ >> n_core) & m;
+    uint64_t b = (occ >> (n_orbs + n_core)) & m;
+    return a | (b << nv);
+}
+
+__device__ __forceinline__ uint64_t d_core_mask(int n_orbs, int n_core) {
+    uint64_t m = (n_core >= 64) ? ~0ull : ((1ull << n_core) - 1);
+    return m | (m << n_orbs);
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct xr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // growable scratch buffers (device)
+    std::vector<void*> ws;
+    std::vector<size_t> ws_size;
+    // cache of K-space occupation masks keyed by (dim, n_elec)
+    std::map<std::pair<int, int>, std::pair<uint64_t*, int64_t>> kspace;
+    std::mutex mu;
+};
+
+int xr_fail(xr_ctx* c, int code, const std::string& msg);
+int xr_hip_check(xr_ctx* c, hipError_t e, const char* what);
+void* xr_workspace(xr_ctx* c, int slot, size_t bytes);
+int xr_fill(xr_ctx* c, double* p, int64_t n, double v);
+
+#define XR_HIP(c, expr)                                         \
+    do {                                                        \
+        hipError_t _e = (expr);                                 \
+        if (_e != hipSuccess) return xr_hip_check((c), _e, #expr); \
+    } while (0)
+
+#define XR_LAUNCH_CHECK(c) XR_HIP(c, hipGetLastError())
+
+static inline int64_t h_binom(int n, int k) {
+    if (k < 0 || n < 0 || k > n) return 0;
+    if (k > n - k) k = n - k;
+    int64_t r = 1;
+    for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+    return r;
+}
+
+static inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }

@@ -1,0 +1,342 @@
+// H_contractions.c families evaluated for all state pairs at once.
+//
+// Every reference contraction is a bilinear form  R1[a] . W . R2[b]  in the
+// densities of two fragments, with W an integral block whose axes are
+// permuted to match the density index order.  Over all state pairs a, b it
+// becomes two FP64 GEMMs:  G = R1 * W  (nA x k2), then  out = G * R2^T.
+#include "xr_common.h"
+#include "xr_gemm.h"
+
+// out[o0,o1,o2,o3] = in[...] with out axis j = in axis perm[j]
+__global__ void k_permute4(const double* __restrict__ in, int64_t d0, int64_t d1, int64_t d2,
+                           int64_t d3, int p0, int p1, int p2, int p3, double* __restrict__ out) {
+    int64_t n = d0 * d1 * d2 * d3;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t din[4] = {d0, d1, d2, d3};
+    int p[4] = {p0, p1, p2, p3};
+    int64_t dout[4] = {din[p[0]], din[p[1]], din[p[2]], din[p[3]]};
+    int64_t o[4];
+    int64_t x = i;
+    for (int j = 3; j >= 0; j--) {
+        o[j] = x % dout[j];
+        x /= dout[j];
+    }
+    int64_t ix[4];
+    for (int j = 0; j < 4; j++) ix[p[j]] = o[j];
+    out[i] = in[((ix[0] * d1 + ix[1]) * d2 + ix[2]) * d3 + ix[3]];
+}
+
+static int permute4(xr_ctx* c, const double* in, int64_t d0, int64_t d1, int64_t d2, int64_t d3,
+                    int p0, int p1, int p2, int p3, double* out) {
+    int64_t n = d0 * d1 * d2 * d3;
+    if (n <= 0) return XR_OK;
+    k_permute4<<<ceil_div(n, 256), 256, 0, c->stream>>>(in, d0, d1, d2, d3, p0, p1, p2, p3, out);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+enum { WS_CW1 = 20, WS_CW2, WS_CG1, WS_CG2 };
+
+#define XR_ENTRY(c)            \
+    if (!(c)) return XR_EINVAL; \
+    std::lock_guard<std::mutex> _g((c)->mu)
+
+extern "C" int xr_monomer_1e(xr_ctx* c, int64_t n, int64_t nA, const double* Rca, const double* h,
+                             double* out) {
+    XR_ENTRY(c);
+    if (n <= 0 || nA < 0) return xr_fail(c, XR_EINVAL, "bad sizes");
+    // out[a] = sum_{pq} Rca[a, pq] h[pq]
+    return xr_dgemm_launch(c, 0, 0, nA, 1, n * n, 1.0, Rca, n * n, 0, h, 1, 0, 0.0, out, 1, 0, 1);
+}
+
+extern "C" int xr_dimer_2min2pls(xr_ctx* c, int64_t n1, int64_t n2, int64_t nA, const double* Rcc1,
+                                 int64_t nB, const double* Raa2, const double* V, double* out) {
+    XR_ENTRY(c);
+    if (n1 <= 0 || n2 <= 0 || nA < 0 || nB < 0) return xr_fail(c, XR_EINVAL, "bad sizes");
+    // W[(p1 q1), (s2 r2)] = V[p1, q1, r2, s2]
+    double* W = (double*)xr_workspace(c, WS_CW1, n1 * n1 * n2 * n2 * sizeof(double));
+    double* G = (double*)xr_workspace(c, WS_CG1, nA * n2 * n2 * sizeof(double));
+    if (!W || !G) return xr_fail(c, XR_ENOMEM, "workspace");
+    int rc;
+    if ((rc = permute4(c, V, n1, n1, n2, n2, 0, 1, 3, 2, W))) return rc;
+    if ((rc = xr_dgemm_launch(c, 0, 0, nA, n2 * n2, n1 * n1, 1.0, Rcc1, n1 * n1, 0, W, n2 * n2, 0,
+                              0.0, G, n2 * n2, 0, 1)))
+        return rc;
+    return xr_dgemm_launch(c, 0, 1, nA, nB, n2 * n2, 1.0, G, n2 * n2, 0, Raa2, n2 * n2, 0, 0.0, out,
+                           nB, 0, 1);
+}
+
+extern "C" int xr_dimer_ExEx(xr_ctx* c, int64_t n1, int64_t n2, int64_t nA, const double* Rca1,
+                             int64_t nB, const double* Rca2, const double* V, double* out) {
+    XR_ENTRY(c);
+    if (n1 <= 0 || n2 <= 0 || nA < 0 || nB < 0) return xr_fail(c, XR_EINVAL, "bad sizes");
+    // W[(p1 r1), (q2 s2)] = V[p1, q2, r1, s2]
+    double* W = (double*)xr_workspace(c, WS_CW1, n1 * n1 * n2 * n2 * sizeof(double));
+    double* G = (double*)xr_workspace(c, WS_CG1, nA * n2 * n2 * sizeof(double));
+    if (!W || !G) return xr_fail(c, XR_ENOMEM, "workspace");
+    int rc;
+    if ((rc = permute4(c, V, n1, n2, n1, n2, 0, 2, 1, 3, W))) return rc;
+    if ((rc = xr_dgemm_launch(c, 0, 0, nA, n2 * n2, n1 * n1, 1.0, Rca1, n1 * n1, 0, W, n2 * n2, 0,
+                              0.0, G, n2 * n2, 0, 1)))
+        return rc;
+    return xr_dgemm_launch(c, 0, 1, nA, nB, n2 * n2, 4.0, G, n2 * n2, 0, Rca2, n2 * n2, 0, 0.0, out,
+                           nB, 0, 1);
+}
+
+extern "C" int xr_dimer_1min1pls_1e(xr_ctx* c, int64_t n1, int64_t n2, int64_t nA, const double* Rc1,
+                                    int64_t nB, const double* Ra2, const double* h, double* out) {
+    XR_ENTRY(c);
+    if (n1 <= 0 || n2 <= 0 || nA < 0 || nB < 0) return xr_fail(c, XR_EINVAL, "bad sizes");
+    double* T = (double*)xr_workspace(c, WS_CG1, nA * n2 * sizeof(double));
+    if (!T) return xr_fail(c, XR_ENOMEM, "workspace");
+    int rc;
+    // T = Rc1 h (nA x n2);  out = T Ra2^T
+    if ((rc = xr_dgemm_launch(c, 0, 0, nA, n2, n1, 1.0, Rc1, n1, 0, h, n2, 0, 0.0, T, n2, 0, 1)))
+        return rc;
+    return xr_dgemm_launch(c, 0, 1, nA, nB, n2, 1.0, T, n2, 0, Ra2, n2, 0, 0.0, out, nB, 0, 1);
+}
+
+extern "C" int xr_dimer_1min1pls_2e(xr_ctx* c, int64_t n1, int64_t n2, int64_t nA, const double* Rc1,
+                                    const double* Rcca1, int64_t nB, const double* Ra2,
+                                    const double* Rcaa2, const double* V1112, const double* V1222,
+                                    double* out) {
+    XR_ENTRY(c);
+    if (n1 <= 0 || n2 <= 0 || nA < 0 || nB < 0) return xr_fail(c, XR_EINVAL, "bad sizes");
+    const int64_t n1c = n1 * n1 * n1, n2c = n2 * n2 * n2;
+    double* W1 = (double*)xr_workspace(c, WS_CW1, n1c * n2 * sizeof(double));
+    double* W2 = (double*)xr_workspace(c, WS_CW2, n1 * n2c * sizeof(double));
+    double* T = (double*)xr_workspace(c, WS_CG1, nA * n2 * sizeof(double));
+    double* S = (double*)xr_workspace(c, WS_CG2, nB * n1 * sizeof(double));
+    if (!W1 || !W2 || !T || !S) return xr_fail(c, XR_ENOMEM, "workspace");
+    int rc;
+    // W1[(q1 p1 r1), s2] = V1112[p1, q1, r1, s2];  W2[p1, (q2 s2 r2)] = V1222[p1, q2, r2, s2]
+    if ((rc = permute4(c, V1112, n1, n1, n1, n2, 1, 0, 2, 3, W1))) return rc;
+    if ((rc = permute4(c, V1222, n1, n2, n2, n2, 0, 1, 3, 2, W2))) return rc;
+    // T = Rcca1 W1 (nA x n2);  S[b, p1] = sum Rcaa2[b, (q2 s2 r2)] W2[p1, (q2 s2 r2)]
+    if ((rc = xr_dgemm_launch(c, 0, 0, nA, n2, n1c, 1.0, Rcca1, n1c, 0, W1, n2, 0, 0.0, T, n2, 0, 1)))
+        return rc;
+    if ((rc = xr_dgemm_launch(c, 0, 1, nB, n1, n2c, 1.0, Rcaa2, n2c, 0, W2, n2c, 0, 0.0, S, n1, 0, 1)))
+        return rc;
+    // out = 2 (T Ra2^T + Rc1 S^T)
+    if ((rc = xr_dgemm_launch(c, 0, 1, nA, nB, n2, 2.0, T, n2, 0, Ra2, n2, 0, 0.0, out, nB, 0, 1)))
+        return rc;
+    return xr_dgemm_launch(c, 0, 1, nA, nB, n1, 2.0, Rc1, n1, 0, S, n1, 0, 1.0, out, nB, 0, 1);
+}
+
+// out[a, b, c] = f * sum_{x y r s} W[(x y), r, s] R1[a, (x y)] R2[b, r] R3[c, s]
+// with W the permuted integral block; the three trimer families differ only
+// in the permutation, the prefactor and which densities they take.
+static int trimer_chain(xr_ctx* c, const double* V, int64_t d0, int64_t d1, int64_t d2, int64_t d3,
+                        int p0, int p1, int p2, int p3, double f, int64_t n1, int64_t n2,
+                        int64_t n3, int64_t nA, const double* R1, int64_t nB, const double* R2,
+                        int64_t nC, const double* R3, double* out) {
+    const int64_t k1 = n1 * n1;
+    double* W = (double*)xr_workspace(c, WS_CW1, k1 * n2 * n3 * sizeof(double));
+    double* G1 = (double*)xr_workspace(c, WS_CG1, nA * n2 * n3 * sizeof(double));
+    double* G2 = (double*)xr_workspace(c, WS_CG2, nA * nB * n3 * sizeof(double));
+    if (!W || !G1 || !G2) return xr_fail(c, XR_ENOMEM, "workspace");
+    int rc;
+    if ((rc = permute4(c, V, d0, d1, d2, d3, p0, p1, p2, p3, W))) return rc;
+    // G1[a, (r s)] = sum_{xy} R1[a, xy] W[xy, (r s)]
+    if ((rc = xr_dgemm_launch(c, 0, 0, nA, n2 * n3, k1, 1.0, R1, k1, 0, W, n2 * n3, 0, 0.0, G1,
+                              n2 * n3, 0, 1)))
+        return rc;
+    // G2[a][b, s] = sum_r R2[b, r] G1[a][r, s]     (batched over a)
+    for (int64_t a0 = 0; a0 < nA; a0 += 65535) {
+        int64_t nb = (nA - a0 < 65535) ? (nA - a0) : 65535;
+        if ((rc = xr_dgemm_launch(c, 0, 0, nB, n3, n2, 1.0, R2, n2, 0, G1 + a0 * n2 * n3, n3, n2 * n3,
+                                  0.0, G2 + a0 * nB * n3, n3, nB * n3, nb)))
+            return rc;
+    }
+    // out[(a b), c] = f * sum_s G2[(a b), s] R3[c, s]
+    return xr_dgemm_launch(c, 0, 1, nA * nB, nC, n3, f, G2, n3, 0, R3, n3, 0, 0.0, out, nC, 0, 1);
+}
+
+extern "C" int xr_trimer_2min1pls1pls(xr_ctx* c, int64_t n1, int64_t n2, int64_t n3, int64_t nA,
+                                      const double* Rcc1, int64_t nB, const double* Ra2,
+                                      int64_t nC, const double* Ra3, const double* V, double* out) {
+    XR_ENTRY(c);
+    if (n1 <= 0 || n2 <= 0 || n3 <= 0 || nA < 0 || nB < 0 || nC < 0) return xr_fail(c, XR_EINVAL, "bad sizes");
+    // W[(q1 p1), r2, s3] = V[p1, q1, r2, s3]   (R1 = Rcc1[a, q1, p1])
+    return trimer_chain(c, V, n1, n1, n2, n3, 1, 0, 2, 3, 2.0, n1, n2, n3, nA, Rcc1, nB, Ra2, nC,
+                        Ra3, out);
+}
+
+extern "C" int xr_trimer_2pls1min1min(xr_ctx* c, int64_t n1, int64_t n2, int64_t n3, int64_t nA,
+                                      const double* Raa1, int64_t nB, const double* Rc2,
+                                      int64_t nC, const double* Rc3, const double* V, double* out) {
+    XR_ENTRY(c);
+    if (n1 <= 0 || n2 <= 0 || n3 <= 0 || nA < 0 || nB < 0 || nC < 0) return xr_fail(c, XR_EINVAL, "bad sizes");
+    // W[(q1 p1), r2, s3] = V[r2, s3, p1, q1]   (R1 = Raa1[a, q1, p1])
+    return trimer_chain(c, V, n2, n3, n1, n1, 3, 2, 0, 1, 2.0, n1, n2, n3, nA, Raa1, nB, Rc2, nC,
+                        Rc3, out);
+}
+
+extern "C" int xr_trimer_Ex1min1pls(xr_ctx* c, int64_t n1, int64_t n2, int64_t n3, int64_t nA,
+                                    const double* Rca1, int64_t nB, const double* Rc2, int64_t nC,
+                                    const double* Ra3, const double* V, double* out) {
+    XR_ENTRY(c);
+    if (n1 <= 0 || n2 <= 0 || n3 <= 0 || nA < 0 || nB < 0 || nC < 0) return xr_fail(c, XR_EINVAL, "bad sizes");
+    // W[(p1 q1), r2, s3] = V[p1, r2, q1, s3]   (R1 = Rca1[a, p1, q1])
+    return trimer_chain(c, V, n1, n2, n1, n3, 0, 2, 1, 3, 4.0, n1, n2, n3, nA, Rca1, nB, Rc2, nC,
+                        Ra3, out);
+}
+
+// ---------------------------------------------------------------------------
+// XR matrix bookkeeping
+// ---------------------------------------------------------------------------
+struct ScatterDesc {
+    int nfrag;
+    int64_t nI[3], nJ[3], stride[3], d[3];
+    int diag[3];
+    const int64_t* rows[3];
+    const int64_t* cols[3];
+};
+
+__global__ void k_block_scatter(ScatterDesc sd, const double* __restrict__ blk, double scale,
+                                double* __restrict__ out, int64_t ldo, int64_t total) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    int64_t x = i, src = 0, row = 0, col = 0;
+    int64_t I[3], J[3];
+    for (int k = sd.nfrag - 1; k >= 0; k--) {
+        int64_t nk = sd.diag[k] ? sd.nI[k] : sd.nI[k] * sd.nJ[k];
+        int64_t e = x % nk;
+        x /= nk;
+        if (sd.diag[k]) { I[k] = e; J[k] = e; }
+        else { I[k] = e / sd.nJ[k]; J[k] = e % sd.nJ[k]; }
+        src += (sd.diag[k] ? 0 : e) * sd.stride[k];
+    }
+    for (int k = 0; k < sd.nfrag; k++) {
+        row = row * sd.d[k] + sd.rows[k][I[k]];
+        col = col * sd.d[k] + sd.cols[k][J[k]];
+    }
+    double v = blk ? scale * blk[src] : scale;
+    out[row * ldo + col] += v;
+}
+
+extern "C" int xr_block_scatter(xr_ctx* c, int nfrag, const int64_t* nI, const int64_t* nJ,
+                                const int64_t* stride, const int* diag_only,
+                                const int64_t* const* d_rows, const int64_t* const* d_cols,
+                                const int64_t* d, const double* d_block, double scale,
+                                double* d_out, int64_t ldo) {
+    XR_ENTRY(c);
+    if (nfrag < 1 || nfrag > 3) return xr_fail(c, XR_EUNSUPPORTED, "1 to 3 fragments per block");
+    ScatterDesc sd;
+    sd.nfrag = nfrag;
+    int64_t total = 1;
+    for (int k = 0; k < nfrag; k++) {
+        sd.nI[k] = nI[k];
+        sd.nJ[k] = nJ[k];
+        sd.stride[k] = stride[k];
+        sd.d[k] = d[k];
+        sd.diag[k] = diag_only[k];
+        sd.rows[k] = d_rows[k];
+        sd.cols[k] = d_cols[k];
+        if (diag_only[k] && nI[k] != nJ[k]) return xr_fail(c, XR_ESHAPE, "diagonal-only fragment needs nI == nJ");
+        total *= diag_only[k] ? nI[k] : nI[k] * nJ[k];
+    }
+    if (total <= 0) return XR_OK;
+    k_block_scatter<<<ceil_div(total, 256), 256, 0, c->stream>>>(sd, d_block, scale, d_out, ldo, total);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+#define XR_MAX_FRAG 8
+struct BraketDesc {
+    int nf;
+    int64_t spf;
+    int64_t pw[XR_MAX_FRAG];  // spf^(nf-1-m)
+};
+
+// One thread per element of the tensor-product matrix: the sums of
+// general-XRCC/hamiltonian.py:21-63 (monomer terms, pair couplings, trimer
+// couplings by number of fragments whose state changes).  With par, one-
+// electron transfers between non-adjacent fragments carry the sign of the
+// electron counts of the spectator fragments between them.
+__global__ void k_braket(BraketDesc bd, const double* const* __restrict__ H1,
+                         const double* const* __restrict__ H2, const double* const* __restrict__ H3,
+                         const int8_t* __restrict__ par, double* __restrict__ out, int64_t n) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * n) return;
+    const int nf = bd.nf;
+    const int64_t s = bd.spf;
+    int64_t I = idx / n, J = idx - (idx / n) * n;
+    int b[XR_MAX_FRAG], k[XR_MAX_FRAG];
+    int tr[3];
+    int nt = 0;
+    for (int m = 0; m < nf; m++) {
+        b[m] = (int)((I / bd.pw[m]) % s);
+        k[m] = (int)((J / bd.pw[m]) % s);
+        if (b[m] != k[m]) {
+            if (nt < 3) tr[nt] = m;
+            nt++;
+        }
+    }
+    double v = 0.0;
+    auto h2 = [&](int M, int N, int64_t r, int64_t c_) { return H2[M * nf + N][r * s * s + c_]; };
+    auto h3 = [&](int M, int N, int O, int64_t r, int64_t c_) {
+        const double* p = H3 ? H3[(M * nf + N) * nf + O] : nullptr;
+        return p ? p[r * s * s * s + c_] : 0.0;
+    };
+    if (nt == 0) {
+        for (int M = 0; M < nf; M++) {
+            v += H1[M][b[M] * s + k[M]];
+            for (int N = M + 1; N < nf; N++) v += h2(M, N, b[M] * s + b[N], k[M] * s + k[N]);
+        }
+    } else if (nt == 1) {
+        int M = tr[0];
+        v += H1[M][b[M] * s + k[M]];
+        for (int N = 0; N < M; N++) v += h2(N, M, b[N] * s + b[M], k[N] * s + k[M]);
+        for (int N = M + 1; N < nf; N++) v += h2(M, N, b[M] * s + b[N], k[M] * s + k[N]);
+    } else if (nt == 2) {
+        int M = tr[0], N = tr[1];
+        double sg = 1.0;
+        if (par && par[M * s + b[M]] != par[M * s + k[M]])
+            for (int O = M + 1; O < N; O++)
+                if (par[O * s + b[O]]) sg = -sg;
+        v += sg * h2(M, N, b[M] * s + b[N], k[M] * s + k[N]);
+        for (int O = 0; O < M; O++)
+            v += h3(O, M, N, (b[O] * s + b[M]) * s + b[N], (k[O] * s + k[M]) * s + k[N]);
+        for (int O = M + 1; O < N; O++)
+            v += h3(M, O, N, (b[M] * s + b[O]) * s + b[N], (k[M] * s + k[O]) * s + k[N]);
+        for (int O = N + 1; O < nf; O++)
+            v += h3(M, N, O, (b[M] * s + b[N]) * s + b[O], (k[M] * s + k[N]) * s + k[O]);
+    } else if (nt == 3) {
+        int M = tr[0], N = tr[1], O = tr[2];
+        v += h3(M, N, O, (b[M] * s + b[N]) * s + b[O], (k[M] * s + k[N]) * s + k[O]);
+    }
+    out[idx] = v;
+}
+
+extern "C" int xr_braket_assemble(xr_ctx* c, int n_frag, int64_t spf, const double* const* H1,
+                                  const double* const* H2, const double* const* H3,
+                                  const int8_t* d_par, double* d_out) {
+    XR_ENTRY(c);
+    if (n_frag < 1 || n_frag > XR_MAX_FRAG) return xr_fail(c, XR_EUNSUPPORTED, "1 to 8 fragments");
+    BraketDesc bd;
+    bd.nf = n_frag;
+    bd.spf = spf;
+    int64_t n = 1;
+    for (int m = n_frag - 1; m >= 0; m--) {
+        bd.pw[m] = n;
+        n *= spf;
+    }
+    if (n > 3037000499LL) return xr_fail(c, XR_EUNSUPPORTED, "matrix too large");
+    // device copies of the pointer tables
+    const int np = n_frag + n_frag * n_frag + (H3 ? n_frag * n_frag * n_frag : 0);
+    const double** tbl = (const double**)xr_workspace(c, 30, np * sizeof(double*));
+    if (!tbl) return xr_fail(c, XR_ENOMEM, "workspace");
+    std::vector<const double*> h(np, nullptr);
+    for (int m = 0; m < n_frag; m++) h[m] = H1[m];
+    for (int m = 0; m < n_frag * n_frag; m++) h[n_frag + m] = H2[m];
+    if (H3)
+        for (int m = 0; m < n_frag * n_frag * n_frag; m++) h[n_frag + n_frag * n_frag + m] = H3[m];
+    XR_HIP(c, hipMemcpyAsync(tbl, h.data(), np * sizeof(double*), hipMemcpyHostToDevice, c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    k_braket<<<ceil_div(n * n, 256), 256, 0, c->stream>>>(
+        bd, tbl, tbl + n_frag, H3 ? tbl + n_frag + n_frag * n_frag : nullptr, d_par, d_out, n);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}

@@ -1,0 +1,189 @@
+// Context, errors, workspaces, binomial table, configuration masks and ranks.
+#include "xr_common.h"
+#include <cstring>
+
+__constant__ uint64_t c_binom[65 * 65];
+
+int xr_fail(xr_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int xr_hip_check(xr_ctx* c, hipError_t e, const char* what) {
+    std::string m = std::string("HIP error ") + hipGetErrorString(e) + " in " + what;
+    return xr_fail(c, e == hipErrorOutOfMemory ? XR_ENOMEM : XR_EHIP, m);
+}
+
+// Returns a device buffer of at least `bytes` for scratch slot `slot`,
+// growing it when needed (the old contents are not kept).
+void* xr_workspace(xr_ctx* c, int slot, size_t bytes) {
+    if ((int)c->ws.size() <= slot) {
+        c->ws.resize(slot + 1, nullptr);
+        c->ws_size.resize(slot + 1, 0);
+    }
+    if (c->ws_size[slot] >= bytes && c->ws[slot]) return c->ws[slot];
+    if (c->ws[slot]) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(c->ws[slot]);
+        c->ws[slot] = nullptr;
+        c->ws_size[slot] = 0;
+    }
+    size_t sz = bytes < 256 ? 256 : bytes;
+    void* p = nullptr;
+    if (hipMalloc(&p, sz) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    c->ws[slot] = p;
+    c->ws_size[slot] = sz;
+    return p;
+}
+
+// Plain fill kernel, used instead of hipMemsetAsync so the work can sit
+// inside a captured HIP graph.
+__global__ void k_fill(double* p, int64_t n, double v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int xr_fill(xr_ctx* c, double* p, int64_t n, double v) {
+    if (n <= 0) return XR_OK;
+    k_fill<<<ceil_div(n, 256), 256, 0, c->stream>>>(p, n, v);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// configuration lists -> masks, with checks
+// ---------------------------------------------------------------------------
+// flags bit 0: a row is not strictly ascending / repeats an orbital (Pauli)
+//       bit 1: orbital index out of range
+//       bit 2: a core spin orbital is empty
+//       bit 3: row i does not have lexical rank i
+__global__ void k_cfg_to_masks(const int64_t* cfg, int64_t n_cfg, int n_elec, int n_orbs,
+                               int n_core, uint64_t* masks, unsigned int* flags) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_cfg) return;
+    int dim = 2 * n_orbs;
+    uint64_t m = 0;
+    unsigned int f = 0;
+    int64_t prev = -1;
+    for (int e = 0; e < n_elec; e++) {
+        int64_t p = cfg[i * n_elec + e];
+        if (p < 0 || p >= dim) { f |= 2u; continue; }
+        if (p <= prev) f |= 1u;
+        prev = p;
+        m |= 1ull << p;
+    }
+    uint64_t core = d_core_mask(n_orbs, n_core);
+    if ((m & core) != core) f |= 4u;
+    if (!f) {
+        int64_t r = d_lex_rank(d_to_valence(m, n_orbs, n_core), dim - 2 * n_core);
+        if (r != i) f |= 8u;
+    }
+    masks[i] = m;
+    if (f) atomicOr(flags, f);
+}
+
+extern "C" int xr_configs_to_masks(xr_ctx* c, const int64_t* d_cfg, int64_t n_cfg, int n_elec,
+                                   int n_orbs, int n_core, uint64_t* d_masks) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (n_orbs <= 0 || 2 * n_orbs > XR_MAX_SO || n_core < 0 || n_core > n_orbs)
+        return xr_fail(c, XR_EUNSUPPORTED, "need 0 < 2*n_orbs <= 64 and 0 <= n_core <= n_orbs");
+    if (n_elec < 2 * n_core || n_elec > 2 * n_orbs)
+        return xr_fail(c, XR_EINVAL, "electron count incompatible with orbitals / core");
+    int64_t expect = h_binom(2 * (n_orbs - n_core), n_elec - 2 * n_core);
+    if (n_cfg != expect)
+        return xr_fail(c, XR_EINVAL_CONFIG_ORDER,
+                       "config list is not the complete determinant space (expected " +
+                           std::to_string(expect) + " rows, got " + std::to_string(n_cfg) + ")");
+    unsigned int* flags = (unsigned int*)xr_workspace(c, 15, sizeof(unsigned int));
+    if (!flags) return xr_fail(c, XR_ENOMEM, "workspace");
+    XR_HIP(c, hipMemsetAsync(flags, 0, sizeof(unsigned int), c->stream));
+    k_cfg_to_masks<<<ceil_div(n_cfg, 256), 256, 0, c->stream>>>(d_cfg, n_cfg, n_elec, n_orbs,
+                                                                 n_core, d_masks, flags);
+    XR_LAUNCH_CHECK(c);
+    unsigned int hf = 0;
+    XR_HIP(c, hipMemcpyAsync(&hf, flags, sizeof(hf), hipMemcpyDeviceToHost, c->stream));
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    if (hf & 1u) return xr_fail(c, XR_EPAULI, "a configuration repeats a spin orbital or is not ascending");
+    if (hf & 2u) return xr_fail(c, XR_EINVAL, "spin-orbital index out of range");
+    if (hf & 4u) return xr_fail(c, XR_EINVAL_CONFIG_ORDER, "a configuration has an empty core orbital");
+    if (hf & 8u) return xr_fail(c, XR_EINVAL_CONFIG_ORDER, "configurations are not in lexical order");
+    return XR_OK;
+}
+
+__global__ void k_rank(const uint64_t* masks, int64_t n, int n_orbs, int n_core, int64_t* out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = d_lex_rank(d_to_valence(masks[i], n_orbs, n_core), 2 * (n_orbs - n_core));
+}
+
+extern "C" int xr_config_rank(xr_ctx* c, const uint64_t* d_masks, int64_t n, int n_orbs,
+                              int n_core, int64_t* d_rank) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (n_orbs <= 0 || 2 * n_orbs > XR_MAX_SO) return xr_fail(c, XR_EUNSUPPORTED, "dim > 64");
+    if (n <= 0) return XR_OK;
+    k_rank<<<ceil_div(n, 256), 256, 0, c->stream>>>(d_masks, n, n_orbs, n_core, d_rank);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// context lifecycle
+// ---------------------------------------------------------------------------
+extern "C" int xr_create(int device, void* stream, xr_ctx** out) {
+    if (!out) return XR_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return XR_EHIP;
+    if (hipSetDevice(device) != hipSuccess) return XR_EHIP;
+    xr_ctx* c = new xr_ctx();
+    c->device = device;
+    c->stream = (hipStream_t)stream;
+    static uint64_t tbl[65 * 65];
+    for (int n = 0; n <= 64; n++)
+        for (int k = 0; k <= 64; k++) {
+            // Pascal's rule in 64-bit; every entry up to C(64,32) fits
+            if (k == 0 || k == n) tbl[n * 65 + k] = 1;
+            else if (k > n) tbl[n * 65 + k] = 0;
+            else tbl[n * 65 + k] = tbl[(n - 1) * 65 + k - 1] + tbl[(n - 1) * 65 + k];
+        }
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_binom), tbl, sizeof(tbl)) != hipSuccess) {
+        delete c;
+        return XR_EHIP;
+    }
+    *out = c;
+    return XR_OK;
+}
+
+extern "C" int xr_destroy(xr_ctx* c) {
+    if (!c) return XR_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->ws)
+        if (p) (void)hipFree(p);
+    for (auto& kv : c->kspace) (void)hipFree(kv.second.first);
+    delete c;
+    return XR_OK;
+}
+
+extern "C" int xr_set_stream(xr_ctx* c, void* stream) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    c->stream = (hipStream_t)stream;
+    return XR_OK;
+}
+
+extern "C" const char* xr_last_error(xr_ctx* c) {
+    if (!c) return "null context";
+    return c->err.c_str();
+}
+
+extern "C" int xr_synchronize(xr_ctx* c) {
+    if (!c) return XR_EINVAL;
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    return XR_OK;
+}

@@ -1,0 +1,414 @@
+// Transition densities as gather-GEMMs on the matrix cores.
+//
+// The reference walks every (ket determinant, operator-index tuple) and
+// scatters a rank-1 update into all state pairs (general-XRCC/
+// density_tensors.c:142-556).  Here an operator string is split as
+// L * R with L pure creators (possibly empty) and R pure annihilators or
+// pure creators, and a resolution of the identity over the complete
+// determinant space K between them is inserted:
+//
+//   <I| L_l R_t |J> = sum_K <I|L_l|K> <K|R_t|J>
+//                   = sum_K X[K, l, I] * Y[K, t, J]
+//
+//   X[K, l, I] = sum_P <K| L_l^+ |P> zbra[I, P]   (one P per (K, l))
+//   Y[K, t, J] = sum_Q <K| R_t |Q> zket[J, Q]     (one Q per (K, t))
+//
+// l and t run over sorted index tuples only (operators of one kind
+// anticommute), so X and Y are signed gathers of coefficient rows driven by
+// integer excitation tables, and the contraction over K is one dense FP64
+// GEMM.  An expand pass writes the full antisymmetric dim^k layout.  When L
+// is empty, K is the bra space itself and X = zbra^T.
+#include "xr_common.h"
+#include "xr_gemm.h"
+#include "xr_density.h"
+
+// ---------------------------------------------------------------------------
+// operator plans
+// ---------------------------------------------------------------------------
+int xr_plan_op(const char* ops, OpPlan* p) {
+    struct Row { const char* s; OpPlan p; };
+    // side: {k, is_c, slots}; slots name the tuple entry each operator of
+    // the side string uses, leftmost operator first.
+    static const Row rows[] = {
+        {"a",    {1, 0, {0, 0, {0}}, {1, 0, {0}}}},
+        {"c",    {1, 0, {0, 0, {0}}, {1, 1, {0}}}},
+        {"aa",   {2, 0, {0, 0, {0}}, {2, 0, {0, 1}}}},
+        {"cc",   {2, 0, {0, 0, {0}}, {2, 1, {0, 1}}}},
+        {"ca",   {2, 1, {1, 0, {0}}, {1, 0, {0}}}},
+        {"caa",  {3, 1, {1, 0, {0}}, {2, 0, {0, 1}}}},
+        {"cca",  {3, 1, {2, 0, {1, 0}}, {1, 0, {0}}}},
+        {"ccaa", {4, 1, {2, 0, {1, 0}}, {2, 0, {0, 1}}}},
+        {"caaa", {4, 1, {1, 0, {0}}, {3, 0, {0, 1, 2}}}},
+        {"ccca", {4, 1, {3, 0, {2, 1, 0}}, {1, 0, {0}}}},
+    };
+    if (!ops) return 0;
+    for (const Row& r : rows) {
+        if (strcmp(ops, r.s) == 0) {
+            *p = r.p;
+            int d = 0;
+            for (const char* q = ops; *q; q++) d += (*q == 'c') ? 1 : -1;
+            p->delta = d;
+            return 1;
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t d_from_valence(uint64_t v, int n_orbs, int n_core) {
+    if (n_core == 0) return v;
+    int nv = n_orbs - n_core;
+    uint64_t m = (nv >= 64) ? ~0ull : ((1ull << nv) - 1);
+    uint64_t a = (v & m) << n_core;
+    uint64_t b = ((v >> nv) & m) << (n_orbs + n_core);
+    return a | b | d_core_mask(n_orbs, n_core);
+}
+
+__global__ void k_unrank_space(uint64_t* masks, int64_t n, int n_orbs, int n_core, int n_elec) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int nval = 2 * (n_orbs - n_core);
+    masks[i] = d_from_valence(d_lex_unrank(i, nval, n_elec - 2 * n_core), n_orbs, n_core);
+}
+
+// One entry per (K, t): the source determinant the side string maps onto K,
+// as sign * (valence rank + 1), or 0 when the string annihilates it.
+__global__ void k_side_table(SidePlan sd, const uint64_t* __restrict__ Km, int64_t nK, int dim,
+                             int n_orbs, int n_core, int64_t nT, int32_t* __restrict__ tab) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nK * nT) return;
+    int64_t K = i / nT, t = i - K * nT;
+    uint64_t km = Km[K];
+    uint64_t tm = d_lex_unrank(t, dim, sd.k);
+    int orb[3] = {0, 0, 0};
+    {
+        uint64_t x = tm;
+        for (int j = 0; j < sd.k; j++) {
+            orb[j] = __ffsll((unsigned long long)x) - 1;
+            x &= x - 1;
+        }
+    }
+    uint64_t src;
+    if (!sd.is_c) {
+        if (km & tm) { tab[i] = 0; return; }
+        src = km | tm;
+    } else {
+        if ((km & tm) != tm) { tab[i] = 0; return; }
+        src = km & ~tm;
+    }
+    uint64_t core = d_core_mask(n_orbs, n_core);
+    if ((src & core) != core) { tab[i] = 0; return; }
+    uint64_t cur = src;
+    int sg = 1;
+    for (int j = sd.k - 1; j >= 0; j--) {
+        int o = orb[sd.slot[j]];
+        sg *= d_sign_above(cur, o);
+        cur ^= 1ull << o;
+    }
+    int64_t r = d_lex_rank(d_to_valence(src, n_orbs, n_core), dim - 2 * n_core);
+    tab[i] = sg * (int32_t)(r + 1);
+}
+
+// zT[c * nS + s] = z[s * nC + c]
+__global__ void k_transpose(const double* __restrict__ z, int64_t nS, int64_t nC,
+                            double* __restrict__ zT) {
+    __shared__ double tile[32][33];
+    int64_t c0 = (int64_t)blockIdx.x * 32, s0 = (int64_t)blockIdx.y * 32;
+    int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+    for (int j = ty; j < 32; j += 8) {
+        int64_t s = s0 + j, c = c0 + tx;
+        if (s < nS && c < nC) tile[j][tx] = z[s * nC + c];
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {
+        int64_t c = c0 + j, s = s0 + tx;
+        if (s < nS && c < nC) zT[c * nS + s] = tile[tx][j];
+    }
+}
+
+// Signed row gather.  layout 0: out[K][t][s]; layout 1: out[t][K][s].
+__global__ void k_gather(const int32_t* __restrict__ tab, int64_t nK, int64_t nT,
+                         const double* __restrict__ zT, int64_t nS, double* __restrict__ out,
+                         int layout) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nK * nT * nS) return;
+    int64_t s = i % nS;
+    int64_t r = i / nS;
+    int64_t K = r / nT, t = r - K * nT;
+    int32_t e = tab[r];
+    double v = 0.0;
+    if (e > 0) v = zT[(int64_t)(e - 1) * nS + s];
+    else if (e < 0) v = -zT[(int64_t)(-e - 1) * nS + s];
+    if (layout == 0) out[i] = v;
+    else out[(t * nK + K) * nS + s] = v;
+}
+
+// For each flat index (d_0 ... d_{k-1}) in [0, dim^k): the lexical rank of
+// the sorted tuple and the parity of the sort, as sign * (rank + 1); 0 if an
+// index repeats.
+__global__ void k_index_table(int k, int dim, int64_t n, int32_t* tab) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int d[3];
+    int64_t x = i;
+    for (int j = k - 1; j >= 0; j--) {
+        d[j] = (int)(x % dim);
+        x /= dim;
+    }
+    int inv = 0;
+    uint64_t m = 0;
+    for (int a = 0; a < k; a++) {
+        for (int b = a + 1; b < k; b++) {
+            if (d[a] == d[b]) { tab[i] = 0; return; }
+            if (d[a] > d[b]) inv++;
+        }
+        m |= 1ull << d[a];
+    }
+    int32_t r = (int32_t)d_lex_rank(m, dim);
+    tab[i] = ((inv & 1) ? -1 : 1) * (r + 1);
+}
+
+// out[((I*nJ + J)*nL + iL)*nR + iR] from the compressed GEMM result
+// Cc[(l*nI + I) * ldc + t*nJ + J].
+__global__ void k_expand(const double* __restrict__ Cc, int64_t ldc, int64_t nI, int64_t nJ,
+                         const int32_t* __restrict__ tabL, int64_t nL,
+                         const int32_t* __restrict__ tabR, int64_t nR, double* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t total = nI * nJ * nL * nR;
+    if (i >= total) return;
+    int64_t iR = i % nR;
+    int64_t r = i / nR;
+    int64_t iL = r % nL;
+    r /= nL;
+    int64_t J = r % nJ, I = r / nJ;
+    int32_t eL = tabL ? tabL[iL] : 1;
+    int32_t eR = tabR[iR];
+    double v = 0.0;
+    if (eL != 0 && eR != 0) {
+        int64_t l = (eL > 0 ? eL : -eL) - 1;
+        int64_t t = (eR > 0 ? eR : -eR) - 1;
+        v = Cc[(l * nI + I) * ldc + t * nJ + J];
+        if ((eL < 0) != (eR < 0)) v = -v;
+    }
+    out[i] = v;
+}
+
+// Veff[u, w] for sorted pairs u = (p<q), w = (a<b):
+//   V[p,q,b,a] - V[q,p,b,a] - V[p,q,a,b] + V[q,p,a,b]
+__global__ void k_veff(const double* __restrict__ V, int dim, int64_t nU,
+                       double* __restrict__ Ve) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nU * nU) return;
+    int64_t u = i / nU, w = i - u * nU;
+    uint64_t mu = d_lex_unrank(u, dim, 2), mw = d_lex_unrank(w, dim, 2);
+    int64_t p = __ffsll((unsigned long long)mu) - 1;
+    int64_t q = 63 - __clzll((long long)mu);
+    int64_t a = __ffsll((unsigned long long)mw) - 1;
+    int64_t b = 63 - __clzll((long long)mw);
+    int64_t D = dim;
+    auto at = [&](int64_t x0, int64_t x1, int64_t x2, int64_t x3) {
+        return V[((x0 * D + x1) * D + x2) * D + x3];
+    };
+    Ve[i] = at(p, q, b, a) - at(q, p, b, a) - at(p, q, a, b) + at(q, p, a, b);
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+int xr_space_masks(xr_ctx* c, int n_orbs, int n_core, int n_elec, const uint64_t** masks,
+                   int64_t* n) {
+    auto key = std::make_pair(n_orbs * 1024 + n_core, n_elec);
+    auto it = c->kspace.find(key);
+    if (it != c->kspace.end()) {
+        *masks = it->second.first;
+        *n = it->second.second;
+        return XR_OK;
+    }
+    int64_t cnt = h_binom(2 * (n_orbs - n_core), n_elec - 2 * n_core);
+    if (cnt <= 0) return xr_fail(c, XR_EINVAL, "empty determinant space");
+    uint64_t* p = nullptr;
+    XR_HIP(c, hipMalloc(&p, cnt * sizeof(uint64_t)));
+    k_unrank_space<<<ceil_div(cnt, 256), 256, 0, c->stream>>>(p, cnt, n_orbs, n_core, n_elec);
+    XR_LAUNCH_CHECK(c);
+    c->kspace[key] = std::make_pair(p, cnt);
+    *masks = p;
+    *n = cnt;
+    return XR_OK;
+}
+
+int xr_transpose(xr_ctx* c, const double* z, int64_t nS, int64_t nC, double* zT) {
+    if (nS <= 0 || nC <= 0) return XR_OK;
+    dim3 grid((unsigned)ceil_div(nC, 32), (unsigned)ceil_div(nS, 32));
+    k_transpose<<<grid, dim3(32, 8), 0, c->stream>>>(z, nS, nC, zT);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+int xr_side_table(xr_ctx* c, const SidePlan& sd, const uint64_t* Km, int64_t nK, int n_orbs,
+                  int n_core, int32_t* tab) {
+    int64_t nT = h_binom(2 * n_orbs, sd.k);
+    int64_t n = nK * nT;
+    if (n <= 0) return XR_OK;
+    k_side_table<<<ceil_div(n, 256), 256, 0, c->stream>>>(sd, Km, nK, 2 * n_orbs, n_orbs, n_core,
+                                                           nT, tab);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+int xr_gather(xr_ctx* c, const int32_t* tab, int64_t nK, int64_t nT, const double* zT,
+              int64_t nS, double* out, int layout) {
+    int64_t n = nK * nT * nS;
+    if (n <= 0) return XR_OK;
+    k_gather<<<ceil_div(n, 256), 256, 0, c->stream>>>(tab, nK, nT, zT, nS, out, layout);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+int xr_index_table(xr_ctx* c, int k, int dim, int32_t* tab) {
+    int64_t n = 1;
+    for (int j = 0; j < k; j++) n *= dim;
+    k_index_table<<<ceil_div(n, 256), 256, 0, c->stream>>>(k, dim, n, tab);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+static int check_spaces(xr_ctx* c, const OpPlan& pl, int n_orbs, int n_core, int n_elec_ket,
+                        int64_t nP, int64_t nQ, int* n_elec_bra) {
+    if (n_orbs <= 0 || 2 * n_orbs > XR_MAX_SO) return xr_fail(c, XR_EUNSUPPORTED, "need 0 < 2*n_orbs <= 64");
+    if (n_core < 0 || n_core > n_orbs) return xr_fail(c, XR_EINVAL, "bad n_core");
+    int nb = n_elec_ket + pl.delta;
+    if (n_elec_ket < 2 * n_core || n_elec_ket > 2 * n_orbs || nb < 2 * n_core || nb > 2 * n_orbs)
+        return xr_fail(c, XR_EINVAL, "electron counts incompatible with orbitals / core");
+    int nval = 2 * (n_orbs - n_core);
+    if (nQ != h_binom(nval, n_elec_ket - 2 * n_core))
+        return xr_fail(c, XR_ESHAPE, "ket coefficient columns != size of the ket determinant space");
+    if (nP != h_binom(nval, nb - 2 * n_core))
+        return xr_fail(c, XR_ESHAPE, "bra coefficient columns != size of the bra determinant space");
+    if (nP >= 0x7fffffff || nQ >= 0x7fffffff) return xr_fail(c, XR_EUNSUPPORTED, "determinant space >= 2^31");
+    *n_elec_bra = nb;
+    return XR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// entry points
+// ---------------------------------------------------------------------------
+enum { WS_ZTB = 0, WS_ZTK, WS_TABL, WS_TABR, WS_X, WS_Y, WS_C, WS_IDXL, WS_IDXR, WS_VE, WS_W };
+
+int xr_density_impl(xr_ctx* c, const char* ops, int n_orbs, int n_core, int n_elec_ket,
+                    int64_t nI, int64_t nP, const double* zbra, int64_t nJ, int64_t nQ,
+                    const double* zket, double* out) {
+    OpPlan pl;
+    if (!xr_plan_op(ops, &pl)) return xr_fail(c, XR_EUNSUPPORTED, std::string("operator string not supported: ") + (ops ? ops : "(null)"));
+    int nb = 0;
+    int rc = check_spaces(c, pl, n_orbs, n_core, n_elec_ket, nP, nQ, &nb);
+    if (rc) return rc;
+    const int dim = 2 * n_orbs;
+    if (nI <= 0 || nJ <= 0) return XR_OK;
+    int64_t out_size = nI * nJ;
+    for (int j = 0; j < pl.k; j++) out_size *= dim;
+    if (pl.L.k && (n_elec_ket - pl.R.k < 0))  // R annihilates more electrons than the ket has
+        return xr_fill(c, out, out_size, 0.0);
+    double* zTb = (double*)xr_workspace(c, WS_ZTB, nP * nI * sizeof(double));
+    double* zTk = (double*)xr_workspace(c, WS_ZTK, nQ * nJ * sizeof(double));
+    if (!zTb || !zTk) return xr_fail(c, XR_ENOMEM, "workspace (coefficients)");
+    if ((rc = xr_transpose(c, zbra, nI, nP, zTb))) return rc;
+    if ((rc = xr_transpose(c, zket, nJ, nQ, zTk))) return rc;
+
+    const int64_t nT = h_binom(dim, pl.R.k);
+    const int64_t nLt = pl.L.k ? h_binom(dim, pl.L.k) : 1;
+    const uint64_t* Km = nullptr;
+    int64_t nK = 0;
+    const double* X = nullptr;
+    if (pl.L.k == 0) {
+        // K is the bra space: X = zbra^T
+        if ((rc = xr_space_masks(c, n_orbs, n_core, nb, &Km, &nK))) return rc;
+        X = zTb;
+    } else {
+        // K is the complete space between L and R (core holes allowed)
+        int nK_e = n_elec_ket - pl.R.k;
+        if ((rc = xr_space_masks(c, n_orbs, 0, nK_e, &Km, &nK))) return rc;
+        int32_t* tabL = (int32_t*)xr_workspace(c, WS_TABL, nK * nLt * sizeof(int32_t));
+        double* Xb = (double*)xr_workspace(c, WS_X, nK * nLt * nI * sizeof(double));
+        if (!tabL || !Xb) return xr_fail(c, XR_ENOMEM, "workspace (bra gather)");
+        if ((rc = xr_side_table(c, pl.L, Km, nK, n_orbs, n_core, tabL))) return rc;
+        if ((rc = xr_gather(c, tabL, nK, nLt, zTb, nI, Xb, 0))) return rc;
+        X = Xb;
+    }
+    int32_t* tabR = (int32_t*)xr_workspace(c, WS_TABR, nK * nT * sizeof(int32_t));
+    double* Y = (double*)xr_workspace(c, WS_Y, nK * nT * nJ * sizeof(double));
+    if (!tabR || !Y) return xr_fail(c, XR_ENOMEM, "workspace (ket gather)");
+    if ((rc = xr_side_table(c, pl.R, Km, nK, n_orbs, n_core, tabR))) return rc;
+    if ((rc = xr_gather(c, tabR, nK, nT, zTk, nJ, Y, 0))) return rc;
+
+    const int64_t Mg = nLt * nI, Ng = nT * nJ;
+    double* Cc = (double*)xr_workspace(c, WS_C, Mg * Ng * sizeof(double));
+    if (!Cc) return xr_fail(c, XR_ENOMEM, "workspace (compressed density)");
+    if ((rc = xr_dgemm_launch(c, 1, 0, Mg, Ng, nK, 1.0, X, Mg, 0, Y, Ng, 0, 0.0, Cc, Ng, 0, 1)))
+        return rc;
+
+    int64_t nIL = 1, nIR = 1;
+    for (int j = 0; j < pl.L.k; j++) nIL *= dim;
+    for (int j = 0; j < pl.R.k; j++) nIR *= dim;
+    int32_t* idxL = pl.L.k ? (int32_t*)xr_workspace(c, WS_IDXL, nIL * sizeof(int32_t)) : nullptr;
+    int32_t* idxR = (int32_t*)xr_workspace(c, WS_IDXR, nIR * sizeof(int32_t));
+    if ((pl.L.k && !idxL) || !idxR) return xr_fail(c, XR_ENOMEM, "workspace (index tables)");
+    if (pl.L.k && (rc = xr_index_table(c, pl.L.k, dim, idxL))) return rc;
+    if ((rc = xr_index_table(c, pl.R.k, dim, idxR))) return rc;
+    int64_t total = nI * nJ * nIL * nIR;
+    k_expand<<<ceil_div(total, 256), 256, 0, c->stream>>>(Cc, Ng, nI, nJ, idxL, nIL, idxR, nIR, out);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+extern "C" int xr_density(xr_ctx* c, const char* ops, int n_orbs, int n_core, int n_elec_ket,
+                          int64_t nI, int64_t nP, const double* zbra, int64_t nJ, int64_t nQ,
+                          const double* zket, double* out) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    return xr_density_impl(c, ops, n_orbs, n_core, n_elec_ket, nI, nP, zbra, nJ, nQ, zket, out);
+}
+
+extern "C" int xr_monomer_2e(xr_ctx* c, int n_orbs, int n_core, int n_elec, int64_t nI, int64_t nP,
+                             const double* zbra, int64_t nJ, int64_t nQ, const double* zket,
+                             const double* V, double* out) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    OpPlan pl;
+    xr_plan_op("ccaa", &pl);
+    int nb = 0;
+    int rc = check_spaces(c, pl, n_orbs, n_core, n_elec, nP, nQ, &nb);
+    if (rc) return rc;
+    if (nI <= 0 || nJ <= 0) return XR_OK;
+    const int dim = 2 * n_orbs;
+    if (n_elec < 2) return xr_fill(c, out, nI * nJ, 0.0);
+    double* zTb = (double*)xr_workspace(c, WS_ZTB, nP * nI * sizeof(double));
+    double* zTk = (double*)xr_workspace(c, WS_ZTK, nQ * nJ * sizeof(double));
+    if (!zTb || !zTk) return xr_fail(c, XR_ENOMEM, "workspace (coefficients)");
+    if ((rc = xr_transpose(c, zbra, nI, nP, zTb))) return rc;
+    if ((rc = xr_transpose(c, zket, nJ, nQ, zTk))) return rc;
+    const uint64_t* Km = nullptr;
+    int64_t nK = 0;
+    if ((rc = xr_space_masks(c, n_orbs, 0, n_elec - 2, &Km, &nK))) return rc;
+    const int64_t nU = h_binom(dim, 2);
+    int32_t* tabL = (int32_t*)xr_workspace(c, WS_TABL, nK * nU * sizeof(int32_t));
+    int32_t* tabR = (int32_t*)xr_workspace(c, WS_TABR, nK * nU * sizeof(int32_t));
+    double* X = (double*)xr_workspace(c, WS_X, nU * nK * nI * sizeof(double));
+    double* Y = (double*)xr_workspace(c, WS_Y, nU * nK * nJ * sizeof(double));
+    double* W = (double*)xr_workspace(c, WS_W, nU * nK * nJ * sizeof(double));
+    double* Ve = (double*)xr_workspace(c, WS_VE, nU * nU * sizeof(double));
+    if (!tabL || !tabR || !X || !Y || !W || !Ve) return xr_fail(c, XR_ENOMEM, "workspace (sigma)");
+    if ((rc = xr_side_table(c, pl.L, Km, nK, n_orbs, n_core, tabL))) return rc;
+    if ((rc = xr_side_table(c, pl.R, Km, nK, n_orbs, n_core, tabR))) return rc;
+    if ((rc = xr_gather(c, tabL, nK, nU, zTb, nI, X, 1))) return rc;   // X[u][K][I]
+    if ((rc = xr_gather(c, tabR, nK, nU, zTk, nJ, Y, 1))) return rc;   // Y[w][K][J]
+    k_veff<<<ceil_div(nU * nU, 256), 256, 0, c->stream>>>(V, dim, nU, Ve);
+    XR_LAUNCH_CHECK(c);
+    // W[u][K*nJ + J] = sum_w Ve[u,w] Y[w][K*nJ + J]
+    if ((rc = xr_dgemm_launch(c, 0, 0, nU, nK * nJ, nU, 1.0, Ve, nU, 0, Y, nK * nJ, 0, 0.0, W,
+                              nK * nJ, 0, 1)))
+        return rc;
+    // E[I,J] = sum_{(u,K)} X[(u,K), I] W[(u,K), J]
+    return xr_dgemm_launch(c, 1, 0, nI, nJ, nU * nK, 1.0, X, nI, 0, W, nJ, 0, 0.0, out, nJ, 0, 1);
+}

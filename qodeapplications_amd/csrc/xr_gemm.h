@@ -1,0 +1,9 @@
+// Internal GEMM launchers shared by the density and contraction drivers.
+#pragma once
+#include "xr_common.h"
+
+// FP64 MFMA GEMM (no locking; callers hold the context lock).
+// ta = 1: A is K x M (A[k*lda+m]); tb = 1: B is N x K (B[n*ldb+k]).
+int xr_dgemm_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
+                    const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
+                    int64_t sB, double beta, double* C, int64_t ldc, int64_t sC, int64_t batch);

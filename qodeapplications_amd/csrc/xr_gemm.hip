@@ -1,0 +1,222 @@
+// FP64 GEMM on the CDNA4 matrix cores (v_mfma_f64_16x16x4f64).
+//
+// C_b = alpha * op(A_b) op(B_b) + beta * C_b.  256 threads = 4 waves per
+// workgroup in a 2 x 2 arrangement; each wave owns a (BM/2) x (BN/2) block of
+// 16 x 16 MFMA tiles.  K advances 16 at a time through one LDS stage while the
+// next stage is prefetched into registers.  Workgroups are remapped so each
+// XCD (own L2) receives a contiguous run of output tiles.
+#include "xr_common.h"
+#include "xr_gemm.h"
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+#define GEMM_BK 16
+#define GEMM_THREADS 256
+
+// blockIdx.x -> logical tile id, so that the ids one XCD executes (hardware
+// hands out workgroups round-robin over the 8 XCDs) are contiguous.
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t total) {
+    const int64_t nx = 8;
+    int64_t q = total / nx, r = total % nx;
+    int64_t x = b % nx, j = b / nx;
+    int64_t base = (x < r) ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+    return base + j;
+}
+
+template <int BM, int BN, bool TA, bool TB>
+__global__ __launch_bounds__(GEMM_THREADS) void k_dgemm(
+    int64_t M, int64_t N, int64_t K, double alpha, const double* __restrict__ A, int64_t lda,
+    int64_t sA, const double* __restrict__ B, int64_t ldb, int64_t sB, double beta,
+    double* __restrict__ C, int64_t ldc, int64_t sC, int64_t tiles_m, int64_t tiles_n) {
+    constexpr int PAD = 4;
+    constexpr int WM = BM / 2, WN = BN / 2;    // wave tile
+    constexpr int FM = WM / 16, FN = WN / 16;  // MFMA tiles per wave
+    constexpr int A_PER = BM * GEMM_BK / GEMM_THREADS;
+    constexpr int B_PER = BN * GEMM_BK / GEMM_THREADS;
+    __shared__ double As[GEMM_BK][BM + PAD];
+    __shared__ double Bs[GEMM_BK][BN + PAD];
+
+    const int64_t batch = blockIdx.z;
+    A += batch * sA;
+    B += batch * sB;
+    C += batch * sC;
+    const int64_t tile = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+    // groups of 8 tile rows walk the tile columns together (L2 reuse of B)
+    const int64_t GROUP = 8;
+    const int64_t per_group = GROUP * tiles_n;
+    const int64_t g = tile / per_group;
+    const int64_t first_m = g * GROUP;
+    const int64_t gsize = (tiles_m - first_m < GROUP) ? (tiles_m - first_m) : GROUP;
+    const int64_t tm = first_m + (tile % per_group) % gsize;
+    const int64_t tn = (tile % per_group) / gsize;
+    const int64_t m0 = tm * BM, n0 = tn * BN;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+
+    double ra[A_PER], rb[B_PER];
+
+    auto load_a = [&](int64_t k0) {
+#pragma unroll
+        for (int i = 0; i < A_PER; i++) {
+            int mm, kk;
+            if (TA) {  // A[k][m]
+                mm = tid % BM;
+                kk = tid / BM + i * (GEMM_THREADS / BM);
+            } else {   // A[m][k]
+                kk = tid % GEMM_BK;
+                mm = tid / GEMM_BK + i * (GEMM_THREADS / GEMM_BK);
+            }
+            int64_t gm = m0 + mm, gk = k0 + kk;
+            double v = 0.0;
+            if (gm < M && gk < K) v = TA ? A[gk * lda + gm] : A[gm * lda + gk];
+            ra[i] = v;
+        }
+    };
+    auto load_b = [&](int64_t k0) {
+#pragma unroll
+        for (int i = 0; i < B_PER; i++) {
+            int nn, kk;
+            if (!TB) {  // B[k][n]
+                nn = tid % BN;
+                kk = tid / BN + i * (GEMM_THREADS / BN);
+            } else {    // B[n][k]
+                kk = tid % GEMM_BK;
+                nn = tid / GEMM_BK + i * (GEMM_THREADS / GEMM_BK);
+            }
+            int64_t gn = n0 + nn, gk = k0 + kk;
+            double v = 0.0;
+            if (gn < N && gk < K) v = TB ? B[gn * ldb + gk] : B[gk * ldb + gn];
+            rb[i] = v;
+        }
+    };
+    auto store_ab = [&]() {
+#pragma unroll
+        for (int i = 0; i < A_PER; i++) {
+            int mm, kk;
+            if (TA) { mm = tid % BM; kk = tid / BM + i * (GEMM_THREADS / BM); }
+            else { kk = tid % GEMM_BK; mm = tid / GEMM_BK + i * (GEMM_THREADS / GEMM_BK); }
+            As[kk][mm] = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < B_PER; i++) {
+            int nn, kk;
+            if (!TB) { nn = tid % BN; kk = tid / BN + i * (GEMM_THREADS / BN); }
+            else { kk = tid % GEMM_BK; nn = tid / GEMM_BK + i * (GEMM_THREADS / GEMM_BK); }
+            Bs[kk][nn] = rb[i];
+        }
+    };
+
+    d4_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+
+    const int64_t nk = (K + GEMM_BK - 1) / GEMM_BK;
+    if (nk > 0) {
+        load_a(0);
+        load_b(0);
+        store_ab();
+        __syncthreads();
+    }
+    const int fr = lane & 15;   // row / column inside a 16-wide fragment
+    const int fk = lane >> 4;   // k inside a 4-deep step
+    for (int64_t kt = 0; kt < nk; kt++) {
+        if (kt + 1 < nk) {
+            load_a((kt + 1) * GEMM_BK);
+            load_b((kt + 1) * GEMM_BK);
+        }
+#pragma unroll
+        for (int ks = 0; ks < GEMM_BK / 4; ks++) {
+            double af[FM], bf[FN];
+#pragma unroll
+            for (int i = 0; i < FM; i++) af[i] = As[ks * 4 + fk][wm * WM + i * 16 + fr];
+#pragma unroll
+            for (int j = 0; j < FN; j++) bf[j] = Bs[ks * 4 + fk][wn * WN + j * 16 + fr];
+#pragma unroll
+            for (int i = 0; i < FM; i++)
+#pragma unroll
+                for (int j = 0; j < FN; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+        if (kt + 1 < nk) {
+            store_ab();
+            __syncthreads();
+        }
+    }
+
+    // D[row][col] of a 16x16 tile (v_mfma_f64_16x16x4f64, checked on gfx950):
+    // col = lane % 16, row = lane / 16 + 4 * r for accumulator element r
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int64_t gm = m0 + wm * WM + i * 16 + fk + 4 * r;
+                int64_t gn = n0 + wn * WN + j * 16 + fr;
+                if (gm < M && gn < N) {
+                    double v = alpha * acc[i][j][r];
+                    double* cp = C + gm * ldc + gn;
+                    if (beta != 0.0) v += beta * (*cp);
+                    *cp = v;
+                }
+            }
+}
+
+template <int BM, int BN>
+static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
+                        const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
+                        int64_t sB, double beta, double* C, int64_t ldc, int64_t sC,
+                        int64_t batch) {
+    int64_t tm = ceil_div(M, BM), tn = ceil_div(N, BN);
+    int64_t tiles = tm * tn;
+    if (tiles > 0x7fffffff || batch > 65535)
+        return xr_fail(c, XR_EUNSUPPORTED, "GEMM grid too large");
+    dim3 grid((unsigned)tiles, 1, (unsigned)batch);
+#define XR_GEMM_LAUNCH(TA_, TB_)                                                               \
+    k_dgemm<BM, BN, TA_, TB_><<<grid, GEMM_THREADS, 0, c->stream>>>(M, N, K, alpha, A, lda, sA, \
+                                                                     B, ldb, sB, beta, C, ldc,  \
+                                                                     sC, tm, tn)
+    if (ta && tb) XR_GEMM_LAUNCH(true, true);
+    else if (ta) XR_GEMM_LAUNCH(true, false);
+    else if (tb) XR_GEMM_LAUNCH(false, true);
+    else XR_GEMM_LAUNCH(false, false);
+#undef XR_GEMM_LAUNCH
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+// Chooses the tile shape: 128x128 when the grid still fills the chip,
+// smaller tiles for skinny or small problems.
+int xr_dgemm_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
+                    const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
+                    int64_t sB, double beta, double* C, int64_t ldc, int64_t sC, int64_t batch) {
+    if (M <= 0 || N <= 0 || batch <= 0) return XR_OK;
+    const int64_t want = 512;  // workgroups that keep 256 CUs busy
+    int64_t big = ceil_div(M, 128) * ceil_div(N, 128) * batch;
+    if (big >= want) return launch_tiles<128, 128>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+    if (N <= 64 && M > 64)
+        return launch_tiles<128, 64>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+    if (M <= 64 && N > 64)
+        return launch_tiles<64, 128>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+    return launch_tiles<64, 64>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+}
+
+extern "C" int xr_gemm_f64(xr_ctx* c, int transA, int transB, int64_t M, int64_t N, int64_t K,
+                           double alpha, const double* A, int64_t lda, int64_t sA,
+                           const double* B, int64_t ldb, int64_t sB, double beta, double* C,
+                           int64_t ldc, int64_t sC, int64_t batch) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (M < 0 || N < 0 || K < 0 || batch < 0) return xr_fail(c, XR_EINVAL, "negative GEMM size");
+    if (ldc < N) return xr_fail(c, XR_ESHAPE, "ldc < N");
+    if ((transA ? lda < M : lda < K) && M > 0 && K > 0) return xr_fail(c, XR_ESHAPE, "lda too small");
+    if ((transB ? ldb < K : ldb < N) && N > 0 && K > 0) return xr_fail(c, XR_ESHAPE, "ldb too small");
+    return xr_dgemm_launch(c, transA, transB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc,
+                           sC, batch);
+}

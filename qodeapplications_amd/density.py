@@ -1,0 +1,150 @@
+"""Device-resident fragment state blocks and transition densities.
+
+Thin host layer over libxrhip: coefficients and determinant masks live in
+HBM as torch tensors (the container), every density, excitation list and
+monomer energy is computed by the HIP kernels behind the C ABI.
+"""
+import math
+
+import numpy as np
+import torch
+
+from ._lib import context, ptr
+
+# operator strings built for each charge difference bra_chg - ket_chg
+# (general-XRCC/build_density_tensors.py:90-157)
+OPS_BY_CHG_DIFF = {+2: ("aa",), +1: ("a", "caa"), 0: ("ca", "ccaa"), -1: ("c", "cca"), -2: ("cc",)}
+# with the 4-operator strings of Be-states/_attic_/deprecated/density_tensors.c:560-688
+OPS_BY_CHG_DIFF_FULL = {+2: ("aa", "caaa"), +1: ("a", "caa"), 0: ("ca", "ccaa"),
+                        -1: ("c", "cca"), -2: ("cc", "ccca")}
+SUPPORTED_OPS = ("a", "c", "ca", "aa", "cc", "caa", "cca", "ccaa", "caaa", "ccca")
+
+
+def n_elec_change(op):
+    return sum(+1 if o == "c" else -1 for o in op)
+
+
+def _device(device):
+    return torch.device("cuda", device if isinstance(device, int) else torch.device(device).index or 0)
+
+
+class ChargeBlock:
+    """One charge block of one fragment on the GPU: the states' coefficients
+    (n_states x n_configs, float64) and the determinants as 64-bit masks,
+    checked to be the complete lexical enumeration."""
+
+    def __init__(self, n_orbs, n_core, configs, coeffs, device=0, check=True):
+        self.device = _device(device)
+        self.n_orbs = int(n_orbs)
+        self.n_core = int(n_core)
+        self.dim = 2 * self.n_orbs
+        cfg = np.ascontiguousarray(np.asarray(configs, dtype=np.int64))
+        if cfg.ndim != 2:
+            raise ValueError("configs must be (n_configs x n_elec)")
+        self.n_elec = cfg.shape[1]
+        self.coeffs = torch.as_tensor(np.ascontiguousarray(np.asarray(coeffs, dtype=np.float64)),
+                                      device=self.device)
+        if self.coeffs.ndim != 2 or self.coeffs.shape[1] != cfg.shape[0]:
+            raise ValueError("coeffs must be (n_states x n_configs) matching configs")
+        self.n_states, self.n_configs = self.coeffs.shape
+        self.masks = torch.empty(cfg.shape[0], dtype=torch.int64, device=self.device)
+        if check:
+            ctx = context(self.device.index)
+            dcfg = torch.as_tensor(cfg, device=self.device)
+            ctx.call("xr_configs_to_masks", ptr(dcfg), cfg.shape[0], self.n_elec, self.n_orbs,
+                     self.n_core, ptr(self.masks))
+
+    @classmethod
+    def from_states(cls, states, n_orbs, n_core, device=0):
+        return cls(n_orbs, n_core, states.configs, states.coeffs, device)
+
+    def with_coeffs(self, coeffs):
+        """Same determinants, new coefficients (already on the device)."""
+        new = object.__new__(ChargeBlock)
+        new.__dict__.update(self.__dict__)
+        new.coeffs = coeffs.contiguous()
+        new.n_states = coeffs.shape[0]
+        return new
+
+
+def density(op, bra, ket, out=None):
+    """<I| op |J> for all state pairs: tensor (nI, nJ, dim, ..., dim) float64."""
+    if op not in SUPPORTED_OPS:
+        raise ValueError(f"unsupported operator string {op!r}")
+    if bra.n_orbs != ket.n_orbs or bra.n_core != ket.n_core:
+        raise ValueError("bra and ket blocks belong to different fragments")
+    if bra.n_elec != ket.n_elec + n_elec_change(op):
+        raise ValueError(f"{op}: bra has {bra.n_elec} electrons, ket {ket.n_elec}")
+    dim = ket.dim
+    shape = (bra.n_states, ket.n_states) + (dim,) * len(op)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float64, device=ket.device)
+    elif tuple(out.shape) != shape or not out.is_contiguous() or out.dtype != torch.float64:
+        raise ValueError("out has the wrong shape / layout")
+    ctx = context(ket.device.index)
+    ctx.call("xr_density", op.encode(), ket.n_orbs, ket.n_core, ket.n_elec,
+             bra.n_states, bra.n_configs, ptr(bra.coeffs),
+             ket.n_states, ket.n_configs, ptr(ket.coeffs), ptr(out))
+    return out
+
+
+def monomer_2e(bra, ket, V):
+    """E[I,J] = sum V[p,q,r,s] <I|p+ q+ s r|J> (nI x nJ), V as (dim,)*4."""
+    if bra.n_elec != ket.n_elec:
+        raise ValueError("monomer_2e needs one charge")
+    V = torch.as_tensor(V, dtype=torch.float64, device=ket.device).contiguous()
+    if tuple(V.shape) != (ket.dim,) * 4:
+        raise ValueError("V must be (dim, dim, dim, dim)")
+    out = torch.empty((bra.n_states, ket.n_states), dtype=torch.float64, device=ket.device)
+    ctx = context(ket.device.index)
+    ctx.call("xr_monomer_2e", ket.n_orbs, ket.n_core, ket.n_elec, bra.n_states, bra.n_configs,
+             ptr(bra.coeffs), ket.n_states, ket.n_configs, ptr(ket.coeffs), ptr(V), ptr(out))
+    return out
+
+
+def excitation_list(op, ket, q0=0, q1=None):
+    """(Q, idx, P, sign) for ket determinants [q0, q1) in the reference loop
+    order (general-XRCC/density_tensors.c:162-556), as device tensors."""
+    if q1 is None:
+        q1 = ket.n_configs
+    ctx = context(ket.device.index)
+    offsets = torch.empty(q1 - q0 + 1, dtype=torch.int64, device=ket.device)
+    import ctypes
+    total = ctypes.c_int64(0)
+    ctx.call("xr_excitation_count", op.encode(), ket.n_orbs, ket.n_core, ket.n_elec,
+             ptr(ket.masks), q0, q1, ptr(offsets), ctypes.byref(total))
+    n = total.value
+    e_q = torch.empty(n, dtype=torch.int64, device=ket.device)
+    e_idx = torch.empty(n, dtype=torch.int64, device=ket.device)
+    e_p = torch.empty(n, dtype=torch.int64, device=ket.device)
+    e_s = torch.empty(n, dtype=torch.int8, device=ket.device)
+    if n:
+        ctx.call("xr_excitation_fill", op.encode(), ket.n_orbs, ket.n_core, ket.n_elec,
+                 ptr(ket.masks), q0, q1, ptr(offsets), ptr(e_q), ptr(e_idx), ptr(e_p), ptr(e_s))
+    return e_q, e_idx, e_p, e_s
+
+
+def config_rank(block):
+    out = torch.empty(block.n_configs, dtype=torch.int64, device=block.device)
+    context(block.device.index).call("xr_config_rank", ptr(block.masks), block.n_configs,
+                                     block.n_orbs, block.n_core, ptr(out))
+    return out
+
+
+def charge_blocks(z_lists, n_orbs, n_core, device=0):
+    return {chg: ChargeBlock.from_states(st, n_orbs, n_core, device) for chg, st in z_lists.items()}
+
+
+def build_density_arrays(blocks, ops_table=OPS_BY_CHG_DIFF_FULL, skip=("ccaa",)):
+    """All densities of one fragment on the device:
+    rho[op][(bra_chg, ket_chg)] = tensor (nI, nJ, dim, ...).  Operator strings
+    in `skip` are not built (the monomer 4-index term goes through
+    monomer_2e instead)."""
+    rho = {}
+    for bra_chg, bb in blocks.items():
+        for ket_chg, kb in blocks.items():
+            for op in ops_table.get(bra_chg - ket_chg, ()):
+                if op in skip:
+                    continue
+                rho.setdefault(op, {})[(bra_chg, ket_chg)] = density(op, bb, kb)
+    return rho

@@ -1,0 +1,44 @@
+"""CPU checks of the C ABI: the library loads and exports every symbol that
+include/xr_hip.h declares (no compute calls without a GPU)."""
+import ctypes
+import os
+import re
+
+from qodeapplications_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "xr_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(xr_[A-Za-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    assert "xr_density" in syms and "xr_gemm_f64" in syms
+    assert len(syms) >= 10
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(_lib.LIB_PATH), "libxrhip.so not built"
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_binding_covers_every_declared_symbol():
+    assert set(declared_symbols()) == set(_lib.SIGNATURES)
+
+
+def test_no_gpu_means_loud_failure():
+    import torch
+    if torch.cuda.is_available():
+        return
+    try:
+        _lib.Context(0)
+    except RuntimeError as e:
+        assert "no GPU" in str(e)
+    else:
+        raise AssertionError("context creation must fail without a GPU")

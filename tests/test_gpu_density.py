@@ -1,0 +1,126 @@
+"""GPU parity of the density path against the CPU oracle.
+
+Integer work (masks, ranks, excitation lists) must be bit-exact; FP64
+densities within 1e-12 absolute."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+from qodeapplications_amd import density as D
+from qodeapplications_amd import synthetic as syn
+from qodeapplications_amd._lib import XRError
+
+pytestmark = pytest.mark.gpu
+ALL_OPS = D.SUPPORTED_OPS
+TOL = 1e-12
+
+
+def _block(rng, n_orbs, n_elec, n_states, n_core=0):
+    st = syn.random_states(n_orbs, n_elec, n_states, rng, n_core)
+    return st, D.ChargeBlock.from_states(st, n_orbs, n_core)
+
+
+@pytest.mark.parametrize("n_orbs,n_core,n_elec", [(4, 1, 4), (9, 0, 4), (13, 0, 5), (18, 0, 3)])
+def test_masks_and_ranks(n_orbs, n_core, n_elec):
+    rng = np.random.default_rng(1)
+    st, b = _block(rng, n_orbs, n_elec, 2, n_core)
+    bits = np.zeros(st.configs.shape[0], dtype=np.uint64)
+    for e in range(st.configs.shape[1]):
+        bits |= np.left_shift(np.uint64(1), st.configs[:, e].astype(np.uint64))
+    assert np.array_equal(b.masks.cpu().numpy().view(np.uint64), bits)
+    assert np.array_equal(D.config_rank(b).cpu().numpy(), np.arange(st.configs.shape[0]))
+
+
+def test_bad_config_lists_raise():
+    rng = np.random.default_rng(2)
+    st = syn.random_states(4, 3, 2, rng)
+    swapped = st.configs.copy()
+    swapped[[0, 1]] = swapped[[1, 0]]
+    with pytest.raises(XRError, match="CONFIG_ORDER"):
+        D.ChargeBlock(4, 0, swapped, st.coeffs)
+    with pytest.raises(XRError, match="CONFIG_ORDER"):
+        D.ChargeBlock(4, 0, st.configs[:-1], st.coeffs[:, :-1])
+    pauli = st.configs.copy()
+    pauli[3, 1] = pauli[3, 0]
+    with pytest.raises(XRError, match="PAULI"):
+        D.ChargeBlock(4, 0, pauli, st.coeffs)
+
+
+@pytest.mark.parametrize("op", ALL_OPS)
+@pytest.mark.parametrize("n_orbs,n_core,n_ket", [(4, 1, 4), (9, 0, 4), (13, 0, 3)])
+def test_excitation_lists_bit_exact(op, n_orbs, n_core, n_ket):
+    rng = np.random.default_rng(3)
+    st, b = _block(rng, n_orbs, n_ket, 1, n_core)
+    nq = st.configs.shape[0]
+    q0, q1 = (0, nq) if nq <= 400 else (nq // 3, nq // 3 + 150)
+    ref = orc.excitations(op, n_orbs, n_core, st.configs, q0, q1)
+    got = [t.cpu().numpy() for t in D.excitation_list(op, b, q0, q1)]
+    for r, g in zip(ref, got):
+        assert r.shape == g.shape
+        assert np.array_equal(r, g.astype(r.dtype))
+
+
+@pytest.mark.parametrize("op", ALL_OPS)
+@pytest.mark.parametrize("n_orbs,n_core,n_ket,ns", [(4, 1, 4, 3), (4, 0, 2, 5), (9, 0, 4, 8), (6, 0, 3, 17)])
+def test_density_matches_oracle(op, n_orbs, n_core, n_ket, ns):
+    n_bra = n_ket + D.n_elec_change(op)
+    if n_bra < 2 * n_core or n_bra > 2 * n_orbs:
+        pytest.skip("empty bra space")
+    rng = np.random.default_rng(4)
+    sk, kb = _block(rng, n_orbs, n_ket, ns, n_core)
+    sb, bb = _block(rng, n_orbs, n_bra, ns + 2, n_core)
+    ref = orc.density(op, n_orbs, n_core, sb.coeffs, sk.coeffs, sk.configs)
+    got = D.density(op, bb, kb).cpu().numpy()
+    assert got.shape == ref.shape
+    np.testing.assert_allclose(got, ref, atol=TOL, rtol=0)
+
+
+@pytest.mark.parametrize("n_orbs,n_core,n_elec", [(4, 1, 4), (6, 0, 4), (9, 0, 5)])
+def test_monomer_2e_matches_oracle(n_orbs, n_core, n_elec):
+    rng = np.random.default_rng(5)
+    st, b = _block(rng, n_orbs, n_elec, 6, n_core)
+    ints = syn.FragmentBlocked([n_orbs], syn.spatial_h(n_orbs, rng), syn.spatial_V(n_orbs, rng))
+    V = ints.V[0, 0, 0, 0]
+    R = orc.density("ccaa", n_orbs, n_core, st.coeffs, st.coeffs, st.configs)
+    ref = np.array([[orc.monomer_2e(2 * n_orbs, R[i, j], V) for j in range(6)] for i in range(6)])
+    got = D.monomer_2e(b, b, V).cpu().numpy()
+    np.testing.assert_allclose(got, ref, atol=TOL, rtol=0)
+
+
+def test_build_density_tensors_drop_in():
+    from qodeapplications_amd.build_density_tensors import build_density_tensors
+    rng = np.random.default_rng(6)
+    frag = syn.make_fragment(4, [3, 4, 5], 3, 4, rng, n_core=1)
+    ints = syn.FragmentBlocked([4], syn.spatial_h(4, rng), syn.spatial_V(4, rng))
+    V = ints.V[0, 0, 0, 0]
+    got, gsize = build_density_tensors(frag.states, 4, V, 1)
+    ref, rsize = orc.build_density_tensors(frag.states, 4, V, 1)
+    assert gsize == rsize
+    assert set(got) == set(ref)
+    for op in ref:
+        assert set(got[op]) == set(ref[op])
+        for key in ref[op]:
+            for i, row in enumerate(ref[op][key]):
+                for j, r in enumerate(row):
+                    g = got[op][key][i][j]
+                    if op == "ccaa":
+                        assert isinstance(g, float)
+                        assert abs(g - r) < TOL
+                    else:
+                        assert g.shape == r.shape
+                        np.testing.assert_allclose(g, r, atol=TOL, rtol=0)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("M,N,K,batch", [(1, 1, 1, 1), (37, 129, 17, 3), (200, 64, 300, 1), (64, 513, 70, 2), (300, 300, 64, 1)])
+def test_gemm_f64(ta, tb, M, N, K, batch):
+    from qodeapplications_amd import gemm
+    g = torch.Generator().manual_seed(M * 7 + N)
+    A = torch.randn(batch, *((K, M) if ta else (M, K)), dtype=torch.float64, generator=g)
+    B = torch.randn(batch, *((N, K) if tb else (K, N)), dtype=torch.float64, generator=g)
+    C = torch.randn(batch, M, N, dtype=torch.float64, generator=g)
+    ref = 0.7 * (A.transpose(1, 2) if ta else A) @ (B.transpose(1, 2) if tb else B) - 0.5 * C
+    Cd = C.cuda()
+    gemm.gemm_f64(A.cuda(), B.cuda(), Cd, transA=ta, transB=tb, alpha=0.7, beta=-0.5)
+    np.testing.assert_allclose(Cd.cpu().numpy(), ref.numpy(), atol=1e-12 * max(1, K) ** 0.5, rtol=1e-12)
