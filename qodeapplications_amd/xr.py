@@ -1,0 +1,217 @@
+"""Batched XR Hamiltonian on the GPU.
+
+XRSystem builds, for a supersystem of fragments, every density the XR blocks
+need (one HIP call per operator string and charge pair), the monomer blocks
+H1[m], the pair blocks H2[m][n] and optionally the trimer blocks H3, all in
+the layouts of general-XRCC/test_H.py:63-108, and assembles the tensor-
+product XR matrix (general-XRCC/hamiltonian.py:21-83) on the device.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import blocks as B
+from ._lib import context, ptr
+from .density import OPS_BY_CHG_DIFF, ChargeBlock, build_density_arrays, monomer_2e
+
+
+class XRFragment:
+    """One fragment on the device: its charge blocks, densities and the
+    position of every (charge, index) state in the fragment's state list."""
+
+    def __init__(self, frag, device):
+        self.device = device
+        self.n_orbs, self.n_core, self.n_elec_ref = frag.n_orbs, frag.n_core, frag.n_elec_ref
+        self.dim = 2 * frag.n_orbs
+        self.charges = list(frag.states)
+        self.blocks = {chg: ChargeBlock.from_states(st, frag.n_orbs, frag.n_core, device)
+                       for chg, st in frag.states.items()}
+        self.state_indices = list(frag.state_indices)
+        self.d = len(self.state_indices)
+        where = {si: k for k, si in enumerate(self.state_indices)}
+        self.pos = {}
+        for chg, b in self.blocks.items():
+            idx = [where.get((chg, i), -1) for i in range(b.n_states)]
+            if min(idx) < 0:
+                raise ValueError(f"state_indices misses states of charge {chg}")
+            self.pos[chg] = torch.tensor(idx, dtype=torch.int64, device=device)
+        self.n_states = {chg: b.n_states for chg, b in self.blocks.items()}
+        self.rho = None
+        self.E2 = None
+
+    def build(self, V_mm):
+        """Densities for all charge pairs (not the 4-index one) and the
+        monomer two-electron matrices."""
+        self.rho = build_density_arrays(self.blocks, OPS_BY_CHG_DIFF, skip=("ccaa",))
+        self.E2 = {chg: monomer_2e(b, b, V_mm) for chg, b in self.blocks.items()}
+
+    def parity(self):
+        return torch.tensor([(self.n_elec_ref - chg) % 2 for chg, _ in self.state_indices],
+                            dtype=torch.int8, device=self.device)
+
+
+class XRSystem:
+    """Device-resident XR model of a supersystem (fragments with .n_orbs,
+    .n_core, .n_elec_ref, .states, .state_indices; fragment-blocked
+    integrals with T, U, V; nuclear repulsion matrix)."""
+
+    def __init__(self, fragments, integrals, nuc, device=0):
+        self.device = torch.device("cuda", device)
+        self.frags = [XRFragment(f, self.device) for f in fragments]
+        self.ints = B.IntegralCache(integrals, self.device)
+        self.nuc = np.asarray(nuc, dtype=np.float64)
+        self.built = False
+
+    def build_densities(self):
+        for m, F in enumerate(self.frags):
+            F.build(self.ints.V(m, m, m, m))
+        self.built = True
+
+    def _need(self):
+        if not self.built:
+            self.build_densities()
+
+    # -- blocks -------------------------------------------------------------
+    def monomer_matrix(self, m):
+        """H1[m][I,J] = build_matrix_elements.monomer (build_H.py:42-55)."""
+        self._need()
+        F = self.frags[m]
+        out = torch.zeros(F.d, F.d, dtype=torch.float64, device=self.device)
+        h = self.ints.h_sum((m, m, "mono"), [("T", (m, m)), ("U", (m, m, m))])
+        for chg in F.charges:
+            n = F.n_states[chg]
+            e1 = B.monomer_1e(F.rho["ca"][(chg, chg)], h)
+            p = F.pos[chg]
+            B.scatter(out, F.d, [(n, n)], [p], [p], [F.d], e1, [1])
+            B.scatter(out, F.d, [(n, n)], [p], [p], [F.d], F.E2[chg], [1])
+            if self.nuc[m, m] != 0.0:
+                B.scatter(out, F.d, [(n, n)], [p], [p], [F.d], None, [0], diag=[1], scale=self.nuc[m, m])
+        return out
+
+    def pair_charge_combos(self, m1, m2):
+        F1, F2 = self.frags[m1], self.frags[m2]
+        for c1b in F1.charges:
+            for c1k in F1.charges:
+                if abs(c1b - c1k) > 2:
+                    continue
+                for c2b in F2.charges:
+                    c2k = c1b + c2b - c1k
+                    if c2k in F2.charges:
+                        yield (c1b, c1k), (c2b, c2k)
+
+    def pair_matrix(self, m1, m2):
+        """H2[m1][m2] over the product basis [(a, b) for a in states(m1) for b
+        in states(m2)] = build_matrix_elements.dimer (build_H.py:56-102)."""
+        self._need()
+        F1, F2 = self.frags[m1], self.frags[m2]
+        D = F1.d * F2.d
+        out = torch.zeros(D, D, dtype=torch.float64, device=self.device)
+        dims = [F1.d, F2.d]
+        for k1, k2 in self.pair_charge_combos(m1, m2):
+            nI1, nJ1 = F1.n_states[k1[0]], F1.n_states[k1[1]]
+            nI2, nJ2 = F2.n_states[k2[0]], F2.n_states[k2[1]]
+            rows = [F1.pos[k1[0]], F2.pos[k2[0]]]
+            cols = [F1.pos[k1[1]], F2.pos[k2[1]]]
+            fr = [(nI1, nJ1), (nI2, nJ2)]
+            terms, extras = B.dimer_block(m1, m2, F2.n_elec_ref, F1.rho, F2.rho, k1, k2, self.ints)
+            nA, nB = nI1 * nJ1, nI2 * nJ2
+            for Hb, swapped, scale in terms:
+                strides = [1, nA] if swapped else [nB, 1]
+                B.scatter(out, D, fr, rows, cols, dims, Hb, strides, scale=scale)
+            for kind, v in extras:
+                if kind == "e2":   # where I1 == J1
+                    B.scatter(out, D, fr, rows, cols, dims, v, [0, 1], diag=[1, 0])
+                else:              # "e1", where I2 == J2
+                    B.scatter(out, D, fr, rows, cols, dims, v, [1, 0], diag=[0, 1])
+            if k1[0] == k1[1] and k2[0] == k2[1] and self.nuc[m1, m2] != 0.0:
+                B.scatter(out, D, fr, rows, cols, dims, None, [0, 0], diag=[1, 1], scale=self.nuc[m1, m2])
+        return out
+
+    def trimer_matrix(self, m1, m2, m3):
+        """H3[m1][m2][m3] = build_matrix_elements.trimer (build_H.py:103-188)."""
+        self._need()
+        ms = (m1, m2, m3)
+        Fs = [self.frags[m] for m in ms]
+        dims = [F.d for F in Fs]
+        D = dims[0] * dims[1] * dims[2]
+        out = torch.zeros(D, D, dtype=torch.float64, device=self.device)
+        charges = [F.charges for F in Fs]
+        for cb in [(a, b, c) for a in charges[0] for b in charges[1] for c in charges[2]]:
+            for ck in [(a, b, c) for a in charges[0] for b in charges[1] for c in charges[2]]:
+                if sum(cb) != sum(ck):
+                    continue
+                keys = list(zip(cb, ck))
+                res = B.trimer_block(ms, [F.n_elec_ref for F in Fs], [F.rho for F in Fs], keys, self.ints)
+                if res is None:
+                    continue
+                Hb, order, scale, extra = res
+                fr = [(Fs[k].n_states[cb[k]], Fs[k].n_states[ck[k]]) for k in range(3)]
+                npair = [fr[k][0] * fr[k][1] for k in range(3)]
+                rows = [Fs[k].pos[cb[k]] for k in range(3)]
+                cols = [Fs[k].pos[ck[k]] for k in range(3)]
+                strides = [0, 0, 0]
+                strides[order[0]] = npair[order[1]] * npair[order[2]]
+                strides[order[1]] = npair[order[2]]
+                strides[order[2]] = 1
+                B.scatter(out, D, fr, rows, cols, dims, Hb, strides, scale=scale)
+                if extra is not None:
+                    spect, u, (y, z), sc = extra
+                    if fr[spect][0] == fr[spect][1] and cb[spect] == ck[spect]:
+                        st = [0, 0, 0]
+                        st[y] = npair[z]
+                        st[z] = 1
+                        dg = [0, 0, 0]
+                        dg[spect] = 1
+                        B.scatter(out, D, fr, rows, cols, dims, u, st, diag=dg, scale=sc)
+        return out
+
+    # -- XR matrix ----------------------------------------------------------
+    def blocks(self, with_trimers=False):
+        nf = len(self.frags)
+        H1 = [self.monomer_matrix(m) for m in range(nf)]
+        H2 = [[None] * nf for _ in range(nf)]
+        for m in range(nf):
+            for n in range(m + 1, nf):
+                H2[m][n] = self.pair_matrix(m, n)
+        H3 = None
+        if with_trimers:
+            H3 = [[[None] * nf for _ in range(nf)] for _ in range(nf)]
+            for a in range(nf):
+                for b in range(a + 1, nf):
+                    for c in range(b + 1, nf):
+                        H3[a][b][c] = self.trimer_matrix(a, b, c)
+        return H1, H2, H3
+
+    def xr_matrix(self, with_trimers=False, spectator_sign=False, H=None):
+        """Tensor-product XR matrix (all fragments need the same number of
+        states).  spectator_sign adds the sign the reference assembly omits
+        for one-electron transfers across spectator fragments."""
+        H1, H2, H3 = H if H is not None else self.blocks(with_trimers)
+        return assemble(H1, H2, H3, [F.parity() for F in self.frags] if spectator_sign else None)
+
+
+def assemble(H1, H2, H3=None, parities=None):
+    """braket_loops on the device (general-XRCC/hamiltonian.py:82-83)."""
+    nf = len(H1)
+    spf = H1[0].shape[0]
+    for h in H1:
+        if h.shape[0] != spf:
+            raise AssertionError("all fragments need the same number of states")
+    dev = H1[0].device
+    n = spf ** nf
+    out = torch.empty(n, n, dtype=torch.float64, device=dev)
+    p1 = (ctypes.c_void_p * nf)(*[h.contiguous().data_ptr() for h in H1])
+    p2 = (ctypes.c_void_p * (nf * nf))(*[
+        (H2[m][k].data_ptr() if (m < k and H2[m][k] is not None) else None)
+        for m in range(nf) for k in range(nf)])
+    p3 = None
+    if H3 is not None:
+        p3 = (ctypes.c_void_p * (nf ** 3))(*[
+            (H3[a][b][c].data_ptr() if (a < b < c and H3[a][b][c] is not None) else None)
+            for a in range(nf) for b in range(nf) for c in range(nf)])
+    par = None
+    if parities is not None:
+        par = torch.cat([p.reshape(-1) for p in parities]).contiguous()
+    context(dev.index).call("xr_braket_assemble", nf, spf, p1, p2, p3, ptr(par), ptr(out))
+    return out

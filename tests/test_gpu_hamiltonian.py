@@ -1,0 +1,128 @@
+"""GPU parity of the XR Hamiltonian blocks, assembly and energy against the
+CPU oracle (general-XRCC/build_H.py, hamiltonian.py, excitonic.py)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import brute
+from oracle import oracle as orc
+from qodeapplications_amd import synthetic as syn
+from qodeapplications_amd.xr import XRSystem
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _system(frag_orbs, elec, n_states, n_ref, seed, n_core=0, U=True, nuc=True):
+    rng = np.random.default_rng(seed)
+    frags = [syn.make_fragment(n, elec, n_states, n_ref, rng, n_core) for n in frag_orbs]
+    n_sp = sum(frag_orbs)
+    h = syn.spatial_h(n_sp, rng)
+    V = syn.spatial_V(n_sp, rng)
+    U_sp = np.stack([0.2 * syn.spatial_h(n_sp, rng) for _ in frag_orbs]) if U else None
+    ints = syn.FragmentBlocked(frag_orbs, h, V, U_sp)
+    nf = len(frag_orbs)
+    nucm = rng.standard_normal((nf, nf)) if nuc else np.zeros((nf, nf))
+    nucm = nucm + nucm.T
+    return frags, ints, nucm
+
+
+def _oracle_blocks(frags, ints, nuc, trimers=False):
+    for m, f in enumerate(frags):
+        f.rho, _ = orc.build_density_tensors(f.states, f.n_orbs, ints.V[m, m, m, m], f.n_core)
+    return orc.compute_H_blocks(frags, ints, nuc, with_trimers=trimers)
+
+
+def test_dimer_blocks_match_oracle():
+    frags, ints, nuc = _system([3, 4], [1, 2, 3], 3, 2, seed=21)
+    H1, H2, _ = _oracle_blocks(frags, ints, nuc)
+    X = XRSystem(frags, ints, nuc)
+    for m in range(2):
+        np.testing.assert_allclose(X.monomer_matrix(m).cpu().numpy(), H1[m], atol=TOL, rtol=0)
+    np.testing.assert_allclose(X.pair_matrix(0, 1).cpu().numpy(), H2[0][1], atol=TOL, rtol=0)
+
+
+def test_dimer_with_core_matches_oracle():
+    frags, ints, nuc = _system([4, 4], [3, 4, 5], 2, 4, seed=22, n_core=1)
+    H1, H2, _ = _oracle_blocks(frags, ints, nuc)
+    X = XRSystem(frags, ints, nuc)
+    np.testing.assert_allclose(X.pair_matrix(0, 1).cpu().numpy(), H2[0][1], atol=TOL, rtol=0)
+    np.testing.assert_allclose(X.monomer_matrix(1).cpu().numpy(), H1[1], atol=TOL, rtol=0)
+
+
+def test_trimer_blocks_and_assembly_match_oracle():
+    frags, ints, nuc = _system([2, 2, 2], [1, 2, 3], 2, 2, seed=23)
+    H1, H2, H3 = _oracle_blocks(frags, ints, nuc, trimers=True)
+    X = XRSystem(frags, ints, nuc)
+    np.testing.assert_allclose(X.trimer_matrix(0, 1, 2).cpu().numpy(), H3[0][1][2], atol=TOL, rtol=0)
+    for sign in (False, True):
+        par = orc.state_parities(frags) if sign else None
+        _, Href = orc.fci((H1, H2, H3), par=par)
+        got = X.xr_matrix(with_trimers=True, spectator_sign=sign).cpu().numpy()
+        np.testing.assert_allclose(got, Href, atol=TOL, rtol=0)
+
+
+def test_build_matrix_elements_drop_in():
+    from qodeapplications_amd.build_H import build_matrix_elements
+    frags, ints, nuc = _system([2, 3, 2], [1, 2, 3], 2, 2, seed=24)
+    for m, f in enumerate(frags):
+        f.rho, _ = orc.build_density_tensors(f.states, f.n_orbs, ints.V[m, m, m, m], 0)
+    ref = orc.build_matrix_elements(frags, ints, nuc)
+    got = build_matrix_elements(frags, ints, nuc)
+    si = [f.state_indices for f in frags]
+    rng = np.random.default_rng(0)
+    for _ in range(300):
+        m = int(rng.integers(3))
+        I, J = si[m][rng.integers(len(si[m]))], si[m][rng.integers(len(si[m]))]
+        assert abs(got.monomer(m, I, J) - ref.monomer(m, I, J)) < TOL
+        I = (si[0][rng.integers(len(si[0]))], si[1][rng.integers(len(si[1]))])
+        J = (si[0][rng.integers(len(si[0]))], si[1][rng.integers(len(si[1]))])
+        assert abs(got.dimer((0, 1), I, J) - ref.dimer((0, 1), I, J)) < TOL
+        I3 = tuple(si[k][rng.integers(len(si[k]))] for k in range(3))
+        J3 = tuple(si[k][rng.integers(len(si[k]))] for k in range(3))
+        assert abs(got.trimer((0, 1, 2), I3, J3) - ref.trimer((0, 1, 2), I3, J3)) < TOL
+
+
+def test_gpu_chain_equals_fci_for_complete_fragments():
+    rng = np.random.default_rng(25)
+    frags = []
+    for _ in range(2):
+        states = {}
+        for ne in range(5):
+            cfg = syn.lex_configs(2, ne, 0)
+            Q, _ = np.linalg.qr(rng.standard_normal((cfg.shape[0], cfg.shape[0])))
+            states[2 - ne] = syn.ChargeStates(configs=cfg, coeffs=Q.T.copy())
+        frags.append(syn.Fragment(n_orbs=2, n_core=0, n_elec_ref=2, states=states))
+    ints = syn.FragmentBlocked([2, 2], syn.spatial_h(4, rng), syn.spatial_V(4, rng),
+                               np.stack([0.3 * syn.spatial_h(4, rng) for _ in range(2)]))
+    nuc = np.zeros((2, 2))
+    X = XRSystem(frags, ints, nuc)
+    xr = np.sort(np.linalg.eigvalsh(X.xr_matrix().cpu().numpy()))
+    hf, Vf = ints.full_spin_orbital()
+    exact = np.sort(np.linalg.eigvalsh(brute.hamiltonian(hf, Vf)))
+    np.testing.assert_allclose(xr, exact, atol=1e-10)
+
+
+def test_excitonic_fci_drop_in():
+    from qodeapplications_amd.excitonic import fci
+    frags, ints, nuc = _system([3, 3], [1, 2, 3], 3, 2, seed=26)
+    H1, H2, H3 = _oracle_blocks(frags, ints, nuc)
+    E_ref, _ = orc.fci((H1, H2, H3))
+    assert abs(fci((H1, H2, H3)) - E_ref) < 1e-10
+
+
+def test_config0_dimer_matches_oracle_fixture():
+    """Plain FP64 dimer (2 x 9 spatial orbitals, 3/4/5 electrons, 8 states):
+    blocks to 1e-12, lowest 10 XR eigenvalues to 1e-9 Eh
+    (fixture written by tests/golden/make_oracle_fixtures.py)."""
+    import os
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "c0_oracle.npz"))
+    w = syn.config0(seed=0)
+    X = XRSystem(w.fragments, w.integrals, w.nuc)
+    H = X.blocks()
+    np.testing.assert_allclose(H[0][0].cpu().numpy(), ref["H1_0"], atol=TOL, rtol=0)
+    np.testing.assert_allclose(H[0][1].cpu().numpy(), ref["H1_1"], atol=TOL, rtol=0)
+    np.testing.assert_allclose(H[1][0][1].cpu().numpy(), ref["H2"], atol=1e-11, rtol=0)
+    M = X.xr_matrix(H=H)
+    ev = torch.linalg.eigvalsh(0.5 * (M + M.T))[:10].cpu().numpy()
+    np.testing.assert_allclose(ev, ref["evals"], atol=1e-9, rtol=0)
