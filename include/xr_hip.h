@@ -55,6 +55,18 @@ int xr_set_stream(xr_ctx *ctx, void *stream);
 const char *xr_last_error(xr_ctx *ctx);
 int xr_synchronize(xr_ctx *ctx);
 
+/* Kernel timing with HIP events on the context stream.  xr_profile(ctx, 1)
+ * clears the records and starts bracketing every launch; xr_profile(ctx, 0)
+ * stops.  xr_profile_stats sums the records of one kernel class
+ * (0 FP64 GEMM, 1 gather, 2 expand, 3 excitation table, 4 block scatter,
+ * 5 braket assembly, 6 other, 7 split-FP32 GEMM): launches, total ms, and the
+ * algorithmic flops and bytes of those launches.  Synchronises the stream.
+ * No reference counterpart (the reference has no timers,
+ * density_tensors.c:177-178). */
+int xr_profile(xr_ctx *ctx, int enable);
+int xr_profile_stats(xr_ctx *ctx, int kind, int64_t *launches, double *total_ms, double *flops,
+                     double *bytes);
+
 /* Occupation lists (n_cfg x n_elec int64, ascending rows) -> 64-bit masks,
  * checking Pauli, core occupancy and that row i has lexical rank i.
  * Replaces the config marshalling of build_density_tensors.py:71-78, which
@@ -95,6 +107,19 @@ int xr_density(xr_ctx *ctx, const char *ops, int n_orbs, int n_core, int n_elec_
                int64_t nI, int64_t nP, const double *d_zbra,
                int64_t nJ, int64_t nQ, const double *d_zket, double *d_out);
 
+/* Adjoint of xr_density: given Rbar = dE/d(density) in the dense layout,
+ * ADDS dE/dzbra (nI x nP) into d_zbar_bra and dE/dzket (nJ x nQ) into
+ * d_zbar_ket (either may be NULL).  Runs the factorised density backwards:
+ * Rbar is folded onto sorted index tuples, two FP64 GEMMs give the adjoints
+ * of the bra and ket gathers, and each coefficient's adjoint is collected
+ * by one thread over the excitations that reach it (no atomics).  No
+ * reference counterpart: it provides the dE/dz the state optimizer needs
+ * (StateSpaceOptimizer/state_gradients.py:121-244). */
+int xr_density_adjoint(xr_ctx *ctx, const char *ops, int n_orbs, int n_core, int n_elec_ket,
+                       int64_t nI, int64_t nP, const double *d_zbra, int64_t nJ, int64_t nQ,
+                       const double *d_zket, const double *d_Rbar, double *d_zbar_bra,
+                       double *d_zbar_ket);
+
 /* Monomer two-electron energy matrix E[I,J] = sum_pqrs V[p,q,r,s] <I|p+q+sr|J>
  * for one charge block, without forming the 4-index density (replaces the
  * ccaa build plus per-(I,J) monomer_2e, build_density_tensors.py:122-133 and
@@ -103,6 +128,14 @@ int xr_monomer_2e(xr_ctx *ctx, int n_orbs, int n_core, int n_elec,
                   int64_t nI, int64_t nP, const double *d_zbra,
                   int64_t nJ, int64_t nQ, const double *d_zket,
                   const double *d_V, double *d_out);
+
+/* Adjoint of xr_monomer_2e for Ebar = dE/dE2 (nI x nJ): ADDS dE/dzbra,
+ * dE/dzket and dE/dV (dim^4) into the given buffers (any may be NULL). */
+int xr_monomer_2e_adjoint(xr_ctx *ctx, int n_orbs, int n_core, int n_elec,
+                          int64_t nI, int64_t nP, const double *d_zbra,
+                          int64_t nJ, int64_t nQ, const double *d_zket,
+                          const double *d_V, const double *d_Ebar, double *d_zbar_bra,
+                          double *d_zbar_ket, double *d_Vbar);
 
 /* Strided-batched FP64 GEMM on the matrix cores:
  *   C_b = alpha * op(A_b) * op(B_b) + beta * C_b,  b < batch.
@@ -188,6 +221,14 @@ int xr_block_scatter(xr_ctx *ctx, int nfrag, const int64_t *nI, const int64_t *n
 int xr_braket_assemble(xr_ctx *ctx, int n_frag, int64_t spf, const double *const *H1,
                        const double *const *H2, const double *const *H3, const int8_t *d_par,
                        double *d_out);
+
+/* dE/d(pair block) for a dimer XR eigenvector v (d1*d2 entries):
+ * G[(I1*nJ1+J1)*nB + I2*nJ2+J2] = v[r1[I1]*d2 + r2[I2]] * v[c1[J1]*d2 + c2[J2]]
+ * (G[b*nA + a] when swapped), the transpose of xr_block_scatter applied to
+ * v v^T without forming it. */
+int xr_pair_outer(xr_ctx *ctx, const double *d_v, int64_t d2, const int64_t *d_r1,
+                  const int64_t *d_c1, const int64_t *d_r2, const int64_t *d_c2, int64_t nI1,
+                  int64_t nJ1, int64_t nI2, int64_t nJ2, int swapped, double *d_G);
 
 #ifdef __cplusplus
 }

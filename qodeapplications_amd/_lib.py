@@ -11,6 +11,7 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libxrhip.so")
 
+K_GEMM_F64, K_GATHER, K_EXPAND, K_TABLE, K_SCATTER, K_BRAKET, K_OTHER, K_GEMM_SPLIT = range(8)
 XR_OK, XR_EINVAL, XR_ESHAPE, XR_EINVAL_CONFIG_ORDER, XR_EPAULI, XR_ENOMEM, XR_EHIP, XR_EUNSUPPORTED = range(8)
 _CODE_NAMES = {1: "XR_EINVAL", 2: "XR_ESHAPE", 3: "XR_EINVAL_CONFIG_ORDER", 4: "XR_EPAULI",
                5: "XR_ENOMEM", 6: "XR_EHIP", 7: "XR_EUNSUPPORTED"}
@@ -27,12 +28,17 @@ SIGNATURES = {
     "xr_set_stream": (_I, [_P, _P]),
     "xr_last_error": (_S, [_P]),
     "xr_synchronize": (_I, [_P]),
+    "xr_profile": (_I, [_P, _I]),
+    "xr_profile_stats": (_I, [_P, _I, ctypes.POINTER(_L), ctypes.POINTER(_D), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "xr_configs_to_masks": (_I, [_P, _P, _L, _I, _I, _I, _P]),
     "xr_config_rank": (_I, [_P, _P, _L, _I, _I, _P]),
     "xr_excitation_count": (_I, [_P, _S, _I, _I, _I, _P, _L, _L, _P, ctypes.POINTER(_L)]),
     "xr_excitation_fill": (_I, [_P, _S, _I, _I, _I, _P, _L, _L, _P, _P, _P, _P, _P]),
     "xr_density": (_I, [_P, _S, _I, _I, _I, _L, _L, _P, _L, _L, _P, _P]),
     "xr_monomer_2e": (_I, [_P, _I, _I, _I, _L, _L, _P, _L, _L, _P, _P, _P]),
+    "xr_density_adjoint": (_I, [_P, _S, _I, _I, _I, _L, _L, _P, _L, _L, _P, _P, _P, _P]),
+    "xr_monomer_2e_adjoint": (_I, [_P, _I, _I, _I, _L, _L, _P, _L, _L, _P, _P, _P, _P, _P, _P]),
+    "xr_pair_outer": (_I, [_P, _P, _L, _P, _P, _P, _P, _L, _L, _L, _L, _I, _P]),
     "xr_gemm_f64": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
     "xr_monomer_1e": (_I, [_P, _L, _L, _P, _P, _P]),
     "xr_dimer_2min2pls": (_I, [_P, _L, _L, _L, _P, _L, _P, _P, _P]),
@@ -112,6 +118,16 @@ class Context:
         import torch
         stream = torch.cuda.current_stream(self.device).cuda_stream
         self.check(self.L.xr_set_stream(self.h, _P(stream)))
+
+    def profile(self, enable):
+        self.check(self.L.xr_profile(self.h, 1 if enable else 0))
+
+    def profile_stats(self, kind):
+        """(launches, total_ms, flops, bytes) of one kernel class since profile(True)."""
+        n, ms, fl, by = _L(0), _D(0), _D(0), _D(0)
+        self.check(self.L.xr_profile_stats(self.h, kind, ctypes.byref(n), ctypes.byref(ms),
+                                           ctypes.byref(fl), ctypes.byref(by)))
+        return n.value, ms.value, fl.value, by.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -45,10 +45,17 @@ class IntegralCache:
         def make():
             acc = None
             for kind, k in parts:
-                a = np.asarray((self.ints.T if kind == "T" else self.ints.U)[k], dtype=np.float64)
-                acc = a.copy() if acc is None else acc + a
+                a = getattr(self, kind)(*k)
+                acc = a.clone() if acc is None else acc + a
             return acc
         return self._get(("h",) + key, make)
+
+    def set_block(self, key, t):
+        """Replaces one cached block (e.g. with rotated integrals) and drops
+        the sums built from the old ones."""
+        self._c[key] = t.detach().to(device=self.device, dtype=torch.float64).contiguous()
+        for k in [k for k in self._c if k[0] == "h"]:
+            del self._c[k]
 
     def _get(self, key, make):
         t = self._c.get(key)

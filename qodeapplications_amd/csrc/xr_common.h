@@ -82,8 +82,21 @@ __device__ __forceinline__ uint64_t d_core_mask(int n_orbs, int n_core) {
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
+// kernel classes timed by the profiler (xr_profile / xr_profile_stats)
+enum { XR_K_GEMM_F64 = 0, XR_K_GATHER = 1, XR_K_EXPAND = 2, XR_K_TABLE = 3, XR_K_SCATTER = 4,
+       XR_K_BRAKET = 5, XR_K_OTHER = 6, XR_K_GEMM_SPLIT = 7, XR_K_COUNT = 8 };
+
+struct ProfRec {
+    int kind;
+    double flops, bytes;
+    hipEvent_t a, b;
+};
+
 struct xr_ctx {
     int device = 0;
+    bool prof = false;
+    std::vector<ProfRec> recs;
+    std::vector<hipEvent_t> ev_pool;
     hipStream_t stream = nullptr;
     std::string err;
     // growable scratch buffers (device)
@@ -95,6 +108,15 @@ struct xr_ctx {
 };
 
 int xr_fail(xr_ctx* c, int code, const std::string& msg);
+
+// Brackets one kernel launch with HIP events on the context stream when the
+// profiler is on (kind: XR_K_*; flops / bytes: algorithmic work of the launch).
+struct ProfScope {
+    xr_ctx* c;
+    int idx = -1;
+    ProfScope(xr_ctx* c_, int kind, double flops, double bytes);
+    ~ProfScope();
+};
 int xr_hip_check(xr_ctx* c, hipError_t e, const char* what);
 void* xr_workspace(xr_ctx* c, int slot, size_t bytes);
 int xr_fill(xr_ctx* c, double* p, int64_t n, double v);
@@ -116,3 +138,15 @@ static inline int64_t h_binom(int n, int k) {
 }
 
 static inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Elementwise kernels loop with a grid stride over a capped grid: a launch
+// may not exceed 2^32 work-items per grid dimension on this hardware.
+#define XR_GRID_STRIDE(i, n) \
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); \
+         i += (int64_t)gridDim.x * blockDim.x)
+
+static inline unsigned xr_grid(int64_t n, int bs = 256) {
+    int64_t b = ceil_div(n, bs);
+    if (b < 1) b = 1;
+    return (unsigned)(b < (1 << 20) ? b : (1 << 20));
+}

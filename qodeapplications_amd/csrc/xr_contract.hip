@@ -8,11 +8,9 @@
 #include "xr_gemm.h"
 
 // out[o0,o1,o2,o3] = in[...] with out axis j = in axis perm[j]
-__global__ void k_permute4(const double* __restrict__ in, int64_t d0, int64_t d1, int64_t d2,
+__device__ __forceinline__ void k_permute4_elem(int64_t i, const double* __restrict__ in, int64_t d0, int64_t d1, int64_t d2,
                            int64_t d3, int p0, int p1, int p2, int p3, double* __restrict__ out) {
     int64_t n = d0 * d1 * d2 * d3;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
     int64_t din[4] = {d0, d1, d2, d3};
     int p[4] = {p0, p1, p2, p3};
     int64_t dout[4] = {din[p[0]], din[p[1]], din[p[2]], din[p[3]]};
@@ -27,11 +25,17 @@ __global__ void k_permute4(const double* __restrict__ in, int64_t d0, int64_t d1
     out[i] = in[((ix[0] * d1 + ix[1]) * d2 + ix[2]) * d3 + ix[3]];
 }
 
+__global__ void k_permute4(const double* __restrict__ in, int64_t d0, int64_t d1, int64_t d2,
+                           int64_t d3, int p0, int p1, int p2, int p3, double* __restrict__ out) {
+    int64_t n = d0 * d1 * d2 * d3;
+    XR_GRID_STRIDE(i, n) k_permute4_elem(i, in, d0, d1, d2, d3, p0, p1, p2, p3, out);
+}
+
 static int permute4(xr_ctx* c, const double* in, int64_t d0, int64_t d1, int64_t d2, int64_t d3,
                     int p0, int p1, int p2, int p3, double* out) {
     int64_t n = d0 * d1 * d2 * d3;
     if (n <= 0) return XR_OK;
-    k_permute4<<<ceil_div(n, 256), 256, 0, c->stream>>>(in, d0, d1, d2, d3, p0, p1, p2, p3, out);
+    k_permute4<<<xr_grid(n), 256, 0, c->stream>>>(in, d0, d1, d2, d3, p0, p1, p2, p3, out);
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }
@@ -194,10 +198,8 @@ struct ScatterDesc {
     const int64_t* cols[3];
 };
 
-__global__ void k_block_scatter(ScatterDesc sd, const double* __restrict__ blk, double scale,
+__device__ __forceinline__ void k_block_scatter_elem(int64_t i, ScatterDesc sd, const double* __restrict__ blk, double scale,
                                 double* __restrict__ out, int64_t ldo, int64_t total) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
     int64_t x = i, src = 0, row = 0, col = 0;
     int64_t I[3], J[3];
     for (int k = sd.nfrag - 1; k >= 0; k--) {
@@ -214,6 +216,11 @@ __global__ void k_block_scatter(ScatterDesc sd, const double* __restrict__ blk, 
     }
     double v = blk ? scale * blk[src] : scale;
     out[row * ldo + col] += v;
+}
+
+__global__ void k_block_scatter(ScatterDesc sd, const double* __restrict__ blk, double scale,
+                                double* __restrict__ out, int64_t ldo, int64_t total) {
+    XR_GRID_STRIDE(i, total) k_block_scatter_elem(i, sd, blk, scale, out, ldo, total);
 }
 
 extern "C" int xr_block_scatter(xr_ctx* c, int nfrag, const int64_t* nI, const int64_t* nJ,
@@ -238,7 +245,8 @@ extern "C" int xr_block_scatter(xr_ctx* c, int nfrag, const int64_t* nI, const i
         total *= diag_only[k] ? nI[k] : nI[k] * nJ[k];
     }
     if (total <= 0) return XR_OK;
-    k_block_scatter<<<ceil_div(total, 256), 256, 0, c->stream>>>(sd, d_block, scale, d_out, ldo, total);
+    ProfScope prof(c, XR_K_SCATTER, 0.0, 16.0 * total);
+    k_block_scatter<<<xr_grid(total), 256, 0, c->stream>>>(sd, d_block, scale, d_out, ldo, total);
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }
@@ -255,11 +263,9 @@ struct BraketDesc {
 // couplings by number of fragments whose state changes).  With par, one-
 // electron transfers between non-adjacent fragments carry the sign of the
 // electron counts of the spectator fragments between them.
-__global__ void k_braket(BraketDesc bd, const double* const* __restrict__ H1,
+__device__ __forceinline__ void k_braket_elem(int64_t idx, BraketDesc bd, const double* const* __restrict__ H1,
                          const double* const* __restrict__ H2, const double* const* __restrict__ H3,
                          const int8_t* __restrict__ par, double* __restrict__ out, int64_t n) {
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n * n) return;
     const int nf = bd.nf;
     const int64_t s = bd.spf;
     int64_t I = idx / n, J = idx - (idx / n) * n;
@@ -310,6 +316,12 @@ __global__ void k_braket(BraketDesc bd, const double* const* __restrict__ H1,
     out[idx] = v;
 }
 
+__global__ void k_braket(BraketDesc bd, const double* const* __restrict__ H1,
+                         const double* const* __restrict__ H2, const double* const* __restrict__ H3,
+                         const int8_t* __restrict__ par, double* __restrict__ out, int64_t n) {
+    XR_GRID_STRIDE(idx, n * n) k_braket_elem(idx, bd, H1, H2, H3, par, out, n);
+}
+
 extern "C" int xr_braket_assemble(xr_ctx* c, int n_frag, int64_t spf, const double* const* H1,
                                   const double* const* H2, const double* const* H3,
                                   const int8_t* d_par, double* d_out) {
@@ -335,8 +347,46 @@ extern "C" int xr_braket_assemble(xr_ctx* c, int n_frag, int64_t spf, const doub
         for (int m = 0; m < n_frag * n_frag * n_frag; m++) h[n_frag + n_frag * n_frag + m] = H3[m];
     XR_HIP(c, hipMemcpyAsync(tbl, h.data(), np * sizeof(double*), hipMemcpyHostToDevice, c->stream));
     XR_HIP(c, hipStreamSynchronize(c->stream));
-    k_braket<<<ceil_div(n * n, 256), 256, 0, c->stream>>>(
+    ProfScope prof(c, XR_K_BRAKET, 0.0, 8.0 * n * n);
+    k_braket<<<xr_grid(n * n), 256, 0, c->stream>>>(
         bd, tbl, tbl + n_frag, H3 ? tbl + n_frag + n_frag * n_frag : nullptr, d_par, d_out, n);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+// Adjoint of the pair-block scatter for a rank-1 XR derivative v v^T on a
+// dimer: G[(I1*nJ1+J1), (I2*nJ2+J2)] = v[r1[I1]*d2 + r2[I2]] * v[c1[J1]*d2 + c2[J2]]
+// (transposed layout when swapped).
+__device__ __forceinline__ void k_pair_outer_elem(int64_t i, const double* __restrict__ v, int64_t d2, const int64_t* __restrict__ r1,
+                             const int64_t* __restrict__ c1, const int64_t* __restrict__ r2,
+                             const int64_t* __restrict__ c2, int64_t nI1, int64_t nJ1, int64_t nI2,
+                             int64_t nJ2, int swapped, double* __restrict__ G) {
+    int64_t nA = nI1 * nJ1, nB = nI2 * nJ2;
+    int64_t a, b;
+    if (swapped) { b = i / nA; a = i - b * nA; }
+    else { a = i / nB; b = i - a * nB; }
+    int64_t I1 = a / nJ1, J1 = a - I1 * nJ1, I2 = b / nJ2, J2 = b - I2 * nJ2;
+    G[i] = v[r1[I1] * d2 + r2[I2]] * v[c1[J1] * d2 + c2[J2]];
+}
+
+__global__ void k_pair_outer(const double* __restrict__ v, int64_t d2, const int64_t* __restrict__ r1,
+                             const int64_t* __restrict__ c1, const int64_t* __restrict__ r2,
+                             const int64_t* __restrict__ c2, int64_t nI1, int64_t nJ1, int64_t nI2,
+                             int64_t nJ2, int swapped, double* __restrict__ G) {
+    int64_t nA = nI1 * nJ1, nB = nI2 * nJ2;
+    XR_GRID_STRIDE(i, nA * nB) k_pair_outer_elem(i, v, d2, r1, c1, r2, c2, nI1, nJ1, nI2, nJ2, swapped, G);
+}
+
+extern "C" int xr_pair_outer(xr_ctx* c, const double* d_v, int64_t d2, const int64_t* d_r1,
+                             const int64_t* d_c1, const int64_t* d_r2, const int64_t* d_c2,
+                             int64_t nI1, int64_t nJ1, int64_t nI2, int64_t nJ2, int swapped,
+                             double* d_G) {
+    XR_ENTRY(c);
+    int64_t n = nI1 * nJ1 * nI2 * nJ2;
+    if (n <= 0) return XR_OK;
+    ProfScope prof(c, XR_K_SCATTER, 0.0, 8.0 * n);
+    k_pair_outer<<<xr_grid(n), 256, 0, c->stream>>>(d_v, d2, d_r1, d_c1, d_r2, d_c2, nI1, nJ1,
+                                                          nI2, nJ2, swapped, d_G);
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }

@@ -39,16 +39,85 @@ void* xr_workspace(xr_ctx* c, int slot, size_t bytes) {
     return p;
 }
 
+// ---------------------------------------------------------------------------
+// profiler
+// ---------------------------------------------------------------------------
+static hipEvent_t ev_get(xr_ctx* c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+ProfScope::ProfScope(xr_ctx* c_, int kind, double flops, double bytes) : c(c_) {
+    if (!c->prof) return;
+    ProfRec r;
+    r.kind = kind;
+    r.flops = flops;
+    r.bytes = bytes;
+    r.a = ev_get(c);
+    r.b = ev_get(c);
+    if (!r.a || !r.b) return;
+    (void)hipEventRecord(r.a, c->stream);
+    c->recs.push_back(r);
+    idx = (int)c->recs.size() - 1;
+}
+
+ProfScope::~ProfScope() {
+    if (idx >= 0) (void)hipEventRecord(c->recs[idx].b, c->stream);
+}
+
+extern "C" int xr_profile(xr_ctx* c, int enable) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (enable) {
+        XR_HIP(c, hipStreamSynchronize(c->stream));
+        for (auto& r : c->recs) {
+            c->ev_pool.push_back(r.a);
+            c->ev_pool.push_back(r.b);
+        }
+        c->recs.clear();
+    }
+    c->prof = enable != 0;
+    return XR_OK;
+}
+
+extern "C" int xr_profile_stats(xr_ctx* c, int kind, int64_t* launches, double* total_ms,
+                                double* flops, double* bytes) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    int64_t n = 0;
+    double ms = 0, fl = 0, by = 0;
+    for (auto& r : c->recs) {
+        if (r.kind != kind) continue;
+        float t = 0;
+        XR_HIP(c, hipEventElapsedTime(&t, r.a, r.b));
+        n++;
+        ms += t;
+        fl += r.flops;
+        by += r.bytes;
+    }
+    if (launches) *launches = n;
+    if (total_ms) *total_ms = ms;
+    if (flops) *flops = fl;
+    if (bytes) *bytes = by;
+    return XR_OK;
+}
+
 // Plain fill kernel, used instead of hipMemsetAsync so the work can sit
 // inside a captured HIP graph.
 __global__ void k_fill(double* p, int64_t n, double v) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    XR_GRID_STRIDE(i, n) p[i] = v;
 }
 
 int xr_fill(xr_ctx* c, double* p, int64_t n, double v) {
     if (n <= 0) return XR_OK;
-    k_fill<<<ceil_div(n, 256), 256, 0, c->stream>>>(p, n, v);
+    k_fill<<<xr_grid(n), 256, 0, c->stream>>>(p, n, v);
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }
@@ -60,10 +129,8 @@ int xr_fill(xr_ctx* c, double* p, int64_t n, double v) {
 //       bit 1: orbital index out of range
 //       bit 2: a core spin orbital is empty
 //       bit 3: row i does not have lexical rank i
-__global__ void k_cfg_to_masks(const int64_t* cfg, int64_t n_cfg, int n_elec, int n_orbs,
+__device__ __forceinline__ void k_cfg_to_masks_elem(int64_t i, const int64_t* cfg, int64_t n_cfg, int n_elec, int n_orbs,
                                int n_core, uint64_t* masks, unsigned int* flags) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_cfg) return;
     int dim = 2 * n_orbs;
     uint64_t m = 0;
     unsigned int f = 0;
@@ -85,6 +152,11 @@ __global__ void k_cfg_to_masks(const int64_t* cfg, int64_t n_cfg, int n_elec, in
     if (f) atomicOr(flags, f);
 }
 
+__global__ void k_cfg_to_masks(const int64_t* cfg, int64_t n_cfg, int n_elec, int n_orbs,
+                               int n_core, uint64_t* masks, unsigned int* flags) {
+    XR_GRID_STRIDE(i, n_cfg) k_cfg_to_masks_elem(i, cfg, n_cfg, n_elec, n_orbs, n_core, masks, flags);
+}
+
 extern "C" int xr_configs_to_masks(xr_ctx* c, const int64_t* d_cfg, int64_t n_cfg, int n_elec,
                                    int n_orbs, int n_core, uint64_t* d_masks) {
     if (!c) return XR_EINVAL;
@@ -101,7 +173,7 @@ extern "C" int xr_configs_to_masks(xr_ctx* c, const int64_t* d_cfg, int64_t n_cf
     unsigned int* flags = (unsigned int*)xr_workspace(c, 15, sizeof(unsigned int));
     if (!flags) return xr_fail(c, XR_ENOMEM, "workspace");
     XR_HIP(c, hipMemsetAsync(flags, 0, sizeof(unsigned int), c->stream));
-    k_cfg_to_masks<<<ceil_div(n_cfg, 256), 256, 0, c->stream>>>(d_cfg, n_cfg, n_elec, n_orbs,
+    k_cfg_to_masks<<<xr_grid(n_cfg), 256, 0, c->stream>>>(d_cfg, n_cfg, n_elec, n_orbs,
                                                                  n_core, d_masks, flags);
     XR_LAUNCH_CHECK(c);
     unsigned int hf = 0;
@@ -114,10 +186,12 @@ extern "C" int xr_configs_to_masks(xr_ctx* c, const int64_t* d_cfg, int64_t n_cf
     return XR_OK;
 }
 
-__global__ void k_rank(const uint64_t* masks, int64_t n, int n_orbs, int n_core, int64_t* out) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void k_rank_elem(int64_t i, const uint64_t* masks, int64_t n, int n_orbs, int n_core, int64_t* out) {
     out[i] = d_lex_rank(d_to_valence(masks[i], n_orbs, n_core), 2 * (n_orbs - n_core));
+}
+
+__global__ void k_rank(const uint64_t* masks, int64_t n, int n_orbs, int n_core, int64_t* out) {
+    XR_GRID_STRIDE(i, n) k_rank_elem(i, masks, n, n_orbs, n_core, out);
 }
 
 extern "C" int xr_config_rank(xr_ctx* c, const uint64_t* d_masks, int64_t n, int n_orbs,
@@ -126,7 +200,7 @@ extern "C" int xr_config_rank(xr_ctx* c, const uint64_t* d_masks, int64_t n, int
     std::lock_guard<std::mutex> g(c->mu);
     if (n_orbs <= 0 || 2 * n_orbs > XR_MAX_SO) return xr_fail(c, XR_EUNSUPPORTED, "dim > 64");
     if (n <= 0) return XR_OK;
-    k_rank<<<ceil_div(n, 256), 256, 0, c->stream>>>(d_masks, n, n_orbs, n_core, d_rank);
+    k_rank<<<xr_grid(n), 256, 0, c->stream>>>(d_masks, n, n_orbs, n_core, d_rank);
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }
@@ -166,6 +240,11 @@ extern "C" int xr_destroy(xr_ctx* c) {
     for (void* p : c->ws)
         if (p) (void)hipFree(p);
     for (auto& kv : c->kspace) (void)hipFree(kv.second.first);
+    for (auto& r : c->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     delete c;
     return XR_OK;
 }

@@ -31,9 +31,9 @@ int xr_plan_op(const char* ops, OpPlan* p) {
     // the side string uses, leftmost operator first.
     static const Row rows[] = {
         {"a",    {1, 0, {0, 0, {0}}, {1, 0, {0}}}},
-        {"c",    {1, 0, {0, 0, {0}}, {1, 1, {0}}}},
+        {"c",    {1, 1, {1, 0, {0}}, {0, 0, {0}}}},
         {"aa",   {2, 0, {0, 0, {0}}, {2, 0, {0, 1}}}},
-        {"cc",   {2, 0, {0, 0, {0}}, {2, 1, {0, 1}}}},
+        {"cc",   {2, 1, {2, 0, {1, 0}}, {0, 0, {0}}}},
         {"ca",   {2, 1, {1, 0, {0}}, {1, 0, {0}}}},
         {"caa",  {3, 1, {1, 0, {0}}, {2, 0, {0, 1}}}},
         {"cca",  {3, 1, {2, 0, {1, 0}}, {1, 0, {0}}}},
@@ -66,19 +66,19 @@ __device__ __forceinline__ uint64_t d_from_valence(uint64_t v, int n_orbs, int n
     return a | b | d_core_mask(n_orbs, n_core);
 }
 
-__global__ void k_unrank_space(uint64_t* masks, int64_t n, int n_orbs, int n_core, int n_elec) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void k_unrank_space_elem(int64_t i, uint64_t* masks, int64_t n, int n_orbs, int n_core, int n_elec) {
     int nval = 2 * (n_orbs - n_core);
     masks[i] = d_from_valence(d_lex_unrank(i, nval, n_elec - 2 * n_core), n_orbs, n_core);
 }
 
+__global__ void k_unrank_space(uint64_t* masks, int64_t n, int n_orbs, int n_core, int n_elec) {
+    XR_GRID_STRIDE(i, n) k_unrank_space_elem(i, masks, n, n_orbs, n_core, n_elec);
+}
+
 // One entry per (K, t): the source determinant the side string maps onto K,
 // as sign * (valence rank + 1), or 0 when the string annihilates it.
-__global__ void k_side_table(SidePlan sd, const uint64_t* __restrict__ Km, int64_t nK, int dim,
+__device__ __forceinline__ void k_side_table_elem(int64_t i, SidePlan sd, const uint64_t* __restrict__ Km, int64_t nK, int dim,
                              int n_orbs, int n_core, int64_t nT, int32_t* __restrict__ tab) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nK * nT) return;
     int64_t K = i / nT, t = i - K * nT;
     uint64_t km = Km[K];
     uint64_t tm = d_lex_unrank(t, dim, sd.k);
@@ -111,6 +111,11 @@ __global__ void k_side_table(SidePlan sd, const uint64_t* __restrict__ Km, int64
     tab[i] = sg * (int32_t)(r + 1);
 }
 
+__global__ void k_side_table(SidePlan sd, const uint64_t* __restrict__ Km, int64_t nK, int dim,
+                             int n_orbs, int n_core, int64_t nT, int32_t* __restrict__ tab) {
+    XR_GRID_STRIDE(i, nK * nT) k_side_table_elem(i, sd, Km, nK, dim, n_orbs, n_core, nT, tab);
+}
+
 // zT[c * nS + s] = z[s * nC + c]
 __global__ void k_transpose(const double* __restrict__ z, int64_t nS, int64_t nC,
                             double* __restrict__ zT) {
@@ -129,11 +134,9 @@ __global__ void k_transpose(const double* __restrict__ z, int64_t nS, int64_t nC
 }
 
 // Signed row gather.  layout 0: out[K][t][s]; layout 1: out[t][K][s].
-__global__ void k_gather(const int32_t* __restrict__ tab, int64_t nK, int64_t nT,
+__device__ __forceinline__ void k_gather_elem(int64_t i, const int32_t* __restrict__ tab, int64_t nK, int64_t nT,
                          const double* __restrict__ zT, int64_t nS, double* __restrict__ out,
                          int layout) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nK * nT * nS) return;
     int64_t s = i % nS;
     int64_t r = i / nS;
     int64_t K = r / nT, t = r - K * nT;
@@ -145,12 +148,16 @@ __global__ void k_gather(const int32_t* __restrict__ tab, int64_t nK, int64_t nT
     else out[(t * nK + K) * nS + s] = v;
 }
 
+__global__ void k_gather(const int32_t* __restrict__ tab, int64_t nK, int64_t nT,
+                         const double* __restrict__ zT, int64_t nS, double* __restrict__ out,
+                         int layout) {
+    XR_GRID_STRIDE(i, nK * nT * nS) k_gather_elem(i, tab, nK, nT, zT, nS, out, layout);
+}
+
 // For each flat index (d_0 ... d_{k-1}) in [0, dim^k): the lexical rank of
 // the sorted tuple and the parity of the sort, as sign * (rank + 1); 0 if an
 // index repeats.
-__global__ void k_index_table(int k, int dim, int64_t n, int32_t* tab) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void k_index_table_elem(int64_t i, int k, int dim, int64_t n, int32_t* tab) {
     int d[3];
     int64_t x = i;
     for (int j = k - 1; j >= 0; j--) {
@@ -170,21 +177,23 @@ __global__ void k_index_table(int k, int dim, int64_t n, int32_t* tab) {
     tab[i] = ((inv & 1) ? -1 : 1) * (r + 1);
 }
 
+__global__ void k_index_table(int k, int dim, int64_t n, int32_t* tab) {
+    XR_GRID_STRIDE(i, n) k_index_table_elem(i, k, dim, n, tab);
+}
+
 // out[((I*nJ + J)*nL + iL)*nR + iR] from the compressed GEMM result
 // Cc[(l*nI + I) * ldc + t*nJ + J].
-__global__ void k_expand(const double* __restrict__ Cc, int64_t ldc, int64_t nI, int64_t nJ,
+__device__ __forceinline__ void k_expand_elem(int64_t i, const double* __restrict__ Cc, int64_t ldc, int64_t nI, int64_t nJ,
                          const int32_t* __restrict__ tabL, int64_t nL,
                          const int32_t* __restrict__ tabR, int64_t nR, double* __restrict__ out) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t total = nI * nJ * nL * nR;
-    if (i >= total) return;
     int64_t iR = i % nR;
     int64_t r = i / nR;
     int64_t iL = r % nL;
     r /= nL;
     int64_t J = r % nJ, I = r / nJ;
     int32_t eL = tabL ? tabL[iL] : 1;
-    int32_t eR = tabR[iR];
+    int32_t eR = tabR ? tabR[iR] : 1;
     double v = 0.0;
     if (eL != 0 && eR != 0) {
         int64_t l = (eL > 0 ? eL : -eL) - 1;
@@ -195,12 +204,17 @@ __global__ void k_expand(const double* __restrict__ Cc, int64_t ldc, int64_t nI,
     out[i] = v;
 }
 
+__global__ void k_expand(const double* __restrict__ Cc, int64_t ldc, int64_t nI, int64_t nJ,
+                         const int32_t* __restrict__ tabL, int64_t nL,
+                         const int32_t* __restrict__ tabR, int64_t nR, double* __restrict__ out) {
+    int64_t total = nI * nJ * nL * nR;
+    XR_GRID_STRIDE(i, total) k_expand_elem(i, Cc, ldc, nI, nJ, tabL, nL, tabR, nR, out);
+}
+
 // Veff[u, w] for sorted pairs u = (p<q), w = (a<b):
 //   V[p,q,b,a] - V[q,p,b,a] - V[p,q,a,b] + V[q,p,a,b]
-__global__ void k_veff(const double* __restrict__ V, int dim, int64_t nU,
+__device__ __forceinline__ void k_veff_elem(int64_t i, const double* __restrict__ V, int dim, int64_t nU,
                        double* __restrict__ Ve) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nU * nU) return;
     int64_t u = i / nU, w = i - u * nU;
     uint64_t mu = d_lex_unrank(u, dim, 2), mw = d_lex_unrank(w, dim, 2);
     int64_t p = __ffsll((unsigned long long)mu) - 1;
@@ -212,6 +226,11 @@ __global__ void k_veff(const double* __restrict__ V, int dim, int64_t nU,
         return V[((x0 * D + x1) * D + x2) * D + x3];
     };
     Ve[i] = at(p, q, b, a) - at(q, p, b, a) - at(p, q, a, b) + at(q, p, a, b);
+}
+
+__global__ void k_veff(const double* __restrict__ V, int dim, int64_t nU,
+                       double* __restrict__ Ve) {
+    XR_GRID_STRIDE(i, nU * nU) k_veff_elem(i, V, dim, nU, Ve);
 }
 
 // ---------------------------------------------------------------------------
@@ -230,7 +249,7 @@ int xr_space_masks(xr_ctx* c, int n_orbs, int n_core, int n_elec, const uint64_t
     if (cnt <= 0) return xr_fail(c, XR_EINVAL, "empty determinant space");
     uint64_t* p = nullptr;
     XR_HIP(c, hipMalloc(&p, cnt * sizeof(uint64_t)));
-    k_unrank_space<<<ceil_div(cnt, 256), 256, 0, c->stream>>>(p, cnt, n_orbs, n_core, n_elec);
+    k_unrank_space<<<xr_grid(cnt), 256, 0, c->stream>>>(p, cnt, n_orbs, n_core, n_elec);
     XR_LAUNCH_CHECK(c);
     c->kspace[key] = std::make_pair(p, cnt);
     *masks = p;
@@ -251,7 +270,8 @@ int xr_side_table(xr_ctx* c, const SidePlan& sd, const uint64_t* Km, int64_t nK,
     int64_t nT = h_binom(2 * n_orbs, sd.k);
     int64_t n = nK * nT;
     if (n <= 0) return XR_OK;
-    k_side_table<<<ceil_div(n, 256), 256, 0, c->stream>>>(sd, Km, nK, 2 * n_orbs, n_orbs, n_core,
+    ProfScope prof(c, XR_K_TABLE, 0.0, 4.0 * n);
+    k_side_table<<<xr_grid(n), 256, 0, c->stream>>>(sd, Km, nK, 2 * n_orbs, n_orbs, n_core,
                                                            nT, tab);
     XR_LAUNCH_CHECK(c);
     return XR_OK;
@@ -261,7 +281,8 @@ int xr_gather(xr_ctx* c, const int32_t* tab, int64_t nK, int64_t nT, const doubl
               int64_t nS, double* out, int layout) {
     int64_t n = nK * nT * nS;
     if (n <= 0) return XR_OK;
-    k_gather<<<ceil_div(n, 256), 256, 0, c->stream>>>(tab, nK, nT, zT, nS, out, layout);
+    ProfScope prof(c, XR_K_GATHER, 0.0, 8.0 * n + 4.0 * nK * nT);
+    k_gather<<<xr_grid(n), 256, 0, c->stream>>>(tab, nK, nT, zT, nS, out, layout);
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }
@@ -269,7 +290,7 @@ int xr_gather(xr_ctx* c, const int32_t* tab, int64_t nK, int64_t nT, const doubl
 int xr_index_table(xr_ctx* c, int k, int dim, int32_t* tab) {
     int64_t n = 1;
     for (int j = 0; j < k; j++) n *= dim;
-    k_index_table<<<ceil_div(n, 256), 256, 0, c->stream>>>(k, dim, n, tab);
+    k_index_table<<<xr_grid(n), 256, 0, c->stream>>>(k, dim, n, tab);
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }
@@ -295,6 +316,63 @@ static int check_spaces(xr_ctx* c, const OpPlan& pl, int n_orbs, int n_core, int
 // entry points
 // ---------------------------------------------------------------------------
 enum { WS_ZTB = 0, WS_ZTK, WS_TABL, WS_TABR, WS_X, WS_Y, WS_C, WS_IDXL, WS_IDXR, WS_VE, WS_W };
+enum { WS_GC = 40, WS_XB, WS_YB, WS_ZBT, WS_WB, WS_VEB };
+
+// The two gathered factors of a density: X (nK x nLt*nI) from the bra
+// coefficients and Y (nK x nT*nJ) from the ket coefficients, over the K
+// space that sits between L and R: the bra space when there is no L, the
+// ket space when there is no R (then X or Y is just the transposed
+// coefficients), else every determinant with n_ket - |R| electrons.
+struct Factors {
+    const uint64_t* Km = nullptr;
+    int64_t nK = 0, nLt = 1, nT = 1;
+    int kspace_core = 0;
+    const double* X = nullptr;
+    const double* Y = nullptr;
+};
+
+static int prepare_factors(xr_ctx* c, const OpPlan& pl, int n_orbs, int n_core, int n_elec_ket,
+                           int nb, int64_t nI, int64_t nP, const double* zbra, int64_t nJ,
+                           int64_t nQ, const double* zket, Factors* f) {
+    const int dim = 2 * n_orbs;
+    int rc;
+    double* zTb = (double*)xr_workspace(c, WS_ZTB, nP * nI * sizeof(double));
+    double* zTk = (double*)xr_workspace(c, WS_ZTK, nQ * nJ * sizeof(double));
+    if (!zTb || !zTk) return xr_fail(c, XR_ENOMEM, "workspace (coefficients)");
+    if ((rc = xr_transpose(c, zbra, nI, nP, zTb))) return rc;
+    if ((rc = xr_transpose(c, zket, nJ, nQ, zTk))) return rc;
+    f->nT = pl.R.k ? h_binom(dim, pl.R.k) : 1;
+    f->nLt = pl.L.k ? h_binom(dim, pl.L.k) : 1;
+    if (!pl.L.k) {
+        if ((rc = xr_space_masks(c, n_orbs, n_core, nb, &f->Km, &f->nK))) return rc;
+        f->X = zTb;
+        f->kspace_core = n_core;
+    } else if (!pl.R.k) {
+        if ((rc = xr_space_masks(c, n_orbs, n_core, n_elec_ket, &f->Km, &f->nK))) return rc;
+        f->Y = zTk;
+        f->kspace_core = n_core;
+    } else {
+        if ((rc = xr_space_masks(c, n_orbs, 0, n_elec_ket - pl.R.k, &f->Km, &f->nK))) return rc;
+        f->kspace_core = 0;
+    }
+    if (pl.L.k) {
+        int32_t* tabL = (int32_t*)xr_workspace(c, WS_TABL, f->nK * f->nLt * sizeof(int32_t));
+        double* Xb = (double*)xr_workspace(c, WS_X, f->nK * f->nLt * nI * sizeof(double));
+        if (!tabL || !Xb) return xr_fail(c, XR_ENOMEM, "workspace (bra gather)");
+        if ((rc = xr_side_table(c, pl.L, f->Km, f->nK, n_orbs, n_core, tabL))) return rc;
+        if ((rc = xr_gather(c, tabL, f->nK, f->nLt, zTb, nI, Xb, 0))) return rc;
+        f->X = Xb;
+    }
+    if (pl.R.k) {
+        int32_t* tabR = (int32_t*)xr_workspace(c, WS_TABR, f->nK * f->nT * sizeof(int32_t));
+        double* Yb = (double*)xr_workspace(c, WS_Y, f->nK * f->nT * nJ * sizeof(double));
+        if (!tabR || !Yb) return xr_fail(c, XR_ENOMEM, "workspace (ket gather)");
+        if ((rc = xr_side_table(c, pl.R, f->Km, f->nK, n_orbs, n_core, tabR))) return rc;
+        if ((rc = xr_gather(c, tabR, f->nK, f->nT, zTk, nJ, Yb, 0))) return rc;
+        f->Y = Yb;
+    }
+    return XR_OK;
+}
 
 int xr_density_impl(xr_ctx* c, const char* ops, int n_orbs, int n_core, int n_elec_ket,
                     int64_t nI, int64_t nP, const double* zbra, int64_t nJ, int64_t nQ,
@@ -308,56 +386,27 @@ int xr_density_impl(xr_ctx* c, const char* ops, int n_orbs, int n_core, int n_el
     if (nI <= 0 || nJ <= 0) return XR_OK;
     int64_t out_size = nI * nJ;
     for (int j = 0; j < pl.k; j++) out_size *= dim;
-    if (pl.L.k && (n_elec_ket - pl.R.k < 0))  // R annihilates more electrons than the ket has
+    if (pl.L.k && pl.R.k && (n_elec_ket - pl.R.k < 0))  // R annihilates more electrons than the ket has
         return xr_fill(c, out, out_size, 0.0);
-    double* zTb = (double*)xr_workspace(c, WS_ZTB, nP * nI * sizeof(double));
-    double* zTk = (double*)xr_workspace(c, WS_ZTK, nQ * nJ * sizeof(double));
-    if (!zTb || !zTk) return xr_fail(c, XR_ENOMEM, "workspace (coefficients)");
-    if ((rc = xr_transpose(c, zbra, nI, nP, zTb))) return rc;
-    if ((rc = xr_transpose(c, zket, nJ, nQ, zTk))) return rc;
-
-    const int64_t nT = h_binom(dim, pl.R.k);
-    const int64_t nLt = pl.L.k ? h_binom(dim, pl.L.k) : 1;
-    const uint64_t* Km = nullptr;
-    int64_t nK = 0;
-    const double* X = nullptr;
-    if (pl.L.k == 0) {
-        // K is the bra space: X = zbra^T
-        if ((rc = xr_space_masks(c, n_orbs, n_core, nb, &Km, &nK))) return rc;
-        X = zTb;
-    } else {
-        // K is the complete space between L and R (core holes allowed)
-        int nK_e = n_elec_ket - pl.R.k;
-        if ((rc = xr_space_masks(c, n_orbs, 0, nK_e, &Km, &nK))) return rc;
-        int32_t* tabL = (int32_t*)xr_workspace(c, WS_TABL, nK * nLt * sizeof(int32_t));
-        double* Xb = (double*)xr_workspace(c, WS_X, nK * nLt * nI * sizeof(double));
-        if (!tabL || !Xb) return xr_fail(c, XR_ENOMEM, "workspace (bra gather)");
-        if ((rc = xr_side_table(c, pl.L, Km, nK, n_orbs, n_core, tabL))) return rc;
-        if ((rc = xr_gather(c, tabL, nK, nLt, zTb, nI, Xb, 0))) return rc;
-        X = Xb;
-    }
-    int32_t* tabR = (int32_t*)xr_workspace(c, WS_TABR, nK * nT * sizeof(int32_t));
-    double* Y = (double*)xr_workspace(c, WS_Y, nK * nT * nJ * sizeof(double));
-    if (!tabR || !Y) return xr_fail(c, XR_ENOMEM, "workspace (ket gather)");
-    if ((rc = xr_side_table(c, pl.R, Km, nK, n_orbs, n_core, tabR))) return rc;
-    if ((rc = xr_gather(c, tabR, nK, nT, zTk, nJ, Y, 0))) return rc;
-
-    const int64_t Mg = nLt * nI, Ng = nT * nJ;
+    Factors f;
+    if ((rc = prepare_factors(c, pl, n_orbs, n_core, n_elec_ket, nb, nI, nP, zbra, nJ, nQ, zket, &f)))
+        return rc;
+    const int64_t Mg = f.nLt * nI, Ng = f.nT * nJ;
     double* Cc = (double*)xr_workspace(c, WS_C, Mg * Ng * sizeof(double));
     if (!Cc) return xr_fail(c, XR_ENOMEM, "workspace (compressed density)");
-    if ((rc = xr_dgemm_launch(c, 1, 0, Mg, Ng, nK, 1.0, X, Mg, 0, Y, Ng, 0, 0.0, Cc, Ng, 0, 1)))
+    if ((rc = xr_dgemm_launch(c, 1, 0, Mg, Ng, f.nK, 1.0, f.X, Mg, 0, f.Y, Ng, 0, 0.0, Cc, Ng, 0, 1)))
         return rc;
-
     int64_t nIL = 1, nIR = 1;
     for (int j = 0; j < pl.L.k; j++) nIL *= dim;
     for (int j = 0; j < pl.R.k; j++) nIR *= dim;
     int32_t* idxL = pl.L.k ? (int32_t*)xr_workspace(c, WS_IDXL, nIL * sizeof(int32_t)) : nullptr;
-    int32_t* idxR = (int32_t*)xr_workspace(c, WS_IDXR, nIR * sizeof(int32_t));
-    if ((pl.L.k && !idxL) || !idxR) return xr_fail(c, XR_ENOMEM, "workspace (index tables)");
+    int32_t* idxR = pl.R.k ? (int32_t*)xr_workspace(c, WS_IDXR, nIR * sizeof(int32_t)) : nullptr;
+    if ((pl.L.k && !idxL) || (pl.R.k && !idxR)) return xr_fail(c, XR_ENOMEM, "workspace (index tables)");
     if (pl.L.k && (rc = xr_index_table(c, pl.L.k, dim, idxL))) return rc;
-    if ((rc = xr_index_table(c, pl.R.k, dim, idxR))) return rc;
+    if (pl.R.k && (rc = xr_index_table(c, pl.R.k, dim, idxR))) return rc;
     int64_t total = nI * nJ * nIL * nIR;
-    k_expand<<<ceil_div(total, 256), 256, 0, c->stream>>>(Cc, Ng, nI, nJ, idxL, nIL, idxR, nIR, out);
+    ProfScope prof(c, XR_K_EXPAND, 0.0, 8.0 * total);
+    k_expand<<<xr_grid(total), 256, 0, c->stream>>>(Cc, Ng, nI, nJ, idxL, nIL, idxR, nIR, out);
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }
@@ -403,7 +452,7 @@ extern "C" int xr_monomer_2e(xr_ctx* c, int n_orbs, int n_core, int n_elec, int6
     if ((rc = xr_side_table(c, pl.R, Km, nK, n_orbs, n_core, tabR))) return rc;
     if ((rc = xr_gather(c, tabL, nK, nU, zTb, nI, X, 1))) return rc;   // X[u][K][I]
     if ((rc = xr_gather(c, tabR, nK, nU, zTk, nJ, Y, 1))) return rc;   // Y[w][K][J]
-    k_veff<<<ceil_div(nU * nU, 256), 256, 0, c->stream>>>(V, dim, nU, Ve);
+    k_veff<<<xr_grid(nU * nU), 256, 0, c->stream>>>(V, dim, nU, Ve);
     XR_LAUNCH_CHECK(c);
     // W[u][K*nJ + J] = sum_w Ve[u,w] Y[w][K*nJ + J]
     if ((rc = xr_dgemm_launch(c, 0, 0, nU, nK * nJ, nU, 1.0, Ve, nU, 0, Y, nK * nJ, 0, 0.0, W,
@@ -411,4 +460,322 @@ extern "C" int xr_monomer_2e(xr_ctx* c, int n_orbs, int n_core, int n_elec, int6
         return rc;
     // E[I,J] = sum_{(u,K)} X[(u,K), I] W[(u,K), J]
     return xr_dgemm_launch(c, 1, 0, nI, nJ, nU * nK, 1.0, X, nI, 0, W, nJ, 0, 0.0, out, nJ, 0, 1);
+}
+
+// ---------------------------------------------------------------------------
+// adjoints
+// ---------------------------------------------------------------------------
+// Transpose of k_expand: Gc[(l*nI + I) * ldc + t*nJ + J] = sum over the index
+// orderings of l and t of sign * Rbar[I, J, iL, iR].
+__device__ __forceinline__ void k_compress_elem(int64_t i, const double* __restrict__ Rbar, int64_t nI, int64_t nJ, int kL, int kR,
+                           int dim, int64_t nLt, int64_t nT, double* __restrict__ Gc, int64_t ldc) {
+    int64_t total = nLt * nI * nT * nJ;
+    int64_t col = i % (nT * nJ), row = i / (nT * nJ);
+    int64_t l = row / nI, I = row - l * nI;
+    int64_t t = col / nJ, J = col - t * nJ;
+    int lo[3] = {0, 0, 0}, ro[3] = {0, 0, 0};
+    if (kL) {
+        uint64_t m = d_lex_unrank(l, dim, kL);
+        for (int j = 0; j < kL; j++) { lo[j] = __ffsll((unsigned long long)m) - 1; m &= m - 1; }
+    }
+    if (kR) {
+        uint64_t m = d_lex_unrank(t, dim, kR);
+        for (int j = 0; j < kR; j++) { ro[j] = __ffsll((unsigned long long)m) - 1; m &= m - 1; }
+    }
+    int64_t nIR = 1, nIL = 1;
+    for (int j = 0; j < kR; j++) nIR *= dim;
+    for (int j = 0; j < kL; j++) nIL *= dim;
+    // permutations of up to 3 items with their parities
+    const int P3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    const int S3[6] = {1, -1, -1, 1, 1, -1};
+    const int P2[2][3] = {{0, 1, 0}, {1, 0, 0}};
+    const int S2[2] = {1, -1};
+    int nPL = kL == 3 ? 6 : (kL == 2 ? 2 : 1);
+    int nPR = kR == 3 ? 6 : (kR == 2 ? 2 : 1);
+    const double* base = Rbar + (I * nJ + J) * nIL * nIR;
+    double acc = 0.0;
+    for (int a = 0; a < nPL; a++) {
+        int64_t iL = 0;
+        int sL = 1;
+        for (int j = 0; j < kL; j++) {
+            int src = kL == 3 ? P3[a][j] : (kL == 2 ? P2[a][j] : 0);
+            iL = iL * dim + lo[src];
+        }
+        if (kL == 3) sL = S3[a];
+        else if (kL == 2) sL = S2[a];
+        for (int b = 0; b < nPR; b++) {
+            int64_t iR = 0;
+            int sR = 1;
+            for (int j = 0; j < kR; j++) {
+                int src = kR == 3 ? P3[b][j] : (kR == 2 ? P2[b][j] : 0);
+                iR = iR * dim + ro[src];
+            }
+            if (kR == 3) sR = S3[b];
+            else if (kR == 2) sR = S2[b];
+            acc += (double)(sL * sR) * base[iL * nIR + iR];
+        }
+    }
+    Gc[row * ldc + col] = acc;
+}
+
+__global__ void k_compress(const double* __restrict__ Rbar, int64_t nI, int64_t nJ, int kL, int kR,
+                           int dim, int64_t nLt, int64_t nT, double* __restrict__ Gc, int64_t ldc) {
+    int64_t total = nLt * nI * nT * nJ;
+    XR_GRID_STRIDE(i, total) k_compress_elem(i, Rbar, nI, nJ, kL, kR, dim, nLt, nT, Gc, ldc);
+}
+
+// Transpose of k_gather through the side table: for each source determinant
+// S (state space, mask Sm[S]) and state s,
+//   zbar[S, s] = sum_t sign(K, t) * Ybar[K(S,t), t, s]
+// where t runs over the tuples the side string removes from S (annihilators)
+// or adds to it (creators), and K is their image in the K space.  Each
+// output is owned by one thread: deterministic, no atomics.
+__device__ __forceinline__ void k_side_adjoint_elem(int64_t i, SidePlan sd, const uint64_t* __restrict__ Sm, int64_t nS_cfg,
+                               int n_orbs, int n_core, int kspace_core, int64_t nK, int64_t nT,
+                               const double* __restrict__ Ybar, int64_t nS, int layout,
+                               double* __restrict__ zbarT) {
+    const int dim = 2 * n_orbs;
+    int64_t S = i / nS, s = i - (i / nS) * nS;
+    uint64_t src = Sm[S];
+    uint64_t full = (dim >= 64) ? ~0ull : ((1ull << dim) - 1);
+    uint64_t pool = sd.is_c ? (~src & full) : src;  // orbitals the string may touch
+    uint64_t kcore = d_core_mask(n_orbs, kspace_core);
+    int nK_val = dim - 2 * kspace_core;
+    double acc = 0.0;
+    int orbs[64];
+    int np = 0;
+    for (uint64_t x = pool; x; x &= x - 1) orbs[np++] = __ffsll((unsigned long long)x) - 1;
+    int idx[3] = {0, 1, 2};
+    const int k = sd.k;
+    if (np >= k) {
+        while (true) {
+            uint64_t tm = 0;
+            int orb[3];
+            for (int j = 0; j < k; j++) { orb[j] = orbs[idx[j]]; tm |= 1ull << orb[j]; }
+            uint64_t km = sd.is_c ? (src | tm) : (src & ~tm);
+            if ((km & kcore) == kcore) {
+                uint64_t cur = src;
+                int sg = 1;
+                for (int j = k - 1; j >= 0; j--) {
+                    int o = orb[sd.slot[j]];
+                    sg *= d_sign_above(cur, o);
+                    cur ^= 1ull << o;
+                }
+                int64_t K = d_lex_rank(d_to_valence(km, n_orbs, kspace_core), nK_val);
+                int64_t t = d_lex_rank(tm, dim);
+                double y = layout == 0 ? Ybar[(K * nT + t) * nS + s] : Ybar[(t * nK + K) * nS + s];
+                acc += sg * y;
+            }
+            // next combination of k indices out of np
+            int j = k - 1;
+            while (j >= 0 && idx[j] == np - k + j) j--;
+            if (j < 0) break;
+            idx[j]++;
+            for (int q = j + 1; q < k; q++) idx[q] = idx[q - 1] + 1;
+        }
+    }
+    zbarT[S * nS + s] = acc;
+}
+
+__global__ void k_side_adjoint(SidePlan sd, const uint64_t* __restrict__ Sm, int64_t nS_cfg,
+                               int n_orbs, int n_core, int kspace_core, int64_t nK, int64_t nT,
+                               const double* __restrict__ Ybar, int64_t nS, int layout,
+                               double* __restrict__ zbarT) {
+    XR_GRID_STRIDE(i, nS_cfg * nS) k_side_adjoint_elem(i, sd, Sm, nS_cfg, n_orbs, n_core, kspace_core, nK, nT, Ybar, nS, layout, zbarT);
+}
+
+// zbar[s * nC + c] += zbarT[c * nS + s]
+__global__ void k_add_transpose(const double* __restrict__ zT, int64_t nS, int64_t nC,
+                                double* __restrict__ z) {
+    __shared__ double tile[32][33];
+    int64_t c0 = (int64_t)blockIdx.x * 32, s0 = (int64_t)blockIdx.y * 32;
+    int tx = threadIdx.x, ty = threadIdx.y;
+    for (int j = ty; j < 32; j += 8) {
+        int64_t c = c0 + j, s = s0 + tx;
+        if (s < nS && c < nC) tile[j][tx] = zT[c * nS + s];
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {
+        int64_t s = s0 + j, c = c0 + tx;
+        if (s < nS && c < nC) z[s * nC + c] += tile[tx][j];
+    }
+}
+
+static int side_adjoint(xr_ctx* c, const SidePlan& sd, int n_orbs, int n_core, int kspace_core,
+                        int n_elec_src, int64_t nK, int64_t nT, const double* Ybar, int64_t nS,
+                        int layout, double* zbar, double* tmpT) {
+    const uint64_t* Sm = nullptr;
+    int64_t nS_cfg = 0;
+    int rc;
+    if ((rc = xr_space_masks(c, n_orbs, n_core, n_elec_src, &Sm, &nS_cfg))) return rc;
+    int64_t n = nS_cfg * nS;
+    {
+        ProfScope prof(c, XR_K_GATHER, 0.0, 8.0 * n);
+        k_side_adjoint<<<xr_grid(n), 256, 0, c->stream>>>(sd, Sm, nS_cfg, n_orbs, n_core,
+                                                                 kspace_core, nK, nT, Ybar, nS,
+                                                                 layout, tmpT);
+        XR_LAUNCH_CHECK(c);
+    }
+    dim3 grid((unsigned)ceil_div(nS_cfg, 32), (unsigned)ceil_div(nS, 32));
+    k_add_transpose<<<grid, dim3(32, 8), 0, c->stream>>>(tmpT, nS, nS_cfg, zbar);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+// Vbar[p,q,b,a] += Veb[u,w], Vbar[q,p,b,a] -= .., Vbar[p,q,a,b] -= .., Vbar[q,p,a,b] += ..
+// (transpose of k_veff); one thread per (u, w), distinct targets.
+__device__ __forceinline__ void k_veff_adjoint_elem(int64_t i, const double* __restrict__ Veb, int dim, int64_t nU,
+                               double* __restrict__ Vbar) {
+    int64_t u = i / nU, w = i - u * nU;
+    uint64_t mu = d_lex_unrank(u, dim, 2), mw = d_lex_unrank(w, dim, 2);
+    int64_t p = __ffsll((unsigned long long)mu) - 1;
+    int64_t q = 63 - __clzll((long long)mu);
+    int64_t a = __ffsll((unsigned long long)mw) - 1;
+    int64_t b = 63 - __clzll((long long)mw);
+    int64_t D = dim;
+    double g = Veb[i];
+    auto at = [&](int64_t x0, int64_t x1, int64_t x2, int64_t x3) -> double& {
+        return Vbar[((x0 * D + x1) * D + x2) * D + x3];
+    };
+    at(p, q, b, a) += g;
+    at(q, p, b, a) -= g;
+    at(p, q, a, b) -= g;
+    at(q, p, a, b) += g;
+}
+
+__global__ void k_veff_adjoint(const double* __restrict__ Veb, int dim, int64_t nU,
+                               double* __restrict__ Vbar) {
+    XR_GRID_STRIDE(i, nU * nU) k_veff_adjoint_elem(i, Veb, dim, nU, Vbar);
+}
+
+extern "C" int xr_monomer_2e_adjoint(xr_ctx* c, int n_orbs, int n_core, int n_elec, int64_t nI,
+                                     int64_t nP, const double* zbra, int64_t nJ, int64_t nQ,
+                                     const double* zket, const double* V, const double* Ebar,
+                                     double* zbar_bra, double* zbar_ket, double* Vbar) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    OpPlan pl;
+    xr_plan_op("ccaa", &pl);
+    int nb = 0;
+    int rc = check_spaces(c, pl, n_orbs, n_core, n_elec, nP, nQ, &nb);
+    if (rc) return rc;
+    if (nI <= 0 || nJ <= 0 || n_elec < 2) return XR_OK;
+    const int dim = 2 * n_orbs;
+    double* zTb = (double*)xr_workspace(c, WS_ZTB, nP * nI * sizeof(double));
+    double* zTk = (double*)xr_workspace(c, WS_ZTK, nQ * nJ * sizeof(double));
+    if (!zTb || !zTk) return xr_fail(c, XR_ENOMEM, "workspace (coefficients)");
+    if ((rc = xr_transpose(c, zbra, nI, nP, zTb))) return rc;
+    if ((rc = xr_transpose(c, zket, nJ, nQ, zTk))) return rc;
+    const uint64_t* Km = nullptr;
+    int64_t nK = 0;
+    if ((rc = xr_space_masks(c, n_orbs, 0, n_elec - 2, &Km, &nK))) return rc;
+    const int64_t nU = h_binom(dim, 2);
+    int32_t* tabL = (int32_t*)xr_workspace(c, WS_TABL, nK * nU * sizeof(int32_t));
+    int32_t* tabR = (int32_t*)xr_workspace(c, WS_TABR, nK * nU * sizeof(int32_t));
+    double* X = (double*)xr_workspace(c, WS_X, nU * nK * nI * sizeof(double));
+    double* Y = (double*)xr_workspace(c, WS_Y, nU * nK * nJ * sizeof(double));
+    double* W = (double*)xr_workspace(c, WS_W, nU * nK * nJ * sizeof(double));
+    double* Ve = (double*)xr_workspace(c, WS_VE, nU * nU * sizeof(double));
+    double* Xbar = (double*)xr_workspace(c, WS_XB, nU * nK * nI * sizeof(double));
+    double* Ybar = (double*)xr_workspace(c, WS_YB, nU * nK * nJ * sizeof(double));
+    double* Wbar = (double*)xr_workspace(c, WS_WB, nU * nK * nJ * sizeof(double));
+    double* Veb = (double*)xr_workspace(c, WS_VEB, nU * nU * sizeof(double));
+    int64_t nmax = nP * nI > nQ * nJ ? nP * nI : nQ * nJ;
+    double* zbT = (double*)xr_workspace(c, WS_ZBT, nmax * sizeof(double));
+    if (!tabL || !tabR || !X || !Y || !W || !Ve || !Xbar || !Ybar || !Wbar || !Veb || !zbT)
+        return xr_fail(c, XR_ENOMEM, "workspace (sigma adjoint)");
+    if ((rc = xr_side_table(c, pl.L, Km, nK, n_orbs, n_core, tabL))) return rc;
+    if ((rc = xr_side_table(c, pl.R, Km, nK, n_orbs, n_core, tabR))) return rc;
+    if ((rc = xr_gather(c, tabL, nK, nU, zTb, nI, X, 1))) return rc;   // X[u][K][I]
+    if ((rc = xr_gather(c, tabR, nK, nU, zTk, nJ, Y, 1))) return rc;   // Y[w][K][J]
+    k_veff<<<xr_grid(nU * nU), 256, 0, c->stream>>>(V, dim, nU, Ve);
+    XR_LAUNCH_CHECK(c);
+    const int64_t KJ = nK * nJ;
+    // W = Ve Y (nU x KJ)
+    if ((rc = xr_dgemm_launch(c, 0, 0, nU, KJ, nU, 1.0, Ve, nU, 0, Y, KJ, 0, 0.0, W, KJ, 0, 1))) return rc;
+    // Xbar[(u,K), I] = sum_J W[(u,K), J] Ebar[I, J]    ((nU nK) x nI)
+    if ((rc = xr_dgemm_launch(c, 0, 1, nU * nK, nI, nJ, 1.0, W, nJ, 0, Ebar, nJ, 0, 0.0, Xbar, nI, 0, 1)))
+        return rc;
+    // Wbar[(u,K), J] = sum_I X[(u,K), I] Ebar[I, J]
+    if ((rc = xr_dgemm_launch(c, 0, 0, nU * nK, nJ, nI, 1.0, X, nI, 0, Ebar, nJ, 0, 0.0, Wbar, nJ, 0, 1)))
+        return rc;
+    // Ybar = Ve^T Wbar (nU x KJ)
+    if ((rc = xr_dgemm_launch(c, 1, 0, nU, KJ, nU, 1.0, Ve, nU, 0, Wbar, KJ, 0, 0.0, Ybar, KJ, 0, 1)))
+        return rc;
+    if (Vbar) {
+        // Veb = Wbar Y^T (nU x nU), then scatter into Vbar
+        if ((rc = xr_dgemm_launch(c, 0, 1, nU, nU, KJ, 1.0, Wbar, KJ, 0, Y, KJ, 0, 0.0, Veb, nU, 0, 1)))
+            return rc;
+        k_veff_adjoint<<<xr_grid(nU * nU), 256, 0, c->stream>>>(Veb, dim, nU, Vbar);
+        XR_LAUNCH_CHECK(c);
+    }
+    if (zbar_ket)
+        if ((rc = side_adjoint(c, pl.R, n_orbs, n_core, 0, n_elec, nK, nU, Ybar, nJ, 1, zbar_ket, zbT)))
+            return rc;
+    if (zbar_bra)
+        if ((rc = side_adjoint(c, pl.L, n_orbs, n_core, 0, n_elec, nK, nU, Xbar, nI, 1, zbar_bra, zbT)))
+            return rc;
+    return XR_OK;
+}
+
+extern "C" int xr_density_adjoint(xr_ctx* c, const char* ops, int n_orbs, int n_core,
+                                  int n_elec_ket, int64_t nI, int64_t nP, const double* zbra,
+                                  int64_t nJ, int64_t nQ, const double* zket, const double* Rbar,
+                                  double* zbar_bra, double* zbar_ket) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    OpPlan pl;
+    if (!xr_plan_op(ops, &pl)) return xr_fail(c, XR_EUNSUPPORTED, "operator string not supported");
+    int nb = 0;
+    int rc = check_spaces(c, pl, n_orbs, n_core, n_elec_ket, nP, nQ, &nb);
+    if (rc) return rc;
+    if (nI <= 0 || nJ <= 0) return XR_OK;
+    if (pl.L.k && pl.R.k && n_elec_ket - pl.R.k < 0) return XR_OK;  // density identically zero
+    const int dim = 2 * n_orbs;
+    Factors f;
+    if ((rc = prepare_factors(c, pl, n_orbs, n_core, n_elec_ket, nb, nI, nP, zbra, nJ, nQ, zket, &f)))
+        return rc;
+    const int64_t Mg = f.nLt * nI, Ng = f.nT * nJ;
+    double* Gc = (double*)xr_workspace(c, WS_GC, Mg * Ng * sizeof(double));
+    double* Xbar = (double*)xr_workspace(c, WS_XB, f.nK * Mg * sizeof(double));
+    double* Ybar = (double*)xr_workspace(c, WS_YB, f.nK * Ng * sizeof(double));
+    int64_t nmax = nP * nI > nQ * nJ ? nP * nI : nQ * nJ;
+    double* zbT = (double*)xr_workspace(c, WS_ZBT, nmax * sizeof(double));
+    if (!Gc || !Xbar || !Ybar || !zbT) return xr_fail(c, XR_ENOMEM, "workspace (adjoint)");
+    {
+        int64_t n = Mg * Ng;
+        ProfScope prof(c, XR_K_EXPAND, 0.0, 8.0 * n * (pl.k > 2 ? 4 : 2));
+        k_compress<<<xr_grid(n), 256, 0, c->stream>>>(Rbar, nI, nJ, pl.L.k, pl.R.k, dim,
+                                                            f.nLt, f.nT, Gc, Ng);
+        XR_LAUNCH_CHECK(c);
+    }
+    // Ybar = X Gc  (nK x Ng);  Xbar = Y Gc^T  (nK x Mg)
+    if ((rc = xr_dgemm_launch(c, 0, 0, f.nK, Ng, Mg, 1.0, f.X, Mg, 0, Gc, Ng, 0, 0.0, Ybar, Ng, 0, 1)))
+        return rc;
+    if ((rc = xr_dgemm_launch(c, 0, 1, f.nK, Mg, Ng, 1.0, f.Y, Ng, 0, Gc, Ng, 0, 0.0, Xbar, Mg, 0, 1)))
+        return rc;
+    if (zbar_ket) {
+        if (pl.R.k) {
+            if ((rc = side_adjoint(c, pl.R, n_orbs, n_core, f.kspace_core, n_elec_ket, f.nK, f.nT, Ybar,
+                                   nJ, 0, zbar_ket, zbT)))
+                return rc;
+        } else {
+            // Y = zket^T, so zbar_ket[J, Q] += Ybar[Q, J]
+            dim3 grid((unsigned)ceil_div(nQ, 32), (unsigned)ceil_div(nJ, 32));
+            k_add_transpose<<<grid, dim3(32, 8), 0, c->stream>>>(Ybar, nJ, nQ, zbar_ket);
+            XR_LAUNCH_CHECK(c);
+        }
+    }
+    if (zbar_bra) {
+        if (pl.L.k) {
+            if ((rc = side_adjoint(c, pl.L, n_orbs, n_core, f.kspace_core, nb, f.nK, f.nLt, Xbar, nI, 0,
+                                   zbar_bra, zbT)))
+                return rc;
+        } else {
+            dim3 grid((unsigned)ceil_div(nP, 32), (unsigned)ceil_div(nI, 32));
+            k_add_transpose<<<grid, dim3(32, 8), 0, c->stream>>>(Xbar, nI, nP, zbar_bra);
+            XR_LAUNCH_CHECK(c);
+        }
+    }
+    return XR_OK;
 }

@@ -36,8 +36,8 @@ template <bool FILL>
 __global__ void k_excite(OpString os, int dim, int n_orbs, int n_core, const uint64_t* ket,
                          int64_t q0, int64_t q1, int64_t* counts, const int64_t* offsets,
                          int64_t* e_q, int64_t* e_idx, int64_t* e_p, int8_t* e_sign) {
-    int64_t Q = q0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (Q >= q1) return;
+  for (int64_t Q = q0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; Q < q1;
+       Q += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t full = (dim >= 64) ? ~0ull : ((1ull << dim) - 1);
     const uint64_t core = d_core_mask(n_orbs, n_core);
     const int nval = dim - 2 * n_core;
@@ -81,6 +81,7 @@ __global__ void k_excite(OpString os, int dim, int n_orbs, int n_core, const uin
         }
     }
     if (!FILL) counts[Q - q0] = cnt;
+  }
 }
 
 static int check_common(xr_ctx* c, const char* ops, int n_orbs, int n_core, int n_elec_ket,
@@ -108,7 +109,7 @@ extern "C" int xr_excitation_count(xr_ctx* c, const char* ops, int n_orbs, int n
     if (!counts) return xr_fail(c, XR_ENOMEM, "workspace");
     XR_HIP(c, hipMemsetAsync(counts, 0, (n + 1) * sizeof(int64_t), c->stream));
     if (n > 0) {
-        k_excite<false><<<ceil_div(n, 128), 128, 0, c->stream>>>(
+        k_excite<false><<<xr_grid(n, 128), 128, 0, c->stream>>>(
             os, 2 * n_orbs, n_orbs, n_core, d_ket, q0, q1, counts, nullptr, nullptr, nullptr,
             nullptr, nullptr);
         XR_LAUNCH_CHECK(c);
@@ -139,7 +140,7 @@ extern "C" int xr_excitation_fill(xr_ctx* c, const char* ops, int n_orbs, int n_
     if (q0 < 0 || q1 < q0) return xr_fail(c, XR_EINVAL, "bad ket range");
     int64_t n = q1 - q0;
     if (n == 0) return XR_OK;
-    k_excite<true><<<ceil_div(n, 128), 128, 0, c->stream>>>(
+    k_excite<true><<<xr_grid(n, 128), 128, 0, c->stream>>>(
         os, 2 * n_orbs, n_orbs, n_core, d_ket, q0, q1, nullptr, d_offsets, d_q, d_idx, d_p,
         d_sign);
     XR_LAUNCH_CHECK(c);

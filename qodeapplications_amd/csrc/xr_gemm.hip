@@ -178,6 +178,7 @@ static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t
     if (tiles > 0x7fffffff || batch > 65535)
         return xr_fail(c, XR_EUNSUPPORTED, "GEMM grid too large");
     dim3 grid((unsigned)tiles, 1, (unsigned)batch);
+    ProfScope prof(c, XR_K_GEMM_F64, 2.0 * M * N * K * batch, 8.0 * (M * K + K * N + M * N) * batch);
 #define XR_GEMM_LAUNCH(TA_, TB_)                                                               \
     k_dgemm<BM, BN, TA_, TB_><<<grid, GEMM_THREADS, 0, c->stream>>>(M, N, K, alpha, A, lda, sA, \
                                                                      B, ldb, sB, beta, C, ldc,  \

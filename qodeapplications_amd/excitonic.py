@@ -23,7 +23,8 @@ def fci(H, out=None, device=0):
     braket_loops(Hmat, N_frag, states_per_frag, H, device=device)
     if out is not None:
         out.log("Diagonalizing ...")
-    w, _ = lowest_eigenpairs(0.5 * (Hmat + Hmat.T), k=1)
+    H1d = [torch.as_tensor(h, dtype=torch.float64, device=Hmat.device) for h in monomer_Hamiltonians]
+    w, _ = lowest_eigenpairs(0.5 * (Hmat + Hmat.T), k=1, H1=H1d)
     E = float(w[0])
     if out is not None:
         out.log("FCI Energy  =", E)

@@ -1,0 +1,293 @@
+"""Analytic gradient of the lowest XR eigenvalue of a dimer.
+
+E = lowest eigenvalue of the XR matrix H(z, integrals) (Hellmann-Feynman:
+dE = v^T dH v).  The backward pass runs every forward contraction in
+reverse on the GPU:
+
+* dE/dH: the reduced matrices D_m = V V^T, V^T V of the eigenvector V
+  (d1 x d2) for the monomer blocks, and v v^T gathered block by block
+  (xr_pair_outer) for the pair blocks;
+* every block term is a bilinear form s A W B^T (general-XRCC/
+  H_contractions.c), whose adjoint is three FP64 GEMMs giving dE/dA, dE/dB
+  and dE/dW (W an integral block);
+* density adjoints go back to the CI coefficients through
+  xr_density_adjoint / xr_monomer_2e_adjoint (the factorised densities
+  run backwards, no atomics).
+
+With lagrangian=True the coefficient gradient is that of the generalised
+eigenvalue with the overlap of the tensor-product states, dE = v^T (dH - E dS) v,
+which is what a state optimiser needs when the states drift from
+orthonormality (the E term of StateSpaceOptimizer/state_gradients.py:230).
+The reference computes the bra-side half of it (state_gradients.py:205-243).
+
+XREnergy wraps forward and backward as a torch.autograd.Function whose
+inputs are the coefficient tensors and the integral blocks; orbital-rotation
+gradients follow by ordinary autograd through the integral transformation
+(orbitals.py).
+"""
+import numpy as np
+import torch
+
+from . import blocks as B
+from ._lib import context, ptr
+from .eigen import lowest_eigenpairs
+from .gemm import gemm_f64
+
+
+def mm(A, Bm, transA=0, transB=0, alpha=1.0, out=None, beta=0.0):
+    """2-D FP64 GEMM on libxrhip: out = alpha op(A) op(B) + beta out."""
+    A = A.contiguous()
+    Bm = Bm.contiguous()
+    M = A.shape[1] if transA else A.shape[0]
+    N = Bm.shape[0] if transB else Bm.shape[1]
+    if out is None:
+        out = torch.zeros(M, N, dtype=torch.float64, device=A.device)
+        beta = 0.0
+    gemm_f64(A, Bm, out, transA, transB, alpha, beta)
+    return out
+
+
+def bilinear_adjoint(G, A, W, Bm, s, Abar=None, Bbar=None, Wbar=None):
+    """For H = s A W B^T (A: nA x p, W: p x q, B: nB x q) and G = dE/dH
+    (nA x nB): Abar += s G B W^T, Bbar += s G^T A W, Wbar += s A^T G B."""
+    if Abar is not None:
+        mm(G, mm(Bm, W, transB=1), alpha=s, out=Abar, beta=1.0)
+    if Bbar is not None:
+        mm(G, mm(A, W), transA=1, alpha=s, out=Bbar, beta=1.0)
+    if Wbar is not None:
+        mm(A, mm(G, Bm), transA=1, alpha=s, out=Wbar, beta=1.0)
+
+
+class _Acc:
+    """Zero-initialised accumulators keyed like their forward tensors."""
+
+    def __init__(self, device):
+        self.device = device
+        self.d = {}
+
+    def get(self, key, shape):
+        t = self.d.get(key)
+        if t is None:
+            t = self.d[key] = torch.zeros(shape, dtype=torch.float64, device=self.device)
+        return t
+
+
+def _flat(R):
+    return R.view(R.shape[0] * R.shape[1], -1)
+
+
+class DimerAdjoint:
+    """Forward + backward of E(z, integrals) for a two-fragment XRSystem."""
+
+    def __init__(self, system):
+        if len(system.frags) != 2:
+            raise ValueError("the analytic gradient is implemented for dimers")
+        self.X = system
+
+    def forward(self):
+        X = self.X
+        X.build_densities()
+        H1, H2, _ = X.blocks()
+        M = X.xr_matrix(H=(H1, H2, None))
+        w, v = lowest_eigenpairs(M, k=1, H1=H1)
+        self.E = float(w[0])
+        self.v = v[:, 0].contiguous()
+        return self.E
+
+    def backward(self, lagrangian=True):
+        """Returns (zbar, ibar): zbar[m][chg] = dE/dz (n_states x n_configs),
+        ibar[key] = dE/d(integral block) for keys ('T', m, n), ('U', a, m, n),
+        ('V', m, n, o, p)."""
+        X, E, v = self.X, self.E, self.v
+        F1, F2 = X.frags
+        dev = X.device
+        d1, d2 = F1.d, F2.d
+        Vm = v.view(d1, d2)
+        D = [mm(Vm, Vm, transB=1), mm(Vm, Vm, transA=1)]
+        rbar = [_Acc(dev), _Acc(dev)]
+        ibar = _Acc(dev)
+        zbar = [{c: torch.zeros_like(F.blocks[c].coeffs) for c in F.charges} for F in X.frags]
+        ctx = context(dev.index)
+
+        def ib(kind, *key):
+            t = getattr(X.ints, kind)(*key)
+            return ibar.get((kind,) + key, t.shape)
+
+        def h_bar(parts, g):
+            for kind, key in parts:
+                ib(kind, *key).add_(g)
+
+        # ---- monomer blocks: e1 = Rca . h, E2 (sigma), nuclear constant
+        for m, F in enumerate(X.frags):
+            parts = [("T", (m, m)), ("U", (m, m, m))]
+            h = X.ints.h_sum((m, m, "mono"), parts)
+            hb = torch.zeros_like(h)
+            for c in F.charges:
+                p = F.pos[c]
+                G1 = D[m][p][:, p].contiguous()
+                Rca = F.rho["ca"][(c, c)]
+                Rb = rbar[m].get(("ca", (c, c)), Rca.shape)
+                # e1[a] = sum Rca[a, pq] h[pq]
+                mm(G1.view(-1, 1), h.view(1, -1), out=_flat(Rb), beta=1.0)
+                mm(_flat(Rca), G1.view(-1, 1), transA=1, out=hb.view(-1, 1), beta=1.0)
+                blk = F.blocks[c]
+                Vb = ib("V", m, m, m, m)
+                ctx.call("xr_monomer_2e_adjoint", blk.n_orbs, blk.n_core, blk.n_elec, blk.n_states,
+                         blk.n_configs, ptr(blk.coeffs), blk.n_states, blk.n_configs, ptr(blk.coeffs),
+                         ptr(X.ints.V(m, m, m, m)), ptr(G1), ptr(zbar[m][c]), ptr(zbar[m][c]), ptr(Vb))
+            h_bar(parts, hb)
+
+        # ---- pair blocks
+        for k1, k2 in X.pair_charge_combos(0, 1):
+            self._pair_adjoint(k1, k2, v, rbar, ib, h_bar)
+
+        # ---- densities -> coefficients
+        for m, F in enumerate(X.frags):
+            for (op, key), Rb in rbar[m].d.items():
+                bb, kb = F.blocks[key[0]], F.blocks[key[1]]
+                ctx.call("xr_density_adjoint", op.encode(), kb.n_orbs, kb.n_core, kb.n_elec,
+                         bb.n_states, bb.n_configs, ptr(bb.coeffs), kb.n_states, kb.n_configs,
+                         ptr(kb.coeffs), ptr(Rb), ptr(zbar[m][key[0]]), ptr(zbar[m][key[1]]))
+        if lagrangian:
+            for m, F in enumerate(X.frags):
+                for c in F.charges:
+                    p = F.pos[c]
+                    Dc = D[m][p][:, p].contiguous()
+                    mm(Dc, F.blocks[c].coeffs, alpha=-2.0 * E, out=zbar[m][c], beta=1.0)
+        self.D = D
+        return zbar, ibar.d
+
+    def _pair_adjoint(self, k1, k2, v, rbar, ib, h_bar):
+        X = self.X
+        F1, F2 = X.frags
+        m1, m2 = 0, 1
+        (i1c, j1c), (i2c, j2c) = k1, k2
+        nI1, nJ1, nI2, nJ2 = F1.n_states[i1c], F1.n_states[j1c], F2.n_states[i2c], F2.n_states[j2c]
+        nA, nB = nI1 * nJ1, nI2 * nJ2
+        d = i1c - j1c
+        sign2 = float((-1) ** ((F2.n_elec_ref - i2c) % 2))
+        ctx = context(X.device.index)
+
+        def G(swapped):
+            out = torch.empty(nA * nB, dtype=torch.float64, device=X.device)
+            ctx.call("xr_pair_outer", ptr(v), F2.d, ptr(F1.pos[i1c]), ptr(F1.pos[j1c]), ptr(F2.pos[i2c]),
+                     ptr(F2.pos[j2c]), nI1, nJ1, nI2, nJ2, int(swapped), ptr(out))
+            return out.view(nB, nA) if swapped else out.view(nA, nB)
+
+        R1 = lambda op: F1.rho[op][k1]
+        R2 = lambda op: F2.rho[op][k2]
+        R1b = lambda op: _flat(rbar[0].get((op, k1), F1.rho[op][k1].shape))
+        R2b = lambda op: _flat(rbar[1].get((op, k2), F2.rho[op][k2].shape))
+        n1, n2 = F1.dim, F2.dim
+
+        def vperm(key, perm, shape2):
+            """W = V[key] with axes permuted and flattened, plus its accumulator
+            mapped back through the same permutation."""
+            Vt = X.ints.V(*key)
+            W = Vt.permute(*perm).contiguous().view(*shape2)
+            Wb = torch.zeros_like(W)
+            return W, Wb, lambda: ib("V", *key).add_(Wb.view([Vt.shape[i] for i in perm]).permute(*_inv(perm)))
+
+        if d in (-2, +2):
+            if d == -2:
+                W, Wb, fin = vperm((m1, m1, m2, m2), (0, 1, 3, 2), (n1 * n1, n2 * n2))
+                bilinear_adjoint(G(False), _flat(R1("cc")), W, _flat(R2("aa")), 1.0,
+                                 R1b("cc"), R2b("aa"), Wb)
+            else:
+                W, Wb, fin = vperm((m2, m2, m1, m1), (0, 1, 3, 2), (n2 * n2, n1 * n1))
+                bilinear_adjoint(G(True), _flat(R2("cc")), W, _flat(R1("aa")), 1.0,
+                                 R2b("cc"), R1b("aa"), Wb)
+            fin()
+            return
+        if d in (-1, +1):
+            if d == -1:
+                # creating fragment 1, annihilating fragment 2
+                cre, ann, nc, na, s = (R1, R1b, n1), (R2, R2b, n2), n1, n2, sign2
+                parts = [("T", (m1, m2)), ("U", (m1, m1, m2)), ("U", (m2, m1, m2))]
+                hkey = (m1, m2, "1min")
+                k11, k22 = (m1, m1, m1, m2), (m1, m2, m2, m2)
+                Gm = G(False)
+            else:
+                cre, ann, nc, na, s = (R2, R2b, n2), (R1, R1b, n1), n2, n1, -sign2
+                parts = [("T", (m2, m1)), ("U", (m1, m2, m1)), ("U", (m2, m2, m1))]
+                hkey = (m2, m1, "1pls")
+                k11, k22 = (m2, m2, m2, m1), (m2, m1, m1, m1)
+                Gm = G(True)
+            Rc, Rcb = cre[0], cre[1]
+            Ra, Rab = ann[0], ann[1]
+            h = X.ints.h_sum(hkey, parts)
+            hb = torch.zeros_like(h)
+            bilinear_adjoint(Gm, _flat(Rc("c")), h, _flat(Ra("a")), s, Rcb("c"), Rab("a"), hb)
+            h_bar(parts, hb)
+            W1, W1b, fin1 = vperm(k11, (1, 0, 2, 3), (nc ** 3, na))
+            bilinear_adjoint(Gm, _flat(Rc("cca")), W1, _flat(Ra("a")), 2.0 * s, Rcb("cca"), Rab("a"), W1b)
+            fin1()
+            W2, W2b, fin2 = vperm(k22, (0, 1, 3, 2), (nc, na ** 3))
+            bilinear_adjoint(Gm, _flat(Rc("c")), W2, _flat(Ra("caa")), 2.0 * s, Rcb("c"), Rab("caa"), W2b)
+            fin2()
+            return
+        # d == 0
+        Gm = G(False)
+        W, Wb, fin = vperm((m1, m2, m1, m2), (0, 2, 1, 3), (n1 * n1, n2 * n2))
+        bilinear_adjoint(Gm, _flat(R1("ca")), W, _flat(R2("ca")), 4.0, R1b("ca"), R2b("ca"), Wb)
+        fin()
+        # extPot terms: e2[b] on I1 == J1 rows, e1[a] on I2 == J2 columns
+        dia1 = torch.zeros(nA, 1, dtype=torch.float64, device=X.device)
+        dia1.view(nI1, nJ1).diagonal().fill_(1.0)
+        dia2 = torch.zeros(nB, 1, dtype=torch.float64, device=X.device)
+        dia2.view(nI2, nJ2).diagonal().fill_(1.0)
+        e2b = mm(Gm, dia1, transA=1)           # (nB x 1)
+        e1b = mm(Gm, dia2)                     # (nA x 1)
+        U12 = X.ints.U(m1, m2, m2)
+        U21 = X.ints.U(m2, m1, m1)
+        mm(e2b, U12.view(1, -1), out=R2b("ca"), beta=1.0)
+        mm(_flat(R2("ca")), e2b, transA=1, out=ib("U", m1, m2, m2).view(-1, 1), beta=1.0)
+        mm(e1b, U21.view(1, -1), out=R1b("ca"), beta=1.0)
+        mm(_flat(R1("ca")), e1b, transA=1, out=ib("U", m2, m1, m1).view(-1, 1), beta=1.0)
+
+
+def _inv(perm):
+    inv = [0] * len(perm)
+    for i, p in enumerate(perm):
+        inv[p] = i
+    return inv
+
+
+class XREnergy(torch.autograd.Function):
+    """E = lowest XR eigenvalue of a dimer as a function of the coefficient
+    tensors and the integral blocks; backward is the hand-written adjoint.
+
+    apply(system, lagrangian, z_keys, int_keys, *tensors): tensors are the
+    coefficient tensors for z_keys [(m, chg)] followed by the integral blocks
+    for int_keys (IntegralCache keys)."""
+
+    @staticmethod
+    def forward(ctx, system, lagrangian, z_keys, int_keys, *tensors):
+        nz = len(z_keys)
+        for (m, c), t in zip(z_keys, tensors[:nz]):
+            F = system.frags[m]
+            F.blocks[c] = F.blocks[c].with_coeffs(t.detach())
+        for key, t in zip(int_keys, tensors[nz:]):
+            system.ints.set_block(key, t)
+        adj = DimerAdjoint(system)
+        E = adj.forward()
+        ctx.adj, ctx.lag, ctx.z_keys, ctx.int_keys = adj, lagrangian, z_keys, int_keys
+        return torch.tensor(E, dtype=torch.float64, device=system.device)
+
+    @staticmethod
+    def backward(ctx, gE):
+        zbar, ibar = ctx.adj.backward(lagrangian=ctx.lag)
+        g = float(gE)
+        out = [zbar[m][c] * g for (m, c) in ctx.z_keys]
+        for key in ctx.int_keys:
+            t = ibar.get(key)
+            out.append(None if t is None else t * g)
+        return (None, None, None, None) + tuple(out)
+
+
+def energy_and_gradient(system, lagrangian=True):
+    """(E, v, zbar, ibar) for a dimer XRSystem without autograd plumbing."""
+    adj = DimerAdjoint(system)
+    E = adj.forward()
+    zbar, ibar = adj.backward(lagrangian=lagrangian)
+    return E, adj.v, zbar, ibar

@@ -124,3 +124,17 @@ def test_gemm_f64(ta, tb, M, N, K, batch):
     Cd = C.cuda()
     gemm.gemm_f64(A.cuda(), B.cuda(), Cd, transA=ta, transB=tb, alpha=0.7, beta=-0.5)
     np.testing.assert_allclose(Cd.cpu().numpy(), ref.numpy(), atol=1e-12 * max(1, K) ** 0.5, rtol=1e-12)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_gemm_f64_large_tiles(ta, tb):
+    """Shapes whose grids take the 128 x 128 tile path."""
+    from qodeapplications_amd import gemm
+    M, N, K = 3000, 2900, 333
+    g = torch.Generator().manual_seed(11)
+    A = torch.randn(*((K, M) if ta else (M, K)), dtype=torch.float64, generator=g)
+    B = torch.randn(*((N, K) if tb else (K, N)), dtype=torch.float64, generator=g)
+    ref = (A.T if ta else A) @ (B.T if tb else B)
+    C = torch.empty(M, N, dtype=torch.float64, device="cuda")
+    gemm.gemm_f64(A.cuda(), B.cuda(), C, transA=ta, transB=tb)
+    np.testing.assert_allclose(C.cpu().numpy(), ref.numpy(), atol=1e-11, rtol=1e-12)
