@@ -138,3 +138,22 @@ def test_gemm_f64_large_tiles(ta, tb):
     C = torch.empty(M, N, dtype=torch.float64, device="cuda")
     gemm.gemm_f64(A.cuda(), B.cuda(), C, transA=ta, transB=tb)
     np.testing.assert_allclose(C.cpu().numpy(), ref.numpy(), atol=1e-11, rtol=1e-12)
+
+
+def test_adjoint_pairs_agree_at_c3_size():
+    """<I|p+ q+|J> = <J|q p|I> and <I|p+|J> = <J|p|I> for 64 states over 36
+    spin orbitals: the size at which an unbounded launch once truncated the
+    creator-string gathers."""
+    rng = np.random.default_rng(41)
+    b3 = D.ChargeBlock.from_states(syn.random_states(18, 3, 64, rng), 18, 0)
+    b4 = D.ChargeBlock.from_states(syn.random_states(18, 4, 64, rng), 18, 0)
+    b5 = D.ChargeBlock.from_states(syn.random_states(18, 5, 64, rng), 18, 0)
+    cc = D.density("cc", b5, b3)
+    aa = D.density("aa", b3, b5)
+    assert float((cc - aa.permute(1, 0, 3, 2)).abs().max()) < 1e-14
+    c = D.density("c", b5, b4)
+    a = D.density("a", b4, b5)
+    assert float((c - a.permute(1, 0, 2)).abs().max()) < 1e-14
+    cca = D.density("cca", b5, b4)
+    caa = D.density("caa", b4, b5)
+    assert float((cca - caa.permute(1, 0, 4, 3, 2)).abs().max()) < 1e-14
