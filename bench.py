@@ -36,6 +36,7 @@ def parse():
     ap.add_argument("--no-grad", action="store_true", help="skip the adjoint (diagnostics only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--dump", default=None, help="write every timed launch as CSV")
     return ap.parse_args()
 
 
@@ -185,6 +186,8 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ctx.profile(False)
+    if args.dump:
+        ctx.profile_dump(args.dump)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     n, gms, gfl, gby = ctx.profile_stats(_lib.K_GEMM_F64)
     avg_ms = gms / max(n, 1)

@@ -66,6 +66,8 @@ int xr_synchronize(xr_ctx *ctx);
 int xr_profile(xr_ctx *ctx, int enable);
 int xr_profile_stats(xr_ctx *ctx, int kind, int64_t *launches, double *total_ms, double *flops,
                      double *bytes);
+/* Writes every profiled launch (kind, GEMM shape, ms, flops, bytes) as CSV. */
+int xr_profile_dump(xr_ctx *ctx, const char *path);
 
 /* Occupation lists (n_cfg x n_elec int64, ascending rows) -> 64-bit masks,
  * checking Pauli, core occupancy and that row i has lexical rank i.

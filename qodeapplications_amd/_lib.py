@@ -30,6 +30,7 @@ SIGNATURES = {
     "xr_synchronize": (_I, [_P]),
     "xr_profile": (_I, [_P, _I]),
     "xr_profile_stats": (_I, [_P, _I, ctypes.POINTER(_L), ctypes.POINTER(_D), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "xr_profile_dump": (_I, [_P, _S]),
     "xr_configs_to_masks": (_I, [_P, _P, _L, _I, _I, _I, _P]),
     "xr_config_rank": (_I, [_P, _P, _L, _I, _I, _P]),
     "xr_excitation_count": (_I, [_P, _S, _I, _I, _I, _P, _L, _L, _P, ctypes.POINTER(_L)]),
@@ -128,6 +129,9 @@ class Context:
         self.check(self.L.xr_profile_stats(self.h, kind, ctypes.byref(n), ctypes.byref(ms),
                                            ctypes.byref(fl), ctypes.byref(by)))
         return n.value, ms.value, fl.value, by.value
+
+    def profile_dump(self, path):
+        self.check(self.L.xr_profile_dump(self.h, path.encode()))
 
     def close(self):
         if getattr(self, "h", None):

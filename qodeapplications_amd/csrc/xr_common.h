@@ -88,6 +88,7 @@ enum { XR_K_GEMM_F64 = 0, XR_K_GATHER = 1, XR_K_EXPAND = 2, XR_K_TABLE = 3, XR_K
 
 struct ProfRec {
     int kind;
+    int64_t m = 0, n = 0, k = 0, batch = 0;
     double flops, bytes;
     hipEvent_t a, b;
 };
@@ -115,6 +116,7 @@ struct ProfScope {
     xr_ctx* c;
     int idx = -1;
     ProfScope(xr_ctx* c_, int kind, double flops, double bytes);
+    void shape(int64_t m, int64_t n, int64_t k, int64_t b);
     ~ProfScope();
 };
 int xr_hip_check(xr_ctx* c, hipError_t e, const char* what);

@@ -27,7 +27,8 @@ template <int BM, int BN, bool TA, bool TB>
 __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm(
     int64_t M, int64_t N, int64_t K, double alpha, const double* __restrict__ A, int64_t lda,
     int64_t sA, const double* __restrict__ B, int64_t ldb, int64_t sB, double beta,
-    double* __restrict__ C, int64_t ldc, int64_t sC, int64_t tiles_m, int64_t tiles_n) {
+    double* __restrict__ C, int64_t ldc, int64_t sC, int64_t tiles_m, int64_t tiles_n,
+    int64_t kchunk, int64_t sSplit) {
     constexpr int PAD = 4;
     constexpr int WM = BM / 2, WN = BN / 2;    // wave tile
     constexpr int FM = WM / 16, FN = WN / 16;  // MFMA tiles per wave
@@ -39,7 +40,12 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm(
     const int64_t batch = blockIdx.z;
     A += batch * sA;
     B += batch * sB;
-    C += batch * sC;
+    C += batch * sC + (int64_t)blockIdx.y * sSplit;
+    // split K: this workgroup covers [kbeg, kbeg + K) of the full K range
+    const int64_t kbeg = (int64_t)blockIdx.y * kchunk;
+    K = (K - kbeg < kchunk) ? (K - kbeg) : kchunk;
+    if (TA) A += kbeg * lda; else A += kbeg;
+    if (TB) B += kbeg; else B += kbeg * ldb;
     const int64_t tile = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
     // groups of 8 tile rows walk the tile columns together (L2 reuse of B)
     const int64_t GROUP = 8;
@@ -168,6 +174,73 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm(
             }
 }
 
+// C = alpha * sum_s P[s] + beta * C, summed in split order.
+__global__ void k_splitk_reduce(const double* __restrict__ P, int64_t splits, int64_t batch,
+                                int64_t M, int64_t N, double alpha, double beta,
+                                double* __restrict__ C, int64_t ldc, int64_t sC) {
+    const int64_t n = batch * M * N;
+    XR_GRID_STRIDE(i, n) {
+        int64_t b = i / (M * N), r = i - b * M * N;
+        int64_t m = r / N, col = r - m * N;
+        double acc = 0.0;
+        for (int64_t s = 0; s < splits; s++) acc += P[s * n + i];
+        double* cp = C + b * sC + m * ldc + col;
+        double v = alpha * acc;
+        if (beta != 0.0) v += beta * (*cp);
+        *cp = v;
+    }
+}
+
+// Tall-skinny product for N <= 8 with A row-major (M x K, the Davidson
+// H X): memory bound on A, so each wave streams 4 rows of A once, reusing
+// every loaded row of B for all 4, and reduces across lanes at the end.
+template <int NB, bool TB>
+__global__ __launch_bounds__(256) void k_dgemm_skinny(int64_t M, int64_t N, int64_t K, double alpha,
+                                                      const double* __restrict__ A, int64_t lda,
+                                                      const double* __restrict__ B, int64_t ldb,
+                                                      double beta, double* __restrict__ C,
+                                                      int64_t ldc) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t r0 = wave * 4; r0 < M; r0 += nwaves * 4) {
+        double acc[4][NB];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) acc[r][j] = 0.0;
+        const int nr = (M - r0 < 4) ? (int)(M - r0) : 4;
+        for (int64_t k = lane; k < K; k += 64) {
+            double b[NB];
+#pragma unroll
+            for (int j = 0; j < NB; j++) b[j] = (j < N) ? (TB ? B[j * ldb + k] : B[k * ldb + j]) : 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                double a = (r < nr) ? A[(r0 + r) * lda + k] : 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; j++) acc[r][j] = fma(a, b[j], acc[r][j]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) {
+                double v = acc[r][j];
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+                acc[r][j] = v;
+            }
+        if (lane == 0) {
+            for (int r = 0; r < nr; r++)
+                for (int j = 0; j < N; j++) {
+                    double* cp = C + (r0 + r) * ldc + j;
+                    double v = alpha * acc[r][j];
+                    if (beta != 0.0) v += beta * (*cp);
+                    *cp = v;
+                }
+        }
+    }
+}
+
 template <int BM, int BN>
 static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
                         const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
@@ -177,18 +250,49 @@ static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t
     int64_t tiles = tm * tn;
     if (tiles > 0x7fffffff || batch > 65535)
         return xr_fail(c, XR_EUNSUPPORTED, "GEMM grid too large");
-    dim3 grid((unsigned)tiles, 1, (unsigned)batch);
+    // split K when the output tiles alone cannot fill the chip; the partial
+    // products go to scratch and are summed in a fixed order (deterministic)
+    int64_t splits = 1, kchunk = K;
+    const int64_t target = 512;
+    if (tiles * batch < 256 && K >= 1024) {
+        splits = ceil_div(target, tiles * batch);
+        int64_t max_split = K / 512;
+        if (splits > max_split) splits = max_split;
+        if (splits < 1) splits = 1;
+        kchunk = ceil_div(ceil_div(K, splits), GEMM_BK) * GEMM_BK;
+        splits = ceil_div(K, kchunk);
+    }
+    double* Cw = C;
+    int64_t ldw = ldc, sCw = sC, sSplit = 0;
+    double aw = alpha, bw = beta;
+    if (splits > 1) {
+        Cw = (double*)xr_workspace(c, 50, splits * batch * M * N * sizeof(double));
+        if (!Cw) return xr_fail(c, XR_ENOMEM, "workspace (split-K)");
+        ldw = N;
+        sCw = M * N;
+        sSplit = batch * M * N;
+        aw = 1.0;
+        bw = 0.0;
+    }
+    dim3 grid((unsigned)tiles, (unsigned)splits, (unsigned)batch);
     ProfScope prof(c, XR_K_GEMM_F64, 2.0 * M * N * K * batch, 8.0 * (M * K + K * N + M * N) * batch);
-#define XR_GEMM_LAUNCH(TA_, TB_)                                                               \
-    k_dgemm<BM, BN, TA_, TB_><<<grid, GEMM_THREADS, 0, c->stream>>>(M, N, K, alpha, A, lda, sA, \
-                                                                     B, ldb, sB, beta, C, ldc,  \
-                                                                     sC, tm, tn)
+    prof.shape(M, N, K, batch);
+#define XR_GEMM_LAUNCH(TA_, TB_)                                                                 \
+    k_dgemm<BM, BN, TA_, TB_><<<grid, GEMM_THREADS, 0, c->stream>>>(M, N, K, aw, A, lda, sA, B,   \
+                                                                     ldb, sB, bw, Cw, ldw, sCw,   \
+                                                                     tm, tn, kchunk, sSplit)
     if (ta && tb) XR_GEMM_LAUNCH(true, true);
     else if (ta) XR_GEMM_LAUNCH(true, false);
     else if (tb) XR_GEMM_LAUNCH(false, true);
     else XR_GEMM_LAUNCH(false, false);
 #undef XR_GEMM_LAUNCH
     XR_LAUNCH_CHECK(c);
+    if (splits > 1) {
+        int64_t n = batch * M * N;
+        k_splitk_reduce<<<xr_grid(n), 256, 0, c->stream>>>(Cw, splits, batch, M, N, alpha, beta, C, ldc,
+                                                            sC);
+        XR_LAUNCH_CHECK(c);
+    }
     return XR_OK;
 }
 
@@ -198,6 +302,21 @@ int xr_dgemm_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, 
                     const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
                     int64_t sB, double beta, double* C, int64_t ldc, int64_t sC, int64_t batch) {
     if (M <= 0 || N <= 0 || batch <= 0) return XR_OK;
+    if (N <= 8 && !ta && batch == 1 && M >= 1024 && K >= 1024) {
+        ProfScope prof(c, XR_K_GEMM_F64, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
+        prof.shape(M, N, K, batch);
+        unsigned grid = xr_grid(ceil_div(M, 4) * 64);
+#define XR_SKINNY(NB_)                                                                                  \
+        if (tb) k_dgemm_skinny<NB_, true><<<grid, 256, 0, c->stream>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc); \
+        else k_dgemm_skinny<NB_, false><<<grid, 256, 0, c->stream>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc)
+        if (N <= 1) { XR_SKINNY(1); }
+        else if (N <= 2) { XR_SKINNY(2); }
+        else if (N <= 4) { XR_SKINNY(4); }
+        else { XR_SKINNY(8); }
+#undef XR_SKINNY
+        XR_LAUNCH_CHECK(c);
+        return XR_OK;
+    }
     const int64_t want = 512;  // workgroups that keep 256 CUs busy
     int64_t big = ceil_div(M, 128) * ceil_div(N, 128) * batch;
     if (big >= want) return launch_tiles<128, 128>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);

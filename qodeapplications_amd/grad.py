@@ -49,13 +49,25 @@ def mm(A, Bm, transA=0, transB=0, alpha=1.0, out=None, beta=0.0):
 
 def bilinear_adjoint(G, A, W, Bm, s, Abar=None, Bbar=None, Wbar=None):
     """For H = s A W B^T (A: nA x p, W: p x q, B: nB x q) and G = dE/dH
-    (nA x nB): Abar += s G B W^T, Bbar += s G^T A W, Wbar += s A^T G B."""
+    (nA x nB): Abar += s G B W^T, Bbar += s G^T A W, Wbar += s A^T G B.
+    Each triple product is evaluated in the cheaper association."""
+    nA, p = A.shape
+    nB, q = Bm.shape
     if Abar is not None:
-        mm(G, mm(Bm, W, transB=1), alpha=s, out=Abar, beta=1.0)
+        if nA * nB * q + nA * q * p <= nB * q * p + nA * nB * p:
+            mm(mm(G, Bm), W, transB=1, alpha=s, out=Abar, beta=1.0)        # (G B) W^T
+        else:
+            mm(G, mm(Bm, W, transB=1), alpha=s, out=Abar, beta=1.0)        # G (B W^T)
     if Bbar is not None:
-        mm(G, mm(A, W), transA=1, alpha=s, out=Bbar, beta=1.0)
+        if nB * nA * p + nB * p * q <= nA * p * q + nB * nA * q:
+            mm(mm(G, A, transA=1), W, alpha=s, out=Bbar, beta=1.0)         # (G^T A) W
+        else:
+            mm(G, mm(A, W), transA=1, alpha=s, out=Bbar, beta=1.0)         # G^T (A W)
     if Wbar is not None:
-        mm(A, mm(G, Bm), transA=1, alpha=s, out=Wbar, beta=1.0)
+        if p * nA * nB + p * nB * q <= nA * nB * q + p * nA * q:
+            mm(mm(A, G, transA=1), Bm, alpha=s, out=Wbar, beta=1.0)        # (A^T G) B
+        else:
+            mm(A, mm(G, Bm), transA=1, alpha=s, out=Wbar, beta=1.0)        # A^T (G B)
 
 
 class _Acc:

@@ -113,7 +113,10 @@ def test_build_density_tensors_drop_in():
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-@pytest.mark.parametrize("M,N,K,batch", [(1, 1, 1, 1), (37, 129, 17, 3), (200, 64, 300, 1), (64, 513, 70, 2), (300, 300, 64, 1)])
+@pytest.mark.parametrize("M,N,K,batch", [(1, 1, 1, 1), (37, 129, 17, 3), (200, 64, 300, 1), (64, 513, 70, 2),
+                                         (300, 300, 64, 1), (64, 64, 50000, 1), (70, 50, 9000, 2),
+                                         (1500, 36, 20000, 1), (5000, 4, 3000, 1), (2048, 7, 1111, 1),
+                                         (1030, 1, 2049, 1)])
 def test_gemm_f64(ta, tb, M, N, K, batch):
     from qodeapplications_amd import gemm
     g = torch.Generator().manual_seed(M * 7 + N)
