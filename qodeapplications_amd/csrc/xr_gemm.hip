@@ -1,6 +1,11 @@
 // FP64 GEMM on the CDNA4 matrix cores (v_mfma_f64_16x16x4f64).
 //
-// C_b = alpha * op(A_b) op(B_b) + beta * C_b.  256 threads = 4 waves per
+// C_b = alpha * op(A_b) op(B_b) + beta * C_b.  The tile kernel computes
+// alpha * op(A) op(B) only (alpha is applied while A is staged in LDS) and
+// stores the accumulators untouched: any arithmetic on them in the epilogue
+// makes the compiler copy all 128 accumulator registers out of the AGPRs,
+// which halves occupancy.  beta != 0 and split-K go through a workspace and
+// a fixed-order reduction.  256 threads = 4 waves per
 // workgroup in a 2 x 2 arrangement; each wave owns a (BM/2) x (BN/2) block of
 // 16 x 16 MFMA tiles.  K advances 16 at a time through one LDS stage while the
 // next stage is prefetched into registers.  Workgroups are remapped so each
@@ -104,7 +109,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm(
             int mm, kk;
             if (TA) { mm = tid % BM; kk = tid / BM + i * (GEMM_THREADS / BM); }
             else { kk = tid % GEMM_BK; mm = tid / GEMM_BK + i * (GEMM_THREADS / GEMM_BK); }
-            As[kk][mm] = ra[i];
+            As[kk][mm] = alpha * ra[i];
         }
 #pragma unroll
         for (int i = 0; i < B_PER; i++) {
@@ -122,6 +127,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm(
         for (int j = 0; j < FN; j++) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
 
     const int64_t nk = (K + GEMM_BK - 1) / GEMM_BK;
+    (void)beta;
     if (nk > 0) {
         load_a(0);
         load_b(0);
@@ -165,12 +171,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm(
             for (int r = 0; r < 4; r++) {
                 int64_t gm = m0 + wm * WM + i * 16 + fk + 4 * r;
                 int64_t gn = n0 + wn * WN + j * 16 + fr;
-                if (gm < M && gn < N) {
-                    double v = alpha * acc[i][j][r];
-                    double* cp = C + gm * ldc + gn;
-                    if (beta != 0.0) v += beta * (*cp);
-                    *cp = v;
-                }
+                if (gm < M && gn < N) C[gm * ldc + gn] = acc[i][j][r];
             }
 }
 
@@ -254,7 +255,7 @@ static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t
     // products go to scratch and are summed in a fixed order (deterministic)
     int64_t splits = 1, kchunk = K;
     const int64_t target = 512;
-    if (tiles * batch < 256 && K >= 1024) {
+    if (tiles * batch < 256 && K >= 1024) {  // (splits may stay 1)
         splits = ceil_div(target, tiles * batch);
         int64_t max_split = K / 512;
         if (splits > max_split) splits = max_split;
@@ -265,7 +266,8 @@ static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t
     double* Cw = C;
     int64_t ldw = ldc, sCw = sC, sSplit = 0;
     double aw = alpha, bw = beta;
-    if (splits > 1) {
+    const bool staged = splits > 1 || beta != 0.0;
+    if (staged) {
         Cw = (double*)xr_workspace(c, 50, splits * batch * M * N * sizeof(double));
         if (!Cw) return xr_fail(c, XR_ENOMEM, "workspace (split-K)");
         ldw = N;
@@ -287,7 +289,7 @@ static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t
     else XR_GEMM_LAUNCH(false, false);
 #undef XR_GEMM_LAUNCH
     XR_LAUNCH_CHECK(c);
-    if (splits > 1) {
+    if (staged) {
         int64_t n = batch * M * N;
         k_splitk_reduce<<<xr_grid(n), 256, 0, c->stream>>>(Cw, splits, batch, M, N, alpha, beta, C, ldc,
                                                             sC);

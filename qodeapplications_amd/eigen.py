@@ -7,9 +7,10 @@ the monomer part H0 = sum_m H1_m (a Kronecker sum, inverted through the
 small eigendecompositions of the H1_m), and the start vectors are the lowest
 product states of H0.
 """
+import numpy as np
 import torch
 
-from .gemm import gemm_f64
+from .gemm import gemm_f64, mm
 
 DENSE_MAX = 6144
 
@@ -64,56 +65,93 @@ def lowest_eigenpairs(H, k=1, tol=1e-9, max_iter=400, H1=None, block=None):
                     max_iter, block, H.device, precond=pre)
 
 
+def _orthonormalize(T, V=None, passes=2):
+    """Columns of T made orthonormal (and orthogonal to V's columns) by
+    projected Cholesky QR; the tiny b x b factorisations run on the host.
+    Returns the kept columns (rank-revealing through the Cholesky pivots)."""
+    for _ in range(passes):
+        if V is not None and V.shape[1]:
+            mm(V, mm(V, T, transA=1), alpha=-1.0, out=T, beta=1.0)
+        G = mm(T, T, transA=1).cpu().numpy()
+        d = np.sqrt(np.maximum(np.diag(G), 0.0))
+        keep = d > 1e-10 * max(float(d.max()), 1e-300)
+        if not keep.any():
+            return T[:, :0]
+        if not keep.all():
+            T = T[:, torch.as_tensor(np.nonzero(keep)[0], device=T.device)].contiguous()
+            G = G[np.ix_(keep, keep)]
+        try:
+            L = np.linalg.cholesky(G)
+        except np.linalg.LinAlgError:
+            w, U = np.linalg.eigh(G)
+            good = w > 1e-14 * w.max()
+            Minv = U[:, good] / np.sqrt(w[good])
+            T = mm(T, torch.as_tensor(np.ascontiguousarray(Minv), device=T.device))
+            continue
+        Linv = np.ascontiguousarray(np.linalg.inv(L).T)
+        T = mm(T, torch.as_tensor(Linv, device=T.device))
+    return T.contiguous()
+
+
 def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=None,
              max_space=None, precond=None):
     """Block Davidson.  matmul(X, Y) writes H X into Y for X of shape (n, b);
     precond(R, theta) approximates (H - theta)^-1 R (default: diagonal).
-    Converged when every residual norm is below tol * max(1, |theta|)."""
+    Converged when every residual norm is below tol * max(1, |theta|).
+    The subspace lives in preallocated device buffers; the Rayleigh-Ritz
+    eigenproblem and the b x b factorisations are solved on the host."""
     b = block or max(2 * k, 4)
     max_space = max_space or max(10 * b, 48)
     dt = torch.float64
     if precond is not None:
-        V = precond.guess(b)
+        V0 = precond.guess(b)
     else:
         idx = torch.argsort(diag)[:b]
-        V = torch.zeros(n, b, dtype=dt, device=device)
-        V[idx, torch.arange(b, device=device)] = 1.0
-    V, _ = torch.linalg.qr(V)
-    V = V.contiguous()
-    AV = torch.empty(V.shape, dtype=dt, device=device)
-    matmul(V, AV)
+        V0 = torch.zeros(n, b, dtype=dt, device=device)
+        V0[idx, torch.arange(b, device=device)] = 1.0
+    Vbuf = torch.empty(n, max_space + b, dtype=dt, device=device)
+    AVbuf = torch.empty(n, max_space + b, dtype=dt, device=device)
+    V0 = _orthonormalize(V0.contiguous())
+    m = V0.shape[1]
+    Vbuf[:, :m] = V0
+    AX0 = torch.empty(n, m, dtype=dt, device=device)
+    matmul(V0, AX0)
+    AVbuf[:, :m] = AX0
+    S = mm(V0, AX0, transA=1).cpu().numpy()
     for it in range(max_iter):
-        S = V.T @ AV
+        V, AV = Vbuf[:, :m], AVbuf[:, :m]
         S = 0.5 * (S + S.T)
-        w, y = torch.linalg.eigh(S)
-        theta = w[:b]
-        X = V @ y[:, :b]
-        AX = AV @ y[:, :b]
+        w, y = np.linalg.eigh(S)
+        nb = min(b, m)
+        yb = torch.as_tensor(np.ascontiguousarray(y[:, :nb]), device=device)
+        theta = torch.as_tensor(w[:nb], device=device)
+        X = mm(V, yb)
+        AX = mm(AV, yb)
         R = AX - X * theta
-        rn = torch.linalg.norm(R[:, :k], dim=0)
-        if bool((rn <= tol * torch.clamp(theta[:k].abs(), min=1.0)).all()):
+        rn = torch.linalg.norm(R[:, :k], dim=0).cpu().numpy()
+        if np.all(rn <= tol * np.maximum(np.abs(w[:k]), 1.0)):
             return theta[:k].clone(), X[:, :k].contiguous()
-        if precond is not None:
-            T = precond(R, theta)
-        else:
-            den = theta[None, :] - diag[:, None]
-            den = torch.where(den.abs() < 1e-8, torch.full_like(den, 1e-8), den)
-            T = R / den
-        if V.shape[1] + b > max_space:      # thick restart on the current Ritz vectors
-            V, AV = X.contiguous(), AX.contiguous()
-        for _ in range(2):
-            T = T - V @ (V.T @ T)
-        Q, Rq = torch.linalg.qr(T)
-        keep = torch.abs(torch.diagonal(Rq)) > 1e-10 * float(torch.abs(torch.diagonal(Rq)).max() + 1e-300)
-        Q = Q[:, keep]
-        if Q.shape[1] == 0:
+        T = precond(R, theta) if precond is not None else R / torch.where(
+            (theta[None, :] - diag[:, None]).abs() < 1e-8, torch.full_like(R, 1e-8),
+            theta[None, :] - diag[:, None])
+        if m + nb > max_space:              # thick restart on the current Ritz vectors
+            Vbuf[:, :nb] = X
+            AVbuf[:, :nb] = AX
+            S = np.diag(w[:nb])
+            m = nb
+            V = Vbuf[:, :m]
+        Q = _orthonormalize(T.contiguous(), V)
+        q = Q.shape[1]
+        if q == 0:
             return theta[:k].clone(), X[:, :k].contiguous()
-        Q = Q - V @ (V.T @ Q)
-        Q, _ = torch.linalg.qr(Q)
-        Q = Q.contiguous()
-        AQ = torch.empty(Q.shape, dtype=dt, device=Q.device)
+        AQ = torch.empty(n, q, dtype=dt, device=device)
         matmul(Q, AQ)
-        V = torch.cat([V, Q], dim=1)
-        AV = torch.cat([AV, AQ], dim=1)
+        # grow the projected matrix by the new rows / columns
+        VtAQ = mm(Vbuf[:, :m], AQ, transA=1).cpu().numpy()
+        QtAQ = mm(Q, AQ, transA=1).cpu().numpy()
+        S = np.block([[S, VtAQ], [VtAQ.T, QtAQ]])
+        Vbuf[:, m:m + q] = Q
+        AVbuf[:, m:m + q] = AQ
+        m += q
     raise RuntimeError(f"Davidson did not converge in {max_iter} iterations "
                        f"(residual {float(rn.max()):.2e})")

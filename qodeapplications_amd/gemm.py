@@ -27,3 +27,30 @@ def gemm_f64(A, B, C, transA=0, transB=0, alpha=1.0, beta=0.0):
     ctx.call("xr_gemm_f64", int(transA), int(transB), M, N, K, float(alpha), ptr(A3), A3.shape[2], sA,
              ptr(B3), B3.shape[2], sB, float(beta), ptr(C3), N, M * N, nb)
     return C
+
+
+def mm(A, B, transA=0, transB=0, alpha=1.0, out=None, beta=0.0):
+    """out = alpha op(A) op(B) + beta out for 2-D float64 CUDA tensors that
+    are row-major with any leading dimension (views of wider buffers work
+    without copies).  transA = 1 uses A^T, transB = 1 uses B^T."""
+    for t in (A, B):
+        if t.dim() != 2 or t.dtype != torch.float64 or not t.is_cuda:
+            raise ValueError("mm needs 2-D float64 CUDA tensors")
+    if A.stride(1) != 1:
+        A = A.contiguous()
+    if B.stride(1) != 1:
+        B = B.contiguous()
+    M = A.shape[1] if transA else A.shape[0]
+    K = A.shape[0] if transA else A.shape[1]
+    N = B.shape[0] if transB else B.shape[1]
+    if (B.shape[1] if transB else B.shape[0]) != K:
+        raise ValueError("inner dimensions differ")
+    if out is None:
+        out = torch.empty(M, N, dtype=torch.float64, device=A.device)
+        beta = 0.0
+    if out.stride(1) != 1 or tuple(out.shape) != (M, N):
+        raise ValueError("out must be row-major (M, N)")
+    ctx = context(A.device.index)
+    ctx.call("xr_gemm_f64", int(transA), int(transB), M, N, K, float(alpha), ptr(A), A.stride(0), 0,
+             ptr(B), B.stride(0), 0, float(beta), ptr(out), out.stride(0), 0, 1)
+    return out
