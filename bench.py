@@ -37,6 +37,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--dump", default=None, help="write every timed launch as CSV")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run one device-to-device copy of this many bytes (calibrates "
+                         "the FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
 
 
@@ -149,6 +152,15 @@ def main():
                                    "in DESIGN.md but not built"}))
         sys.exit(2)
     import torch
+    if args.copy_bytes:
+        n = args.copy_bytes // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 8}))
+        return
     from qodeapplications_amd import _lib
     from qodeapplications_amd.xr import XRSystem
     from qodeapplications_amd.eigen import lowest_eigenpairs
@@ -200,6 +212,13 @@ def main():
         stats[name] = {"launches": ln, "ms_per_step": round(ms / args.steps, 3)}
     print(f"bench: E={E:.12f} ms/step={ms_per_step:.1f} kernels={stats}", file=sys.stderr, flush=True)
 
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "c3_dimer_traffic.json")
+    if args.config == "c3" and os.path.exists(tfile):
+        # HBM bytes per FP64 GEMM launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+        # on this workload (the profile summary names the commands)
+        traffic = json.load(open(tfile))["traffic_bytes_per_launch"]
+
     cpu = None
     if not args.no_cpu:
         t_cpu, sample = cpu_baseline(w, args.cpu_seconds)
@@ -223,7 +242,8 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": MFMA_F64_PEAK / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / MFMA_F64_PEAK,
                      "kernel": "k_dgemm (FP64 MFMA GEMM)", "launches_per_step": n / args.steps,
-                     "avg_launch_ms": avg_ms, "traffic": None},
+                     "avg_launch_ms": avg_ms, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": gby / max(n, 1)},
         "kernels": stats,
         "cpu_baseline": cpu,
     }
