@@ -33,6 +33,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c3", choices=["c0", "c3"])
+    ap.add_argument("--precision", default="both", choices=["f64", "split_f32", "both"],
+                    help="two-electron contraction precision; 'both' times FP64 (the value) "
+                         "and split-FP32 (reported beside it with its |dE|)")
     ap.add_argument("--no-grad", action="store_true", help="skip the adjoint (diagnostics only)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -167,50 +170,75 @@ def main():
     from qodeapplications_amd.orbitals import OrbitalRotation, energy_and_gradients
 
     w, wdesc = workload(args.config)
-    torch.cuda.synchronize()
-    X = XRSystem(w.fragments, w.integrals, w.nuc)
     I = w.integrals
     rot = OrbitalRotation(I.frag_orbs, I.h_sp, I.V_sp, I.U_sp)
     ctx = _lib.context(0)
 
-    def step():
-        if args.no_grad:
-            X.build_densities()
-            H = X.blocks()
-            M = X.xr_matrix(H=H)
-            E, v = lowest_eigenpairs(M, k=1, H1=H[0])
-            return float(E[0]), None
-        E, gz, gk = energy_and_gradients(X, rot, lagrangian=True)
-        gnorm = math.sqrt(sum(float((g * g).sum()) for g in gz.values()) +
-                          sum(float((g * g).sum()) for g in gk))
-        return E, gnorm
+    def measure(prec):
+        """Warm-up, then the timed steps of one precision path; returns the
+        per-step time, energy, gradient and the profiler's kernel stats."""
+        X = XRSystem(w.fragments, w.integrals, w.nuc, precision=prec)
 
-    print(f"bench: workload {wdesc}; warmup {args.warmup}", file=sys.stderr, flush=True)
-    for i in range(args.warmup):
-        E, gn = step()
+        def step():
+            if args.no_grad:
+                X.build_densities()
+                H = X.blocks()
+                M = X.xr_matrix(H=H)
+                E, v = lowest_eigenpairs(M, k=1, H1=H[0])
+                return float(E[0]), None
+            E, gz, gk = energy_and_gradients(X, rot, lagrangian=True)
+            g = torch.cat([x.reshape(-1) for x in gz.values()] + [x.reshape(-1) for x in gk])
+            return E, g
+
+        print(f"bench: workload {wdesc}; precision {prec}; warmup {args.warmup}", file=sys.stderr, flush=True)
+        for i in range(args.warmup):
+            E, g = step()
+            torch.cuda.synchronize()
+            print(f"bench: warmup {i} E={E:.12f}", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
-        print(f"bench: warmup {i} E={E:.12f} |grad|={gn}", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    ctx.profile(True)
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        E, gn = step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ctx.profile(False)
-    if args.dump:
-        ctx.profile_dump(args.dump)
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    n, gms, gfl, gby = ctx.profile_stats(_lib.K_GEMM_F64)
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            E, g = step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ctx.profile(False)
+        if args.dump:
+            ctx.profile_dump(args.dump.replace(".csv", f"_{prec}.csv"))
+        ms = (t1 - t0) * 1e3 / args.steps
+        kind = _lib.K_GEMM_F64 if prec == "f64" else _lib.K_GEMM_SPLIT
+        stats = {}
+        for name, k in (("gemm_f64", _lib.K_GEMM_F64), ("gemm_split_f32", _lib.K_GEMM_SPLIT),
+                        ("gather", _lib.K_GATHER), ("expand", _lib.K_EXPAND),
+                        ("excitation_table", _lib.K_TABLE), ("block_scatter", _lib.K_SCATTER),
+                        ("braket", _lib.K_BRAKET)):
+            ln, kms, fl, by = ctx.profile_stats(k)
+            stats[name] = {"launches": ln, "ms_per_step": round(kms / args.steps, 3),
+                           "tflops": round(fl / (kms * 1e-3) / 1e12, 2) if kms > 0 and fl > 0 else None}
+        n, gms, gfl, gby = ctx.profile_stats(_lib.K_GEMM_F64)
+        print(f"bench: {prec} E={E:.12f} ms/step={ms:.1f} kernels={stats}", file=sys.stderr, flush=True)
+        del X
+        torch.cuda.empty_cache()
+        return {"ms": ms, "E": E, "g": g, "stats": stats, "gemm": (n, gms, gfl, gby)}
+
+    precs = ["f64", "split_f32"] if args.precision == "both" else [args.precision]
+    res = {p: measure(p) for p in precs}
+    main_p = precs[0]
+    r0 = res[main_p]
+    ms_per_step, E = r0["ms"], r0["E"]
+    n, gms, gfl, gby = r0["gemm"]
     avg_ms = gms / max(n, 1)
     achieved = gfl / (gms * 1e-3) if gms > 0 else 0.0
-    stats = {}
-    for name, k in (("gemm_f64", _lib.K_GEMM_F64), ("gather", _lib.K_GATHER), ("expand", _lib.K_EXPAND),
-                    ("excitation_table", _lib.K_TABLE), ("block_scatter", _lib.K_SCATTER),
-                    ("braket", _lib.K_BRAKET)):
-        ln, ms, fl, by = ctx.profile_stats(k)
-        stats[name] = {"launches": ln, "ms_per_step": round(ms / args.steps, 3)}
-    print(f"bench: E={E:.12f} ms/step={ms_per_step:.1f} kernels={stats}", file=sys.stderr, flush=True)
+    stats = r0["stats"]
+    gn = None if r0["g"] is None else float(torch.linalg.norm(r0["g"]))
+    split = None
+    if "split_f32" in res and main_p == "f64":
+        rs = res["split_f32"]
+        split = {"s_per_iter": rs["ms"] / 1e3, "E0": rs["E"], "abs_dE_vs_f64": abs(rs["E"] - E),
+                 "speedup_vs_f64": ms_per_step / rs["ms"], "kernels": rs["stats"]}
+        if rs["g"] is not None and r0["g"] is not None:
+            split["grad_max_abs_err_vs_f64"] = float((rs["g"] - r0["g"]).abs().max())
+            split["grad_rel_err_vs_f64"] = float(torch.linalg.norm(rs["g"] - r0["g"]) / torch.linalg.norm(r0["g"]))
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "c3_dimer_traffic.json")
@@ -237,7 +265,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded random orthonormal CI vectors, 8-fold symmetric V)",
-        "config": {"workload": wdesc, "precision_path": "fp64", "gradient": not args.no_grad,
+        "config": {"workload": wdesc, "precision_path": main_p, "gradient": not args.no_grad,
                    "E0": E, "grad_norm": gn},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": MFMA_F64_PEAK / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / MFMA_F64_PEAK,
@@ -245,6 +273,7 @@ def main():
                      "avg_launch_ms": avg_ms, "traffic": traffic,
                      "algorithmic_bytes_per_launch": gby / max(n, 1)},
         "kernels": stats,
+        "split_fp32": split,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -46,6 +46,9 @@ enum {
 
 typedef struct xr_ctx xr_ctx;
 
+/* Precision of the two-electron contraction GEMMs (xr_set_precision). */
+enum { XR_PREC_F64 = 0, XR_PREC_SPLIT_F32 = 1 };
+
 /* Context on `device`, queuing on `stream` (a hipStream_t; NULL = default).
  * No reference counterpart: the reference is a set of stateless C functions
  * compiled at import (general-XRCC/build_density_tensors.py:23-24). */
@@ -148,6 +151,21 @@ int xr_gemm_f64(xr_ctx *ctx, int transA, int transB, int64_t M, int64_t N, int64
                 double alpha, const double *d_A, int64_t lda, int64_t strideA,
                 const double *d_B, int64_t ldb, int64_t strideB,
                 double beta, double *d_C, int64_t ldc, int64_t strideC, int64_t batch);
+
+/* Split-precision GEMM: each FP64 operand is split into hi = fp32(x) and
+ * lo = fp32(x - hi); the FP32 matrix cores form hi*hi and hi*lo + lo*hi per
+ * 4-deep step and every partial is added into FP64 accumulators at once.
+ * Same arguments as xr_gemm_f64.  No reference counterpart. */
+int xr_gemm_split_f32(xr_ctx *ctx, int transA, int transB, int64_t M, int64_t N, int64_t K,
+                      double alpha, const double *d_A, int64_t lda, int64_t strideA,
+                      const double *d_B, int64_t ldb, int64_t strideB,
+                      double beta, double *d_C, int64_t ldc, int64_t strideC, int64_t batch);
+
+/* Selects FP64 (XR_PREC_F64, default) or split-FP32 (XR_PREC_SPLIT_F32)
+ * GEMMs for the two-electron contractions (xr_monomer_2e, xr_dimer_2min2pls,
+ * xr_dimer_1min1pls_2e, xr_dimer_ExEx, xr_trimer_*).  Densities, one-electron
+ * terms and the eigenproblem stay FP64. */
+int xr_set_precision(xr_ctx *ctx, int mode);
 
 /* Batched H_contractions.c families.  Each evaluates one reference scalar
  * contraction for every pair (a, b) of state pairs at once:

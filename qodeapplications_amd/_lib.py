@@ -11,6 +11,7 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libxrhip.so")
 
+PREC_F64, PREC_SPLIT_F32 = 0, 1
 K_GEMM_F64, K_GATHER, K_EXPAND, K_TABLE, K_SCATTER, K_BRAKET, K_OTHER, K_GEMM_SPLIT = range(8)
 XR_OK, XR_EINVAL, XR_ESHAPE, XR_EINVAL_CONFIG_ORDER, XR_EPAULI, XR_ENOMEM, XR_EHIP, XR_EUNSUPPORTED = range(8)
 _CODE_NAMES = {1: "XR_EINVAL", 2: "XR_ESHAPE", 3: "XR_EINVAL_CONFIG_ORDER", 4: "XR_EPAULI",
@@ -41,6 +42,8 @@ SIGNATURES = {
     "xr_monomer_2e_adjoint": (_I, [_P, _I, _I, _I, _L, _L, _P, _L, _L, _P, _P, _P, _P, _P, _P]),
     "xr_pair_outer": (_I, [_P, _P, _L, _P, _P, _P, _P, _L, _L, _L, _L, _I, _P]),
     "xr_gemm_f64": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
+    "xr_gemm_split_f32": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
+    "xr_set_precision": (_I, [_P, _I]),
     "xr_monomer_1e": (_I, [_P, _L, _L, _P, _P, _P]),
     "xr_dimer_2min2pls": (_I, [_P, _L, _L, _L, _P, _L, _P, _P, _P]),
     "xr_dimer_1min1pls_1e": (_I, [_P, _L, _L, _L, _P, _L, _P, _P, _P]),
@@ -107,6 +110,7 @@ class Context:
         if rc != XR_OK:
             raise XRError(rc, "xr_create failed")
         self.h = h
+        self.precision = PREC_F64
 
     def check(self, rc):
         if rc != XR_OK:
@@ -119,6 +123,11 @@ class Context:
         import torch
         stream = torch.cuda.current_stream(self.device).cuda_stream
         self.check(self.L.xr_set_stream(self.h, _P(stream)))
+
+    def set_precision(self, mode):
+        """PREC_F64 or PREC_SPLIT_F32 for the two-electron contraction GEMMs."""
+        self.check(self.L.xr_set_precision(self.h, int(mode)))
+        self.precision = int(mode)
 
     def profile(self, enable):
         self.check(self.L.xr_profile(self.h, 1 if enable else 0))

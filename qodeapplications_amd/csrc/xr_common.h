@@ -95,6 +95,7 @@ struct ProfRec {
 
 struct xr_ctx {
     int device = 0;
+    int precision = XR_PREC_F64;  // two-electron contraction GEMMs
     bool prof = false;
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> ev_pool;

@@ -64,10 +64,10 @@ extern "C" int xr_dimer_2min2pls(xr_ctx* c, int64_t n1, int64_t n2, int64_t nA, 
     if (!W || !G) return xr_fail(c, XR_ENOMEM, "workspace");
     int rc;
     if ((rc = permute4(c, V, n1, n1, n2, n2, 0, 1, 3, 2, W))) return rc;
-    if ((rc = xr_dgemm_launch(c, 0, 0, nA, n2 * n2, n1 * n1, 1.0, Rcc1, n1 * n1, 0, W, n2 * n2, 0,
+    if ((rc = xr_gemm_2e(c, 0, 0, nA, n2 * n2, n1 * n1, 1.0, Rcc1, n1 * n1, 0, W, n2 * n2, 0,
                               0.0, G, n2 * n2, 0, 1)))
         return rc;
-    return xr_dgemm_launch(c, 0, 1, nA, nB, n2 * n2, 1.0, G, n2 * n2, 0, Raa2, n2 * n2, 0, 0.0, out,
+    return xr_gemm_2e(c, 0, 1, nA, nB, n2 * n2, 1.0, G, n2 * n2, 0, Raa2, n2 * n2, 0, 0.0, out,
                            nB, 0, 1);
 }
 
@@ -81,10 +81,10 @@ extern "C" int xr_dimer_ExEx(xr_ctx* c, int64_t n1, int64_t n2, int64_t nA, cons
     if (!W || !G) return xr_fail(c, XR_ENOMEM, "workspace");
     int rc;
     if ((rc = permute4(c, V, n1, n2, n1, n2, 0, 2, 1, 3, W))) return rc;
-    if ((rc = xr_dgemm_launch(c, 0, 0, nA, n2 * n2, n1 * n1, 1.0, Rca1, n1 * n1, 0, W, n2 * n2, 0,
+    if ((rc = xr_gemm_2e(c, 0, 0, nA, n2 * n2, n1 * n1, 1.0, Rca1, n1 * n1, 0, W, n2 * n2, 0,
                               0.0, G, n2 * n2, 0, 1)))
         return rc;
-    return xr_dgemm_launch(c, 0, 1, nA, nB, n2 * n2, 4.0, G, n2 * n2, 0, Rca2, n2 * n2, 0, 0.0, out,
+    return xr_gemm_2e(c, 0, 1, nA, nB, n2 * n2, 4.0, G, n2 * n2, 0, Rca2, n2 * n2, 0, 0.0, out,
                            nB, 0, 1);
 }
 
@@ -118,14 +118,14 @@ extern "C" int xr_dimer_1min1pls_2e(xr_ctx* c, int64_t n1, int64_t n2, int64_t n
     if ((rc = permute4(c, V1112, n1, n1, n1, n2, 1, 0, 2, 3, W1))) return rc;
     if ((rc = permute4(c, V1222, n1, n2, n2, n2, 0, 1, 3, 2, W2))) return rc;
     // T = Rcca1 W1 (nA x n2);  S[b, p1] = sum Rcaa2[b, (q2 s2 r2)] W2[p1, (q2 s2 r2)]
-    if ((rc = xr_dgemm_launch(c, 0, 0, nA, n2, n1c, 1.0, Rcca1, n1c, 0, W1, n2, 0, 0.0, T, n2, 0, 1)))
+    if ((rc = xr_gemm_2e(c, 0, 0, nA, n2, n1c, 1.0, Rcca1, n1c, 0, W1, n2, 0, 0.0, T, n2, 0, 1)))
         return rc;
-    if ((rc = xr_dgemm_launch(c, 0, 1, nB, n1, n2c, 1.0, Rcaa2, n2c, 0, W2, n2c, 0, 0.0, S, n1, 0, 1)))
+    if ((rc = xr_gemm_2e(c, 0, 1, nB, n1, n2c, 1.0, Rcaa2, n2c, 0, W2, n2c, 0, 0.0, S, n1, 0, 1)))
         return rc;
     // out = 2 (T Ra2^T + Rc1 S^T)
-    if ((rc = xr_dgemm_launch(c, 0, 1, nA, nB, n2, 2.0, T, n2, 0, Ra2, n2, 0, 0.0, out, nB, 0, 1)))
+    if ((rc = xr_gemm_2e(c, 0, 1, nA, nB, n2, 2.0, T, n2, 0, Ra2, n2, 0, 0.0, out, nB, 0, 1)))
         return rc;
-    return xr_dgemm_launch(c, 0, 1, nA, nB, n1, 2.0, Rc1, n1, 0, S, n1, 0, 1.0, out, nB, 0, 1);
+    return xr_gemm_2e(c, 0, 1, nA, nB, n1, 2.0, Rc1, n1, 0, S, n1, 0, 1.0, out, nB, 0, 1);
 }
 
 // out[a, b, c] = f * sum_{x y r s} W[(x y), r, s] R1[a, (x y)] R2[b, r] R3[c, s]
@@ -143,18 +143,18 @@ static int trimer_chain(xr_ctx* c, const double* V, int64_t d0, int64_t d1, int6
     int rc;
     if ((rc = permute4(c, V, d0, d1, d2, d3, p0, p1, p2, p3, W))) return rc;
     // G1[a, (r s)] = sum_{xy} R1[a, xy] W[xy, (r s)]
-    if ((rc = xr_dgemm_launch(c, 0, 0, nA, n2 * n3, k1, 1.0, R1, k1, 0, W, n2 * n3, 0, 0.0, G1,
+    if ((rc = xr_gemm_2e(c, 0, 0, nA, n2 * n3, k1, 1.0, R1, k1, 0, W, n2 * n3, 0, 0.0, G1,
                               n2 * n3, 0, 1)))
         return rc;
     // G2[a][b, s] = sum_r R2[b, r] G1[a][r, s]     (batched over a)
     for (int64_t a0 = 0; a0 < nA; a0 += 65535) {
         int64_t nb = (nA - a0 < 65535) ? (nA - a0) : 65535;
-        if ((rc = xr_dgemm_launch(c, 0, 0, nB, n3, n2, 1.0, R2, n2, 0, G1 + a0 * n2 * n3, n3, n2 * n3,
+        if ((rc = xr_gemm_2e(c, 0, 0, nB, n3, n2, 1.0, R2, n2, 0, G1 + a0 * n2 * n3, n3, n2 * n3,
                                   0.0, G2 + a0 * nB * n3, n3, nB * n3, nb)))
             return rc;
     }
     // out[(a b), c] = f * sum_s G2[(a b), s] R3[c, s]
-    return xr_dgemm_launch(c, 0, 1, nA * nB, nC, n3, f, G2, n3, 0, R3, n3, 0, 0.0, out, nC, 0, 1);
+    return xr_gemm_2e(c, 0, 1, nA * nB, nC, n3, f, G2, n3, 0, R3, n3, 0, 0.0, out, nC, 0, 1);
 }
 
 extern "C" int xr_trimer_2min1pls1pls(xr_ctx* c, int64_t n1, int64_t n2, int64_t n3, int64_t nA,

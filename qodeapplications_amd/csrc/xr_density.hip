@@ -455,11 +455,11 @@ extern "C" int xr_monomer_2e(xr_ctx* c, int n_orbs, int n_core, int n_elec, int6
     k_veff<<<xr_grid(nU * nU), 256, 0, c->stream>>>(V, dim, nU, Ve);
     XR_LAUNCH_CHECK(c);
     // W[u][K*nJ + J] = sum_w Ve[u,w] Y[w][K*nJ + J]
-    if ((rc = xr_dgemm_launch(c, 0, 0, nU, nK * nJ, nU, 1.0, Ve, nU, 0, Y, nK * nJ, 0, 0.0, W,
+    if ((rc = xr_gemm_2e(c, 0, 0, nU, nK * nJ, nU, 1.0, Ve, nU, 0, Y, nK * nJ, 0, 0.0, W,
                               nK * nJ, 0, 1)))
         return rc;
     // E[I,J] = sum_{(u,K)} X[(u,K), I] W[(u,K), J]
-    return xr_dgemm_launch(c, 1, 0, nI, nJ, nU * nK, 1.0, X, nI, 0, W, nJ, 0, 0.0, out, nJ, 0, 1);
+    return xr_gemm_2e(c, 1, 0, nI, nJ, nU * nK, 1.0, X, nI, 0, W, nJ, 0, 0.0, out, nJ, 0, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -692,19 +692,19 @@ extern "C" int xr_monomer_2e_adjoint(xr_ctx* c, int n_orbs, int n_core, int n_el
     XR_LAUNCH_CHECK(c);
     const int64_t KJ = nK * nJ;
     // W = Ve Y (nU x KJ)
-    if ((rc = xr_dgemm_launch(c, 0, 0, nU, KJ, nU, 1.0, Ve, nU, 0, Y, KJ, 0, 0.0, W, KJ, 0, 1))) return rc;
+    if ((rc = xr_gemm_2e(c, 0, 0, nU, KJ, nU, 1.0, Ve, nU, 0, Y, KJ, 0, 0.0, W, KJ, 0, 1))) return rc;
     // Xbar[(u,K), I] = sum_J W[(u,K), J] Ebar[I, J]    ((nU nK) x nI)
-    if ((rc = xr_dgemm_launch(c, 0, 1, nU * nK, nI, nJ, 1.0, W, nJ, 0, Ebar, nJ, 0, 0.0, Xbar, nI, 0, 1)))
+    if ((rc = xr_gemm_2e(c, 0, 1, nU * nK, nI, nJ, 1.0, W, nJ, 0, Ebar, nJ, 0, 0.0, Xbar, nI, 0, 1)))
         return rc;
     // Wbar[(u,K), J] = sum_I X[(u,K), I] Ebar[I, J]
-    if ((rc = xr_dgemm_launch(c, 0, 0, nU * nK, nJ, nI, 1.0, X, nI, 0, Ebar, nJ, 0, 0.0, Wbar, nJ, 0, 1)))
+    if ((rc = xr_gemm_2e(c, 0, 0, nU * nK, nJ, nI, 1.0, X, nI, 0, Ebar, nJ, 0, 0.0, Wbar, nJ, 0, 1)))
         return rc;
     // Ybar = Ve^T Wbar (nU x KJ)
-    if ((rc = xr_dgemm_launch(c, 1, 0, nU, KJ, nU, 1.0, Ve, nU, 0, Wbar, KJ, 0, 0.0, Ybar, KJ, 0, 1)))
+    if ((rc = xr_gemm_2e(c, 1, 0, nU, KJ, nU, 1.0, Ve, nU, 0, Wbar, KJ, 0, 0.0, Ybar, KJ, 0, 1)))
         return rc;
     if (Vbar) {
         // Veb = Wbar Y^T (nU x nU), then scatter into Vbar
-        if ((rc = xr_dgemm_launch(c, 0, 1, nU, nU, KJ, 1.0, Wbar, KJ, 0, Y, KJ, 0, 0.0, Veb, nU, 0, 1)))
+        if ((rc = xr_gemm_2e(c, 0, 1, nU, nU, KJ, 1.0, Wbar, KJ, 0, Y, KJ, 0, 0.0, Veb, nU, 0, 1)))
             return rc;
         k_veff_adjoint<<<xr_grid(nU * nU), 256, 0, c->stream>>>(Veb, dim, nU, Vbar);
         XR_LAUNCH_CHECK(c);

@@ -17,6 +17,9 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 
 #define GEMM_BK 16
 #define GEMM_THREADS 256
+// output row of accumulator element r for lane group fk (lane / 16) of
+// v_mfma_f32_16x16x4f32 (checked on gfx950; the f64 variant differs)
+#define XR_F32_ROW(fk, r) (4 * (fk) + (r))
 
 // blockIdx.x -> logical tile id, so that the ids one XCD executes (hardware
 // hands out workgroups round-robin over the 8 XCDs) are contiguous.
@@ -341,4 +344,251 @@ extern "C" int xr_gemm_f64(xr_ctx* c, int transA, int transB, int64_t M, int64_t
     if ((transB ? ldb < K : ldb < N) && N > 0 && K > 0) return xr_fail(c, XR_ESHAPE, "ldb too small");
     return xr_dgemm_launch(c, transA, transB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc,
                            sC, batch);
+}
+
+// ---------------------------------------------------------------------------
+// Split-FP32 GEMM on v_mfma_f32_16x16x4f32 with FP64 accumulation.
+//
+// While an operand tile is staged in LDS, every group of 8 consecutive k
+// values of one row of A (one column of B) is put on a common power-of-two
+// grid g = 2^(e-10), e = exponent of the group's largest |x|:
+//   hi = rint(x / g) * g      (|hi / g| <= 2^10: 11 significant bits)
+//   lo = fp32(x - hi)         (|lo| <= g/2, 24 significant bits)
+// A hi*hi product is an integer multiple of ga*gb below 2^20 of it, so the
+// 8 products of one group (two chained MFMAs) sum to < 2^23 grid units and
+// the FP32 matrix core forms them EXACTLY; that sum is then added into the
+// FP64 accumulator.  The remaining terms hi_a*lo_b + lo_a*(hi_b + lo_b) are
+// ~2^-11 of the group's scale and go through FP32 accumulation over one
+// 16-deep LDS tile; their rounding, and the 24-bit lo, leave a relative
+// error of a few 2^-35 of max|a|*max|b| per group.
+// ---------------------------------------------------------------------------
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+// Biased exponent field of |x|.
+__device__ __forceinline__ int xr_expo(double x) {
+    return (int)((__double_as_longlong(x) >> 52) & 0x7ff);
+}
+
+// Split x on the grid of a group whose largest biased exponent is eb.  A
+// group whose largest value is below 2^-1012 (or zero) is flushed to zero.
+__device__ __forceinline__ void xr_split(double x, int eb, float& hi, float& lo) {
+    if (eb < 12) { hi = 0.f; lo = 0.f; return; }
+    const double s = __longlong_as_double((long long)(2055 - eb) << 52);   // 2^(1032-eb) = 1/g
+    const double gi = __longlong_as_double((long long)(eb - 9) << 52);     // 2^(eb-1032) = g
+    const double h = rint(x * s) * gi;
+    hi = (float)h;
+    lo = (float)(x - h);
+}
+
+// Max over the 8 lanes (aligned) that hold one group.
+__device__ __forceinline__ int xr_group8_max(int e) {
+    e = max(e, __shfl_xor(e, 1));
+    e = max(e, __shfl_xor(e, 2));
+    e = max(e, __shfl_xor(e, 4));
+    return e;
+}
+
+#define SPLIT_THREADS 512
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(SPLIT_THREADS) void k_dgemm_split(
+    int64_t M, int64_t N, int64_t K, const double* __restrict__ A, int64_t lda, int64_t sA,
+    const double* __restrict__ B, int64_t ldb, int64_t sB, double* __restrict__ C, int64_t ldc,
+    int64_t sC, int64_t tiles_m, int64_t tiles_n, int64_t kchunk, int64_t sSplit) {
+    // 128x128 tile, 8 waves as 2 x 4, each wave 64 x 32 (4 x 2 MFMA blocks).
+    constexpr int BM = 128, BN = 128, PAD = 4;
+    constexpr int WM = 64, WN = 32, FM = WM / 16, FN = WN / 16;
+    constexpr int PER = BM * GEMM_BK / SPLIT_THREADS;
+    static_assert(PER == 4 && BM == BN, "4 elements per thread and operand");
+    __shared__ float Ah[GEMM_BK][BM + PAD], Al[GEMM_BK][BM + PAD];
+    __shared__ float Bh[GEMM_BK][BN + PAD], Bl[GEMM_BK][BN + PAD];
+    const int64_t batch = blockIdx.z;
+    A += batch * sA;
+    B += batch * sB;
+    C += batch * sC + (int64_t)blockIdx.y * sSplit;
+    const int64_t kbeg = (int64_t)blockIdx.y * kchunk;
+    K = (K - kbeg < kchunk) ? (K - kbeg) : kchunk;
+    if (TA) A += kbeg * lda; else A += kbeg;
+    if (TB) B += kbeg; else B += kbeg * ldb;
+    const int64_t tile = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+    const int64_t GROUP = 8, per_group = GROUP * tiles_n, g = tile / per_group, first_m = g * GROUP;
+    const int64_t gsize = (tiles_m - first_m < GROUP) ? (tiles_m - first_m) : GROUP;
+    const int64_t tm = first_m + (tile % per_group) % gsize, tn = (tile % per_group) / gsize;
+    const int64_t m0 = tm * BM, n0 = tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wm = wave >> 2, wn = wave & 3;
+    // Element i of a thread as (row, k) inside a 128 x 16 tile.  Operands
+    // stored k-major (A transposed, B plain) give lanes t and t^1 the two
+    // halves of one row's 8-deep group; row-major ones give 8 aligned lanes
+    // one k each of a row's group.
+    auto pos = [&](bool kmajor, int i, int& row, int& kk) {
+        if (kmajor) { row = (tid >> 1) & 127; kk = (tid >> 8) * 8 + (tid & 1) * 4 + i; }
+        else { kk = tid % GEMM_BK; row = tid / GEMM_BK + i * (SPLIT_THREADS / GEMM_BK); }
+    };
+    auto group_max = [&](bool kmajor, int* e) {
+        if (kmajor) {
+            int m = max(max(e[0], e[1]), max(e[2], e[3]));
+            m = max(m, __shfl_xor(m, 1));
+#pragma unroll
+            for (int i = 0; i < PER; i++) e[i] = m;
+        } else {
+#pragma unroll
+            for (int i = 0; i < PER; i++) e[i] = xr_group8_max(e[i]);
+        }
+    };
+    double ra[PER], rb[PER];
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            int mm, kk;
+            pos(TA, i, mm, kk);
+            int64_t gm = m0 + mm, gk = k0 + kk;
+            ra[i] = (gm < M && gk < K) ? (TA ? A[gk * lda + gm] : A[gm * lda + gk]) : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            int nn, kk;
+            pos(!TB, i, nn, kk);
+            int64_t gn = n0 + nn, gk = k0 + kk;
+            rb[i] = (gn < N && gk < K) ? (TB ? B[gn * ldb + gk] : B[gk * ldb + gn]) : 0.0;
+        }
+    };
+    auto store = [&]() {
+        int ea[PER], eb[PER];
+#pragma unroll
+        for (int i = 0; i < PER; i++) { ea[i] = xr_expo(ra[i]); eb[i] = xr_expo(rb[i]); }
+        group_max(TA, ea);
+        group_max(!TB, eb);
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            int mm, kk;
+            pos(TA, i, mm, kk);
+            xr_split(ra[i], ea[i], Ah[kk][mm], Al[kk][mm]);
+            pos(!TB, i, mm, kk);
+            xr_split(rb[i], eb[i], Bh[kk][mm], Bl[kk][mm]);
+        }
+    };
+    d4_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+    const int64_t nk = (K + GEMM_BK - 1) / GEMM_BK;
+    const int fr = lane & 15, fk = lane >> 4;
+    if (nk > 0) { load(0); store(); __syncthreads(); }
+    const f4_t z4 = {0.f, 0.f, 0.f, 0.f};
+    for (int64_t kt = 0; kt < nk; kt++) {
+        if (kt + 1 < nk) load((kt + 1) * GEMM_BK);
+#pragma unroll
+        for (int grp = 0; grp < GEMM_BK / 8; grp++) {
+            float ah[2][FM], al[2][FM], bh[2][FN], bl[2][FN], bs[2][FN];
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                const int k = grp * 8 + s * 4 + fk;
+#pragma unroll
+                for (int i = 0; i < FM; i++) {
+                    ah[s][i] = Ah[k][wm * WM + i * 16 + fr];
+                    al[s][i] = Al[k][wm * WM + i * 16 + fr];
+                }
+#pragma unroll
+                for (int j = 0; j < FN; j++) {
+                    bh[s][j] = Bh[k][wn * WN + j * 16 + fr];
+                    bl[s][j] = Bl[k][wn * WN + j * 16 + fr];
+                    bs[s][j] = bh[s][j] + bl[s][j];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < FM; i++)
+#pragma unroll
+                for (int j = 0; j < FN; j++) {
+                    f4_t hh = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[0][i], bh[0][j], z4, 0, 0, 0);
+                    hh = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[1][i], bh[1][j], hh, 0, 0, 0);
+                    f4_t x = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[0][i], bl[0][j], z4, 0, 0, 0);
+                    x = __builtin_amdgcn_mfma_f32_16x16x4f32(al[0][i], bs[0][j], x, 0, 0, 0);
+                    x = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[1][i], bl[1][j], x, 0, 0, 0);
+                    x = __builtin_amdgcn_mfma_f32_16x16x4f32(al[1][i], bs[1][j], x, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; r++) acc[i][j][r] += (double)hh[r] + (double)x[r];
+                }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();
+        if (kt + 1 < nk) { store(); __syncthreads(); }
+    }
+    // D[row][col] of v_mfma_f32_16x16x4f32: col = lane % 16, row = XR_F32_ROW(fk, r)
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int64_t gm = m0 + wm * WM + i * 16 + XR_F32_ROW(fk, r);
+                int64_t gn = n0 + wn * WN + j * 16 + fr;
+                if (gm < M && gn < N) C[gm * ldc + gn] = acc[i][j][r];
+            }
+}
+
+// Split-FP32 product (always staged: partial tiles go to a workspace and the
+// fixed-order reduction applies beta).
+int xr_dgemm_split_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
+                          const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
+                          int64_t sB, double beta, double* C, int64_t ldc, int64_t sC, int64_t batch) {
+    if (M <= 0 || N <= 0 || batch <= 0) return XR_OK;
+    int64_t tm = ceil_div(M, 128), tn = ceil_div(N, 128), tiles = tm * tn;
+    if (tiles > 0x7fffffff || batch > 65535) return xr_fail(c, XR_EUNSUPPORTED, "GEMM grid too large");
+    int64_t splits = 1, kchunk = K;
+    if (tiles * batch < 256 && K >= 1024) {
+        splits = ceil_div(512, tiles * batch);
+        int64_t max_split = K / 512;
+        if (splits > max_split) splits = max_split;
+        if (splits < 1) splits = 1;
+        kchunk = ceil_div(ceil_div(K, splits), GEMM_BK) * GEMM_BK;
+        splits = ceil_div(K, kchunk);
+    }
+    double* Cw = (double*)xr_workspace(c, 51, splits * batch * M * N * sizeof(double));
+    if (!Cw) return xr_fail(c, XR_ENOMEM, "workspace (split-FP32)");
+    dim3 grid((unsigned)tiles, (unsigned)splits, (unsigned)batch);
+    {
+        ProfScope prof(c, XR_K_GEMM_SPLIT, 2.0 * M * N * K * batch, 8.0 * (M * K + K * N + M * N) * batch);
+        prof.shape(M, N, K, batch);
+        if (ta && tb) k_dgemm_split<true, true><<<grid, SPLIT_THREADS, 0, c->stream>>>(M, N, K, A, lda, sA, B, ldb, sB, Cw, N, M * N, tm, tn, kchunk, batch * M * N);
+        else if (ta) k_dgemm_split<true, false><<<grid, SPLIT_THREADS, 0, c->stream>>>(M, N, K, A, lda, sA, B, ldb, sB, Cw, N, M * N, tm, tn, kchunk, batch * M * N);
+        else if (tb) k_dgemm_split<false, true><<<grid, SPLIT_THREADS, 0, c->stream>>>(M, N, K, A, lda, sA, B, ldb, sB, Cw, N, M * N, tm, tn, kchunk, batch * M * N);
+        else k_dgemm_split<false, false><<<grid, SPLIT_THREADS, 0, c->stream>>>(M, N, K, A, lda, sA, B, ldb, sB, Cw, N, M * N, tm, tn, kchunk, batch * M * N);
+        XR_LAUNCH_CHECK(c);
+    }
+    int64_t n = batch * M * N;
+    k_splitk_reduce<<<xr_grid(n), 256, 0, c->stream>>>(Cw, splits, batch, M, N, alpha, beta, C, ldc, sC);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+// The contraction GEMMs of the two-electron terms go through here: FP64 or
+// split-FP32 depending on the context's precision setting.
+int xr_gemm_2e(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
+               const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB,
+               double beta, double* C, int64_t ldc, int64_t sC, int64_t batch) {
+    if (c->precision == XR_PREC_SPLIT_F32)
+        return xr_dgemm_split_launch(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc,
+                                     sC, batch);
+    return xr_dgemm_launch(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+}
+
+extern "C" int xr_set_precision(xr_ctx* c, int mode) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (mode != XR_PREC_F64 && mode != XR_PREC_SPLIT_F32) return xr_fail(c, XR_EINVAL, "unknown precision mode");
+    c->precision = mode;
+    return XR_OK;
+}
+
+extern "C" int xr_gemm_split_f32(xr_ctx* c, int transA, int transB, int64_t M, int64_t N, int64_t K,
+                                 double alpha, const double* A, int64_t lda, int64_t sA,
+                                 const double* B, int64_t ldb, int64_t sB, double beta, double* C,
+                                 int64_t ldc, int64_t sC, int64_t batch) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (M < 0 || N < 0 || K < 0 || batch < 0) return xr_fail(c, XR_EINVAL, "negative GEMM size");
+    if (ldc < N) return xr_fail(c, XR_ESHAPE, "ldc < N");
+    return xr_dgemm_split_launch(c, transA, transB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc,
+                                 sC, batch);
 }

@@ -31,43 +31,39 @@ import torch
 from . import blocks as B
 from ._lib import context, ptr
 from .eigen import lowest_eigenpairs
-from .gemm import gemm_f64
+from .gemm import mm as _mm
+from .xr import precision
 
 
-def mm(A, Bm, transA=0, transB=0, alpha=1.0, out=None, beta=0.0):
-    """2-D FP64 GEMM on libxrhip: out = alpha op(A) op(B) + beta out."""
-    A = A.contiguous()
-    Bm = Bm.contiguous()
-    M = A.shape[1] if transA else A.shape[0]
-    N = Bm.shape[0] if transB else Bm.shape[1]
-    if out is None:
-        out = torch.zeros(M, N, dtype=torch.float64, device=A.device)
-        beta = 0.0
-    gemm_f64(A, Bm, out, transA, transB, alpha, beta)
-    return out
+def mm(A, Bm, transA=0, transB=0, alpha=1.0, out=None, beta=0.0, split=False):
+    """2-D GEMM on libxrhip: out = alpha op(A) op(B) + beta out (split-FP32
+    when split is set)."""
+    return _mm(A.contiguous(), Bm.contiguous(), transA, transB, alpha, out, beta, split)
 
 
-def bilinear_adjoint(G, A, W, Bm, s, Abar=None, Bbar=None, Wbar=None):
+def bilinear_adjoint(G, A, W, Bm, s, Abar=None, Bbar=None, Wbar=None, split=False):
     """For H = s A W B^T (A: nA x p, W: p x q, B: nB x q) and G = dE/dH
     (nA x nB): Abar += s G B W^T, Bbar += s G^T A W, Wbar += s A^T G B.
-    Each triple product is evaluated in the cheaper association."""
+    Each triple product is evaluated in the cheaper association; split runs
+    the products on the split-FP32 path (two-electron terms in that mode)."""
     nA, p = A.shape
     nB, q = Bm.shape
+    x = split
     if Abar is not None:
         if nA * nB * q + nA * q * p <= nB * q * p + nA * nB * p:
-            mm(mm(G, Bm), W, transB=1, alpha=s, out=Abar, beta=1.0)        # (G B) W^T
+            mm(mm(G, Bm, split=x), W, transB=1, alpha=s, out=Abar, beta=1.0, split=x)   # (G B) W^T
         else:
-            mm(G, mm(Bm, W, transB=1), alpha=s, out=Abar, beta=1.0)        # G (B W^T)
+            mm(G, mm(Bm, W, transB=1, split=x), alpha=s, out=Abar, beta=1.0, split=x)   # G (B W^T)
     if Bbar is not None:
         if nB * nA * p + nB * p * q <= nA * p * q + nB * nA * q:
-            mm(mm(G, A, transA=1), W, alpha=s, out=Bbar, beta=1.0)         # (G^T A) W
+            mm(mm(G, A, transA=1, split=x), W, alpha=s, out=Bbar, beta=1.0, split=x)    # (G^T A) W
         else:
-            mm(G, mm(A, W), transA=1, alpha=s, out=Bbar, beta=1.0)         # G^T (A W)
+            mm(G, mm(A, W, split=x), transA=1, alpha=s, out=Bbar, beta=1.0, split=x)    # G^T (A W)
     if Wbar is not None:
         if p * nA * nB + p * nB * q <= nA * nB * q + p * nA * q:
-            mm(mm(A, G, transA=1), Bm, alpha=s, out=Wbar, beta=1.0)        # (A^T G) B
+            mm(mm(A, G, transA=1, split=x), Bm, alpha=s, out=Wbar, beta=1.0, split=x)   # (A^T G) B
         else:
-            mm(A, mm(G, Bm), transA=1, alpha=s, out=Wbar, beta=1.0)        # A^T (G B)
+            mm(A, mm(G, Bm, split=x), transA=1, alpha=s, out=Wbar, beta=1.0, split=x)   # A^T (G B)
 
 
 class _Acc:
@@ -110,6 +106,10 @@ class DimerAdjoint:
         """Returns (zbar, ibar): zbar[m][chg] = dE/dz (n_states x n_configs),
         ibar[key] = dE/d(integral block) for keys ('T', m, n), ('U', a, m, n),
         ('V', m, n, o, p)."""
+        with precision(self.X.precision, self.X.device.index):
+            return self._backward(lagrangian)
+
+    def _backward(self, lagrangian):
         X, E, v = self.X, self.E, self.v
         F1, F2 = X.frags
         dev = X.device
@@ -174,6 +174,7 @@ class DimerAdjoint:
         F1, F2 = X.frags
         m1, m2 = 0, 1
         (i1c, j1c), (i2c, j2c) = k1, k2
+        sp = X.precision == "split_f32"
         nI1, nJ1, nI2, nJ2 = F1.n_states[i1c], F1.n_states[j1c], F2.n_states[i2c], F2.n_states[j2c]
         nA, nB = nI1 * nJ1, nI2 * nJ2
         d = i1c - j1c
@@ -204,11 +205,11 @@ class DimerAdjoint:
             if d == -2:
                 W, Wb, fin = vperm((m1, m1, m2, m2), (0, 1, 3, 2), (n1 * n1, n2 * n2))
                 bilinear_adjoint(G(False), _flat(R1("cc")), W, _flat(R2("aa")), 1.0,
-                                 R1b("cc"), R2b("aa"), Wb)
+                                 R1b("cc"), R2b("aa"), Wb, split=sp)
             else:
                 W, Wb, fin = vperm((m2, m2, m1, m1), (0, 1, 3, 2), (n2 * n2, n1 * n1))
                 bilinear_adjoint(G(True), _flat(R2("cc")), W, _flat(R1("aa")), 1.0,
-                                 R2b("cc"), R1b("aa"), Wb)
+                                 R2b("cc"), R1b("aa"), Wb, split=sp)
             fin()
             return
         if d in (-1, +1):
@@ -232,16 +233,18 @@ class DimerAdjoint:
             bilinear_adjoint(Gm, _flat(Rc("c")), h, _flat(Ra("a")), s, Rcb("c"), Rab("a"), hb)
             h_bar(parts, hb)
             W1, W1b, fin1 = vperm(k11, (1, 0, 2, 3), (nc ** 3, na))
-            bilinear_adjoint(Gm, _flat(Rc("cca")), W1, _flat(Ra("a")), 2.0 * s, Rcb("cca"), Rab("a"), W1b)
+            bilinear_adjoint(Gm, _flat(Rc("cca")), W1, _flat(Ra("a")), 2.0 * s, Rcb("cca"), Rab("a"), W1b,
+                             split=sp)
             fin1()
             W2, W2b, fin2 = vperm(k22, (0, 1, 3, 2), (nc, na ** 3))
-            bilinear_adjoint(Gm, _flat(Rc("c")), W2, _flat(Ra("caa")), 2.0 * s, Rcb("c"), Rab("caa"), W2b)
+            bilinear_adjoint(Gm, _flat(Rc("c")), W2, _flat(Ra("caa")), 2.0 * s, Rcb("c"), Rab("caa"), W2b,
+                             split=sp)
             fin2()
             return
         # d == 0
         Gm = G(False)
         W, Wb, fin = vperm((m1, m2, m1, m2), (0, 2, 1, 3), (n1 * n1, n2 * n2))
-        bilinear_adjoint(Gm, _flat(R1("ca")), W, _flat(R2("ca")), 4.0, R1b("ca"), R2b("ca"), Wb)
+        bilinear_adjoint(Gm, _flat(R1("ca")), W, _flat(R2("ca")), 4.0, R1b("ca"), R2b("ca"), Wb, split=sp)
         fin()
         # extPot terms: e2[b] on I1 == J1 rows, e1[a] on I2 == J2 columns
         dia1 = torch.zeros(nA, 1, dtype=torch.float64, device=X.device)

@@ -7,6 +7,7 @@ the layouts of general-XRCC/test_H.py:63-108, and assembles the tensor-
 product XR matrix (general-XRCC/hamiltonian.py:21-83) on the device.
 """
 import ctypes
+from contextlib import contextmanager
 
 import numpy as np
 import torch
@@ -14,6 +15,22 @@ import torch
 from . import blocks as B
 from ._lib import context, ptr
 from .density import OPS_BY_CHG_DIFF, ChargeBlock, build_density_arrays, monomer_2e
+
+
+PRECISIONS = {"f64": 0, "split_f32": 1}
+
+
+@contextmanager
+def precision(mode, device=0):
+    """Runs the enclosed two-electron contractions in `mode` ('f64' or
+    'split_f32') on the device's context, restoring the previous mode."""
+    ctx = context(device)
+    old = ctx.precision
+    ctx.set_precision(PRECISIONS[mode] if isinstance(mode, str) else int(mode))
+    try:
+        yield
+    finally:
+        ctx.set_precision(old)
 
 
 class XRFragment:
@@ -56,7 +73,12 @@ class XRSystem:
     .n_core, .n_elec_ref, .states, .state_indices; fragment-blocked
     integrals with T, U, V; nuclear repulsion matrix)."""
 
-    def __init__(self, fragments, integrals, nuc, device=0):
+    def __init__(self, fragments, integrals, nuc, device=0, precision="f64"):
+        """precision: 'f64', or 'split_f32' for the two-electron contraction
+        GEMMs (hi/lo FP32 matrix-core products accumulated in FP64)."""
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        self.precision = precision
         self.device = torch.device("cuda", device)
         self.frags = [XRFragment(f, self.device) for f in fragments]
         self.ints = B.IntegralCache(integrals, self.device)
@@ -64,8 +86,9 @@ class XRSystem:
         self.built = False
 
     def build_densities(self):
-        for m, F in enumerate(self.frags):
-            F.build(self.ints.V(m, m, m, m))
+        with precision(self.precision, self.device.index):
+            for m, F in enumerate(self.frags):
+                F.build(self.ints.V(m, m, m, m))
         self.built = True
 
     def _need(self):
@@ -168,6 +191,10 @@ class XRSystem:
 
     # -- XR matrix ----------------------------------------------------------
     def blocks(self, with_trimers=False):
+        with precision(self.precision, self.device.index):
+            return self._blocks(with_trimers)
+
+    def _blocks(self, with_trimers):
         nf = len(self.frags)
         H1 = [self.monomer_matrix(m) for m in range(nf)]
         H2 = [[None] * nf for _ in range(nf)]
@@ -188,7 +215,13 @@ class XRSystem:
         states).  spectator_sign adds the sign the reference assembly omits
         for one-electron transfers across spectator fragments."""
         H1, H2, H3 = H if H is not None else self.blocks(with_trimers)
-        return assemble(H1, H2, H3, [F.parity() for F in self.frags] if spectator_sign else None)
+        M = assemble(H1, H2, H3, [F.parity() for F in self.frags] if spectator_sign else None)
+        if self.precision != "f64":
+            # bra/ket-mirrored blocks come from different split products, so
+            # the matrix is symmetric only to the split error; the eigensolver
+            # gets its symmetric part (same v^T M v for every v)
+            M = torch.add(M, M.T).mul_(0.5)
+        return M
 
 
 def assemble(H1, H2, H3=None, parities=None):

@@ -160,3 +160,45 @@ def test_adjoint_pairs_agree_at_c3_size():
     cca = D.density("cca", b5, b4)
     caa = D.density("caa", b4, b5)
     assert float((cca - caa.permute(1, 0, 4, 3, 2)).abs().max()) < 1e-14
+
+
+def _group_max(X):
+    """max |x| over aligned groups of 8 along the last axis (zero padded)."""
+    K = X.shape[-1]
+    pad = (-K) % 8
+    Xp = torch.nn.functional.pad(X.abs(), (0, pad))
+    return Xp.view(*X.shape[:-1], -1, 8).amax(-1)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("M,N,K,wide", [(300, 260, 700, False), (64, 64, 40000, False),
+                                        (1000, 1200, 333, False), (200, 300, 517, True)])
+def test_gemm_split_f32_accuracy(ta, tb, M, N, K, wide):
+    """Split-FP32 product with FP64 accumulation.  hi*hi sums are exact (common
+    power-of-two grid per 8-deep group, 11-bit hi); the hi*lo terms and the
+    24-bit lo leave at most ~2^-29 of the group's max|a|*max|b| per group, so
+    err <= 2^-28 * sum_groups max|a| max|b| holds element by element.  For
+    N(0,1) operands the error is also below 1e-10 of sum |a||b| (plain FP32
+    would be ~1e-7 of it).  `wide` spreads the entries over 2^-40..2^40 and
+    plants zero rows and columns to exercise the grouping and zero groups."""
+    from qodeapplications_amd.gemm import mm
+    g = torch.Generator().manual_seed(M + N + K)
+    A = torch.randn(*((K, M) if ta else (M, K)), dtype=torch.float64, generator=g)
+    B = torch.randn(*((N, K) if tb else (K, N)), dtype=torch.float64, generator=g)
+    if wide:
+        A *= torch.exp2(torch.randint(-40, 41, A.shape, generator=g).double())
+        B *= torch.exp2(torch.randint(-40, 41, B.shape, generator=g).double())
+        A[..., 5] = 0.0
+        B[3] = 0.0
+    C0 = torch.randn(M, N, dtype=torch.float64, generator=g)
+    Am, Bm = (A.T if ta else A), (B.T if tb else B)
+    ref = 0.5 * (Am @ Bm) + 2.0 * C0
+    C = C0.cuda()
+    mm(A.cuda(), B.cuda(), transA=ta, transB=tb, alpha=0.5, out=C, beta=2.0, split=True)
+    err = (C.cpu() - ref).abs()
+    noise = 1e-15 * C0.abs()
+    group = 0.5 * _group_max(Am) @ _group_max(Bm.T.contiguous()).T
+    assert bool((err <= 2.0 ** -28 * group + noise).all()), float((err / group).max())
+    if not wide:
+        typical = 1e-10 * 0.5 * (Am.abs() @ Bm.abs()) + noise
+        assert bool((err <= typical).all()), float((err / typical).max())
