@@ -32,7 +32,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c3", choices=["c0", "c3"])
+    ap.add_argument("--config", default="c3", choices=["c0", "c1", "c3"],
+                    help="c3: XR iteration on the 2x18-orbital dimer (the headline line); "
+                         "c0: the same on the 2x9-orbital dimer; c1: density build of one "
+                         "13-orbital fragment, 2-6 electrons, 32 states (separate metric)")
     ap.add_argument("--precision", default="both", choices=["f64", "split_f32", "both"],
                     help="two-electron contraction precision; 'both' times FP64 (the value) "
                          "and split-FP32 (reported beside it with its |dE|)")
@@ -49,8 +52,8 @@ def parse():
 def workload(name):
     from qodeapplications_amd import synthetic as syn
     if name == "c0":
-        return syn.config0(seed=0), "dimer 2x9 spatial orbitals, 3/4/5 e, 8 states/block (configs[0])"
-    return syn.config3(seed=3), "dimer 2x18 spatial orbitals, 3/4/5 e, 64 states/block (configs[3])"
+        return syn.config0(seed=0), "dimer 2x9 spatial orbitals, 3/4/5 e, 8 states/block"
+    return syn.config3(seed=3), "dimer 2x18 spatial orbitals, 3/4/5 e, 64 states/block"
 
 
 def cpu_baseline(w, budget_s):
@@ -151,8 +154,8 @@ def _conserving_fraction(w):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described "
-                                   "in DESIGN.md but not built"}))
+        print(json.dumps({"error": "this build runs on one GPU; sharding over GPUs is "
+                                   "designed but not built"}))
         sys.exit(2)
     import torch
     if args.copy_bytes:
@@ -164,6 +167,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n * 8}))
         return
+    if args.config == "c1":
+        return bench_c1(args)
     from qodeapplications_amd import _lib
     from qodeapplications_amd.xr import XRSystem
     from qodeapplications_amd.eigen import lowest_eigenpairs
@@ -275,6 +280,71 @@ def main():
         "kernels": stats,
         "split_fp32": split,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def bench_c1(args):
+    """Single-fragment density build at scale: every 1-4 index transition
+    density of a 13-spatial-orbital fragment with 2-6 electrons and 32 states
+    per charge block (42.4 GB of FP64 tensors).  One step = the whole build."""
+    import torch
+    from qodeapplications_amd import _lib, synthetic as syn
+    from qodeapplications_amd import density as D
+    frag = syn.config1_fragment(seed=1)
+    blocks = D.charge_blocks(frag.states, frag.n_orbs, frag.n_core, device=0)
+    dim = 2 * frag.n_orbs
+    flops = 0.0          # 2 * nnz * nI * nJ, the reference loop count
+    out_bytes = 0
+    for bc, bb in blocks.items():
+        for kc, kb in blocks.items():
+            for op in D.OPS_BY_CHG_DIFF_FULL.get(bc - kc, ()):
+                flops += 2.0 * _paths_per_config(op, kb.n_elec, dim) * kb.n_configs * bb.n_states * kb.n_states
+                out_bytes += 8 * bb.n_states * kb.n_states * dim ** len(op)
+    in_bytes = sum(8 * b.coeffs.numel() for b in blocks.values())
+    ctx = _lib.context(0)
+    wdesc = "fragment 13 spatial orbitals, 2-6 e, 32 states/block, all 1-4 index densities"
+    print(f"bench: workload {wdesc}; warmup {args.warmup}", file=sys.stderr, flush=True)
+
+    def step():
+        rho = D.build_density_arrays(blocks, D.OPS_BY_CHG_DIFF_FULL, skip=())
+        del rho
+
+    for i in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        print(f"bench: warmup {i} done", file=sys.stderr, flush=True)
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.profile(False)
+    if args.dump:
+        ctx.profile_dump(args.dump)
+    s = (t1 - t0) / args.steps
+    n, gms, gfl, gby = ctx.profile_stats(_lib.K_GEMM_F64)
+    stats = {}
+    for name, k in (("gemm_f64", _lib.K_GEMM_F64), ("gather", _lib.K_GATHER), ("expand", _lib.K_EXPAND),
+                    ("excitation_table", _lib.K_TABLE)):
+        ln, kms, fl, by = ctx.profile_stats(k)
+        stats[name] = {"launches": ln, "ms_per_step": round(kms / args.steps, 3),
+                       "tflops": round(fl / (kms * 1e-3) / 1e12, 2) if kms > 0 and fl > 0 else None}
+    out = {
+        "metric": "single-fragment density build s/build",
+        "value": s, "unit": "s/build", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": s * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded random orthonormal CI vectors)",
+        "config": {"workload": wdesc, "reference_loop_flops": flops, "output_bytes": out_bytes},
+        "reference_loop_tflops": flops / s / 1e12,
+        "output_gbs": (out_bytes + in_bytes) / s / 1e9,
+        "roofline": {"bound": "mfma", "achieved": gfl / (gms * 1e-3) / 1e12 if gms else 0.0,
+                     "peak": MFMA_F64_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": gfl / (gms * 1e-3) / MFMA_F64_PEAK if gms else 0.0,
+                     "kernel": "k_dgemm (FP64 MFMA GEMM)", "launches_per_step": n / args.steps,
+                     "avg_launch_ms": gms / max(n, 1), "traffic": None},
+        "kernels": stats,
     }
     print(json.dumps(out))
 

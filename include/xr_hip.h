@@ -250,6 +250,30 @@ int xr_pair_outer(xr_ctx *ctx, const double *d_v, int64_t d2, const int64_t *d_r
                   const int64_t *d_c1, const int64_t *d_r2, const int64_t *d_c2, int64_t nI1,
                   int64_t nJ1, int64_t nI2, int64_t nJ2, int swapped, double *d_G);
 
+/* Matrix-free XR operator, axis packing (replaces the dense product-space
+ * loops of general-XRCC/hamiltonian.py:21-63 for large fragment counts).
+ * full: row-major tensor [dims[0]]...[dims[nf-1]][ncol]; packed: rows = the
+ * front axes (front0 < front1, front1 = -1 for one axis) reordered through
+ * d_rowmap (row r holds front index rowmap[r] = a*dims[front1] + b; NULL =
+ * identity, then nrows must equal the front size), columns = the other
+ * axes in order and then the column index.  mode 0: packed = gather(full);
+ * mode 1: full += alpha * scatter(packed).  nf <= 8. */
+int xr_axes_pack(xr_ctx *ctx, int nf, const int64_t *dims, int64_t ncol, int front0, int front1,
+                 const int32_t *d_rowmap, int64_t nrows, double *d_full, double *d_packed,
+                 int mode, double alpha);
+
+/* Pair-block adjoint input (the pair-outer layout of xr_pair_outer) read from
+ * a reduced pair matrix G2 = sum over the other fragments of v v^T, whose
+ * rows and columns run over the pair index a*d2 + b, reordered through
+ * d_inv when given; d_G2 may hold only the diagonal block starting at row and
+ * column `base` (general-XRCC/hamiltonian.py:21-63, the dE/dH2 of every
+ * dimer element the loops place). */
+int xr_pair_gather(xr_ctx *ctx, const double *d_G2, int64_t ldg, int64_t d2, int64_t base,
+                   const int32_t *d_inv,
+                   const int64_t *d_r1, const int64_t *d_c1, const int64_t *d_r2,
+                   const int64_t *d_c2, int64_t nI1, int64_t nJ1, int64_t nI2, int64_t nJ2,
+                   int swapped, double *d_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -252,6 +252,125 @@ extern "C" int xr_block_scatter(xr_ctx* c, int nfrag, const int64_t* nI, const i
 }
 
 #define XR_MAX_FRAG 8
+#define XR_MAX_FRAG_AX 8
+
+// ---------------------------------------------------------------------------
+// Axis packing for the matrix-free XR operator.  A vector block over the
+// tensor-product basis is a row-major tensor src[s_0]...[s_{nf-1}][j]
+// (dims[m] states of fragment m, ncol columns).  Packing brings one or two
+// fragment axes ("front") to the rows, optionally reordered through rowmap
+// (packed row r holds front index rowmap[r] = a * dims[f1] + b), and the
+// remaining axes plus the column to the packed columns, in order:
+//   packed[r * xsize + x],  x = ((i_rest0 * d_rest1 + i_rest1) ...) * ncol + j.
+// mode 0 copies src -> packed, mode 1 adds alpha * packed into src.  Every
+// element of src maps to exactly one packed element (no write conflicts).
+// ---------------------------------------------------------------------------
+struct AxesDesc {
+    int nrest;
+    int64_t s0, s1;            // strides of the front axes (s1 unused when one)
+    int64_t d1;                // dims of the second front axis (1 when one)
+    int64_t nfront;            // size of the front index space
+    int64_t rest_dim[XR_MAX_FRAG_AX], rest_stride[XR_MAX_FRAG_AX];
+    int64_t ncol, xsize, nrows;
+};
+
+__global__ void k_axes_pack(AxesDesc ad, const int32_t* __restrict__ rowmap, double* __restrict__ full,
+                            double* __restrict__ packed, int mode, double alpha) {
+    const int64_t total = ad.nrows * ad.xsize;
+    XR_GRID_STRIDE(i, total) {
+        const int64_t r = i / ad.xsize, x = i - r * ad.xsize;
+        const int64_t f = rowmap ? (int64_t)rowmap[r] : r;
+        if (f < 0 || f >= ad.nfront) continue;
+        const int64_t a = f / ad.d1, b = f - a * ad.d1;
+        int64_t off = a * ad.s0 + b * ad.s1;
+        int64_t y = x / ad.ncol;
+        off += x - y * ad.ncol;
+        for (int k = ad.nrest - 1; k >= 0; k--) {
+            const int64_t q = y / ad.rest_dim[k];
+            off += (y - q * ad.rest_dim[k]) * ad.rest_stride[k];
+            y = q;
+        }
+        if (mode == 0) packed[i] = full[off];
+        else full[off] += alpha * packed[i];
+    }
+}
+
+extern "C" int xr_axes_pack(xr_ctx* c, int nf, const int64_t* dims, int64_t ncol, int front0, int front1,
+                            const int32_t* d_rowmap, int64_t nrows, double* d_full, double* d_packed,
+                            int mode, double alpha) {
+    XR_ENTRY(c);
+    if (nf < 1 || nf > XR_MAX_FRAG_AX) return xr_fail(c, XR_EUNSUPPORTED, "1 to 8 fragment axes");
+    if (front0 < 0 || front0 >= nf || front1 >= nf || front1 == front0 || (front1 >= 0 && front1 < front0))
+        return xr_fail(c, XR_EINVAL, "front axes must satisfy 0 <= front0 < front1 < nf (front1 = -1 for one)");
+    if (ncol <= 0 || (mode != 0 && mode != 1)) return xr_fail(c, XR_EINVAL, "bad ncol or mode");
+    AxesDesc ad;
+    int64_t stride[XR_MAX_FRAG_AX];
+    int64_t acc = ncol;
+    for (int m = nf - 1; m >= 0; m--) {
+        if (dims[m] <= 0) return xr_fail(c, XR_EINVAL, "empty fragment axis");
+        stride[m] = acc;
+        acc *= dims[m];
+    }
+    ad.s0 = stride[front0];
+    ad.s1 = front1 >= 0 ? stride[front1] : 0;
+    ad.d1 = front1 >= 0 ? dims[front1] : 1;
+    ad.nfront = dims[front0] * ad.d1;
+    ad.nrest = 0;
+    ad.xsize = ncol;
+    for (int m = 0; m < nf; m++) {
+        if (m == front0 || m == front1) continue;
+        ad.rest_dim[ad.nrest] = dims[m];
+        ad.rest_stride[ad.nrest] = stride[m];
+        ad.nrest++;
+        ad.xsize *= dims[m];
+    }
+    ad.ncol = ncol;
+    ad.nrows = nrows;
+    if (!d_rowmap && nrows != ad.nfront) return xr_fail(c, XR_ESHAPE, "without a row map nrows must be the front size");
+    const int64_t total = nrows * ad.xsize;
+    if (total <= 0) return XR_OK;
+    ProfScope prof(c, XR_K_SCATTER, 0.0, (mode ? 24.0 : 16.0) * total);
+    k_axes_pack<<<xr_grid(total), 256, 0, c->stream>>>(ad, d_rowmap, d_full, d_packed, mode, alpha);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+// Pair-block adjoint input from a reduced pair matrix G2 (rows and columns
+// over the pair index a * d2 + b, reordered through inv when given):
+//   out[(I1*nJ1+J1)*nB + I2*nJ2+J2] = G2[row(r1[I1], r2[I2]), row(c1[J1], c2[J2])]
+// (out[b*nA + a] when swapped), row(a, b) = (inv ? inv[a*d2 + b] : a*d2 + b) - base
+// (base: first row of the block of G2 that is passed).
+__global__ void k_pair_gather(const double* __restrict__ G2, int64_t ldg, int64_t d2, int64_t base,
+                              const int32_t* __restrict__ inv, const int64_t* __restrict__ r1,
+                              const int64_t* __restrict__ c1, const int64_t* __restrict__ r2,
+                              const int64_t* __restrict__ c2, int64_t nI1, int64_t nJ1, int64_t nI2,
+                              int64_t nJ2, int swapped, double* __restrict__ out) {
+    const int64_t nA = nI1 * nJ1, nB = nI2 * nJ2;
+    XR_GRID_STRIDE(i, nA * nB) {
+        int64_t a, b;
+        if (swapped) { b = i / nA; a = i - b * nA; }
+        else { a = i / nB; b = i - a * nB; }
+        const int64_t I1 = a / nJ1, J1 = a - I1 * nJ1, I2 = b / nJ2, J2 = b - I2 * nJ2;
+        int64_t row = r1[I1] * d2 + r2[I2], col = c1[J1] * d2 + c2[J2];
+        if (inv) { row = inv[row]; col = inv[col]; }
+        out[i] = G2[(row - base) * ldg + (col - base)];
+    }
+}
+
+extern "C" int xr_pair_gather(xr_ctx* c, const double* d_G2, int64_t ldg, int64_t d2, int64_t base,
+                              const int32_t* d_inv, const int64_t* d_r1, const int64_t* d_c1,
+                              const int64_t* d_r2, const int64_t* d_c2, int64_t nI1, int64_t nJ1,
+                              int64_t nI2, int64_t nJ2, int swapped, double* d_out) {
+    XR_ENTRY(c);
+    const int64_t n = nI1 * nJ1 * nI2 * nJ2;
+    if (n <= 0) return XR_OK;
+    ProfScope prof(c, XR_K_SCATTER, 0.0, 16.0 * n);
+    k_pair_gather<<<xr_grid(n), 256, 0, c->stream>>>(d_G2, ldg, d2, base, d_inv, d_r1, d_c1, d_r2, d_c2,
+                                                     nI1, nJ1, nI2, nJ2, swapped, d_out);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
 struct BraketDesc {
     int nf;
     int64_t spf;

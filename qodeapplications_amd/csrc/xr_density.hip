@@ -182,33 +182,44 @@ __global__ void k_index_table(int k, int dim, int64_t n, int32_t* tab) {
 }
 
 // out[((I*nJ + J)*nL + iL)*nR + iR] from the compressed GEMM result
-// Cc[(l*nI + I) * ldc + t*nJ + J].
-__device__ __forceinline__ void k_expand_elem(int64_t i, const double* __restrict__ Cc, int64_t ldc, int64_t nI, int64_t nJ,
-                         const int32_t* __restrict__ tabL, int64_t nL,
-                         const int32_t* __restrict__ tabR, int64_t nR, double* __restrict__ out) {
-    int64_t total = nI * nJ * nL * nR;
-    int64_t iR = i % nR;
-    int64_t r = i / nR;
-    int64_t iL = r % nL;
-    r /= nL;
-    int64_t J = r % nJ, I = r / nJ;
-    int32_t eL = tabL ? tabL[iL] : 1;
-    int32_t eR = tabR ? tabR[iR] : 1;
-    double v = 0.0;
-    if (eL != 0 && eR != 0) {
-        int64_t l = (eL > 0 ? eL : -eL) - 1;
-        int64_t t = (eR > 0 ? eR : -eR) - 1;
-        v = Cc[(l * nI + I) * ldc + t * nJ + J];
-        if ((eL < 0) != (eR < 0)) v = -v;
+// Cc[(l*nI + I) * ldc + t*nJ + J], through the signed index tables
+// tab[i] = sign * (sorted-tuple rank + 1), 0 for a repeated index.
+// blockIdx.x walks the (I, J) planes (J fastest, remapped so that each XCD
+// takes a contiguous run of planes) and blockIdx.y 1024-element chunks of a
+// plane: the blocks in flight on one XCD write contiguous chunks of
+// neighbouring planes and read the same Cc rows for consecutive J, so one
+// cache line of Cc serves 8 planes in that XCD's L2.  Plane offsets are 32-bit.
+__global__ __launch_bounds__(256) void k_expand_planes(
+    const double* __restrict__ Cc, int64_t ldc, int64_t nI, int64_t nJ,
+    const int32_t* __restrict__ tabL, uint32_t nL, const int32_t* __restrict__ tabR, uint32_t nR,
+    double* __restrict__ out) {
+    const uint32_t plane = nL * nR;
+    const int64_t n_planes = nI * nJ;
+    const int64_t lstride = nI * ldc;
+    for (int64_t b = blockIdx.x; b < n_planes; b += gridDim.x) {
+        const int64_t pl = (b - blockIdx.x) + xcd_remap(blockIdx.x, gridDim.x);
+        if (pl >= n_planes) continue;
+        const int64_t I = pl / nJ, J = pl - I * nJ;
+        const double* src = Cc + I * ldc + J;
+        double* dst = out + pl * (int64_t)plane;
+        for (uint32_t base = blockIdx.y * 1024u; base < plane; base += gridDim.y * 1024u) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t e = base + k * 256u + threadIdx.x;
+                if (e >= plane) break;
+                const uint32_t iL = e / nR, iR = e - iL * nR;
+                const int32_t eL = tabL ? tabL[iL] : 1;
+                const int32_t eR = tabR ? tabR[iR] : 1;
+                double v = 0.0;
+                if (eL != 0 && eR != 0) {
+                    const int64_t l = (eL > 0 ? eL : -eL) - 1, t = (eR > 0 ? eR : -eR) - 1;
+                    v = src[l * lstride + t * nJ];
+                    if ((eL < 0) != (eR < 0)) v = -v;
+                }
+                dst[e] = v;
+            }
+        }
     }
-    out[i] = v;
-}
-
-__global__ void k_expand(const double* __restrict__ Cc, int64_t ldc, int64_t nI, int64_t nJ,
-                         const int32_t* __restrict__ tabL, int64_t nL,
-                         const int32_t* __restrict__ tabR, int64_t nR, double* __restrict__ out) {
-    int64_t total = nI * nJ * nL * nR;
-    XR_GRID_STRIDE(i, total) k_expand_elem(i, Cc, ldc, nI, nJ, tabL, nL, tabR, nR, out);
 }
 
 // Veff[u, w] for sorted pairs u = (p<q), w = (a<b):
@@ -406,7 +417,13 @@ int xr_density_impl(xr_ctx* c, const char* ops, int n_orbs, int n_core, int n_el
     if (pl.R.k && (rc = xr_index_table(c, pl.R.k, dim, idxR))) return rc;
     int64_t total = nI * nJ * nIL * nIR;
     ProfScope prof(c, XR_K_EXPAND, 0.0, 8.0 * total);
-    k_expand<<<xr_grid(total), 256, 0, c->stream>>>(Cc, Ng, nI, nJ, idxL, nIL, idxR, nIR, out);
+    if (nIL * nIR > 0xffffffffLL - 65536LL * 1024) return xr_fail(c, XR_EUNSUPPORTED, "density plane too large");
+    const int64_t planes = nI * nJ;
+    const int64_t chunks = ceil_div(nIL * nIR, 1024);
+    dim3 grid((unsigned)(planes < (1LL << 30) ? planes : (1LL << 30)),
+              (unsigned)(chunks < 65535 ? chunks : 65535));
+    k_expand_planes<<<grid, 256, 0, c->stream>>>(Cc, Ng, nI, nJ, idxL, (uint32_t)nIL, idxR,
+                                                 (uint32_t)nIR, out);
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }

@@ -17,14 +17,34 @@ DENSE_MAX = 6144
 
 class KroneckerSumPreconditioner:
     """(H0 - theta)^-1 for H0 = H1_1 (x) 1 + 1 (x) H1_2 (+ ...), applied in the
-    product eigenbasis of the monomer blocks."""
+    product eigenbasis of the monomer blocks.  With the states' charges
+    given, each monomer block is diagonalised per charge (it conserves
+    charge), so every eigenvector has a definite charge (qcol)."""
 
-    def __init__(self, H1):
-        self.w, self.U = [], []
-        for h in H1:
-            w, U = torch.linalg.eigh(0.5 * (h + h.T))
+    def __init__(self, H1, charges=None):
+        self.w, self.U, self.qcol = [], [], []
+        for m, h in enumerate(H1):
+            hs = 0.5 * (h + h.T)
+            if charges is None:
+                w, U = torch.linalg.eigh(hs)
+                q = np.zeros(h.shape[0], dtype=np.int64)
+            else:
+                qm = np.asarray(charges[m], dtype=np.int64)
+                U = torch.zeros_like(hs)
+                w = torch.empty(h.shape[0], dtype=torch.float64, device=h.device)
+                q = np.empty(h.shape[0], dtype=np.int64)
+                col = 0
+                for c in np.unique(qm):
+                    idx = torch.as_tensor(np.nonzero(qm == c)[0], device=h.device)
+                    wc, Uc = torch.linalg.eigh(hs[idx][:, idx])
+                    nc = idx.numel()
+                    U[idx, col:col + nc] = Uc
+                    w[col:col + nc] = wc
+                    q[col:col + nc] = c
+                    col += nc
             self.w.append(w)
             self.U.append(U)
+            self.qcol.append(q)
         self.dims = [h.shape[0] for h in H1]
         lam = self.w[0]
         for w in self.w[1:]:
@@ -94,16 +114,22 @@ def _orthonormalize(T, V=None, passes=2):
 
 
 def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=None,
-             max_space=None, precond=None):
+             max_space=None, precond=None, start=None, keep=8):
     """Block Davidson.  matmul(X, Y) writes H X into Y for X of shape (n, b);
     precond(R, theta) approximates (H - theta)^-1 R (default: diagonal).
-    Converged when every residual norm is below tol * max(1, |theta|).
+    Converged when every residual norm is below tol * max(1, |theta|), or
+    when it has stalled at the rounding floor within 100x of that.  A full
+    subspace restarts on its `keep` lowest Ritz vectors.
     The subspace lives in preallocated device buffers; the Rayleigh-Ritz
     eigenproblem and the b x b factorisations are solved on the host."""
     b = block or max(2 * k, 4)
     max_space = max_space or max(10 * b, 48)
+    if start is not None:
+        max_space = max(max_space, start.shape[1] + b)
     dt = torch.float64
-    if precond is not None:
+    if start is not None:
+        V0 = start
+    elif precond is not None:
         V0 = precond.guess(b)
     else:
         idx = torch.argsort(diag)[:b]
@@ -118,6 +144,7 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
     matmul(V0, AX0)
     AVbuf[:, :m] = AX0
     S = mm(V0, AX0, transA=1).cpu().numpy()
+    hist = []
     for it in range(max_iter):
         V, AV = Vbuf[:, :m], AVbuf[:, :m]
         S = 0.5 * (S + S.T)
@@ -129,16 +156,25 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         AX = mm(AV, yb)
         R = AX - X * theta
         rn = torch.linalg.norm(R[:, :k], dim=0).cpu().numpy()
-        if np.all(rn <= tol * np.maximum(np.abs(w[:k]), 1.0)):
+        target = tol * np.maximum(np.abs(w[:k]), 1.0)
+        if np.all(rn <= target):
+            return theta[:k].clone(), X[:, :k].contiguous()
+        # rounding floor: the residual is within 100x of the target and has
+        # stopped falling over the last 10 iterations
+        hist.append(float(np.max(rn / target)))
+        if len(hist) > 20 and hist[-1] < 100.0 and min(hist[-10:]) > 0.5 * min(hist[:-10]):
             return theta[:k].clone(), X[:, :k].contiguous()
         T = precond(R, theta) if precond is not None else R / torch.where(
             (theta[None, :] - diag[:, None]).abs() < 1e-8, torch.full_like(R, 1e-8),
             theta[None, :] - diag[:, None])
-        if m + nb > max_space:              # thick restart on the current Ritz vectors
-            Vbuf[:, :nb] = X
-            AVbuf[:, :nb] = AX
-            S = np.diag(w[:nb])
-            m = nb
+        if m + nb > max_space:              # thick restart on the lowest Ritz vectors
+            nk = min(m, max(2 * nb, keep))
+            yk = torch.as_tensor(np.ascontiguousarray(y[:, :nk]), device=device)
+            Xk, AXk = mm(V, yk), mm(AV, yk)
+            Vbuf[:, :nk] = Xk
+            AVbuf[:, :nk] = AXk
+            S = np.diag(w[:nk])
+            m = nk
             V = Vbuf[:, :m]
         Q = _orthonormalize(T.contiguous(), V)
         q = Q.shape[1]

@@ -1,12 +1,14 @@
-"""Analytic gradient of the lowest XR eigenvalue of a dimer.
+"""Analytic gradient of the lowest XR eigenvalue (monomer + pair terms, any
+number of fragments).
 
 E = lowest eigenvalue of the XR matrix H(z, integrals) (Hellmann-Feynman:
 dE = v^T dH v).  The backward pass runs every forward contraction in
 reverse on the GPU:
 
-* dE/dH: the reduced matrices D_m = V V^T, V^T V of the eigenvector V
-  (d1 x d2) for the monomer blocks, and v v^T gathered block by block
-  (xr_pair_outer) for the pair blocks;
+* dE/dH: the reduced matrices of the eigenvector v over the tensor-product
+  basis, D_m = sum over the other fragments of v v^T on fragment m (monomer
+  blocks) and the charge-diagonal blocks of the same sum on each fragment
+  pair (pair blocks, gathered per charge combination by xr_pair_gather);
 * every block term is a bilinear form s A W B^T (general-XRCC/
   H_contractions.c), whose adjoint is three FP64 GEMMs giving dE/dA, dE/dB
   and dE/dW (W an integral block);
@@ -30,7 +32,7 @@ import torch
 
 from . import blocks as B
 from ._lib import context, ptr
-from .eigen import lowest_eigenpairs
+from .eigen import DENSE_MAX
 from .gemm import mm as _mm
 from .xr import precision
 
@@ -84,20 +86,32 @@ def _flat(R):
     return R.view(R.shape[0] * R.shape[1], -1)
 
 
-class DimerAdjoint:
-    """Forward + backward of E(z, integrals) for a two-fragment XRSystem."""
+GRAD_TOL = 1e-11
+
+
+class XRAdjoint:
+    """Forward + backward of E(z, integrals) for an XRSystem of any number of
+    fragments (monomer and pair terms).  The lowest root comes from the
+    matrix-free operator (dense eigendecomposition for small spaces)."""
 
     def __init__(self, system):
-        if len(system.frags) != 2:
-            raise ValueError("the analytic gradient is implemented for dimers")
+        if len(system.frags) < 2:
+            raise ValueError("the XR gradient needs at least two fragments")
         self.X = system
 
     def forward(self):
         X = self.X
         X.build_densities()
         H1, H2, _ = X.blocks()
-        M = X.xr_matrix(H=(H1, H2, None))
-        w, v = lowest_eigenpairs(M, k=1, H1=H1)
+        self.op = op = X.operator(H=(H1, H2, None))
+        if op.n <= DENSE_MAX:
+            M = op.dense()
+            w, v = torch.linalg.eigh(0.5 * (M + M.T))
+            w, v = w[:1], v[:, :1]
+        else:
+            # the gradient is first order in the eigenvector error: converge
+            # the residual well below the energy tolerance
+            w, v = op.lowest(k=1, tol=GRAD_TOL)
         self.E = float(w[0])
         self.v = v[:, 0].contiguous()
         return self.E
@@ -110,13 +124,11 @@ class DimerAdjoint:
             return self._backward(lagrangian)
 
     def _backward(self, lagrangian):
-        X, E, v = self.X, self.E, self.v
-        F1, F2 = X.frags
+        X, E, v, xop = self.X, self.E, self.v, self.op
         dev = X.device
-        d1, d2 = F1.d, F2.d
-        Vm = v.view(d1, d2)
-        D = [mm(Vm, Vm, transB=1), mm(Vm, Vm, transA=1)]
-        rbar = [_Acc(dev), _Acc(dev)]
+        nf = len(X.frags)
+        D = [xop.monomer_density(v, m) for m in range(nf)]
+        rbar = [_Acc(dev) for _ in range(nf)]
         ibar = _Acc(dev)
         zbar = [{c: torch.zeros_like(F.blocks[c].coeffs) for c in F.charges} for F in X.frags]
         ctx = context(dev.index)
@@ -150,8 +162,11 @@ class DimerAdjoint:
             h_bar(parts, hb)
 
         # ---- pair blocks
-        for k1, k2 in X.pair_charge_combos(0, 1):
-            self._pair_adjoint(k1, k2, v, rbar, ib, h_bar)
+        for (m1, m2) in xop.pairs:
+            Gb = xop.pair_density(v, m1, m2)
+            for k1, k2 in X.pair_charge_combos(m1, m2):
+                self._pair_adjoint(m1, m2, k1, k2, Gb, rbar, ib, h_bar)
+            del Gb
 
         # ---- densities -> coefficients
         for m, F in enumerate(X.frags):
@@ -169,10 +184,9 @@ class DimerAdjoint:
         self.D = D
         return zbar, ibar.d
 
-    def _pair_adjoint(self, k1, k2, v, rbar, ib, h_bar):
+    def _pair_adjoint(self, m1, m2, k1, k2, Gb, rbar, ib, h_bar):
         X = self.X
-        F1, F2 = X.frags
-        m1, m2 = 0, 1
+        F1, F2 = X.frags[m1], X.frags[m2]
         (i1c, j1c), (i2c, j2c) = k1, k2
         sp = X.precision == "split_f32"
         nI1, nJ1, nI2, nJ2 = F1.n_states[i1c], F1.n_states[j1c], F2.n_states[i2c], F2.n_states[j2c]
@@ -182,15 +196,12 @@ class DimerAdjoint:
         ctx = context(X.device.index)
 
         def G(swapped):
-            out = torch.empty(nA * nB, dtype=torch.float64, device=X.device)
-            ctx.call("xr_pair_outer", ptr(v), F2.d, ptr(F1.pos[i1c]), ptr(F1.pos[j1c]), ptr(F2.pos[i2c]),
-                     ptr(F2.pos[j2c]), nI1, nJ1, nI2, nJ2, int(swapped), ptr(out))
-            return out.view(nB, nA) if swapped else out.view(nA, nB)
+            return self.op.pair_gather(Gb, m1, m2, (F1.pos, F2.pos), k1, k2, swapped)
 
         R1 = lambda op: F1.rho[op][k1]
         R2 = lambda op: F2.rho[op][k2]
-        R1b = lambda op: _flat(rbar[0].get((op, k1), F1.rho[op][k1].shape))
-        R2b = lambda op: _flat(rbar[1].get((op, k2), F2.rho[op][k2].shape))
+        R1b = lambda op: _flat(rbar[m1].get((op, k1), F1.rho[op][k1].shape))
+        R2b = lambda op: _flat(rbar[m2].get((op, k2), F2.rho[op][k2].shape))
         n1, n2 = F1.dim, F2.dim
 
         def vperm(key, perm, shape2):
@@ -269,7 +280,7 @@ def _inv(perm):
 
 
 class XREnergy(torch.autograd.Function):
-    """E = lowest XR eigenvalue of a dimer as a function of the coefficient
+    """E = lowest XR eigenvalue as a function of the coefficient
     tensors and the integral blocks; backward is the hand-written adjoint.
 
     apply(system, lagrangian, z_keys, int_keys, *tensors): tensors are the
@@ -284,7 +295,7 @@ class XREnergy(torch.autograd.Function):
             F.blocks[c] = F.blocks[c].with_coeffs(t.detach())
         for key, t in zip(int_keys, tensors[nz:]):
             system.ints.set_block(key, t)
-        adj = DimerAdjoint(system)
+        adj = XRAdjoint(system)
         E = adj.forward()
         ctx.adj, ctx.lag, ctx.z_keys, ctx.int_keys = adj, lagrangian, z_keys, int_keys
         return torch.tensor(E, dtype=torch.float64, device=system.device)
@@ -301,8 +312,11 @@ class XREnergy(torch.autograd.Function):
 
 
 def energy_and_gradient(system, lagrangian=True):
-    """(E, v, zbar, ibar) for a dimer XRSystem without autograd plumbing."""
-    adj = DimerAdjoint(system)
+    """(E, v, zbar, ibar) for an XRSystem without autograd plumbing."""
+    adj = XRAdjoint(system)
     E = adj.forward()
     zbar, ibar = adj.backward(lagrangian=lagrangian)
     return E, adj.v, zbar, ibar
+
+
+DimerAdjoint = XRAdjoint

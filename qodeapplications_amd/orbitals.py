@@ -95,21 +95,31 @@ class OrbitalRotation:
         return out
 
 
-def dimer_integral_keys():
-    keys = [("T", m, n) for m in range(2) for n in range(2)]
-    keys += [("U", a, m, n) for a in range(2) for m in range(2) for n in range(2)]
-    keys += [("V",) + k for k in ((0, 0, 0, 0), (1, 1, 1, 1), (0, 0, 1, 1), (1, 1, 0, 0), (0, 0, 0, 1),
-                                  (0, 1, 1, 1), (1, 1, 1, 0), (1, 0, 0, 0), (0, 1, 0, 1))]
+def integral_keys(nf):
+    """Every integral block the monomer and pair terms of nf fragments read
+    (IntegralCache keys)."""
+    keys = [("T", m, n) for m in range(nf) for n in range(nf)]
+    keys += [("U", a, m, n) for a in range(nf) for m in range(nf) for n in range(nf)]
+    v = [(m, m, m, m) for m in range(nf)]
+    for m in range(nf):
+        for n in range(m + 1, nf):
+            v += [(m, m, n, n), (n, n, m, m), (m, m, m, n), (m, n, n, n), (n, n, n, m), (n, m, m, m),
+                  (m, n, m, n)]
+    keys += [("V",) + k for k in v]
     return keys
+
+
+def dimer_integral_keys():
+    return integral_keys(2)
 
 
 def energy_and_gradients(system, rotation, lagrangian=True):
     """E, dE/dz (per fragment and charge) and dE/dkappa (per fragment,
-    antisymmetric n_m x n_m) at kappa = 0, for a dimer XRSystem whose
-    integrals are `rotation`'s."""
+    antisymmetric n_m x n_m) at kappa = 0, for an XRSystem whose integrals
+    are `rotation`'s."""
     from .grad import XREnergy
     kappas = rotation.zero_kappas()
-    keys = dimer_integral_keys()
+    keys = integral_keys(len(system.frags))
     blocks = rotation.blocks(kappas, keys)
     z_keys = [(m, c) for m, F in enumerate(system.frags) for c in F.charges]
     zs = [system.frags[m].blocks[c].coeffs.clone().requires_grad_(True) for m, c in z_keys]

@@ -210,6 +210,20 @@ class XRSystem:
                         H3[a][b][c] = self.trimer_matrix(a, b, c)
         return H1, H2, H3
 
+    def state_charges(self):
+        """Per fragment, the charge of each state in state order."""
+        return [np.array([chg for chg, _ in F.state_indices], dtype=np.int64) for F in self.frags]
+
+    def operator(self, H=None):
+        """Matrix-free XR operator (monomer and pair terms) over the blocks H."""
+        from .xr_operator import XROperator
+        H1, H2, H3 = H if H is not None else self.blocks()
+        if H3 is not None:
+            raise ValueError("the matrix-free operator carries monomer and pair terms only")
+        nf = len(self.frags)
+        pairs = {(m, n): H2[m][n] for m in range(nf) for n in range(m + 1, nf)}
+        return XROperator(H1, pairs, self.state_charges(), symmetrize=self.precision != "f64")
+
     def xr_matrix(self, with_trimers=False, spectator_sign=False, H=None):
         """Tensor-product XR matrix (all fragments need the same number of
         states).  spectator_sign adds the sign the reference assembly omits

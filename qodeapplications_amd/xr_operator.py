@@ -1,0 +1,416 @@
+"""Matrix-free XR Hamiltonian on the tensor-product basis.
+
+H = sum_m H1_m (acting on fragment axis m) + sum_{m<n} H2_mn (acting on the
+axes m, n), the sums general-XRCC/hamiltonian.py:21-63 places element by
+element into a dense matrix of side prod_m d_m.  Here a block of vectors is
+a row-major tensor X[s_0]...[s_{N-1}][j] and H is applied without forming it:
+
+* H1_m: one batched FP64 GEMM along axis m;
+* H2_mn: xr_axes_pack gathers the (m, n) axes to the rows with the pair index
+  sorted by the pair's total charge.  H conserves charge, so H2_mn in that
+  order is block diagonal (one block per charge sum) and the product is one
+  GEMM per block; xr_axes_pack adds the result back.  The blocks hold
+  sum_s n_s^2 of the (sum_s n_s)^2 entries (19 / 81 for three equal charge
+  blocks).
+
+The same packing gives the reduced matrices the gradient needs:
+D_m = sum_rest v v^T on fragment m, and the pair blocks G_mn of
+sum_rest v v^T on (m, n), block diagonal in the same order.
+
+The spectator-fragment sign of one-electron transfers across fragments
+between m and n is not applied, as in the reference loops.
+"""
+import numpy as np
+import torch
+
+from ._lib import context, ptr
+from .eigen import KroneckerSumPreconditioner, davidson
+from .gemm import gemm_f64, mm
+
+
+DENSE_SECTOR = 2048
+
+
+def _i32(a, device):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=device)
+
+
+class XROperator:
+    """H1: list of monomer matrices (d_m x d_m); H2: {(m, n): dense pair
+    matrix (d_m d_n x d_m d_n)} for m < n; charges: per fragment, the charge
+    of each state in state order."""
+
+    def __init__(self, H1, H2, charges, symmetrize=False):
+        self.H1 = [h.contiguous() for h in H1]
+        self.dims = [int(h.shape[0]) for h in H1]
+        self.nf = len(self.dims)
+        self.n = int(np.prod(self.dims))
+        self.device = H1[0].device
+        self.ctx = context(self.device.index)
+        self.charges = [np.asarray(c, dtype=np.int64) for c in charges]
+        self.pairs = {}
+        self._stacks, self._pair_diag = {}, {}
+        for (m, n), H in H2.items():
+            if H is None:
+                continue
+            self.pairs[(m, n)] = self._pair_blocks(m, n, H, symmetrize)
+
+    # -- setup ----------------------------------------------------------------
+    def _order(self, m, n):
+        q = (self.charges[m][:, None] + self.charges[n][None, :]).ravel()
+        order = np.argsort(q, kind="stable")
+        inv = np.empty_like(order)
+        inv[order] = np.arange(order.size)
+        segs, segq = [], []
+        qs = q[order]
+        start = 0
+        for i in range(1, qs.size + 1):
+            if i == qs.size or qs[i] != qs[start]:
+                segs.append((start, i - start))
+                segq.append(int(qs[start]))
+                start = i
+        return order, inv, segs, segq
+
+    def _pair_blocks(self, m, n, H, symmetrize):
+        D = self.dims[m] * self.dims[n]
+        if tuple(H.shape) != (D, D):
+            raise ValueError(f"pair matrix ({m}, {n}) must be {D} x {D}")
+        order, inv, segs, segq = self._order(m, n)
+        H = H.contiguous()
+        blocks = []
+        for start, cnt in segs:
+            rows = _i32(order[start:start + cnt], self.device)
+            # tmp[r, x] = H[x, rows[r]]; blk[r, x] = tmp[x, rows[r]] = H[rows[r], rows[x]]
+            tmp = torch.empty(cnt, D, dtype=torch.float64, device=self.device)
+            self._pack([D, D], 1, 1, -1, rows, cnt, H, tmp, 0)
+            blk = torch.empty(cnt, cnt, dtype=torch.float64, device=self.device)
+            self._pack([cnt, D], 1, 1, -1, rows, cnt, tmp, blk, 0)
+            del tmp
+            if symmetrize:
+                blk = torch.add(blk, blk.T).mul_(0.5)
+            blocks.append(blk)
+        return {"order": _i32(order, self.device), "inv": _i32(inv, self.device), "segs": segs,
+                "segq": segq, "blocks": blocks, "inv_np": inv}
+
+    def extract(self, m, n, rows, cols):
+        """H2_mn[rows][:, cols] for pair indices a * d_n + b of one charge sum."""
+        P = self.pairs[(m, n)]
+        r, c = P["inv_np"][rows], P["inv_np"][cols]
+        for i, (start, cnt) in enumerate(P["segs"]):
+            if start <= r.min() and r.max() < start + cnt:
+                break
+        else:
+            raise ValueError("rows span several charge sums")
+        if c.min() < start or c.max() >= start + cnt:
+            raise ValueError("rows and columns have different charge sums")
+        blk = P["blocks"][i]
+        lc = _i32(c - start, self.device)
+        lr = _i32(r - start, self.device)
+        tmp = torch.empty(c.size, cnt, dtype=torch.float64, device=self.device)
+        self._pack([cnt, cnt], 1, 1, -1, lc, c.size, blk, tmp, 0)       # tmp[x, y] = blk[y, lc[x]]
+        out = torch.empty(r.size, c.size, dtype=torch.float64, device=self.device)
+        self._pack([c.size, cnt], 1, 1, -1, lr, r.size, tmp, out, 0)    # out[x, y] = blk[lr[x], lc[y]]
+        return out
+
+    def _pack(self, dims, ncol, f0, f1, rowmap, nrows, full, packed, mode, alpha=1.0):
+        import ctypes
+        d = (ctypes.c_int64 * len(dims))(*[int(x) for x in dims])
+        self.ctx.call("xr_axes_pack", len(dims), d, int(ncol), int(f0), int(f1),
+                      None if rowmap is None else ptr(rowmap), int(nrows), ptr(full), ptr(packed),
+                      int(mode), float(alpha))
+
+    # -- products -------------------------------------------------------------
+    def matvec(self, X, Y):
+        """Y = H X for X, Y of shape (n, b), contiguous."""
+        b = X.shape[1]
+        dims = self.dims
+        for m, h in enumerate(self.H1):
+            pre = int(np.prod(dims[:m]))
+            post = int(np.prod(dims[m + 1:])) * b
+            X3 = X.view(pre, dims[m], post)
+            Y3 = Y.view(pre, dims[m], post)
+            gemm_f64(h.unsqueeze(0), X3, Y3, beta=0.0 if m == 0 else 1.0)
+        for (m, n), P in self.pairs.items():
+            D = dims[m] * dims[n]
+            R = self.n // D * b
+            Xp = torch.empty(D, R, dtype=torch.float64, device=self.device)
+            self._pack(dims, b, m, n, P["order"], D, X, Xp, 0)
+            Yp = torch.empty_like(Xp)
+            for (start, cnt), blk in zip(P["segs"], P["blocks"]):
+                mm(blk, Xp[start:start + cnt], out=Yp[start:start + cnt])
+            self._pack(dims, b, m, n, P["order"], D, Y, Yp, 1, 1.0)
+        return Y
+
+    def dense(self):
+        """The full matrix (small systems only): H applied to the identity."""
+        E = torch.eye(self.n, dtype=torch.float64, device=self.device)
+        out = torch.empty_like(E)
+        self.matvec(E, out)
+        return out
+
+    # -- eigenpairs -----------------------------------------------------------
+    def sector_space(self, Q):
+        """The states of total charge Q in the compact sector layout."""
+        return SectorSpace(self, Q)
+
+    def total_charges(self):
+        qs = [sorted(set(int(x) for x in c)) for c in self.charges]
+        tot = {0}
+        for q in qs:
+            tot = {t + x for t in tot for x in q}
+        return sorted(tot)
+
+    def lowest(self, k=1, tol=1e-9, max_iter=400, block=None, charges=None, screen_tol=1e-4):
+        """(values[k], vectors[n, k]) of the lowest roots over all total
+        charges (or the given ones), in the product layout.
+
+        H conserves the total charge, so each charge sector is solved on its
+        own compact layout (block size k unless given).  Every sector is
+        first converged loosely (residual screen_tol); only the sectors whose
+        Ritz values can still be among the lowest k (within the residual
+        norms) are then refined to tol from their screened vectors."""
+        cand = []
+        for Q in (charges if charges is not None else self.total_charges()):
+            sp = self.sector_space(Q)
+            kk = min(k, sp.size)
+            if sp.size <= DENSE_SECTOR:
+                M = torch.empty(sp.size, sp.size, dtype=torch.float64, device=self.device)
+                sp.matvec(torch.eye(sp.size, dtype=torch.float64, device=self.device), M)
+                w, v = torch.linalg.eigh(0.5 * (M + M.T))
+                cand.append([sp, w[:kk], v[:, :kk].contiguous(), 0.0, True])
+            else:
+                b = block or kk
+                w, v = davidson(sp.matvec, None, sp.size, kk, max(tol, screen_tol), max_iter, b,
+                                self.device, precond=sp.precondition, start=sp.guess(b))
+                cand.append([sp, w, v, max(tol, screen_tol) * max(1.0, abs(float(w[-1]))) * 10, False])
+        # k-th lowest upper bound over all sectors; a sector whose lowest Ritz
+        # value minus its residual margin lies above it cannot contribute
+        ups = sorted(float(x) for c in cand for x in c[1])
+        bound = ups[min(k, len(ups)) - 1]
+        found = []
+        for c in cand:
+            sp, w, v, margin, exact = c
+            if float(w[0]) - margin > bound:
+                continue
+            if not exact and tol < screen_tol:
+                b = block or v.shape[1]
+                start = v if v.shape[1] >= b else torch.cat([v, sp.guess(b)[:, :b - v.shape[1]]], 1)
+                w, v = davidson(sp.matvec, None, sp.size, v.shape[1], tol, max_iter, b, self.device,
+                                precond=sp.precondition, start=start.contiguous())
+            for i in range(w.shape[0]):
+                found.append((float(w[i]), sp, v[:, i]))
+        found.sort(key=lambda t: t[0])
+        found = found[:k]
+        vals = torch.tensor([f[0] for f in found], dtype=torch.float64, device=self.device)
+        vecs = torch.zeros(self.n, len(found), dtype=torch.float64, device=self.device)
+        for i, (_, sp, x) in enumerate(found):
+            sp.to_product(x.contiguous(), vecs[:, i])
+        return vals, vecs
+
+    # -- reduced matrices for the gradient ------------------------------------
+    def monomer_density(self, v, m):
+        """D_m[a, a'] = sum over the other fragments of v[.., a, ..] v[.., a', ..]."""
+        d = self.dims[m]
+        P = torch.empty(d, self.n // d, dtype=torch.float64, device=self.device)
+        self._pack(self.dims, 1, m, -1, None, d, v, P, 0)
+        return mm(P, P, transB=1)
+
+    def pair_density(self, v, m, n):
+        """Diagonal blocks of G_mn = sum over the other fragments of v v^T on
+        the (m, n) pair index, in the charge-sorted order of the pair blocks."""
+        P = self.pairs[(m, n)]
+        D = self.dims[m] * self.dims[n]
+        Vp = torch.empty(D, self.n // D, dtype=torch.float64, device=self.device)
+        self._pack(self.dims, 1, m, n, P["order"], D, v, Vp, 0)
+        blocks = []
+        for start, cnt in P["segs"]:
+            S = Vp[start:start + cnt]
+            blocks.append(mm(S, S, transB=1))
+        return blocks
+
+    def pair_gather(self, G_blocks, m, n, pos, k1, k2, swapped):
+        """dE/dH of the pair block with charges k1 = (bra, ket) on m, k2 on n,
+        in the layout of the block's bilinear terms ((I1 J1) x (I2 J2), or
+        transposed when swapped)."""
+        P = self.pairs[(m, n)]
+        i = P["segq"].index(k1[0] + k2[0])
+        start, cnt = P["segs"][i]
+        (r1, c1), (r2, c2) = (pos[0][k1[0]], pos[0][k1[1]]), (pos[1][k2[0]], pos[1][k2[1]])
+        nI1, nJ1, nI2, nJ2 = r1.numel(), c1.numel(), r2.numel(), c2.numel()
+        out = torch.empty(nI1 * nJ1 * nI2 * nJ2, dtype=torch.float64, device=self.device)
+        self.ctx.call("xr_pair_gather", ptr(G_blocks[i]), cnt, self.dims[n], start, ptr(P["inv"]),
+                      ptr(r1), ptr(c1), ptr(r2), ptr(c2), nI1, nJ1, nI2, nJ2, int(swapped), ptr(out))
+        return out.view(nI2 * nJ2, nI1 * nJ1) if swapped else out.view(nI1 * nJ1, nI2 * nJ2)
+
+
+class SectorSpace:
+    """States of one total charge Q in the compact sector layout: the
+    fragment-charge tuples c = (c_0, ..., c_{N-1}) with sum Q, each a dense
+    row-major tensor of shape (n_{0,c_0}, ..., n_{N-1,c_{N-1}}), concatenated.
+    H conserves Q, so the Davidson iteration of one sector never leaves it;
+    monomer terms act inside a tuple, pair terms move the (m, n) charges of a
+    tuple with their sum fixed."""
+
+    def __init__(self, op, Q):
+        self.op = op
+        self.Q = Q
+        dev = op.device
+        nf = op.nf
+        self.blocks_of = []          # per fragment: {charge: (start, count)} in state order
+        for m in range(nf):
+            q = op.charges[m]
+            d = {}
+            for c in sorted(set(int(x) for x in q)):
+                idx = np.nonzero(q == c)[0]
+                if idx.size and not np.all(np.diff(idx) == 1):
+                    raise ValueError("the states of one charge must be contiguous")
+                d[c] = (int(idx[0]), int(idx.size))
+            self.blocks_of.append(d)
+        charges = [sorted(d) for d in self.blocks_of]
+        tuples = [()]
+        for m in range(nf):
+            tuples = [t + (c,) for t in tuples for c in charges[m]]
+        self.sectors = [t for t in tuples if sum(t) == Q]
+        self.index = {t: i for i, t in enumerate(self.sectors)}
+        self.shapes = [[self.blocks_of[m][t[m]][1] for m in range(nf)] for t in self.sectors]
+        self.sizes = [int(np.prod(sh)) for sh in self.shapes]
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        self.size = int(self.offsets[-1])
+        # monomer blocks per (fragment, charge) and their eigenbases
+        self.h1 = [{c: op.H1[m][st:st + n, st:st + n].contiguous() for c, (st, n) in self.blocks_of[m].items()}
+                   for m in range(nf)]
+        self.eig = []
+        for m in range(nf):
+            d = {}
+            for c, h in self.h1[m].items():
+                w, U = torch.linalg.eigh(0.5 * (h + h.T))
+                d[c] = (w, U.contiguous())
+            self.eig.append(d)
+        # pair transitions: stacked target blocks per (pair, source charges),
+        # shared by the sectors of every total charge
+        self.stack = op._stacks
+        for (m, n) in op.pairs:
+            for cm in charges[m]:
+                for cn in charges[n]:
+                    if (m, n, cm, cn) in self.stack:
+                        continue
+                    rows, tg = [], []
+                    for tm in charges[m]:
+                        tn = cm + cn - tm
+                        if tn in self.blocks_of[n]:
+                            tg.append((tm, tn, int(sum(r.size for r in rows))))
+                            rows.append(self._pair_index(m, n, tm, tn))
+                    cols = self._pair_index(m, n, cm, cn)
+                    R = np.concatenate(rows)
+                    self.stack[(m, n, cm, cn)] = (op.extract(m, n, R, cols),
+                                                  [(tm, tn, off) for tm, tn, off in tg])
+        # diagonal of each charge-conserving pair block in the monomer
+        # eigenbasis: diag((Um x Un)^T H2[(cm,cn),(cm,cn)] (Um x Un))
+        self.pair_diag = op._pair_diag
+        for (m, n, cm, cn), (Hs, targets) in self.stack.items():
+            if (m, n, cm, cn) in self.pair_diag:
+                continue
+            off = [o for tm, tn, o in targets if tm == cm and tn == cn][0]
+            Dm, Dn = self.blocks_of[m][cm][1], self.blocks_of[n][cn][1]
+            Hself = Hs[off:off + Dm * Dn]
+            K = torch.kron(self.eig[m][cm][1], self.eig[n][cn][1]).contiguous()
+            HK = mm(Hself, K)
+            self.pair_diag[(m, n, cm, cn)] = (K * HK).sum(0).reshape(Dm, Dn)
+        lam = []
+        for i, t in enumerate(self.sectors):
+            l = torch.zeros(self.shapes[i], dtype=torch.float64, device=dev)
+            for m in range(nf):
+                sh = [1] * nf
+                sh[m] = self.shapes[i][m]
+                l = l + self.eig[m][t[m]][0].reshape(sh)
+            for (m, n) in op.pairs:
+                sh = [1] * nf
+                sh[m], sh[n] = self.shapes[i][m], self.shapes[i][n]
+                l = l + self.pair_diag[(m, n, t[m], t[n])].reshape(sh)
+            lam.append(l.reshape(-1))
+        self.lam = torch.cat(lam) if lam else torch.zeros(0, dtype=torch.float64, device=dev)
+
+    def _pair_index(self, m, n, cm, cn):
+        """Pair indices a * d_n + b of the states with charges (cm, cn)."""
+        sm, nm = self.blocks_of[m][cm]
+        sn, nn = self.blocks_of[n][cn]
+        return ((np.arange(sm, sm + nm)[:, None]) * self.op.dims[n] + np.arange(sn, sn + nn)[None, :]).ravel()
+
+    def _view(self, X, i):
+        return X[int(self.offsets[i]):int(self.offsets[i + 1])]
+
+    def matvec(self, X, Y):
+        """Y = H X for X, Y (size, b) in the sector layout."""
+        op = self.op
+        b = X.shape[1]
+        nf = op.nf
+        for i, t in enumerate(self.sectors):
+            sh = self.shapes[i]
+            Xi, Yi = self._view(X, i), self._view(Y, i)
+            for m in range(nf):
+                pre = int(np.prod(sh[:m]))
+                post = int(np.prod(sh[m + 1:])) * b
+                gemm_f64(self.h1[m][t[m]].unsqueeze(0), Xi.view(pre, sh[m], post),
+                         Yi.view(pre, sh[m], post), beta=0.0 if m == 0 else 1.0)
+        for i, t in enumerate(self.sectors):
+            sh = self.shapes[i]
+            Xi = self._view(X, i)
+            for (m, n) in op.pairs:
+                Hs, targets = self.stack[(m, n, t[m], t[n])]
+                D = sh[m] * sh[n]
+                P = torch.empty(D, self.sizes[i] // D * b, dtype=torch.float64, device=op.device)
+                op._pack(sh, b, m, n, None, D, Xi, P, 0)
+                T = mm(Hs, P)
+                for tm, tn, off in targets:
+                    tt = list(t)
+                    tt[m], tt[n] = tm, tn
+                    j = self.index[tuple(tt)]
+                    shj = self.shapes[j]
+                    Dj = shj[m] * shj[n]
+                    op._pack(shj, b, m, n, None, Dj, self._view(Y, j), T[off:off + Dj], 1, 1.0)
+        return Y
+
+    def _axes(self, X, inverse):
+        """Applies U_m^T (or U_m when inverse) along every fragment axis of
+        every tuple: the product eigenbasis of the monomer blocks."""
+        b = X.shape[1]
+        out = X.clone()
+        tmp = torch.empty_like(X)
+        for i, t in enumerate(self.sectors):
+            sh = self.shapes[i]
+            for m in range(self.op.nf):
+                pre = int(np.prod(sh[:m]))
+                post = int(np.prod(sh[m + 1:])) * b
+                U = self.eig[m][t[m]][1]
+                src, dst = self._view(out, i), self._view(tmp, i)
+                gemm_f64(U.unsqueeze(0), src.view(pre, sh[m], post), dst.view(pre, sh[m], post),
+                         transA=0 if inverse else 1)
+                src.copy_(dst)
+        return out
+
+    def precondition(self, R, theta):
+        """(Hd - theta)^-1 R, Hd the diagonal of H in the product eigenbasis of
+        the monomer blocks (monomer part exact, pair terms by their diagonal)."""
+        Y = self._axes(R.contiguous(), False)
+        den = self.lam[:, None] - theta[None, :]
+        den = torch.where(den.abs() < 1e-8, torch.full_like(den, 1e-8), den)
+        return self._axes((Y / den).contiguous(), True)
+
+    def guess(self, b):
+        """The b product eigenstates with the lowest diagonal energy."""
+        b = min(b, self.size)
+        idx = torch.topk(self.lam, b, largest=False).indices
+        E = torch.zeros(self.size, b, dtype=torch.float64, device=self.op.device)
+        E[idx, torch.arange(b, device=E.device)] = 1.0
+        return self._axes(E, True)
+
+    def to_product(self, x, out):
+        """Writes the sector vector x into the product-layout vector out."""
+        dims = self.op.dims
+        full = out.view(dims)
+        for i, t in enumerate(self.sectors):
+            view = full
+            for m in range(self.op.nf):
+                st, n = self.blocks_of[m][t[m]]
+                view = view.narrow(m, st, n)
+            view.copy_(self._view(x, i).view(self.shapes[i]))
