@@ -32,10 +32,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c3", choices=["c0", "c1", "c3"],
+    ap.add_argument("--config", default="c3", choices=["c0", "c1", "c2", "c3"],
                     help="c3: XR iteration on the 2x18-orbital dimer (the headline line); "
                          "c0: the same on the 2x9-orbital dimer; c1: density build of one "
-                         "13-orbital fragment, 2-6 electrons, 32 states (separate metric)")
+                         "13-orbital fragment, 2-6 electrons, 32 states; c2: all blocks of six "
+                         "uneven fragments, eager vs autotuned graph replay (separate metrics)")
     ap.add_argument("--precision", default="both", choices=["f64", "split_f32", "both"],
                     help="two-electron contraction precision; 'both' times FP64 (the value) "
                          "and split-FP32 (reported beside it with its |dE|)")
@@ -169,6 +170,8 @@ def main():
         return
     if args.config == "c1":
         return bench_c1(args)
+    if args.config == "c2":
+        return bench_c2(args)
     from qodeapplications_amd import _lib
     from qodeapplications_amd.xr import XRSystem
     from qodeapplications_amd.eigen import lowest_eigenpairs
@@ -345,6 +348,69 @@ def bench_c1(args):
                      "kernel": "k_dgemm (FP64 MFMA GEMM)", "launches_per_step": n / args.steps,
                      "avg_launch_ms": gms / max(n, 1), "traffic": None},
         "kernels": stats,
+    }
+    print(json.dumps(out))
+
+
+def bench_c2(args):
+    """All densities, monomer and pair blocks (15 pairs) of six fragments of
+    9, 9, 13, 13, 18, 18 spatial orbitals with 3/4/5 electrons and 8-64
+    states per charge block.  One step = one full block build; timed eagerly
+    and as replays of the autotuned, graph-captured build."""
+    import torch
+    from qodeapplications_amd import _lib, synthetic as syn
+    from qodeapplications_amd.graph import CapturedBuild
+    from qodeapplications_amd.xr import XRSystem
+    w = syn.config2(seed=2)
+    X = XRSystem(w.fragments, w.integrals, w.nuc)
+    ctx = _lib.context(0)
+    wdesc = ("six fragments 9,9,13,13,18,18 spatial orbitals, 3/4/5 e, states/block "
+             + ",".join(str(next(iter(f.states.values())).coeffs.shape[0]) for f in w.fragments)
+             + ", all 15 pair blocks")
+    print(f"bench: workload {wdesc}", file=sys.stderr, flush=True)
+
+    def eager():
+        X.build_densities()
+        return X.blocks()
+
+    for i in range(args.warmup):
+        eager()
+        torch.cuda.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        H = eager()
+    torch.cuda.synchronize()
+    t_eager = (time.perf_counter() - t0) / args.steps
+    ctx.profile(False)
+    n, gms, gfl, gby = ctx.profile_stats(_lib.K_GEMM_F64)
+    del H
+    print(f"bench: eager {t_eager * 1e3:.1f} ms/build", file=sys.stderr, flush=True)
+    cap = CapturedBuild(X, tune=True)
+    for i in range(args.warmup):
+        cap.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        cap.replay()
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - t0) / args.steps
+    print(f"bench: graph {t_graph * 1e3:.1f} ms/build, {len(cap.tuned)} tuned shapes",
+          file=sys.stderr, flush=True)
+    out = {
+        "metric": "XR block build s/build (6 uneven fragments, 15 pair blocks)",
+        "value": t_graph, "unit": "s/build", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_graph * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded random orthonormal CI vectors, 8-fold symmetric V)",
+        "config": {"workload": wdesc},
+        "eager_s": t_eager, "graph_speedup": t_eager / t_graph,
+        "tuned_shapes": [{"shape": list(k), "tile": v[0], "splits": v[1], "tuned_ms": v[2], "untuned_ms": v[3]}
+                         for k, v in sorted(cap.tuned.items(), key=lambda kv: -kv[1][3])[:12]],
+        "roofline": {"bound": "mfma", "achieved": gfl / (gms * 1e-3) / 1e12 if gms else 0.0,
+                     "peak": MFMA_F64_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": gfl / (gms * 1e-3) / MFMA_F64_PEAK if gms else 0.0,
+                     "kernel": "k_dgemm (FP64 MFMA GEMM), eager build", "launches_per_step": n / args.steps,
+                     "avg_launch_ms": gms / max(n, 1), "traffic": None},
     }
     print(json.dumps(out))
 

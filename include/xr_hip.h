@@ -250,17 +250,30 @@ int xr_pair_outer(xr_ctx *ctx, const double *d_v, int64_t d2, const int64_t *d_r
                   const int64_t *d_c1, const int64_t *d_r2, const int64_t *d_c2, int64_t nI1,
                   int64_t nJ1, int64_t nI2, int64_t nJ2, int swapped, double *d_G);
 
+/* GEMM autotuner (no counterpart in the reference, whose contractions are
+ * fixed C loops, general-XRCC/H_contractions.c): times the four tile shapes
+ * of xr_gemm_f64 with K splits 1..64 on scratch operands of the given shape
+ * (reps timed launches each) and records the fastest in the context; later
+ * xr_gemm_f64 calls of exactly that shape use it.  Outputs the chosen tile
+ * (0 128x128, 1 128x64, 2 64x128, 3 64x64; -1 = the shape takes the skinny
+ * kernel and is not tuned), splits and mean time.  Synchronises the stream:
+ * call it outside graph capture. */
+int xr_gemm_tune(xr_ctx *ctx, int transA, int transB, int64_t M, int64_t N, int64_t K,
+                 int64_t batch, int reps, int *tile, int *splits, double *ms);
+int xr_gemm_tune_reset(xr_ctx *ctx);
+
 /* Matrix-free XR operator, axis packing (replaces the dense product-space
  * loops of general-XRCC/hamiltonian.py:21-63 for large fragment counts).
  * full: row-major tensor [dims[0]]...[dims[nf-1]][ncol]; packed: rows = the
  * front axes (front0 < front1, front1 = -1 for one axis) reordered through
  * d_rowmap (row r holds front index rowmap[r] = a*dims[front1] + b; NULL =
  * identity, then nrows must equal the front size), columns = the other
- * axes in order and then the column index.  mode 0: packed = gather(full);
- * mode 1: full += alpha * scatter(packed).  nf <= 8. */
+ * axes in order and then the column index, packed rows ld_packed apart
+ * (0 = the row length).  mode 0: packed = gather(full); mode 1: full +=
+ * alpha * scatter(packed).  nf <= 8. */
 int xr_axes_pack(xr_ctx *ctx, int nf, const int64_t *dims, int64_t ncol, int front0, int front1,
                  const int32_t *d_rowmap, int64_t nrows, double *d_full, double *d_packed,
-                 int mode, double alpha);
+                 int64_t ld_packed, int mode, double alpha);
 
 /* Pair-block adjoint input (the pair-outer layout of xr_pair_outer) read from
  * a reduced pair matrix G2 = sum over the other fragments of v v^T, whose

@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 #include <map>
+#include <array>
 #include <tuple>
 #include <mutex>
 
@@ -89,6 +90,7 @@ enum { XR_K_GEMM_F64 = 0, XR_K_GATHER = 1, XR_K_EXPAND = 2, XR_K_TABLE = 3, XR_K
 struct ProfRec {
     int kind;
     int64_t m = 0, n = 0, k = 0, batch = 0;
+    int ta = -1, tb = -1;
     double flops, bytes;
     hipEvent_t a, b;
 };
@@ -106,6 +108,8 @@ struct xr_ctx {
     std::vector<size_t> ws_size;
     // cache of K-space occupation masks keyed by (dim, n_elec)
     std::map<std::pair<int, int>, std::pair<uint64_t*, int64_t>> kspace;
+    // GEMM autotune table: (ta, tb, M, N, K, batch) -> (tile, splits)
+    std::map<std::array<int64_t, 6>, std::pair<int, int>> tune;
     std::mutex mu;
 };
 
@@ -117,7 +121,7 @@ struct ProfScope {
     xr_ctx* c;
     int idx = -1;
     ProfScope(xr_ctx* c_, int kind, double flops, double bytes);
-    void shape(int64_t m, int64_t n, int64_t k, int64_t b);
+    void shape(int64_t m, int64_t n, int64_t k, int64_t b, int ta = -1, int tb = -1);
     ~ProfScope();
 };
 int xr_hip_check(xr_ctx* c, hipError_t e, const char* what);

@@ -261,7 +261,9 @@ extern "C" int xr_block_scatter(xr_ctx* c, int nfrag, const int64_t* nI, const i
 // fragment axes ("front") to the rows, optionally reordered through rowmap
 // (packed row r holds front index rowmap[r] = a * dims[f1] + b), and the
 // remaining axes plus the column to the packed columns, in order:
-//   packed[r * xsize + x],  x = ((i_rest0 * d_rest1 + i_rest1) ...) * ncol + j.
+//   packed[r * ld_packed + x],  x = ((i_rest0 * d_rest1 + i_rest1) ...) * ncol + j
+// (ld_packed = 0: the packed row length), so several tensors can be packed
+// side by side into one wider matrix.
 // mode 0 copies src -> packed, mode 1 adds alpha * packed into src.  Every
 // element of src maps to exactly one packed element (no write conflicts).
 // ---------------------------------------------------------------------------
@@ -275,10 +277,11 @@ struct AxesDesc {
 };
 
 __global__ void k_axes_pack(AxesDesc ad, const int32_t* __restrict__ rowmap, double* __restrict__ full,
-                            double* __restrict__ packed, int mode, double alpha) {
+                            double* __restrict__ packed, int64_t ldp, int mode, double alpha) {
     const int64_t total = ad.nrows * ad.xsize;
     XR_GRID_STRIDE(i, total) {
         const int64_t r = i / ad.xsize, x = i - r * ad.xsize;
+        const int64_t pi = r * ldp + x;
         const int64_t f = rowmap ? (int64_t)rowmap[r] : r;
         if (f < 0 || f >= ad.nfront) continue;
         const int64_t a = f / ad.d1, b = f - a * ad.d1;
@@ -290,14 +293,14 @@ __global__ void k_axes_pack(AxesDesc ad, const int32_t* __restrict__ rowmap, dou
             off += (y - q * ad.rest_dim[k]) * ad.rest_stride[k];
             y = q;
         }
-        if (mode == 0) packed[i] = full[off];
-        else full[off] += alpha * packed[i];
+        if (mode == 0) packed[pi] = full[off];
+        else full[off] += alpha * packed[pi];
     }
 }
 
 extern "C" int xr_axes_pack(xr_ctx* c, int nf, const int64_t* dims, int64_t ncol, int front0, int front1,
                             const int32_t* d_rowmap, int64_t nrows, double* d_full, double* d_packed,
-                            int mode, double alpha) {
+                            int64_t ld_packed, int mode, double alpha) {
     XR_ENTRY(c);
     if (nf < 1 || nf > XR_MAX_FRAG_AX) return xr_fail(c, XR_EUNSUPPORTED, "1 to 8 fragment axes");
     if (front0 < 0 || front0 >= nf || front1 >= nf || front1 == front0 || (front1 >= 0 && front1 < front0))
@@ -327,10 +330,12 @@ extern "C" int xr_axes_pack(xr_ctx* c, int nf, const int64_t* dims, int64_t ncol
     ad.ncol = ncol;
     ad.nrows = nrows;
     if (!d_rowmap && nrows != ad.nfront) return xr_fail(c, XR_ESHAPE, "without a row map nrows must be the front size");
+    if (ld_packed == 0) ld_packed = ad.xsize;
+    if (ld_packed < ad.xsize) return xr_fail(c, XR_ESHAPE, "ld_packed smaller than the packed row");
     const int64_t total = nrows * ad.xsize;
     if (total <= 0) return XR_OK;
     ProfScope prof(c, XR_K_SCATTER, 0.0, (mode ? 24.0 : 16.0) * total);
-    k_axes_pack<<<xr_grid(total), 256, 0, c->stream>>>(ad, d_rowmap, d_full, d_packed, mode, alpha);
+    k_axes_pack<<<xr_grid(total), 256, 0, c->stream>>>(ad, d_rowmap, d_full, d_packed, ld_packed, mode, alpha);
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }

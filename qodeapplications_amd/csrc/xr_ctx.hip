@@ -67,13 +67,15 @@ ProfScope::ProfScope(xr_ctx* c_, int kind, double flops, double bytes) : c(c_) {
     idx = (int)c->recs.size() - 1;
 }
 
-void ProfScope::shape(int64_t m, int64_t n, int64_t k, int64_t b) {
+void ProfScope::shape(int64_t m, int64_t n, int64_t k, int64_t b, int ta, int tb) {
     if (idx < 0) return;
     ProfRec& r = c->recs[idx];
     r.m = m;
     r.n = n;
     r.k = k;
     r.batch = b;
+    r.ta = ta;
+    r.tb = tb;
 }
 
 ProfScope::~ProfScope() {
@@ -124,12 +126,12 @@ extern "C" int xr_profile_dump(xr_ctx* c, const char* path) {
     XR_HIP(c, hipStreamSynchronize(c->stream));
     FILE* f = fopen(path, "w");
     if (!f) return xr_fail(c, XR_EINVAL, std::string("cannot open ") + path);
-    fprintf(f, "kind,m,n,k,batch,ms,flops,bytes\n");
+    fprintf(f, "kind,m,n,k,batch,ta,tb,ms,flops,bytes\n");
     for (auto& r : c->recs) {
         float t = 0;
         if (hipEventElapsedTime(&t, r.a, r.b) != hipSuccess) t = -1;
-        fprintf(f, "%d,%lld,%lld,%lld,%lld,%.6f,%.6g,%.6g\n", r.kind, (long long)r.m, (long long)r.n,
-                (long long)r.k, (long long)r.batch, t, r.flops, r.bytes);
+        fprintf(f, "%d,%lld,%lld,%lld,%lld,%d,%d,%.6f,%.6g,%.6g\n", r.kind, (long long)r.m,
+                (long long)r.n, (long long)r.k, (long long)r.batch, r.ta, r.tb, t, r.flops, r.bytes);
     }
     fclose(f);
     return XR_OK;

@@ -249,7 +249,7 @@ template <int BM, int BN>
 static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
                         const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
                         int64_t sB, double beta, double* C, int64_t ldc, int64_t sC,
-                        int64_t batch) {
+                        int64_t batch, int splits_req = 0) {
     int64_t tm = ceil_div(M, BM), tn = ceil_div(N, BN);
     int64_t tiles = tm * tn;
     if (tiles > 0x7fffffff || batch > 65535)
@@ -258,7 +258,12 @@ static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t
     // products go to scratch and are summed in a fixed order (deterministic)
     int64_t splits = 1, kchunk = K;
     const int64_t target = 512;
-    if (tiles * batch < 256 && K >= 1024) {  // (splits may stay 1)
+    if (splits_req > 0) {                    // from the autotune table
+        kchunk = ceil_div(ceil_div(K, splits_req), GEMM_BK) * GEMM_BK;
+        if (kchunk < GEMM_BK) kchunk = GEMM_BK;
+        splits = ceil_div(K, kchunk);
+        if (splits < 1) { splits = 1; kchunk = K; }
+    } else if (tiles * batch < 256 && K >= 1024) {  // (splits may stay 1)
         splits = ceil_div(target, tiles * batch);
         int64_t max_split = K / 512;
         if (splits > max_split) splits = max_split;
@@ -281,7 +286,7 @@ static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t
     }
     dim3 grid((unsigned)tiles, (unsigned)splits, (unsigned)batch);
     ProfScope prof(c, XR_K_GEMM_F64, 2.0 * M * N * K * batch, 8.0 * (M * K + K * N + M * N) * batch);
-    prof.shape(M, N, K, batch);
+    prof.shape(M, N, K, batch, ta, tb);
 #define XR_GEMM_LAUNCH(TA_, TB_)                                                                 \
     k_dgemm<BM, BN, TA_, TB_><<<grid, GEMM_THREADS, 0, c->stream>>>(M, N, K, aw, A, lda, sA, B,   \
                                                                      ldb, sB, bw, Cw, ldw, sCw,   \
@@ -301,15 +306,30 @@ static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t
     return XR_OK;
 }
 
-// Chooses the tile shape: 128x128 when the grid still fills the chip,
-// smaller tiles for skinny or small problems.
+// One explicit configuration: tile 0 = 128x128, 1 = 128x64, 2 = 64x128,
+// 3 = 64x64; splits = K splits (0 = the heuristic).
+static int launch_cfg(xr_ctx* c, int tile, int splits, int ta, int tb, int64_t M, int64_t N, int64_t K,
+                      double alpha, const double* A, int64_t lda, int64_t sA, const double* B,
+                      int64_t ldb, int64_t sB, double beta, double* C, int64_t ldc, int64_t sC,
+                      int64_t batch) {
+    switch (tile) {
+    case 0: return launch_tiles<128, 128>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, splits);
+    case 1: return launch_tiles<128, 64>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, splits);
+    case 2: return launch_tiles<64, 128>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, splits);
+    default: return launch_tiles<64, 64>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, splits);
+    }
+}
+
+// Chooses the tile shape: the autotuned configuration when the shape has
+// one, else 128x128 when the grid still fills the chip and smaller tiles
+// for skinny or small problems.
 int xr_dgemm_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
                     const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
                     int64_t sB, double beta, double* C, int64_t ldc, int64_t sC, int64_t batch) {
     if (M <= 0 || N <= 0 || batch <= 0) return XR_OK;
     if (N <= 8 && !ta && batch == 1 && M >= 1024 && K >= 1024) {
         ProfScope prof(c, XR_K_GEMM_F64, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
-        prof.shape(M, N, K, batch);
+        prof.shape(M, N, K, batch, ta, tb);
         unsigned grid = xr_grid(ceil_div(M, 4) * 64);
 #define XR_SKINNY(NB_)                                                                                  \
         if (tb) k_dgemm_skinny<NB_, true><<<grid, 256, 0, c->stream>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc); \
@@ -322,6 +342,10 @@ int xr_dgemm_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, 
         XR_LAUNCH_CHECK(c);
         return XR_OK;
     }
+    auto it = c->tune.find(std::array<int64_t, 6>{ta, tb, M, N, K, batch});
+    if (it != c->tune.end())
+        return launch_cfg(c, it->second.first, it->second.second, ta, tb, M, N, K, alpha, A, lda, sA,
+                          B, ldb, sB, beta, C, ldc, sC, batch);
     const int64_t want = 512;  // workgroups that keep 256 CUs busy
     int64_t big = ceil_div(M, 128) * ceil_div(N, 128) * batch;
     if (big >= want) return launch_tiles<128, 128>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
@@ -344,6 +368,74 @@ extern "C" int xr_gemm_f64(xr_ctx* c, int transA, int transB, int64_t M, int64_t
     if ((transB ? ldb < K : ldb < N) && N > 0 && K > 0) return xr_fail(c, XR_ESHAPE, "ldb too small");
     return xr_dgemm_launch(c, transA, transB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc,
                            sC, batch);
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: times every tile shape with a ladder of K splits on scratch
+// operands of the given shape (HIP events on the context stream) and keeps
+// the fastest in the context's table, which xr_dgemm_launch consults.
+// ---------------------------------------------------------------------------
+extern "C" int xr_gemm_tune(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, int64_t batch,
+                            int reps, int* tile_out, int* splits_out, double* ms_out) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (M <= 0 || N <= 0 || K <= 0 || batch <= 0 || reps <= 0) return xr_fail(c, XR_EINVAL, "bad tune shape");
+    if (N <= 8 && !ta && batch == 1 && M >= 1024 && K >= 1024) {   // the skinny kernel, not tiled
+        if (tile_out) *tile_out = -1;
+        if (splits_out) *splits_out = 0;
+        if (ms_out) *ms_out = 0.0;
+        return XR_OK;
+    }
+    const int64_t na = M * K * batch, nb = K * N * batch, nc = M * N * batch;
+    if ((na + nb + nc) * 8 > (int64_t)16 << 30) return xr_fail(c, XR_EUNSUPPORTED, "shape too large to tune");
+    double* A = (double*)xr_workspace(c, 60, na * sizeof(double));
+    double* B = (double*)xr_workspace(c, 61, nb * sizeof(double));
+    double* C = (double*)xr_workspace(c, 62, nc * sizeof(double));
+    if (!A || !B || !C) return xr_fail(c, XR_ENOMEM, "workspace (tune)");
+    int rc;
+    if ((rc = xr_fill(c, A, na, 1e-3)) || (rc = xr_fill(c, B, nb, 1e-3))) return rc;
+    const int64_t lda = ta ? M : K, ldb = tb ? K : N;
+    const int64_t sA = batch > 1 ? M * K : 0, sB = batch > 1 ? K * N : 0, sC = batch > 1 ? M * N : 0;
+    hipEvent_t e0, e1;
+    XR_HIP(c, hipEventCreate(&e0));
+    XR_HIP(c, hipEventCreate(&e1));
+    const bool prof = c->prof;
+    c->prof = false;
+    double best = 1e300;
+    int best_tile = 0, best_split = 0;
+    for (int tile = 0; tile < 4; tile++) {
+        for (int sp = 1; sp <= 64; sp *= 2) {
+            if (sp > 1 && K / sp < 256) break;
+            rc = launch_cfg(c, tile, sp, ta, tb, M, N, K, 1.0, A, lda, sA, B, ldb, sB, 0.0, C, N, sC, batch);
+            if (rc) break;
+            XR_HIP(c, hipEventRecord(e0, c->stream));
+            for (int r = 0; r < reps; r++)
+                rc |= launch_cfg(c, tile, sp, ta, tb, M, N, K, 1.0, A, lda, sA, B, ldb, sB, 0.0, C, N, sC, batch);
+            XR_HIP(c, hipEventRecord(e1, c->stream));
+            XR_HIP(c, hipEventSynchronize(e1));
+            float ms = 0.f;
+            XR_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+            if (rc) break;
+            if (ms / reps < best) { best = ms / reps; best_tile = tile; best_split = sp; }
+        }
+        if (rc) break;
+    }
+    c->prof = prof;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    c->tune[std::array<int64_t, 6>{ta, tb, M, N, K, batch}] = std::make_pair(best_tile, best_split);
+    if (tile_out) *tile_out = best_tile;
+    if (splits_out) *splits_out = best_split;
+    if (ms_out) *ms_out = best;
+    return XR_OK;
+}
+
+extern "C" int xr_gemm_tune_reset(xr_ctx* c) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    c->tune.clear();
+    return XR_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -549,7 +641,7 @@ int xr_dgemm_split_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64
     dim3 grid((unsigned)tiles, (unsigned)splits, (unsigned)batch);
     {
         ProfScope prof(c, XR_K_GEMM_SPLIT, 2.0 * M * N * K * batch, 8.0 * (M * K + K * N + M * N) * batch);
-        prof.shape(M, N, K, batch);
+        prof.shape(M, N, K, batch, ta, tb);
         if (ta && tb) k_dgemm_split<true, true><<<grid, SPLIT_THREADS, 0, c->stream>>>(M, N, K, A, lda, sA, B, ldb, sB, Cw, N, M * N, tm, tn, kchunk, batch * M * N);
         else if (ta) k_dgemm_split<true, false><<<grid, SPLIT_THREADS, 0, c->stream>>>(M, N, K, A, lda, sA, B, ldb, sB, Cw, N, M * N, tm, tn, kchunk, batch * M * N);
         else if (tb) k_dgemm_split<false, true><<<grid, SPLIT_THREADS, 0, c->stream>>>(M, N, K, A, lda, sA, B, ldb, sB, Cw, N, M * N, tm, tn, kchunk, batch * M * N);

@@ -114,12 +114,14 @@ def _orthonormalize(T, V=None, passes=2):
 
 
 def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=None,
-             max_space=None, precond=None, start=None, keep=8):
+             max_space=None, precond=None, start=None, keep=8, olsen=False, strict=True):
     """Block Davidson.  matmul(X, Y) writes H X into Y for X of shape (n, b);
     precond(R, theta) approximates (H - theta)^-1 R (default: diagonal).
     Converged when every residual norm is below tol * max(1, |theta|), or
     when it has stalled at the rounding floor within 100x of that.  A full
-    subspace restarts on its `keep` lowest Ritz vectors.
+    subspace restarts on its `keep` lowest Ritz vectors.  olsen: apply
+    Olsen's correction to the preconditioned residual.  strict=False returns
+    the current Ritz pairs after max_iter instead of raising.
     The subspace lives in preallocated device buffers; the Rayleigh-Ritz
     eigenproblem and the b x b factorisations are solved on the host."""
     b = block or max(2 * k, 4)
@@ -164,9 +166,17 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         hist.append(float(np.max(rn / target)))
         if len(hist) > 20 and hist[-1] < 100.0 and min(hist[-10:]) > 0.5 * min(hist[:-10]):
             return theta[:k].clone(), X[:, :k].contiguous()
-        T = precond(R, theta) if precond is not None else R / torch.where(
-            (theta[None, :] - diag[:, None]).abs() < 1e-8, torch.full_like(R, 1e-8),
-            theta[None, :] - diag[:, None])
+        if precond is not None:
+            T = precond(R, theta)
+            if olsen:
+                # Olsen's correction: remove the part of M^-1 r along M^-1 x,
+                # so the new direction is not dominated by the Ritz vector itself
+                PX = precond(X, theta)
+                eps = (X * T).sum(0) / (X * PX).sum(0)
+                T = T - PX * eps[None, :]
+        else:
+            T = R / torch.where((theta[None, :] - diag[:, None]).abs() < 1e-8, torch.full_like(R, 1e-8),
+                                theta[None, :] - diag[:, None])
         if m + nb > max_space:              # thick restart on the lowest Ritz vectors
             nk = min(m, max(2 * nb, keep))
             yk = torch.as_tensor(np.ascontiguousarray(y[:, :nk]), device=device)
@@ -189,5 +199,7 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         Vbuf[:, m:m + q] = Q
         AVbuf[:, m:m + q] = AQ
         m += q
+    if not strict:
+        return theta[:k].clone(), X[:, :k].contiguous()
     raise RuntimeError(f"Davidson did not converge in {max_iter} iterations "
                        f"(residual {float(rn.max()):.2e})")

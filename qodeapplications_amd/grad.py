@@ -86,7 +86,7 @@ def _flat(R):
     return R.view(R.shape[0] * R.shape[1], -1)
 
 
-GRAD_TOL = 1e-11
+GRAD_TOL = 1e-10
 
 
 class XRAdjoint:

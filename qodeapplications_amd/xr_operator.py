@@ -31,6 +31,15 @@ from .gemm import gemm_f64, mm
 DENSE_SECTOR = 2048
 
 
+def _axis_apply(A, src, dst, pre, n, post, transA=0, beta=0.0):
+    """dst[p, :, q] = op(A) src[p, :, q] (+ beta dst) for row-major (pre, n,
+    post) views: one GEMM src @ op(A)^T when post == 1, else one batched GEMM."""
+    if post == 1:
+        mm(src.view(pre, n), A, transB=0 if transA else 1, out=dst.view(pre, n), beta=beta)
+    else:
+        gemm_f64(A.unsqueeze(0), src.view(pre, n, post), dst.view(pre, n, post), transA=transA, beta=beta)
+
+
 def _i32(a, device):
     return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=device)
 
@@ -50,6 +59,7 @@ class XROperator:
         self.charges = [np.asarray(c, dtype=np.int64) for c in charges]
         self.pairs = {}
         self._stacks, self._pair_diag = {}, {}
+        self.log = None          # set to a list to record the per-sector screening
         for (m, n), H in H2.items():
             if H is None:
                 continue
@@ -112,12 +122,12 @@ class XROperator:
         self._pack([c.size, cnt], 1, 1, -1, lr, r.size, tmp, out, 0)    # out[x, y] = blk[lr[x], lc[y]]
         return out
 
-    def _pack(self, dims, ncol, f0, f1, rowmap, nrows, full, packed, mode, alpha=1.0):
+    def _pack(self, dims, ncol, f0, f1, rowmap, nrows, full, packed, mode, alpha=1.0, ld=0):
         import ctypes
         d = (ctypes.c_int64 * len(dims))(*[int(x) for x in dims])
         self.ctx.call("xr_axes_pack", len(dims), d, int(ncol), int(f0), int(f1),
                       None if rowmap is None else ptr(rowmap), int(nrows), ptr(full), ptr(packed),
-                      int(mode), float(alpha))
+                      int(ld), int(mode), float(alpha))
 
     # -- products -------------------------------------------------------------
     def matvec(self, X, Y):
@@ -160,15 +170,17 @@ class XROperator:
             tot = {t + x for t in tot for x in q}
         return sorted(tot)
 
-    def lowest(self, k=1, tol=1e-9, max_iter=400, block=None, charges=None, screen_tol=1e-4):
+    def lowest(self, k=1, tol=1e-9, max_iter=400, block=None, charges=None, screen_tol=1e-3,
+               screen_iter=12):
         """(values[k], vectors[n, k]) of the lowest roots over all total
         charges (or the given ones), in the product layout.
 
         H conserves the total charge, so each charge sector is solved on its
         own compact layout (block size k unless given).  Every sector is
-        first converged loosely (residual screen_tol); only the sectors whose
-        Ritz values can still be among the lowest k (within the residual
-        norms) are then refined to tol from their screened vectors."""
+        first screened (residual screen_tol or screen_iter iterations); only
+        the sectors whose Ritz values can still be among the lowest k (within
+        twice their residual norms) are then refined to tol from their
+        screened vectors."""
         cand = []
         for Q in (charges if charges is not None else self.total_charges()):
             sp = self.sector_space(Q)
@@ -180,9 +192,15 @@ class XROperator:
                 cand.append([sp, w[:kk], v[:, :kk].contiguous(), 0.0, True])
             else:
                 b = block or kk
-                w, v = davidson(sp.matvec, None, sp.size, kk, max(tol, screen_tol), max_iter, b,
-                                self.device, precond=sp.precondition, start=sp.guess(b))
-                cand.append([sp, w, v, max(tol, screen_tol) * max(1.0, abs(float(w[-1]))) * 10, False])
+                w, v = davidson(sp.matvec, None, sp.size, kk, max(tol, screen_tol), screen_iter, b,
+                                self.device, precond=sp.precondition, start=sp.guess(b), strict=False)
+                # residual norm of the screened roots: an eigenvalue lies within it
+                r = torch.empty_like(v)
+                sp.matvec(v, r)
+                rn = float(torch.linalg.norm(r - v * w[None, :], dim=0).max())
+                cand.append([sp, w, v, 2.0 * rn, False])
+                if self.log is not None:
+                    self.log.append((sp.Q, sp.size, [float(x) for x in w], rn))
         # k-th lowest upper bound over all sectors; a sector whose lowest Ritz
         # value minus its residual margin lies above it cannot contribute
         ups = sorted(float(x) for c in cand for x in c[1])
@@ -304,6 +322,10 @@ class SectorSpace:
                     R = np.concatenate(rows)
                     self.stack[(m, n, cm, cn)] = (op.extract(m, n, R, cols),
                                                   [(tm, tn, off) for tm, tn, off in tg])
+        self.groups = {}
+        for (m, n) in op.pairs:
+            for i, t in enumerate(self.sectors):
+                self.groups.setdefault((m, n, t[m], t[n]), []).append(i)
         # diagonal of each charge-conserving pair block in the monomer
         # eigenbasis: diag((Um x Un)^T H2[(cm,cn),(cm,cn)] (Um x Un))
         self.pair_diag = op._pair_diag
@@ -348,45 +370,49 @@ class SectorSpace:
             sh = self.shapes[i]
             Xi, Yi = self._view(X, i), self._view(Y, i)
             for m in range(nf):
-                pre = int(np.prod(sh[:m]))
-                post = int(np.prod(sh[m + 1:])) * b
-                gemm_f64(self.h1[m][t[m]].unsqueeze(0), Xi.view(pre, sh[m], post),
-                         Yi.view(pre, sh[m], post), beta=0.0 if m == 0 else 1.0)
-        for i, t in enumerate(self.sectors):
-            sh = self.shapes[i]
-            Xi = self._view(X, i)
-            for (m, n) in op.pairs:
-                Hs, targets = self.stack[(m, n, t[m], t[n])]
-                D = sh[m] * sh[n]
-                P = torch.empty(D, self.sizes[i] // D * b, dtype=torch.float64, device=op.device)
-                op._pack(sh, b, m, n, None, D, Xi, P, 0)
-                T = mm(Hs, P)
+                _axis_apply(self.h1[m][t[m]], Xi, Yi, int(np.prod(sh[:m])), sh[m],
+                            int(np.prod(sh[m + 1:])) * b, beta=0.0 if m == 0 else 1.0)
+        # pair terms: the tuples sharing (m, n) charges share the stacked
+        # blocks, so their packed columns go side by side into one GEMM
+        for (m, n, cm, cn), members in self.groups.items():
+            Hs, targets = self.stack[(m, n, cm, cn)]
+            D = self.blocks_of[m][cm][1] * self.blocks_of[n][cn][1]
+            widths = [self.sizes[i] // D * b for i in members]
+            W = int(sum(widths))
+            P = torch.empty(D, W, dtype=torch.float64, device=op.device)
+            col = 0
+            for i, w in zip(members, widths):
+                op._pack(self.shapes[i], b, m, n, None, D, self._view(X, i), P[:, col:], 0, ld=W)
+                col += w
+            T = mm(Hs, P)
+            col = 0
+            for i, w in zip(members, widths):
+                t = self.sectors[i]
                 for tm, tn, off in targets:
                     tt = list(t)
                     tt[m], tt[n] = tm, tn
                     j = self.index[tuple(tt)]
                     shj = self.shapes[j]
                     Dj = shj[m] * shj[n]
-                    op._pack(shj, b, m, n, None, Dj, self._view(Y, j), T[off:off + Dj], 1, 1.0)
+                    op._pack(shj, b, m, n, None, Dj, self._view(Y, j), T[off:off + Dj, col:], 1, 1.0, ld=W)
+                col += w
         return Y
 
     def _axes(self, X, inverse):
         """Applies U_m^T (or U_m when inverse) along every fragment axis of
         every tuple: the product eigenbasis of the monomer blocks."""
         b = X.shape[1]
-        out = X.clone()
-        tmp = torch.empty_like(X)
-        for i, t in enumerate(self.sectors):
-            sh = self.shapes[i]
-            for m in range(self.op.nf):
-                pre = int(np.prod(sh[:m]))
-                post = int(np.prod(sh[m + 1:])) * b
-                U = self.eig[m][t[m]][1]
-                src, dst = self._view(out, i), self._view(tmp, i)
-                gemm_f64(U.unsqueeze(0), src.view(pre, sh[m], post), dst.view(pre, sh[m], post),
-                         transA=0 if inverse else 1)
-                src.copy_(dst)
-        return out
+        bufs = [X.clone(), torch.empty_like(X)]
+        cur = 0
+        for m in range(self.op.nf):
+            src, dst = bufs[cur], bufs[1 - cur]
+            for i, t in enumerate(self.sectors):
+                sh = self.shapes[i]
+                _axis_apply(self.eig[m][t[m]][1], self._view(src, i), self._view(dst, i),
+                            int(np.prod(sh[:m])), sh[m], int(np.prod(sh[m + 1:])) * b,
+                            transA=0 if inverse else 1)
+            cur = 1 - cur
+        return bufs[cur]
 
     def precondition(self, R, theta):
         """(Hd - theta)^-1 R, Hd the diagonal of H in the product eigenbasis of

@@ -42,3 +42,11 @@ def test_no_gpu_means_loud_failure():
         assert "no GPU" in str(e)
     else:
         raise AssertionError("context creation must fail without a GPU")
+
+
+def test_conjugate_strings():
+    from qodeapplications_amd.hermitian import conjugate_string
+    pairs = {"a": "c", "aa": "cc", "caa": "cca", "ca": "ca", "ccaa": "ccaa", "caaa": "ccca", "ccaaa": "cccaa"}
+    for op, adj in pairs.items():
+        assert conjugate_string(op) == adj
+        assert conjugate_string(adj) == op

@@ -1,0 +1,96 @@
+"""Autotuned GEMM configurations and the graph-captured XR block build.
+
+* every tuned configuration gives the product of the untuned one (same
+  algorithm, other tiling and K split: ~1e-15 relative);
+* replaying the captured build after new coefficients are copied in equals
+  an eager build with those coefficients, on several replays with eager
+  work and synchronisation between them (graph memory nodes and library
+  calls inside a capture have given wrong results on later replays).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from qodeapplications_amd import synthetic as syn
+from qodeapplications_amd._lib import context
+from qodeapplications_amd.gemm import mm
+from qodeapplications_amd.graph import CapturedBuild, autotune, gemm_shapes
+from qodeapplications_amd.xr import XRSystem
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("ta,tb,M,N,K", [(0, 0, 300, 200, 5000), (1, 0, 64, 96, 40000),
+                                         (0, 1, 1500, 700, 300), (1, 1, 200, 1100, 900)])
+def test_tuned_gemm_matches_heuristic(ta, tb, M, N, K):
+    ctx = context(0)
+    g = torch.Generator(device="cuda").manual_seed(M + N + K)
+    A = torch.randn(*((K, M) if ta else (M, K)), dtype=torch.float64, device="cuda", generator=g)
+    B = torch.randn(*((N, K) if tb else (K, N)), dtype=torch.float64, device="cuda", generator=g)
+    ctx.call("xr_gemm_tune_reset")
+    ref = mm(A, B, transA=ta, transB=tb)
+    tile, splits, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    ctx.call("xr_gemm_tune", ta, tb, M, N, K, 1, 2, ctypes.byref(tile), ctypes.byref(splits), ctypes.byref(ms))
+    assert 0 <= tile.value <= 3 and splits.value >= 1 and ms.value > 0
+    got = mm(A, B, transA=ta, transB=tb)
+    scale = float(((A.abs().T if ta else A.abs()) @ (B.abs().T if tb else B.abs())).max())
+    assert float((got - ref).abs().max()) < 1e-14 * scale
+    ctx.call("xr_gemm_tune_reset")
+
+
+def test_gemm_shapes_records_transposes():
+    A = torch.randn(400, 300, dtype=torch.float64, device="cuda")
+    B = torch.randn(500, 300, dtype=torch.float64, device="cuda")
+    shapes = gemm_shapes(lambda: mm(A, B, transB=1))
+    assert (0, 1, 400, 500, 300, 1) in shapes
+
+
+def _coeff_keys(X):
+    return [(m, c) for m, F in enumerate(X.frags) for c in F.charges]
+
+
+def _new_coeffs(X, seed):
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    out = {}
+    for m, c in _coeff_keys(X):
+        z = X.frags[m].blocks[c].coeffs
+        q, _ = torch.linalg.qr((z + 0.1 * torch.randn(z.shape, dtype=torch.float64, device="cuda",
+                                                      generator=g)).T)
+        out[(m, c)] = q.T.contiguous()
+    return out
+
+
+@pytest.mark.parametrize("cfg", ["c0", "six"])
+def test_captured_build_replays_match_eager(cfg):
+    if cfg == "c0":
+        w = syn.config0(seed=0)
+    else:   # six uneven small fragments, every pair block
+        rng = np.random.default_rng(5)
+        orbs = [2, 3, 2, 4, 3, 2]
+        frags = [syn.make_fragment(n, [1, 2, 3], int(rng.integers(2, 4)), 2, rng) for n in orbs]
+        w = syn.Workload("six", frags, syn.FragmentBlocked(orbs, syn.spatial_h(sum(orbs), rng),
+                                                           syn.spatial_V(sum(orbs), rng)),
+                         np.zeros((6, 6)), 5)
+    X = XRSystem(w.fragments, w.integrals, w.nuc)
+    cap = CapturedBuild(X, tune=(cfg == "c0"))
+    ref_sys = XRSystem(w.fragments, w.integrals, w.nuc)
+    for rep in range(3):
+        z = _new_coeffs(X, 100 + rep)
+        cap.set_coeffs(z)
+        H1, H2, _ = cap.replay()
+        torch.cuda.synchronize()
+        # eager work and a synchronisation between replays
+        for (m, c), t in z.items():
+            ref_sys.frags[m].blocks[c] = ref_sys.frags[m].blocks[c].with_coeffs(t.clone())
+        ref_sys.build_densities()
+        R1, R2, _ = ref_sys.blocks()
+        torch.cuda.synchronize()
+        for a, b in zip(H1, R1):
+            assert float((a - b).abs().max()) < 1e-12, rep
+        nf = len(H1)
+        for m in range(nf):
+            for n in range(m + 1, nf):
+                assert float((H2[m][n] - R2[m][n]).abs().max()) < 1e-12, (rep, m, n)
+    context(0).call("xr_gemm_tune_reset")
