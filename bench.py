@@ -32,11 +32,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c3", choices=["c0", "c1", "c2", "c3"],
+    ap.add_argument("--config", default="c3", choices=["c0", "c1", "c2", "c3", "c4"],
                     help="c3: XR iteration on the 2x18-orbital dimer (the headline line); "
                          "c0: the same on the 2x9-orbital dimer; c1: density build of one "
                          "13-orbital fragment, 2-6 electrons, 32 states; c2: all blocks of six "
-                         "uneven fragments, eager vs autotuned graph replay (separate metrics)")
+                         "uneven fragments, eager vs autotuned graph replay (separate metrics); "
+                         "c4: XR iteration on the 4x13-orbital chain (no CPU baseline)")
     ap.add_argument("--precision", default="both", choices=["f64", "split_f32", "both"],
                     help="two-electron contraction precision; 'both' times FP64 (the value) "
                          "and split-FP32 (reported beside it with its |dE|)")
@@ -54,6 +55,8 @@ def workload(name):
     from qodeapplications_amd import synthetic as syn
     if name == "c0":
         return syn.config0(seed=0), "dimer 2x9 spatial orbitals, 3/4/5 e, 8 states/block"
+    if name == "c4":
+        return syn.config4(seed=4), "chain 4x13 spatial orbitals, 3/4/5 e, 32 states/block"
     return syn.config3(seed=3), "dimer 2x18 spatial orbitals, 3/4/5 e, 64 states/block"
 
 
@@ -256,7 +259,7 @@ def main():
         traffic = json.load(open(tfile))["traffic_bytes_per_launch"]
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and len(w.fragments) == 2:
         t_cpu, sample = cpu_baseline(w, args.cpu_seconds)
         cpu = {"value": t_cpu, "unit": "s/iter", "cores": 1, "kind": "port", "sample": sample}
 

@@ -30,3 +30,49 @@ def test_hermitian_layout_matches_direct_strings():
     assert dens["n_elec"] == {chg: 3 - chg for chg in frag.states}
     for chg in frag.states:
         assert torch.equal(dens["KetCoeffs"][(chg, chg)].cpu(), torch.as_tensor(frag.states[chg].coeffs))
+
+
+def _dimer_system(seed):
+    from qodeapplications_amd.xr import XRSystem
+    rng = np.random.default_rng(seed)
+    orbs = [3, 3]
+    frags = [syn.make_fragment(n, [1, 2, 3], 3, 2, rng) for n in orbs]
+    h = syn.spatial_h(6, rng)
+    V = syn.spatial_V(6, rng)
+    U = np.stack([0.2 * syn.spatial_h(6, rng) for _ in orbs])
+    return XRSystem(frags, syn.FragmentBlocked(orbs, h, V, U), np.zeros((2, 2)))
+
+
+@pytest.mark.parametrize("which", ["c0", "with_U"])
+def test_order0_diagrams_reproduce_xr_dimer(which):
+    """The order-0 diagrams of the hermitian formulation give the same XR
+    dimer Hamiltonian as the H_contractions path (checked against FCI in
+    test_oracle / test_gpu_hamiltonian): monomer matrices element by
+    element, pair matrix up to the sign gauge of the product basis
+    (|elements| and the spectrum agree)."""
+    from qodeapplications_amd import hermitian as Hm
+    from qodeapplications_amd.xr import XRSystem
+    if which == "c0":
+        w = syn.config0(seed=0)
+        X = XRSystem(w.fragments, w.integrals, np.zeros((2, 2)))
+    else:
+        X = _dimer_system(7)
+    H1, H2, _ = X.blocks()
+    for m in range(2):
+        F = X.frags[m]
+        order = torch.cat([F.pos[c] for c in F.charges])
+        Hh = Hm.monomer_matrix(X, m)
+        assert float((Hh - H1[m][order][:, order]).abs().max()) < 1e-12
+    S2H2 = Hm.dimer_matrix(X, 0, 1)
+    idx = Hm.product_order(X, 0, 1)
+    G = H2[0][1][idx][:, idx]
+    assert float((S2H2.abs() - G.abs()).abs().max()) < 1e-12
+    d0, d1 = X.frags[0].d, X.frags[1].d
+    M_gen = X.xr_matrix(H=(H1, H2, None))
+    I0 = torch.eye(d0, dtype=torch.float64, device="cuda")
+    I1 = torch.eye(d1, dtype=torch.float64, device="cuda")
+    M_h = torch.kron(H1[0], I1) + torch.kron(I0, H1[1])
+    M_h[idx[:, None], idx[None, :]] += S2H2
+    ev_h = torch.linalg.eigvalsh(0.5 * (M_h + M_h.T))
+    ev_g = torch.linalg.eigvalsh(0.5 * (M_gen + M_gen.T))
+    assert float((ev_h - ev_g).abs().max()) < 1e-10
