@@ -245,6 +245,177 @@ __global__ __launch_bounds__(256) void k_dgemm_skinny(int64_t M, int64_t N, int6
     }
 }
 
+// C = alpha A^T B + beta C for A (K x M) and B (K x N) row-major with K
+// large and M <= 64, N <= 8 (Gram products of a tall block of vectors).
+// Each block sums a contiguous run of rows (lane = column of A, the 4 waves
+// take every 4th row) into a partial (64 x NB) tile; a second kernel adds
+// the partials in block order (deterministic).
+template <int NB>
+__global__ __launch_bounds__(256) void k_dgemm_tn_skinny(int64_t M, int64_t N, int64_t K,
+                                                         const double* __restrict__ A, int64_t lda,
+                                                         const double* __restrict__ B, int64_t ldb,
+                                                         int64_t rows, double* __restrict__ part) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t k0 = (int64_t)blockIdx.x * rows;
+    const int64_t k1 = (K - k0 < rows) ? K : k0 + rows;
+    double acc[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) acc[j] = 0.0;
+#pragma unroll 4
+    for (int64_t k = k0 + wave; k < k1; k += 4) {
+        const double a = (lane < M) ? A[k * lda + lane] : 0.0;
+#pragma unroll
+        for (int j = 0; j < NB; j++) acc[j] = fma(a, (j < N) ? B[k * ldb + j] : 0.0, acc[j]);
+    }
+    __shared__ double red[4][64][NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) red[wave][lane][j] = acc[j];
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            part[((int64_t)blockIdx.x * 64 + lane) * NB + j] =
+                red[0][lane][j] + red[1][lane][j] + red[2][lane][j] + red[3][lane][j];
+    }
+}
+
+__global__ void k_tn_reduce(const double* __restrict__ part, int64_t nblk, int64_t M, int64_t N, int nb,
+                            double alpha, double beta, double* __restrict__ C, int64_t ldc) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M * N) return;
+    const int64_t i = t / N, j = t - (t / N) * N;
+    double s = 0.0;
+    for (int64_t b = 0; b < nblk; b++) s += part[(b * 64 + i) * nb + j];
+    double v = alpha * s;
+    if (beta != 0.0) v += beta * C[i * ldc + j];
+    C[i * ldc + j] = v;
+}
+
+// Gram products with M <= 8 columns: the rows are spread over all 256
+// threads (coalesced reads of A and B), each thread keeps the M x NB sums,
+// and the block adds them through LDS into the same partial layout as above.
+template <int MB, int NB>
+__global__ __launch_bounds__(256) void k_dgemm_tn_small(int64_t M, int64_t N, int64_t K,
+                                                        const double* __restrict__ A, int64_t lda,
+                                                        const double* __restrict__ B, int64_t ldb,
+                                                        int64_t rows, double* __restrict__ part) {
+    const int64_t k0 = (int64_t)blockIdx.x * rows;
+    const int64_t k1 = (K - k0 < rows) ? K : k0 + rows;
+    double acc[MB][NB];
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++) acc[i][j] = 0.0;
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += 256) {
+        double a[MB], b[NB];
+#pragma unroll
+        for (int i = 0; i < MB; i++) a[i] = (i < M) ? A[k * lda + i] : 0.0;
+#pragma unroll
+        for (int j = 0; j < NB; j++) b[j] = (j < N) ? B[k * ldb + j] : 0.0;
+#pragma unroll
+        for (int i = 0; i < MB; i++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+    }
+    __shared__ double red[256];
+    for (int i = 0; i < MB; i++)
+        for (int j = 0; j < NB; j++) {
+            double v = acc[i][j];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+            __syncthreads();
+            if (threadIdx.x == 0)
+                part[((int64_t)blockIdx.x * 64 + i) * NB + j] = red[0] + red[1] + red[2] + red[3];
+            __syncthreads();
+        }
+}
+
+// C = alpha A op(B) + beta C for tall A (M x K row-major, row stride lda)
+// with K <= 64 and N <= 8.  A block stages 64 rows of A through LDS with
+// coalesced loads (the rows' K values are contiguous), then 4 threads per
+// row sum interleaved k and combine with two shuffles.
+template <int NB, bool TB>
+__global__ __launch_bounds__(256) void k_dgemm_rows_lds(int64_t M, int64_t N, int64_t K, double alpha,
+                                                        const double* __restrict__ A, int64_t lda,
+                                                        const double* __restrict__ B, int64_t ldb,
+                                                        double beta, double* __restrict__ C, int64_t ldc) {
+    __shared__ double tile[64 * 64];
+    __shared__ double bs[64 * NB];
+    const int tid = threadIdx.x;
+    const int Ki = (int)K;
+    for (int e = tid; e < Ki * NB; e += 256) {
+        const int k = e / NB, j = e - (e / NB) * NB;
+        bs[e] = (j < N) ? (TB ? B[(int64_t)j * ldb + k] : B[(int64_t)k * ldb + j]) : 0.0;
+    }
+    const int row = tid >> 2, part = tid & 3;
+    for (int64_t r0 = (int64_t)blockIdx.x * 64; r0 < M; r0 += (int64_t)gridDim.x * 64) {
+        const int nrows = (M - r0 < 64) ? (int)(M - r0) : 64;
+        __syncthreads();
+        for (int e = tid; e < nrows * Ki; e += 256) {
+            const int r = e / Ki, k = e - (e / Ki) * Ki;
+            tile[e] = A[(r0 + r) * lda + k];
+        }
+        __syncthreads();
+        double acc[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++) acc[j] = 0.0;
+        if (row < nrows) {
+            for (int k = part; k < Ki; k += 4) {
+                const double a = tile[row * Ki + k];
+#pragma unroll
+                for (int j = 0; j < NB; j++) acc[j] = fma(a, bs[k * NB + j], acc[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            acc[j] += __shfl_xor(acc[j], 1, 64);
+            acc[j] += __shfl_xor(acc[j], 2, 64);
+        }
+        if (part == 0 && row < nrows) {
+#pragma unroll
+            for (int j = 0; j < NB; j++) {
+                if (j < N) {
+                    double v = alpha * acc[j];
+                    if (beta != 0.0) v += beta * C[(r0 + row) * ldc + j];
+                    C[(r0 + row) * ldc + j] = v;
+                }
+            }
+        }
+    }
+}
+
+static int launch_tn_skinny(xr_ctx* c, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
+                            int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                            int64_t ldc) {
+    const int nb = N <= 1 ? 1 : (N <= 2 ? 2 : (N <= 4 ? 4 : 8));
+    int64_t nblk = ceil_div(K, 2048);
+    if (nblk > 1024) nblk = 1024;
+    const int64_t rows = ceil_div(K, nblk);
+    nblk = ceil_div(K, rows);
+    double* part = (double*)xr_workspace(c, 52, nblk * 64 * nb * sizeof(double));
+    if (!part) return xr_fail(c, XR_ENOMEM, "workspace (Gram)");
+    ProfScope prof(c, XR_K_GEMM_F64, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
+    prof.shape(M, N, K, 1, 1, 0);
+#define XR_TN_SMALL(MB_)                                                                                  \
+    if (nb == 1) k_dgemm_tn_small<MB_, 1><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part); \
+    else if (nb == 2) k_dgemm_tn_small<MB_, 2><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part); \
+    else if (nb == 4) k_dgemm_tn_small<MB_, 4><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part); \
+    else k_dgemm_tn_small<MB_, 8><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part)
+    if (M <= 1) { XR_TN_SMALL(1); }
+    else if (M <= 2) { XR_TN_SMALL(2); }
+    else if (M <= 4) { XR_TN_SMALL(4); }
+    else if (M <= 8) { XR_TN_SMALL(8); }
+    else if (nb == 1) k_dgemm_tn_skinny<1><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
+    else if (nb == 2) k_dgemm_tn_skinny<2><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
+    else if (nb == 4) k_dgemm_tn_skinny<4><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
+    else k_dgemm_tn_skinny<8><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
+#undef XR_TN_SMALL
+    XR_LAUNCH_CHECK(c);
+    k_tn_reduce<<<(unsigned)ceil_div(M * N, 256), 256, 0, c->stream>>>(part, nblk, M, N, nb, alpha, beta, C, ldc);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
 template <int BM, int BN>
 static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
                         const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
@@ -327,7 +498,24 @@ int xr_dgemm_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, 
                     const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
                     int64_t sB, double beta, double* C, int64_t ldc, int64_t sC, int64_t batch) {
     if (M <= 0 || N <= 0 || batch <= 0) return XR_OK;
-    if (N <= 8 && !ta && batch == 1 && M >= 1024 && K >= 1024) {
+    if (ta && !tb && batch == 1 && M <= 64 && N <= 8 && K >= 4096)
+        return launch_tn_skinny(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (N <= 8 && !ta && batch == 1 && M >= 1024 && K <= 64) {
+        ProfScope prof(c, XR_K_GEMM_F64, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
+        prof.shape(M, N, K, batch, ta, tb);
+        const unsigned rgrid = (unsigned)(ceil_div(M, 64) < (1 << 20) ? ceil_div(M, 64) : (1 << 20));
+#define XR_ROWWISE(NB_)                                                                                 \
+        if (tb) k_dgemm_rows_lds<NB_, true><<<rgrid, 256, 0, c->stream>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc); \
+        else k_dgemm_rows_lds<NB_, false><<<rgrid, 256, 0, c->stream>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc)
+        if (N <= 1) { XR_ROWWISE(1); }
+        else if (N <= 2) { XR_ROWWISE(2); }
+        else if (N <= 4) { XR_ROWWISE(4); }
+        else { XR_ROWWISE(8); }
+#undef XR_ROWWISE
+        XR_LAUNCH_CHECK(c);
+        return XR_OK;
+    }
+    if (N <= 8 && !ta && batch == 1 && M >= 1024 && K > 64) {
         ProfScope prof(c, XR_K_GEMM_F64, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
         prof.shape(M, N, K, batch, ta, tb);
         unsigned grid = xr_grid(ceil_div(M, 4) * 64);
@@ -380,7 +568,8 @@ extern "C" int xr_gemm_tune(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int
     if (!c) return XR_EINVAL;
     std::lock_guard<std::mutex> g(c->mu);
     if (M <= 0 || N <= 0 || K <= 0 || batch <= 0 || reps <= 0) return xr_fail(c, XR_EINVAL, "bad tune shape");
-    if (N <= 8 && !ta && batch == 1 && M >= 1024 && K >= 1024) {   // the skinny kernel, not tiled
+    if ((N <= 8 && !ta && batch == 1 && M >= 1024) ||
+        (ta && !tb && batch == 1 && M <= 64 && N <= 8 && K >= 4096)) {   // skinny kernels, not tiled
         if (tile_out) *tile_out = -1;
         if (splits_out) *splits_out = 0;
         if (ms_out) *ms_out = 0.0;
