@@ -61,8 +61,10 @@ int xr_synchronize(xr_ctx *ctx);
 /* Kernel timing with HIP events on the context stream.  xr_profile(ctx, 1)
  * clears the records and starts bracketing every launch; xr_profile(ctx, 0)
  * stops.  xr_profile_stats sums the records of one kernel class
- * (0 FP64 GEMM, 1 gather, 2 expand, 3 excitation table, 4 block scatter,
- * 5 braket assembly, 6 other, 7 split-FP32 GEMM): launches, total ms, and the
+ * (0 FP64 MFMA tile GEMM, 1 gather, 2 expand, 3 excitation table, 4 block
+ * scatter / axis packing, 5 braket assembly, 6 other (incl. split-K
+ * reductions), 7 split-FP32 GEMM, 8 bandwidth-bound tall-skinny products):
+ * launches, total ms, and the
  * algorithmic flops and bytes of those launches.  Synchronises the stream.
  * No reference counterpart (the reference has no timers,
  * density_tensors.c:177-178). */

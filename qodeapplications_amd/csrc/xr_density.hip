@@ -547,58 +547,73 @@ __global__ void k_compress(const double* __restrict__ Rbar, int64_t nI, int64_t 
 // where t runs over the tuples the side string removes from S (annihilators)
 // or adds to it (creators), and K is their image in the K space.  Each
 // output is owned by one thread: deterministic, no atomics.
-__device__ __forceinline__ void k_side_adjoint_elem(int64_t i, SidePlan sd, const uint64_t* __restrict__ Sm, int64_t nS_cfg,
-                               int n_orbs, int n_core, int kspace_core, int64_t nK, int64_t nT,
-                               const double* __restrict__ Ybar, int64_t nS, int layout,
-                               double* __restrict__ zbarT) {
-    const int dim = 2 * n_orbs;
-    int64_t S = i / nS, s = i - (i / nS) * nS;
-    uint64_t src = Sm[S];
-    uint64_t full = (dim >= 64) ? ~0ull : ((1ull << dim) - 1);
-    uint64_t pool = sd.is_c ? (~src & full) : src;  // orbitals the string may touch
-    uint64_t kcore = d_core_mask(n_orbs, kspace_core);
-    int nK_val = dim - 2 * kspace_core;
-    double acc = 0.0;
-    int orbs[64];
-    int np = 0;
-    for (uint64_t x = pool; x; x &= x - 1) orbs[np++] = __ffsll((unsigned long long)x) - 1;
-    int idx[3] = {0, 1, 2};
-    const int k = sd.k;
-    if (np >= k) {
-        while (true) {
-            uint64_t tm = 0;
-            int orb[3];
-            for (int j = 0; j < k; j++) { orb[j] = orbs[idx[j]]; tm |= 1ull << orb[j]; }
-            uint64_t km = sd.is_c ? (src | tm) : (src & ~tm);
-            if ((km & kcore) == kcore) {
-                uint64_t cur = src;
-                int sg = 1;
-                for (int j = k - 1; j >= 0; j--) {
-                    int o = orb[sd.slot[j]];
-                    sg *= d_sign_above(cur, o);
-                    cur ^= 1ull << o;
-                }
-                int64_t K = d_lex_rank(d_to_valence(km, n_orbs, kspace_core), nK_val);
-                int64_t t = d_lex_rank(tm, dim);
-                double y = layout == 0 ? Ybar[(K * nT + t) * nS + s] : Ybar[(t * nK + K) * nS + s];
-                acc += sg * y;
-            }
-            // next combination of k indices out of np
-            int j = k - 1;
-            while (j >= 0 && idx[j] == np - k + j) j--;
-            if (j < 0) break;
-            idx[j]++;
-            for (int q = j + 1; q < k; q++) idx[q] = idx[q - 1] + 1;
-        }
+// One (tuple of K ascending orbitals o[0] < o[1] < o[2]) term of the side
+// adjoint: sign of applying the string (slot order) to src, K's rank, t's rank.
+template <int K>
+__device__ __forceinline__ double side_term(const SidePlan& sd, uint64_t src, const int* o, int n_orbs,
+                                            uint64_t kcore, int kspace_core, int nK_val, int dim,
+                                            int64_t nK, int64_t nT, const double* __restrict__ Ybar,
+                                            int64_t nS, int64_t s, int layout) {
+    uint64_t tm = 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) tm |= 1ull << o[j];
+    const uint64_t km = sd.is_c ? (src | tm) : (src & ~tm);
+    if ((km & kcore) != kcore) return 0.0;
+    uint64_t cur = src;
+    int sg = 1;
+#pragma unroll
+    for (int j = K - 1; j >= 0; j--) {
+        const int sl = sd.slot[j];
+        const int oo = (sl == 0) ? o[0] : ((sl == 1) ? o[1 < K ? 1 : 0] : o[2 < K ? 2 : 0]);
+        sg *= d_sign_above(cur, oo);
+        cur ^= 1ull << oo;
     }
-    zbarT[S * nS + s] = acc;
+    const int64_t Kr = d_lex_rank(d_to_valence(km, n_orbs, kspace_core), nK_val);
+    const int64_t t = d_lex_rank(tm, dim);
+    const double y = layout == 0 ? Ybar[(Kr * nT + t) * nS + s] : Ybar[(t * nK + Kr) * nS + s];
+    return sg * y;
 }
 
+// Each output (S, s) is owned by one thread (deterministic, no atomics); the
+// state index s is fastest, so the threads of a wave share S and walk the
+// same tuples in lockstep while reading consecutive s.  Tuples are
+// enumerated as nested ascending bit loops over the orbitals the string may
+// touch, in the lexicographic order of (o[0], o[1], o[2]).
+template <int K>
 __global__ void k_side_adjoint(SidePlan sd, const uint64_t* __restrict__ Sm, int64_t nS_cfg,
                                int n_orbs, int n_core, int kspace_core, int64_t nK, int64_t nT,
                                const double* __restrict__ Ybar, int64_t nS, int layout,
                                double* __restrict__ zbarT) {
-    XR_GRID_STRIDE(i, nS_cfg * nS) k_side_adjoint_elem(i, sd, Sm, nS_cfg, n_orbs, n_core, kspace_core, nK, nT, Ybar, nS, layout, zbarT);
+    XR_GRID_STRIDE(i, nS_cfg * nS) {
+        const int dim = 2 * n_orbs;
+        const int64_t S = i / nS, s = i - (i / nS) * nS;
+        const uint64_t src = Sm[S];
+        const uint64_t full = (dim >= 64) ? ~0ull : ((1ull << dim) - 1);
+        const uint64_t pool = sd.is_c ? (~src & full) : src;
+        const uint64_t kcore = d_core_mask(n_orbs, kspace_core);
+        const int nK_val = dim - 2 * kspace_core;
+        double acc = 0.0;
+        int o[3] = {0, 0, 0};
+        for (uint64_t p1 = pool; p1; p1 &= p1 - 1) {
+            o[0] = __ffsll((unsigned long long)p1) - 1;
+            if (K == 1) {
+                acc += side_term<K>(sd, src, o, n_orbs, kcore, kspace_core, nK_val, dim, nK, nT, Ybar, nS, s, layout);
+                continue;
+            }
+            for (uint64_t p2 = p1 & (p1 - 1); p2; p2 &= p2 - 1) {
+                o[1] = __ffsll((unsigned long long)p2) - 1;
+                if (K == 2) {
+                    acc += side_term<K>(sd, src, o, n_orbs, kcore, kspace_core, nK_val, dim, nK, nT, Ybar, nS, s, layout);
+                    continue;
+                }
+                for (uint64_t p3 = p2 & (p2 - 1); p3; p3 &= p3 - 1) {
+                    o[2] = __ffsll((unsigned long long)p3) - 1;
+                    acc += side_term<K>(sd, src, o, n_orbs, kcore, kspace_core, nK_val, dim, nK, nT, Ybar, nS, s, layout);
+                }
+            }
+        }
+        zbarT[S * nS + s] = acc;
+    }
 }
 
 // zbar[s * nC + c] += zbarT[c * nS + s]
@@ -624,13 +639,20 @@ static int side_adjoint(xr_ctx* c, const SidePlan& sd, int n_orbs, int n_core, i
     const uint64_t* Sm = nullptr;
     int64_t nS_cfg = 0;
     int rc;
+    if (sd.k < 1 || sd.k > 3) return xr_fail(c, XR_EINVAL, "side string of 1 to 3 operators expected");
     if ((rc = xr_space_masks(c, n_orbs, n_core, n_elec_src, &Sm, &nS_cfg))) return rc;
     int64_t n = nS_cfg * nS;
     {
         ProfScope prof(c, XR_K_GATHER, 0.0, 8.0 * n);
-        k_side_adjoint<<<xr_grid(n), 256, 0, c->stream>>>(sd, Sm, nS_cfg, n_orbs, n_core,
-                                                                 kspace_core, nK, nT, Ybar, nS,
-                                                                 layout, tmpT);
+        if (sd.k == 1)
+            k_side_adjoint<1><<<xr_grid(n), 256, 0, c->stream>>>(sd, Sm, nS_cfg, n_orbs, n_core, kspace_core,
+                                                                 nK, nT, Ybar, nS, layout, tmpT);
+        else if (sd.k == 2)
+            k_side_adjoint<2><<<xr_grid(n), 256, 0, c->stream>>>(sd, Sm, nS_cfg, n_orbs, n_core, kspace_core,
+                                                                 nK, nT, Ybar, nS, layout, tmpT);
+        else
+            k_side_adjoint<3><<<xr_grid(n), 256, 0, c->stream>>>(sd, Sm, nS_cfg, n_orbs, n_core, kspace_core,
+                                                                 nK, nT, Ybar, nS, layout, tmpT);
         XR_LAUNCH_CHECK(c);
     }
     dim3 grid((unsigned)ceil_div(nS_cfg, 32), (unsigned)ceil_div(nS, 32));

@@ -394,7 +394,7 @@ static int launch_tn_skinny(xr_ctx* c, int64_t M, int64_t N, int64_t K, double a
     nblk = ceil_div(K, rows);
     double* part = (double*)xr_workspace(c, 52, nblk * 64 * nb * sizeof(double));
     if (!part) return xr_fail(c, XR_ENOMEM, "workspace (Gram)");
-    ProfScope prof(c, XR_K_GEMM_F64, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
+    ProfScope prof(c, XR_K_GEMM_BW, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
     prof.shape(M, N, K, 1, 1, 0);
 #define XR_TN_SMALL(MB_)                                                                                  \
     if (nb == 1) k_dgemm_tn_small<MB_, 1><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part); \
@@ -456,19 +456,22 @@ static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t
         bw = 0.0;
     }
     dim3 grid((unsigned)tiles, (unsigned)splits, (unsigned)batch);
-    ProfScope prof(c, XR_K_GEMM_F64, 2.0 * M * N * K * batch, 8.0 * (M * K + K * N + M * N) * batch);
-    prof.shape(M, N, K, batch, ta, tb);
+    {
+        ProfScope prof(c, XR_K_GEMM_F64, 2.0 * M * N * K * batch, 8.0 * (M * K + K * N + M * N) * batch);
+        prof.shape(M, N, K, batch, ta, tb);
 #define XR_GEMM_LAUNCH(TA_, TB_)                                                                 \
     k_dgemm<BM, BN, TA_, TB_><<<grid, GEMM_THREADS, 0, c->stream>>>(M, N, K, aw, A, lda, sA, B,   \
                                                                      ldb, sB, bw, Cw, ldw, sCw,   \
                                                                      tm, tn, kchunk, sSplit)
-    if (ta && tb) XR_GEMM_LAUNCH(true, true);
-    else if (ta) XR_GEMM_LAUNCH(true, false);
-    else if (tb) XR_GEMM_LAUNCH(false, true);
-    else XR_GEMM_LAUNCH(false, false);
+        if (ta && tb) XR_GEMM_LAUNCH(true, true);
+        else if (ta) XR_GEMM_LAUNCH(true, false);
+        else if (tb) XR_GEMM_LAUNCH(false, true);
+        else XR_GEMM_LAUNCH(false, false);
 #undef XR_GEMM_LAUNCH
-    XR_LAUNCH_CHECK(c);
+        XR_LAUNCH_CHECK(c);
+    }
     if (staged) {
+        ProfScope prof(c, XR_K_OTHER, 0.0, 8.0 * (splits + 2) * batch * M * N);
         int64_t n = batch * M * N;
         k_splitk_reduce<<<xr_grid(n), 256, 0, c->stream>>>(Cw, splits, batch, M, N, alpha, beta, C, ldc,
                                                             sC);
@@ -501,7 +504,7 @@ int xr_dgemm_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, 
     if (ta && !tb && batch == 1 && M <= 64 && N <= 8 && K >= 4096)
         return launch_tn_skinny(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
     if (N <= 8 && !ta && batch == 1 && M >= 1024 && K <= 64) {
-        ProfScope prof(c, XR_K_GEMM_F64, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
+        ProfScope prof(c, XR_K_GEMM_BW, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
         prof.shape(M, N, K, batch, ta, tb);
         const unsigned rgrid = (unsigned)(ceil_div(M, 64) < (1 << 20) ? ceil_div(M, 64) : (1 << 20));
 #define XR_ROWWISE(NB_)                                                                                 \
@@ -516,7 +519,7 @@ int xr_dgemm_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, 
         return XR_OK;
     }
     if (N <= 8 && !ta && batch == 1 && M >= 1024 && K > 64) {
-        ProfScope prof(c, XR_K_GEMM_F64, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
+        ProfScope prof(c, XR_K_GEMM_BW, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
         prof.shape(M, N, K, batch, ta, tb);
         unsigned grid = xr_grid(ceil_div(M, 4) * 64);
 #define XR_SKINNY(NB_)                                                                                  \

@@ -66,7 +66,8 @@ def autotune(fn, device=0, reps=3, min_ms=0.05):
         if 8 * (M * K + K * N + M * N) * batch > (16 << 30):
             continue
         tile, splits, best = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-        ctx.call("xr_gemm_tune", ta, tb, M, N, K, batch, reps, ctypes.byref(tile), ctypes.byref(splits),
+        r = max(1, min(reps, int(5.0 / max(ms / n, 1e-3))))     # fewer repeats for long launches
+        ctx.call("xr_gemm_tune", ta, tb, M, N, K, batch, r, ctypes.byref(tile), ctypes.byref(splits),
                  ctypes.byref(best))
         if tile.value >= 0:
             out[key] = (tile.value, splits.value, best.value, ms / n)

@@ -122,6 +122,22 @@ class XROperator:
         self._pack([c.size, cnt], 1, 1, -1, lr, r.size, tmp, out, 0)    # out[x, y] = blk[lr[x], lc[y]]
         return out
 
+    def _monomer_eig(self, h1):
+        """Eigenpairs of every (fragment, charge) monomer block, computed once
+        per operator on the host (blocks of at most a few hundred states; a
+        device solver would launch thousands of tiny kernels)."""
+        if getattr(self, "_eig", None) is None:
+            self._eig = []
+            for blocks in h1:
+                d = {}
+                for c, h in blocks.items():
+                    hh = h.cpu().numpy()
+                    w, U = np.linalg.eigh(0.5 * (hh + hh.T))
+                    d[c] = (torch.as_tensor(w, device=self.device),
+                            torch.as_tensor(np.ascontiguousarray(U), device=self.device))
+                self._eig.append(d)
+        return self._eig
+
     def _pack(self, dims, ncol, f0, f1, rowmap, nrows, full, packed, mode, alpha=1.0, ld=0):
         import ctypes
         d = (ctypes.c_int64 * len(dims))(*[int(x) for x in dims])
@@ -297,13 +313,7 @@ class SectorSpace:
         # monomer blocks per (fragment, charge) and their eigenbases
         self.h1 = [{c: op.H1[m][st:st + n, st:st + n].contiguous() for c, (st, n) in self.blocks_of[m].items()}
                    for m in range(nf)]
-        self.eig = []
-        for m in range(nf):
-            d = {}
-            for c, h in self.h1[m].items():
-                w, U = torch.linalg.eigh(0.5 * (h + h.T))
-                d[c] = (w, U.contiguous())
-            self.eig.append(d)
+        self.eig = op._monomer_eig(self.h1)
         # pair transitions: stacked target blocks per (pair, source charges),
         # shared by the sectors of every total charge
         self.stack = op._stacks
