@@ -114,14 +114,17 @@ def _orthonormalize(T, V=None, passes=2):
 
 
 def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=None,
-             max_space=None, precond=None, start=None, keep=8, olsen=False, strict=True):
+             max_space=None, precond=None, start=None, keep=8, olsen=False, strict=True,
+             state=None):
     """Block Davidson.  matmul(X, Y) writes H X into Y for X of shape (n, b);
     precond(R, theta) approximates (H - theta)^-1 R (default: diagonal).
     Converged when every residual norm is below tol * max(1, |theta|), or
     when it has stalled at the rounding floor within 100x of that.  A full
     subspace restarts on its `keep` lowest Ritz vectors.  olsen: apply
     Olsen's correction to the preconditioned residual.  strict=False returns
-    the current Ritz pairs after max_iter instead of raising.
+    the current Ritz pairs after max_iter instead of raising.  state: a dict
+    that receives the subspace on return and, when it already holds one (same
+    n and block), continues from it instead of from the start vectors.
     The subspace lives in preallocated device buffers; the Rayleigh-Ritz
     eigenproblem and the b x b factorisations are solved on the host."""
     b = block or max(2 * k, 4)
@@ -137,15 +140,24 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         idx = torch.argsort(diag)[:b]
         V0 = torch.zeros(n, b, dtype=dt, device=device)
         V0[idx, torch.arange(b, device=device)] = 1.0
-    Vbuf = torch.empty(n, max_space + b, dtype=dt, device=device)
-    AVbuf = torch.empty(n, max_space + b, dtype=dt, device=device)
-    V0 = _orthonormalize(V0.contiguous())
-    m = V0.shape[1]
-    Vbuf[:, :m] = V0
-    AX0 = torch.empty(n, m, dtype=dt, device=device)
-    matmul(V0, AX0)
-    AVbuf[:, :m] = AX0
-    S = mm(V0, AX0, transA=1).cpu().numpy()
+    if state is not None and state.get("b") == b and state.get("n") == n:
+        Vbuf, AVbuf, m, S = state["V"], state["AV"], state["m"], state["S"]
+        max_space = Vbuf.shape[1] - b
+    else:
+        Vbuf = torch.empty(n, max_space + b, dtype=dt, device=device)
+        AVbuf = torch.empty(n, max_space + b, dtype=dt, device=device)
+        V0 = _orthonormalize(V0.contiguous())
+        m = V0.shape[1]
+        Vbuf[:, :m] = V0
+        AX0 = torch.empty(n, m, dtype=dt, device=device)
+        matmul(V0, AX0)
+        AVbuf[:, :m] = AX0
+        S = mm(V0, AX0, transA=1).cpu().numpy()
+
+    def keep_state():
+        if state is not None:
+            state.update(V=Vbuf, AV=AVbuf, m=m, S=S, b=b, n=n)
+
     hist = []
     for it in range(max_iter):
         V, AV = Vbuf[:, :m], AVbuf[:, :m]
@@ -160,11 +172,13 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         rn = torch.linalg.norm(R[:, :k], dim=0).cpu().numpy()
         target = tol * np.maximum(np.abs(w[:k]), 1.0)
         if np.all(rn <= target):
+            keep_state()
             return theta[:k].clone(), X[:, :k].contiguous()
         # rounding floor: the residual is within 100x of the target and has
         # stopped falling over the last 10 iterations
         hist.append(float(np.max(rn / target)))
         if len(hist) > 20 and hist[-1] < 100.0 and min(hist[-10:]) > 0.5 * min(hist[:-10]):
+            keep_state()
             return theta[:k].clone(), X[:, :k].contiguous()
         if precond is not None:
             T = precond(R, theta)
@@ -189,6 +203,7 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         Q = _orthonormalize(T.contiguous(), V)
         q = Q.shape[1]
         if q == 0:
+            keep_state()
             return theta[:k].clone(), X[:, :k].contiguous()
         AQ = torch.empty(n, q, dtype=dt, device=device)
         matmul(Q, AQ)
@@ -200,6 +215,7 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         AVbuf[:, m:m + q] = AQ
         m += q
     if not strict:
+        keep_state()
         return theta[:k].clone(), X[:, :k].contiguous()
     raise RuntimeError(f"Davidson did not converge in {max_iter} iterations "
                        f"(residual {float(rn.max()):.2e})")

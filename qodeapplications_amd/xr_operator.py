@@ -31,6 +31,13 @@ from .gemm import gemm_f64, mm
 DENSE_SECTOR = 2048
 
 
+def _block_size(k, size):
+    """Davidson block: small sectors are bound by the per-iteration host work,
+    so they take 4 new directions per root and iteration; large ones are bound
+    by the products and take one."""
+    return k * (4 if size <= (1 << 20) else 1)
+
+
 def _axis_apply(A, src, dst, pre, n, post, transA=0, beta=0.0):
     """dst[p, :, q] = op(A) src[p, :, q] (+ beta dst) for row-major (pre, n,
     post) views: one GEMM src @ op(A)^T when post == 1, else one batched GEMM."""
@@ -207,9 +214,12 @@ class XROperator:
                 w, v = torch.linalg.eigh(0.5 * (M + M.T))
                 cand.append([sp, w[:kk], v[:, :kk].contiguous(), 0.0, True])
             else:
-                b = block or kk
+                b = block or _block_size(kk, sp.size)
+                st = {} if sp.size <= (1 << 20) else None     # keep small subspaces for the refinement
                 w, v = davidson(sp.matvec, None, sp.size, kk, max(tol, screen_tol), screen_iter, b,
-                                self.device, precond=sp.precondition, start=sp.guess(b), strict=False)
+                                self.device, precond=sp.precondition, start=sp.guess(b), strict=False,
+                                state=st)
+                sp.dav_state = st
                 # residual norm of the screened roots: an eigenvalue lies within it
                 r = torch.empty_like(v)
                 sp.matvec(v, r)
@@ -227,10 +237,11 @@ class XROperator:
             if float(w[0]) - margin > bound:
                 continue
             if not exact and tol < screen_tol:
-                b = block or v.shape[1]
+                b = block or _block_size(v.shape[1], sp.size)
                 start = v if v.shape[1] >= b else torch.cat([v, sp.guess(b)[:, :b - v.shape[1]]], 1)
                 w, v = davidson(sp.matvec, None, sp.size, v.shape[1], tol, max_iter, b, self.device,
-                                precond=sp.precondition, start=start.contiguous())
+                                precond=sp.precondition, start=start.contiguous(),
+                                state=getattr(sp, "dav_state", None))
             for i in range(w.shape[0]):
                 found.append((float(w[i]), sp, v[:, i]))
         found.sort(key=lambda t: t[0])
