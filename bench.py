@@ -13,7 +13,6 @@ build targets one GPU.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -68,7 +67,7 @@ def cpu_baseline(w, budget_s):
     H_contractions.c, one thread) on bounded samples of the same workload
     and scales by the counted work of one iteration."""
     from oracle import oracle as orc
-    from qodeapplications_amd.density import OPS_BY_CHG_DIFF, n_elec_change
+    from qodeapplications_amd.density import OPS_BY_CHG_DIFF
     f = w.fragments[0]
     n_orbs = f.n_orbs
     dim = 2 * n_orbs

@@ -4,7 +4,6 @@ Thin host layer over libxrhip: coefficients and determinant masks live in
 HBM as torch tensors (the container), every density, excitation list and
 monomer energy is computed by the HIP kernels behind the C ABI.
 """
-import math
 
 import numpy as np
 import torch

@@ -1,5 +1,4 @@
 """Drop-in for the XR-FCI part of general-XRCC/excitonic.py (fci, :56-82)."""
-import numpy as np
 import torch
 
 from .eigen import lowest_eigenpairs

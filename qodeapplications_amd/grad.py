@@ -27,10 +27,8 @@ inputs are the coefficient tensors and the integral blocks; orbital-rotation
 gradients follow by ordinary autograd through the integral transformation
 (orbitals.py).
 """
-import numpy as np
 import torch
 
-from . import blocks as B
 from ._lib import context, ptr
 from .eigen import DENSE_MAX
 from .gemm import mm as _mm
@@ -193,7 +191,6 @@ class XRAdjoint:
         nA, nB = nI1 * nJ1, nI2 * nJ2
         d = i1c - j1c
         sign2 = float((-1) ** ((F2.n_elec_ref - i2c) % 2))
-        ctx = context(X.device.index)
 
         def G(swapped):
             return self.op.pair_gather(Gb, m1, m2, (F1.pos, F2.pos), k1, k2, swapped)

@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from ._lib import context, ptr
-from .eigen import KroneckerSumPreconditioner, davidson
+from .eigen import davidson
 from .gemm import gemm_f64, mm
 
 
