@@ -29,6 +29,7 @@ from .gemm import gemm_f64, mm
 
 
 DENSE_SECTOR = 2048
+KEEP_BYTES = 48 << 30         # device memory for screened Davidson subspaces
 
 
 def _block_size(k, size):
@@ -205,6 +206,7 @@ class XROperator:
         twice their residual norms) are then refined to tol from their
         screened vectors."""
         cand = []
+        kept = 0                      # bytes of screened subspaces kept for the refinement
         for Q in (charges if charges is not None else self.total_charges()):
             sp = self.sector_space(Q)
             kk = min(k, sp.size)
@@ -215,7 +217,9 @@ class XROperator:
                 cand.append([sp, w[:kk], v[:, :kk].contiguous(), 0.0, True])
             else:
                 b = block or _block_size(kk, sp.size)
-                st = {} if sp.size <= (1 << 20) else None     # keep small subspaces for the refinement
+                need = 16 * sp.size * (max(10 * b, 48) + b)
+                st = {} if kept + need <= KEEP_BYTES else None     # subspace kept for the refinement
+                kept += need if st is not None else 0
                 w, v = davidson(sp.matvec, None, sp.size, kk, max(tol, screen_tol), screen_iter, b,
                                 self.device, precond=sp.precondition, start=sp.guess(b), strict=False,
                                 state=st)
@@ -231,6 +235,9 @@ class XROperator:
         # value minus its residual margin lies above it cannot contribute
         ups = sorted(float(x) for c in cand for x in c[1])
         bound = ups[min(k, len(ups)) - 1]
+        for c in cand:                # free the subspaces of sectors that cannot contribute
+            if float(c[1][0]) - c[3] > bound:
+                c[0].dav_state = None
         found = []
         for c in cand:
             sp, w, v, margin, exact = c
