@@ -88,9 +88,15 @@ def lowest_eigenpairs(H, k=1, tol=1e-9, max_iter=400, H1=None, block=None):
 def _orthonormalize(T, V=None, passes=2):
     """Columns of T made orthonormal (and orthogonal to V's columns) by
     projected Cholesky QR; the tiny b x b factorisations run on the host.
-    Returns the kept columns (rank-revealing through the Cholesky pivots)."""
-    for _ in range(passes):
+    The projection is repeated only when it removed more than 3/4 of a
+    column's squared norm (the Daniel-Gragg-Kaufman-Stewart test), since
+    each pass reads all of V.  Returns the kept columns (rank-revealing
+    through the Cholesky pivots)."""
+    n0 = None
+    for p in range(passes):
         if V is not None and V.shape[1]:
+            if p == 0:
+                n0 = np.diag(mm(T, T, transA=1).cpu().numpy()).copy()
             mm(V, mm(V, T, transA=1), alpha=-1.0, out=T, beta=1.0)
         G = mm(T, T, transA=1).cpu().numpy()
         d = np.sqrt(np.maximum(np.diag(G), 0.0))
@@ -100,6 +106,8 @@ def _orthonormalize(T, V=None, passes=2):
         if not keep.all():
             T = T[:, torch.as_tensor(np.nonzero(keep)[0], device=T.device)].contiguous()
             G = G[np.ix_(keep, keep)]
+            if n0 is not None:
+                n0 = n0[keep]
         try:
             L = np.linalg.cholesky(G)
         except np.linalg.LinAlgError:
@@ -107,9 +115,12 @@ def _orthonormalize(T, V=None, passes=2):
             good = w > 1e-14 * w.max()
             Minv = U[:, good] / np.sqrt(w[good])
             T = mm(T, torch.as_tensor(np.ascontiguousarray(Minv), device=T.device))
+            n0 = None
             continue
         Linv = np.ascontiguousarray(np.linalg.inv(L).T)
         T = mm(T, torch.as_tensor(Linv, device=T.device))
+        if n0 is not None and np.all(np.diag(G) > 0.25 * n0):
+            break
     return T.contiguous()
 
 
