@@ -384,6 +384,89 @@ __global__ __launch_bounds__(256) void k_dgemm_rows_lds(int64_t M, int64_t N, in
     }
 }
 
+// C = alpha A^T op(B) + beta C for A stored K x M (row-major, lda) with
+// K <= 64 and M large, N <= 8: the product of a block of vectors stored one
+// per row with a small coefficient matrix.  One thread per output row reads
+// column r of A, so consecutive threads read consecutive addresses.
+template <int NB, bool TB>
+__global__ __launch_bounds__(256) void k_dgemm_tn_tall(int64_t M, int64_t N, int64_t K, double alpha,
+                                                       const double* __restrict__ A, int64_t lda,
+                                                       const double* __restrict__ B, int64_t ldb,
+                                                       double beta, double* __restrict__ C, int64_t ldc) {
+    __shared__ double bs[64 * NB];
+    for (int e = threadIdx.x; e < (int)K * NB; e += 256) {
+        const int k = e / NB, j = e - (e / NB) * NB;
+        bs[e] = (j < N) ? (TB ? B[(int64_t)j * ldb + k] : B[(int64_t)k * ldb + j]) : 0.0;
+    }
+    __syncthreads();
+    XR_GRID_STRIDE(r, M) {
+        double acc[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++) acc[j] = 0.0;
+        for (int k = 0; k < (int)K; k++) {
+            const double a = A[(int64_t)k * lda + r];
+#pragma unroll
+            for (int j = 0; j < NB; j++) acc[j] = fma(a, bs[k * NB + j], acc[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            if (j < N) {
+                double v = alpha * acc[j];
+                if (beta != 0.0) v += beta * C[r * ldc + j];
+                C[r * ldc + j] = v;
+            }
+        }
+    }
+}
+
+// C = alpha A B + beta C for A (M x K row-major) with M <= 64 / NB, N <= NB
+// and K large (Gram products of vectors stored one per row).  Each block sums
+// a contiguous run of k for all M x N entries (consecutive threads read
+// consecutive k of a row of A) into a partial tile; k_tn_reduce adds the
+// partials in block order.
+template <int NB>
+__global__ __launch_bounds__(256) void k_dgemm_nn_gram(int64_t M, int64_t N, int64_t K,
+                                                       const double* __restrict__ A, int64_t lda,
+                                                       const double* __restrict__ B, int64_t ldb,
+                                                       int64_t rows, double* __restrict__ part) {
+    constexpr int MB = 64 / NB;
+    const int64_t k0 = (int64_t)blockIdx.x * rows;
+    const int64_t k1 = (K - k0 < rows) ? K : k0 + rows;
+    double acc[MB][NB];
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++) acc[i][j] = 0.0;
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += 256) {
+        double b[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++) b[j] = (j < N) ? B[k * ldb + j] : 0.0;
+#pragma unroll
+        for (int i = 0; i < MB; i++) {
+            if (i < M) {
+                const double a = A[(int64_t)i * lda + k];
+#pragma unroll
+                for (int j = 0; j < NB; j++) acc[i][j] = fma(a, b[j], acc[i][j]);
+            }
+        }
+    }
+    __shared__ double red[4][MB * NB];
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            double v = acc[i][j];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][i * NB + j] = v;
+        }
+    __syncthreads();
+    for (int e = threadIdx.x; e < MB * NB; e += 256) {
+        const int i = e / NB, j = e - (e / NB) * NB;
+        if (i < M)
+            part[((int64_t)blockIdx.x * 64 + i) * NB + j] = red[0][e] + red[1][e] + red[2][e] + red[3][e];
+    }
+}
+
 static int launch_tn_skinny(xr_ctx* c, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
                             int64_t lda, const double* B, int64_t ldb, double beta, double* C,
                             int64_t ldc) {
@@ -410,6 +493,27 @@ static int launch_tn_skinny(xr_ctx* c, int64_t M, int64_t N, int64_t K, double a
     else if (nb == 4) k_dgemm_tn_skinny<4><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
     else k_dgemm_tn_skinny<8><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
 #undef XR_TN_SMALL
+    XR_LAUNCH_CHECK(c);
+    k_tn_reduce<<<(unsigned)ceil_div(M * N, 256), 256, 0, c->stream>>>(part, nblk, M, N, nb, alpha, beta, C, ldc);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+static int launch_nn_gram(xr_ctx* c, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
+                          int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
+    const int nb = N <= 1 ? 1 : (N <= 2 ? 2 : (N <= 4 ? 4 : 8));
+    int64_t nblk = ceil_div(K, 4096);
+    if (nblk > 1024) nblk = 1024;
+    const int64_t rows = ceil_div(K, nblk);
+    nblk = ceil_div(K, rows);
+    double* part = (double*)xr_workspace(c, 52, nblk * 64 * nb * sizeof(double));
+    if (!part) return xr_fail(c, XR_ENOMEM, "workspace (Gram)");
+    ProfScope prof(c, XR_K_GEMM_BW, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
+    prof.shape(M, N, K, 1, 0, 0);
+    if (nb == 1) k_dgemm_nn_gram<1><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
+    else if (nb == 2) k_dgemm_nn_gram<2><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
+    else if (nb == 4) k_dgemm_nn_gram<4><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
+    else k_dgemm_nn_gram<8><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
     XR_LAUNCH_CHECK(c);
     k_tn_reduce<<<(unsigned)ceil_div(M * N, 256), 256, 0, c->stream>>>(part, nblk, M, N, nb, alpha, beta, C, ldc);
     XR_LAUNCH_CHECK(c);
@@ -503,6 +607,22 @@ int xr_dgemm_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, 
     if (M <= 0 || N <= 0 || batch <= 0) return XR_OK;
     if (ta && !tb && batch == 1 && M <= 64 && N <= 8 && K >= 4096)
         return launch_tn_skinny(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (ta && batch == 1 && M >= 1024 && K <= 64 && N <= 8) {
+        ProfScope prof(c, XR_K_GEMM_BW, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
+        prof.shape(M, N, K, batch, ta, tb);
+#define XR_TN_TALL(NB_)                                                                                 \
+        if (tb) k_dgemm_tn_tall<NB_, true><<<xr_grid(M), 256, 0, c->stream>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc); \
+        else k_dgemm_tn_tall<NB_, false><<<xr_grid(M), 256, 0, c->stream>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc)
+        if (N <= 1) { XR_TN_TALL(1); }
+        else if (N <= 2) { XR_TN_TALL(2); }
+        else if (N <= 4) { XR_TN_TALL(4); }
+        else { XR_TN_TALL(8); }
+#undef XR_TN_TALL
+        XR_LAUNCH_CHECK(c);
+        return XR_OK;
+    }
+    if (!ta && !tb && batch == 1 && N <= 8 && K >= 4096 && M * (N <= 1 ? 1 : (N <= 2 ? 2 : (N <= 4 ? 4 : 8))) <= 64)
+        return launch_nn_gram(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
     if (N <= 8 && !ta && batch == 1 && M >= 1024 && K <= 64) {
         ProfScope prof(c, XR_K_GEMM_BW, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
         prof.shape(M, N, K, batch, ta, tb);
@@ -572,7 +692,10 @@ extern "C" int xr_gemm_tune(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int
     std::lock_guard<std::mutex> g(c->mu);
     if (M <= 0 || N <= 0 || K <= 0 || batch <= 0 || reps <= 0) return xr_fail(c, XR_EINVAL, "bad tune shape");
     if ((N <= 8 && !ta && batch == 1 && M >= 1024) ||
-        (ta && !tb && batch == 1 && M <= 64 && N <= 8 && K >= 4096)) {   // skinny kernels, not tiled
+        (ta && !tb && batch == 1 && M <= 64 && N <= 8 && K >= 4096) ||
+        (ta && batch == 1 && M >= 1024 && K <= 64 && N <= 8) ||
+        (!ta && !tb && batch == 1 && N <= 8 && K >= 4096 &&
+         M * (N <= 1 ? 1 : (N <= 2 ? 2 : (N <= 4 ? 4 : 8))) <= 64)) {   // bandwidth kernels, not tiled
         if (tile_out) *tile_out = -1;
         if (splits_out) *splits_out = 0;
         if (ms_out) *ms_out = 0.0;

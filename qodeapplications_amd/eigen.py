@@ -85,19 +85,23 @@ def lowest_eigenpairs(H, k=1, tol=1e-9, max_iter=400, H1=None, block=None):
                     max_iter, block, H.device, precond=pre)
 
 
-def _orthonormalize(T, V=None, passes=2):
+def _orthonormalize(T, V=None, passes=2, Vt=None):
     """Columns of T made orthonormal (and orthogonal to V's columns) by
     projected Cholesky QR; the tiny b x b factorisations run on the host.
     The projection is repeated only when it removed more than 3/4 of a
     column's squared norm (the Daniel-Gragg-Kaufman-Stewart test), since
-    each pass reads all of V.  Returns the kept columns (rank-revealing
-    through the Cholesky pivots)."""
+    each pass reads all of V.  Vt: the basis stored one vector per row
+    instead of V.  Returns the kept columns (rank-revealing through the
+    Cholesky pivots)."""
     n0 = None
     for p in range(passes):
-        if V is not None and V.shape[1]:
+        if (V is not None and V.shape[1]) or (Vt is not None and Vt.shape[0]):
             if p == 0:
                 n0 = np.diag(mm(T, T, transA=1).cpu().numpy()).copy()
-            mm(V, mm(V, T, transA=1), alpha=-1.0, out=T, beta=1.0)
+            if Vt is not None:          # basis stored one vector per row
+                mm(Vt, mm(Vt, T), transA=1, alpha=-1.0, out=T, beta=1.0)
+            else:
+                mm(V, mm(V, T, transA=1), alpha=-1.0, out=T, beta=1.0)
         G = mm(T, T, transA=1).cpu().numpy()
         d = np.sqrt(np.maximum(np.diag(G), 0.0))
         keep = d > 1e-10 * max(float(d.max()), 1e-300)
@@ -152,33 +156,34 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         V0 = torch.zeros(n, b, dtype=dt, device=device)
         V0[idx, torch.arange(b, device=device)] = 1.0
     if state is not None and state.get("b") == b and state.get("n") == n:
-        Vbuf, AVbuf, m, S = state["V"], state["AV"], state["m"], state["S"]
-        max_space = Vbuf.shape[1] - b
+        Vt, AVt, m, S = state["Vt"], state["AVt"], state["m"], state["S"]
+        max_space = Vt.shape[0] - b
     else:
-        Vbuf = torch.empty(n, max_space + b, dtype=dt, device=device)
-        AVbuf = torch.empty(n, max_space + b, dtype=dt, device=device)
+        # basis and its image stored one vector per row: the tall-skinny
+        # products then read contiguous vectors
+        Vt = torch.empty(max_space + b, n, dtype=dt, device=device)
+        AVt = torch.empty(max_space + b, n, dtype=dt, device=device)
         V0 = _orthonormalize(V0.contiguous())
         m = V0.shape[1]
-        Vbuf[:, :m] = V0
+        Vt[:m] = V0.T
         AX0 = torch.empty(n, m, dtype=dt, device=device)
         matmul(V0, AX0)
-        AVbuf[:, :m] = AX0
+        AVt[:m] = AX0.T
         S = mm(V0, AX0, transA=1).cpu().numpy()
 
     def keep_state():
         if state is not None:
-            state.update(V=Vbuf, AV=AVbuf, m=m, S=S, b=b, n=n)
+            state.update(Vt=Vt, AVt=AVt, m=m, S=S, b=b, n=n)
 
     hist = []
     for it in range(max_iter):
-        V, AV = Vbuf[:, :m], AVbuf[:, :m]
         S = 0.5 * (S + S.T)
         w, y = np.linalg.eigh(S)
         nb = min(b, m)
         yb = torch.as_tensor(np.ascontiguousarray(y[:, :nb]), device=device)
         theta = torch.as_tensor(w[:nb], device=device)
-        X = mm(V, yb)
-        AX = mm(AV, yb)
+        X = mm(Vt[:m], yb, transA=1)
+        AX = mm(AVt[:m], yb, transA=1)
         R = AX - X * theta
         rn = torch.linalg.norm(R[:, :k], dim=0).cpu().numpy()
         target = tol * np.maximum(np.abs(w[:k]), 1.0)
@@ -205,13 +210,12 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         if m + nb > max_space:              # thick restart on the lowest Ritz vectors
             nk = min(m, max(2 * nb, keep))
             yk = torch.as_tensor(np.ascontiguousarray(y[:, :nk]), device=device)
-            Xk, AXk = mm(V, yk), mm(AV, yk)
-            Vbuf[:, :nk] = Xk
-            AVbuf[:, :nk] = AXk
+            Xk, AXk = mm(Vt[:m], yk, transA=1), mm(AVt[:m], yk, transA=1)
+            Vt[:nk] = Xk.T
+            AVt[:nk] = AXk.T
             S = np.diag(w[:nk])
             m = nk
-            V = Vbuf[:, :m]
-        Q = _orthonormalize(T.contiguous(), V)
+        Q = _orthonormalize(T.contiguous(), Vt=Vt[:m])
         q = Q.shape[1]
         if q == 0:
             keep_state()
@@ -219,11 +223,11 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         AQ = torch.empty(n, q, dtype=dt, device=device)
         matmul(Q, AQ)
         # grow the projected matrix by the new rows / columns
-        VtAQ = mm(Vbuf[:, :m], AQ, transA=1).cpu().numpy()
+        VtAQ = mm(Vt[:m], AQ).cpu().numpy()
         QtAQ = mm(Q, AQ, transA=1).cpu().numpy()
         S = np.block([[S, VtAQ], [VtAQ.T, QtAQ]])
-        Vbuf[:, m:m + q] = Q
-        AVbuf[:, m:m + q] = AQ
+        Vt[m:m + q] = Q.T
+        AVt[m:m + q] = AQ.T
         m += q
     if not strict:
         keep_state()
