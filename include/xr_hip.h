@@ -277,6 +277,16 @@ int xr_axes_pack(xr_ctx *ctx, int nf, const int64_t *dims, int64_t ncol, int fro
                  const int32_t *d_rowmap, int64_t nrows, double *d_full, double *d_packed,
                  int64_t ld_packed, int mode, double alpha);
 
+/* n_items packs (mode 0) or unpacks (mode 1) of xr_axes_pack in one launch.
+ * Item q: nf[q] axes with dims[q*8 .. q*8+nf[q]-1], ncol[q], front0[q],
+ * front1[q], d_rowmaps ? d_rowmaps[q] : identity, nrows[q], d_full[q],
+ * d_packed[q], ld_packed[q].  The items of one call must write disjoint
+ * memory. */
+int xr_axes_pack_multi(xr_ctx *ctx, int n_items, const int *nf, const int64_t *dims,
+                       const int64_t *ncol, const int *front0, const int *front1,
+                       const int32_t *const *d_rowmaps, const int64_t *nrows, double *const *d_full,
+                       double *const *d_packed, const int64_t *ld_packed, int mode, double alpha);
+
 /* Pair-block adjoint input (the pair-outer layout of xr_pair_outer) read from
  * a reduced pair matrix G2 = sum over the other fragments of v v^T, whose
  * rows and columns run over the pair index a*d2 + b, reordered through

@@ -44,6 +44,7 @@ SIGNATURES = {
     "xr_gemm_tune": (_I, [_P, _I, _I, _L, _L, _L, _L, _I, _P, _P, _P]),
     "xr_gemm_tune_reset": (_I, [_P]),
     "xr_axes_pack": (_I, [_P, _I, _P, _L, _I, _I, _P, _L, _P, _P, _L, _I, _D]),
+    "xr_axes_pack_multi": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _D]),
     "xr_pair_gather": (_I, [_P, _P, _L, _L, _L, _P, _P, _P, _P, _P, _L, _L, _L, _L, _I, _P]),
     "xr_gemm_f64": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
     "xr_gemm_split_f32": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),

@@ -276,14 +276,14 @@ struct AxesDesc {
     int64_t ncol, xsize, nrows;
 };
 
-__global__ void k_axes_pack(AxesDesc ad, const int32_t* __restrict__ rowmap, double* __restrict__ full,
-                            double* __restrict__ packed, int64_t ldp, int mode, double alpha) {
-    const int64_t total = ad.nrows * ad.xsize;
-    XR_GRID_STRIDE(i, total) {
+__device__ __forceinline__ void axes_pack_elem(const AxesDesc& ad, const int32_t* __restrict__ rowmap,
+                                               double* __restrict__ full, double* __restrict__ packed,
+                                               int64_t ldp, int mode, double alpha, int64_t i) {
+    {
         const int64_t r = i / ad.xsize, x = i - r * ad.xsize;
         const int64_t pi = r * ldp + x;
         const int64_t f = rowmap ? (int64_t)rowmap[r] : r;
-        if (f < 0 || f >= ad.nfront) continue;
+        if (f < 0 || f >= ad.nfront) return;
         const int64_t a = f / ad.d1, b = f - a * ad.d1;
         int64_t off = a * ad.s0 + b * ad.s1;
         int64_t y = x / ad.ncol;
@@ -298,15 +298,18 @@ __global__ void k_axes_pack(AxesDesc ad, const int32_t* __restrict__ rowmap, dou
     }
 }
 
-extern "C" int xr_axes_pack(xr_ctx* c, int nf, const int64_t* dims, int64_t ncol, int front0, int front1,
-                            const int32_t* d_rowmap, int64_t nrows, double* d_full, double* d_packed,
-                            int64_t ld_packed, int mode, double alpha) {
-    XR_ENTRY(c);
+__global__ void k_axes_pack(AxesDesc ad, const int32_t* __restrict__ rowmap, double* __restrict__ full,
+                            double* __restrict__ packed, int64_t ldp, int mode, double alpha) {
+    const int64_t total = ad.nrows * ad.xsize;
+    XR_GRID_STRIDE(i, total) axes_pack_elem(ad, rowmap, full, packed, ldp, mode, alpha, i);
+}
+
+static int axes_desc(xr_ctx* c, int nf, const int64_t* dims, int64_t ncol, int front0, int front1,
+                     const int32_t* d_rowmap, int64_t nrows, AxesDesc* ad) {
     if (nf < 1 || nf > XR_MAX_FRAG_AX) return xr_fail(c, XR_EUNSUPPORTED, "1 to 8 fragment axes");
     if (front0 < 0 || front0 >= nf || front1 >= nf || front1 == front0 || (front1 >= 0 && front1 < front0))
         return xr_fail(c, XR_EINVAL, "front axes must satisfy 0 <= front0 < front1 < nf (front1 = -1 for one)");
-    if (ncol <= 0 || (mode != 0 && mode != 1)) return xr_fail(c, XR_EINVAL, "bad ncol or mode");
-    AxesDesc ad;
+    if (ncol <= 0) return xr_fail(c, XR_EINVAL, "bad ncol");
     int64_t stride[XR_MAX_FRAG_AX];
     int64_t acc = ncol;
     for (int m = nf - 1; m >= 0; m--) {
@@ -314,22 +317,33 @@ extern "C" int xr_axes_pack(xr_ctx* c, int nf, const int64_t* dims, int64_t ncol
         stride[m] = acc;
         acc *= dims[m];
     }
-    ad.s0 = stride[front0];
-    ad.s1 = front1 >= 0 ? stride[front1] : 0;
-    ad.d1 = front1 >= 0 ? dims[front1] : 1;
-    ad.nfront = dims[front0] * ad.d1;
-    ad.nrest = 0;
-    ad.xsize = ncol;
+    ad->s0 = stride[front0];
+    ad->s1 = front1 >= 0 ? stride[front1] : 0;
+    ad->d1 = front1 >= 0 ? dims[front1] : 1;
+    ad->nfront = dims[front0] * ad->d1;
+    ad->nrest = 0;
+    ad->xsize = ncol;
     for (int m = 0; m < nf; m++) {
         if (m == front0 || m == front1) continue;
-        ad.rest_dim[ad.nrest] = dims[m];
-        ad.rest_stride[ad.nrest] = stride[m];
-        ad.nrest++;
-        ad.xsize *= dims[m];
+        ad->rest_dim[ad->nrest] = dims[m];
+        ad->rest_stride[ad->nrest] = stride[m];
+        ad->nrest++;
+        ad->xsize *= dims[m];
     }
-    ad.ncol = ncol;
-    ad.nrows = nrows;
-    if (!d_rowmap && nrows != ad.nfront) return xr_fail(c, XR_ESHAPE, "without a row map nrows must be the front size");
+    ad->ncol = ncol;
+    ad->nrows = nrows;
+    if (!d_rowmap && nrows != ad->nfront) return xr_fail(c, XR_ESHAPE, "without a row map nrows must be the front size");
+    return XR_OK;
+}
+
+extern "C" int xr_axes_pack(xr_ctx* c, int nf, const int64_t* dims, int64_t ncol, int front0, int front1,
+                            const int32_t* d_rowmap, int64_t nrows, double* d_full, double* d_packed,
+                            int64_t ld_packed, int mode, double alpha) {
+    XR_ENTRY(c);
+    if (mode != 0 && mode != 1) return xr_fail(c, XR_EINVAL, "bad mode");
+    AxesDesc ad;
+    int rc = axes_desc(c, nf, dims, ncol, front0, front1, d_rowmap, nrows, &ad);
+    if (rc) return rc;
     if (ld_packed == 0) ld_packed = ad.xsize;
     if (ld_packed < ad.xsize) return xr_fail(c, XR_ESHAPE, "ld_packed smaller than the packed row");
     const int64_t total = nrows * ad.xsize;
@@ -337,6 +351,67 @@ extern "C" int xr_axes_pack(xr_ctx* c, int nf, const int64_t* dims, int64_t ncol
     ProfScope prof(c, XR_K_SCATTER, 0.0, (mode ? 24.0 : 16.0) * total);
     k_axes_pack<<<xr_grid(total), 256, 0, c->stream>>>(ad, d_rowmap, d_full, d_packed, ld_packed, mode, alpha);
     XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+// Several packs (or unpacks) of one mode in one launch: blockIdx.y picks the
+// item, the x dimension strides over its elements.  Items must not write
+// the same memory (unpacks of one launch target distinct elements).
+#define XR_PACK_ITEMS 12
+struct AxesItem {
+    AxesDesc ad;
+    const int32_t* rowmap;
+    double* full;
+    double* packed;
+    int64_t ldp;
+};
+struct AxesBatch {
+    int n, mode;
+    double alpha;
+    AxesItem it[XR_PACK_ITEMS];
+};
+
+__global__ void k_axes_pack_multi(AxesBatch bt) {
+    const AxesItem& I = bt.it[blockIdx.y];
+    const int64_t total = I.ad.nrows * I.ad.xsize;
+    XR_GRID_STRIDE(i, total) axes_pack_elem(I.ad, I.rowmap, I.full, I.packed, I.ldp, bt.mode, bt.alpha, i);
+}
+
+extern "C" int xr_axes_pack_multi(xr_ctx* c, int n_items, const int* nf, const int64_t* dims,
+                                  const int64_t* ncol, const int* front0, const int* front1,
+                                  const int32_t* const* d_rowmaps, const int64_t* nrows,
+                                  double* const* d_full, double* const* d_packed,
+                                  const int64_t* ld_packed, int mode, double alpha) {
+    XR_ENTRY(c);
+    if (mode != 0 && mode != 1) return xr_fail(c, XR_EINVAL, "bad mode");
+    if (n_items < 0) return xr_fail(c, XR_EINVAL, "negative item count");
+    for (int base = 0; base < n_items; base += XR_PACK_ITEMS) {
+        AxesBatch bt;
+        bt.n = (n_items - base < XR_PACK_ITEMS) ? n_items - base : XR_PACK_ITEMS;
+        bt.mode = mode;
+        bt.alpha = alpha;
+        int64_t most = 0, bytes = 0;
+        for (int t = 0; t < bt.n; t++) {
+            const int q = base + t;
+            AxesItem& I = bt.it[t];
+            int rc = axes_desc(c, nf[q], dims + (int64_t)q * XR_MAX_FRAG_AX, ncol[q], front0[q], front1[q],
+                               d_rowmaps ? d_rowmaps[q] : nullptr, nrows[q], &I.ad);
+            if (rc) return rc;
+            I.rowmap = d_rowmaps ? d_rowmaps[q] : nullptr;
+            I.full = d_full[q];
+            I.packed = d_packed[q];
+            I.ldp = ld_packed[q] ? ld_packed[q] : I.ad.xsize;
+            if (I.ldp < I.ad.xsize) return xr_fail(c, XR_ESHAPE, "ld_packed smaller than the packed row");
+            const int64_t tot = I.ad.nrows * I.ad.xsize;
+            most = tot > most ? tot : most;
+            bytes += tot;
+        }
+        if (most <= 0) continue;
+        ProfScope prof(c, XR_K_SCATTER, 0.0, (mode ? 24.0 : 16.0) * bytes);
+        dim3 grid(xr_grid(most), (unsigned)bt.n);
+        k_axes_pack_multi<<<grid, 256, 0, c->stream>>>(bt);
+        XR_LAUNCH_CHECK(c);
+    }
     return XR_OK;
 }
 

@@ -146,6 +146,24 @@ class XROperator:
                 self._eig.append(d)
         return self._eig
 
+    def _pack_multi(self, items, mode, alpha=1.0):
+        """items: [(dims, ncol, f0, f1, nrows, full, packed, ld)], one launch."""
+        import ctypes
+        n = len(items)
+        if n == 0:
+            return
+        I, L, P = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p
+        dims = (L * (8 * n))()
+        for q, it in enumerate(items):
+            for a, d in enumerate(it[0]):
+                dims[8 * q + a] = int(d)
+        arr = lambda T, vals: (T * n)(*vals)
+        self.ctx.call("xr_axes_pack_multi", n, arr(I, [len(it[0]) for it in items]), dims,
+                      arr(L, [int(it[1]) for it in items]), arr(I, [int(it[2]) for it in items]),
+                      arr(I, [int(it[3]) for it in items]), None, arr(L, [int(it[4]) for it in items]),
+                      arr(P, [it[5].data_ptr() for it in items]), arr(P, [it[6].data_ptr() for it in items]),
+                      arr(L, [int(it[7]) for it in items]), int(mode), float(alpha))
+
     def _pack(self, dims, ncol, f0, f1, rowmap, nrows, full, packed, mode, alpha=1.0, ld=0):
         import ctypes
         d = (ctypes.c_int64 * len(dims))(*[int(x) for x in dims])
@@ -408,12 +426,13 @@ class SectorSpace:
             widths = [self.sizes[i] // D * b for i in members]
             W = int(sum(widths))
             P = torch.empty(D, W, dtype=torch.float64, device=op.device)
-            col = 0
+            items, col = [], 0
             for i, w in zip(members, widths):
-                op._pack(self.shapes[i], b, m, n, None, D, self._view(X, i), P[:, col:], 0, ld=W)
+                items.append((self.shapes[i], b, m, n, D, self._view(X, i), P[:, col:], W))
                 col += w
+            op._pack_multi(items, 0)
             T = mm(Hs, P)
-            col = 0
+            items, col = [], 0
             for i, w in zip(members, widths):
                 t = self.sectors[i]
                 for tm, tn, off in targets:
@@ -421,9 +440,9 @@ class SectorSpace:
                     tt[m], tt[n] = tm, tn
                     j = self.index[tuple(tt)]
                     shj = self.shapes[j]
-                    Dj = shj[m] * shj[n]
-                    op._pack(shj, b, m, n, None, Dj, self._view(Y, j), T[off:off + Dj, col:], 1, 1.0, ld=W)
+                    items.append((shj, b, m, n, shj[m] * shj[n], self._view(Y, j), T[off:, col:], W))
                 col += w
+            op._pack_multi(items, 1)
         return Y
 
     def _axes(self, X, inverse):
