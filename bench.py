@@ -227,25 +227,29 @@ def main():
         kind = _lib.K_GEMM_F64 if prec == "f64" else _lib.K_GEMM_SPLIT
         stats = {}
         for name, k in (("gemm_f64", _lib.K_GEMM_F64), ("gemm_split_f32", _lib.K_GEMM_SPLIT),
-                        ("gemm_bandwidth", _lib.K_GEMM_BW), ("other", _lib.K_OTHER),
+                        ("gemm_grouped", _lib.K_GEMM_GROUPED), ("gemm_bandwidth", _lib.K_GEMM_BW),
+                        ("other", _lib.K_OTHER),
                         ("gather", _lib.K_GATHER), ("expand", _lib.K_EXPAND),
                         ("excitation_table", _lib.K_TABLE), ("block_scatter", _lib.K_SCATTER),
                         ("braket", _lib.K_BRAKET)):
             ln, kms, fl, by = ctx.profile_stats(k)
             stats[name] = {"launches": ln, "ms_per_step": round(kms / args.steps, 3),
                            "tflops": round(fl / (kms * 1e-3) / 1e12, 2) if kms > 0 and fl > 0 else None}
-        n, gms, gfl, gby = ctx.profile_stats(_lib.K_GEMM_F64)
+        # the dominant FP64 GEMM kernel: plain tile GEMM or grouped tile GEMM
+        gemm = max((ctx.profile_stats(k) + (name,) for k, name in
+                    ((_lib.K_GEMM_F64, "k_dgemm"), (_lib.K_GEMM_GROUPED, "k_dgemm_grouped"))),
+                   key=lambda r: r[1])
         print(f"bench: {prec} E={E:.12f} ms/step={ms:.1f} kernels={stats}", file=sys.stderr, flush=True)
         del X
         torch.cuda.empty_cache()
-        return {"ms": ms, "E": E, "g": g, "stats": stats, "gemm": (n, gms, gfl, gby)}
+        return {"ms": ms, "E": E, "g": g, "stats": stats, "gemm": gemm}
 
     precs = ["f64", "split_f32"] if args.precision == "both" else [args.precision]
     res = {p: measure(p) for p in precs}
     main_p = precs[0]
     r0 = res[main_p]
     ms_per_step, E = r0["ms"], r0["E"]
-    n, gms, gfl, gby = r0["gemm"]
+    n, gms, gfl, gby, gname = r0["gemm"]
     avg_ms = gms / max(n, 1)
     achieved = gfl / (gms * 1e-3) if gms > 0 else 0.0
     stats = r0["stats"]
@@ -264,7 +268,7 @@ def main():
     if args.config == "c3" and os.path.exists(tfile):
         # HBM bytes per FP64 GEMM launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
         # on this workload (the profile summary names the commands)
-        traffic = json.load(open(tfile))["traffic_bytes_per_launch"]
+        traffic = json.load(open(tfile))["traffic_bytes_per_launch"].get(gname)
 
     cpu = None
     if not args.no_cpu and len(w.fragments) == 2:
@@ -288,7 +292,7 @@ def main():
                    "E0": E, "grad_norm": gn},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": MFMA_F64_PEAK / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / MFMA_F64_PEAK,
-                     "kernel": "k_dgemm (FP64 MFMA GEMM)", "launches_per_step": n / args.steps,
+                     "kernel": f"{gname} (FP64 MFMA GEMM)", "launches_per_step": n / args.steps,
                      "avg_launch_ms": avg_ms, "traffic": traffic,
                      "algorithmic_bytes_per_launch": gby / max(n, 1)},
         "kernels": stats,

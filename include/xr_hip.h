@@ -63,7 +63,8 @@ int xr_synchronize(xr_ctx *ctx);
  * stops.  xr_profile_stats sums the records of one kernel class
  * (0 FP64 MFMA tile GEMM, 1 gather, 2 expand, 3 excitation table, 4 block
  * scatter / axis packing, 5 braket assembly, 6 other (incl. split-K
- * reductions), 7 split-FP32 GEMM, 8 bandwidth-bound tall-skinny products):
+ * reductions), 7 split-FP32 GEMM, 8 bandwidth-bound tall-skinny products,
+ * 9 grouped FP64 tile GEMM):
  * launches, total ms, and the
  * algorithmic flops and bytes of those launches.  Synchronises the stream.
  * No reference counterpart (the reference has no timers,
@@ -153,6 +154,18 @@ int xr_gemm_f64(xr_ctx *ctx, int transA, int transB, int64_t M, int64_t N, int64
                 double alpha, const double *d_A, int64_t lda, int64_t strideA,
                 const double *d_B, int64_t ldb, int64_t strideB,
                 double beta, double *d_C, int64_t ldc, int64_t strideC, int64_t batch);
+
+/* n independent products C_q = alpha op(A_q) op(B_q) of uneven shapes
+ * (host arrays of n shapes, leading dimensions and device pointers; the same
+ * transposes for all), their output tiles in one grid per 24 problems of one
+ * tile shape (tall-skinny products go to the single-product kernels);
+ * every C_q is overwritten.  Used for the pair-term products of one XR
+ * operator application (general-XRCC/hamiltonian.py:21-63, the pair terms
+ * H2_mn (x) 1 of all charge groups at once). */
+int xr_gemm_grouped(xr_ctx *ctx, int n, int transA, int transB, const int64_t *M, const int64_t *N,
+                    const int64_t *K, double alpha, const double *const *d_A, const int64_t *lda,
+                    const double *const *d_B, const int64_t *ldb, double *const *d_C,
+                    const int64_t *ldc);
 
 /* Split-precision GEMM: each FP64 operand is split into hi = fp32(x) and
  * lo = fp32(x - hi); the FP32 matrix cores form hi*hi and hi*lo + lo*hi per
@@ -286,6 +299,15 @@ int xr_axes_pack_multi(xr_ctx *ctx, int n_items, const int *nf, const int64_t *d
                        const int64_t *ncol, const int *front0, const int *front1,
                        const int32_t *const *d_rowmaps, const int64_t *nrows, double *const *d_full,
                        double *const *d_packed, const int64_t *ld_packed, int mode, double alpha);
+
+/* Monomer matrices along one fragment axis of many vector blocks, one launch:
+ * item q is the row-major view (pre[q], n[q], post[q]) at d_src[q];
+ * d_dst[q] = op(U_q) applied on the middle axis (+ beta d_dst[q]), op(U) = U
+ * (trans 0) or U^T (trans 1), U_q n x n row-major, n <= 64, src != dst
+ * (general-XRCC/hamiltonian.py:21-63, the monomer terms H1_m (x) 1). */
+int xr_axis_apply_multi(xr_ctx *ctx, int n_items, const double *const *d_U, const int *n,
+                        const int *trans, const int64_t *pre, const int64_t *post,
+                        const double *const *d_src, double *const *d_dst, double beta);
 
 /* Pair-block adjoint input (the pair-outer layout of xr_pair_outer) read from
  * a reduced pair matrix G2 = sum over the other fragments of v v^T, whose

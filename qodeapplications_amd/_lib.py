@@ -12,7 +12,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libxrhip.so")
 
 PREC_F64, PREC_SPLIT_F32 = 0, 1
-K_GEMM_F64, K_GATHER, K_EXPAND, K_TABLE, K_SCATTER, K_BRAKET, K_OTHER, K_GEMM_SPLIT, K_GEMM_BW = range(9)
+(K_GEMM_F64, K_GATHER, K_EXPAND, K_TABLE, K_SCATTER, K_BRAKET, K_OTHER, K_GEMM_SPLIT, K_GEMM_BW,
+ K_GEMM_GROUPED) = range(10)
 XR_OK, XR_EINVAL, XR_ESHAPE, XR_EINVAL_CONFIG_ORDER, XR_EPAULI, XR_ENOMEM, XR_EHIP, XR_EUNSUPPORTED = range(8)
 _CODE_NAMES = {1: "XR_EINVAL", 2: "XR_ESHAPE", 3: "XR_EINVAL_CONFIG_ORDER", 4: "XR_EPAULI",
                5: "XR_ENOMEM", 6: "XR_EHIP", 7: "XR_EUNSUPPORTED"}
@@ -43,7 +44,9 @@ SIGNATURES = {
     "xr_pair_outer": (_I, [_P, _P, _L, _P, _P, _P, _P, _L, _L, _L, _L, _I, _P]),
     "xr_gemm_tune": (_I, [_P, _I, _I, _L, _L, _L, _L, _I, _P, _P, _P]),
     "xr_gemm_tune_reset": (_I, [_P]),
+    "xr_gemm_grouped": (_I, [_P, _I, _I, _I, _P, _P, _P, _D, _P, _P, _P, _P, _P, _P]),
     "xr_axes_pack": (_I, [_P, _I, _P, _L, _I, _I, _P, _L, _P, _P, _L, _I, _D]),
+    "xr_axis_apply_multi": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _D]),
     "xr_axes_pack_multi": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _D]),
     "xr_pair_gather": (_I, [_P, _P, _L, _L, _L, _P, _P, _P, _P, _P, _L, _L, _L, _L, _I, _P]),
     "xr_gemm_f64": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),

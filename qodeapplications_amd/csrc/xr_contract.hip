@@ -10,7 +10,6 @@
 // out[o0,o1,o2,o3] = in[...] with out axis j = in axis perm[j]
 __device__ __forceinline__ void k_permute4_elem(int64_t i, const double* __restrict__ in, int64_t d0, int64_t d1, int64_t d2,
                            int64_t d3, int p0, int p1, int p2, int p3, double* __restrict__ out) {
-    int64_t n = d0 * d1 * d2 * d3;
     int64_t din[4] = {d0, d1, d2, d3};
     int p[4] = {p0, p1, p2, p3};
     int64_t dout[4] = {din[p[0]], din[p[1]], din[p[2]], din[p[3]]};
@@ -410,6 +409,138 @@ extern "C" int xr_axes_pack_multi(xr_ctx* c, int n_items, const int* nf, const i
         ProfScope prof(c, XR_K_SCATTER, 0.0, (mode ? 24.0 : 16.0) * bytes);
         dim3 grid(xr_grid(most), (unsigned)bt.n);
         k_axes_pack_multi<<<grid, 256, 0, c->stream>>>(bt);
+        XR_LAUNCH_CHECK(c);
+    }
+    return XR_OK;
+}
+
+// dst[p, i, q] = sum_j op(U)[i, j] src[p, j, q] (+ beta dst) for many
+// (pre, n, post) tensor views in one launch: a small matrix (n <= 64) applied
+// along one fragment axis of each item, op(U) = U or U^T.  The columns
+// c = p * post + q are cut into tiles of 64; a workgroup stages op(U)^T once
+// and then, per tile, the n x 64 operand block in LDS, and wave w computes
+// rows [w RB, w RB + RB) of the tile's output, lane = column.  Loads and
+// stores run along q (one contiguous run per p); for post == 1 the tile is
+// one contiguous 64 x n block, moved through LDS in both directions.
+#define XR_AXIS_ITEMS 16
+struct AxisItem {
+    const double* U;
+    const double* src;
+    double* dst;
+    int64_t pre, post;
+    int n, trans;
+};
+struct AxisBatch {
+    int nitems;
+    double beta;
+    AxisItem it[XR_AXIS_ITEMS];
+};
+
+template <int RB>
+__global__ __launch_bounds__(256) void k_axis_apply_multi(AxisBatch bt) {
+    const AxisItem& I = bt.it[blockIdx.y];
+    __shared__ double Ut[64][64 + 1];         // Ut[j][i] = op(U)[i][j]
+    __shared__ double Xs[64][64 + 1];         // Xs[j][c]: operand tile, then output tile (post == 1)
+    const int n = I.n;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int64_t post = I.post, cols = I.pre * post;
+    const int64_t tiles = (cols + 63) >> 6;
+    const double beta = bt.beta;
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int i = e >> 6, j = e & 63;
+        double v = 0.0;
+        if (i < n && j < n) v = I.trans ? I.U[j * n + i] : I.U[i * n + j];
+        Ut[j][i] = v;
+    }
+    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int64_t c0 = t << 6;
+        const int width = (cols - c0 < 64) ? (int)(cols - c0) : 64;
+        __syncthreads();                      // Ut written / previous tile consumed
+        int64_t base = 0;                     // this lane's column: p * n * post + q
+        if (post == 1) {
+            const double* s = I.src + c0 * n;
+            for (int e = tid; e < width * n; e += 256) {
+                const int c = e / n, j = e - c * n;
+                Xs[j][c] = s[e];
+            }
+        } else {
+            const int64_t c = c0 + lane;
+            const int64_t p = c / post, q = c - p * post;
+            base = p * n * post + q;
+            if (lane < width)
+                for (int j = w; j < n; j += 4) Xs[j][lane] = I.src[base + j * post];
+        }
+        __syncthreads();
+        double acc[RB];
+#pragma unroll
+        for (int k = 0; k < RB; k++) acc[k] = 0.0;
+        for (int j = 0; j < n; j++) {
+            const double x = Xs[j][lane];
+#pragma unroll
+            for (int k = 0; k < RB; k++) acc[k] = fma(Ut[j][w * RB + k], x, acc[k]);
+        }
+        if (post == 1) {
+            __syncthreads();                  // every wave has read Xs
+#pragma unroll
+            for (int k = 0; k < RB; k++)
+                if (w * RB + k < n) Xs[w * RB + k][lane] = acc[k];
+            __syncthreads();
+            double* d = I.dst + c0 * n;
+            for (int e = tid; e < width * n; e += 256) {
+                const int c = e / n, i = e - c * n;
+                const double v = Xs[i][c];
+                d[e] = (beta != 0.0) ? v + beta * d[e] : v;
+            }
+        } else if (lane < width) {
+#pragma unroll
+            for (int k = 0; k < RB; k++) {
+                const int i = w * RB + k;
+                if (i < n) {
+                    double* d = I.dst + base + (int64_t)i * post;
+                    *d = (beta != 0.0) ? acc[k] + beta * *d : acc[k];
+                }
+            }
+        }
+    }
+}
+
+extern "C" int xr_axis_apply_multi(xr_ctx* c, int n_items, const double* const* d_U, const int* n,
+                                   const int* trans, const int64_t* pre, const int64_t* post,
+                                   const double* const* d_src, double* const* d_dst, double beta) {
+    XR_ENTRY(c);
+    if (n_items < 0) return xr_fail(c, XR_EINVAL, "negative item count");
+    for (int base = 0; base < n_items; base += XR_AXIS_ITEMS) {
+        AxisBatch bt;
+        bt.nitems = (n_items - base < XR_AXIS_ITEMS) ? n_items - base : XR_AXIS_ITEMS;
+        bt.beta = beta;
+        int64_t most = 0, elems = 0, flops = 0, tiles = 0;
+        int nmax = 1;
+        for (int t = 0; t < bt.nitems; t++) {
+            const int q = base + t;
+            if (n[q] < 1 || n[q] > 64) return xr_fail(c, XR_EUNSUPPORTED, "axis length must be 1..64");
+            if (pre[q] < 0 || post[q] < 1) return xr_fail(c, XR_EINVAL, "bad pre / post");
+            if (d_src[q] == d_dst[q]) return xr_fail(c, XR_EINVAL, "axis apply cannot work in place");
+            bt.it[t] = AxisItem{d_U[q], d_src[q], d_dst[q], pre[q], post[q], n[q], trans[q]};
+            const int64_t tot = pre[q] * n[q] * post[q];
+            most = tot > most ? tot : most;
+            elems += tot;
+            flops += tot * n[q];
+            nmax = n[q] > nmax ? n[q] : nmax;
+            const int64_t tq = ceil_div(pre[q] * post[q], 64);
+            tiles = tq > tiles ? tq : tiles;
+        }
+        if (most <= 0) continue;
+        ProfScope prof(c, XR_K_GEMM_BW, 2.0 * flops, (beta != 0.0 ? 24.0 : 16.0) * elems);
+        // enough workgroups to fill the chip; each loops over its item's tiles
+        int64_t gx = tiles, cap = 4096 / bt.nitems;
+        gx = gx < cap ? gx : cap;
+        dim3 grid((unsigned)(gx > 0 ? gx : 1), (unsigned)bt.nitems);
+        if (nmax <= 4) k_axis_apply_multi<1><<<grid, 256, 0, c->stream>>>(bt);
+        else if (nmax <= 8) k_axis_apply_multi<2><<<grid, 256, 0, c->stream>>>(bt);
+        else if (nmax <= 16) k_axis_apply_multi<4><<<grid, 256, 0, c->stream>>>(bt);
+        else if (nmax <= 32) k_axis_apply_multi<8><<<grid, 256, 0, c->stream>>>(bt);
+        else k_axis_apply_multi<16><<<grid, 256, 0, c->stream>>>(bt);
         XR_LAUNCH_CHECK(c);
     }
     return XR_OK;

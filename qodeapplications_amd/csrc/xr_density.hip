@@ -486,7 +486,6 @@ extern "C" int xr_monomer_2e(xr_ctx* c, int n_orbs, int n_core, int n_elec, int6
 // orderings of l and t of sign * Rbar[I, J, iL, iR].
 __device__ __forceinline__ void k_compress_elem(int64_t i, const double* __restrict__ Rbar, int64_t nI, int64_t nJ, int kL, int kR,
                            int dim, int64_t nLt, int64_t nT, double* __restrict__ Gc, int64_t ldc) {
-    int64_t total = nLt * nI * nT * nJ;
     int64_t col = i % (nT * nJ), row = i / (nT * nJ);
     int64_t l = row / nI, I = row - l * nI;
     int64_t t = col / nJ, J = col - t * nJ;

@@ -12,6 +12,7 @@
 // XCD (own L2) receives a contiguous run of output tiles.
 #include "xr_common.h"
 #include "xr_gemm.h"
+#include <vector>
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
@@ -31,12 +32,12 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t total) {
     return base + j;
 }
 
+// One BM x BN output tile at (m0, n0): C = alpha op(A) op(B) over [0, K).
 template <int BM, int BN, bool TA, bool TB>
-__global__ __launch_bounds__(GEMM_THREADS) void k_dgemm(
-    int64_t M, int64_t N, int64_t K, double alpha, const double* __restrict__ A, int64_t lda,
-    int64_t sA, const double* __restrict__ B, int64_t ldb, int64_t sB, double beta,
-    double* __restrict__ C, int64_t ldc, int64_t sC, int64_t tiles_m, int64_t tiles_n,
-    int64_t kchunk, int64_t sSplit) {
+__device__ __forceinline__ void dgemm_tile(int64_t M, int64_t N, int64_t K, double alpha,
+                                           const double* __restrict__ A, int64_t lda,
+                                           const double* __restrict__ B, int64_t ldb,
+                                           double* __restrict__ C, int64_t ldc, int64_t m0, int64_t n0) {
     constexpr int PAD = 4;
     constexpr int WM = BM / 2, WN = BN / 2;    // wave tile
     constexpr int FM = WM / 16, FN = WN / 16;  // MFMA tiles per wave
@@ -44,27 +45,6 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm(
     constexpr int B_PER = BN * GEMM_BK / GEMM_THREADS;
     __shared__ double As[GEMM_BK][BM + PAD];
     __shared__ double Bs[GEMM_BK][BN + PAD];
-
-    const int64_t batch = blockIdx.z;
-    A += batch * sA;
-    B += batch * sB;
-    C += batch * sC + (int64_t)blockIdx.y * sSplit;
-    // split K: this workgroup covers [kbeg, kbeg + K) of the full K range
-    const int64_t kbeg = (int64_t)blockIdx.y * kchunk;
-    K = (K - kbeg < kchunk) ? (K - kbeg) : kchunk;
-    if (TA) A += kbeg * lda; else A += kbeg;
-    if (TB) B += kbeg; else B += kbeg * ldb;
-    const int64_t tile = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
-    // groups of 8 tile rows walk the tile columns together (L2 reuse of B)
-    const int64_t GROUP = 8;
-    const int64_t per_group = GROUP * tiles_n;
-    const int64_t g = tile / per_group;
-    const int64_t first_m = g * GROUP;
-    const int64_t gsize = (tiles_m - first_m < GROUP) ? (tiles_m - first_m) : GROUP;
-    const int64_t tm = first_m + (tile % per_group) % gsize;
-    const int64_t tn = (tile % per_group) / gsize;
-    const int64_t m0 = tm * BM, n0 = tn * BN;
-
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -130,7 +110,6 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm(
         for (int j = 0; j < FN; j++) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
 
     const int64_t nk = (K + GEMM_BK - 1) / GEMM_BK;
-    (void)beta;
     if (nk > 0) {
         load_a(0);
         load_b(0);
@@ -176,6 +155,72 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm(
                 int64_t gn = n0 + wn * WN + j * 16 + fr;
                 if (gm < M && gn < N) C[gm * ldc + gn] = acc[i][j][r];
             }
+}
+
+
+template <int BM, int BN, bool TA, bool TB>
+__global__ __launch_bounds__(GEMM_THREADS) void k_dgemm(
+    int64_t M, int64_t N, int64_t K, double alpha, const double* __restrict__ A, int64_t lda,
+    int64_t sA, const double* __restrict__ B, int64_t ldb, int64_t sB, double beta,
+    double* __restrict__ C, int64_t ldc, int64_t sC, int64_t tiles_m, int64_t tiles_n,
+    int64_t kchunk, int64_t sSplit) {
+    (void)beta;
+    const int64_t batch = blockIdx.z;
+    A += batch * sA;
+    B += batch * sB;
+    C += batch * sC + (int64_t)blockIdx.y * sSplit;
+    // split K: this workgroup covers [kbeg, kbeg + K) of the full K range
+    const int64_t kbeg = (int64_t)blockIdx.y * kchunk;
+    K = (K - kbeg < kchunk) ? (K - kbeg) : kchunk;
+    if (TA) A += kbeg * lda; else A += kbeg;
+    if (TB) B += kbeg; else B += kbeg * ldb;
+    const int64_t tile = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+    // groups of 8 tile rows walk the tile columns together (L2 reuse of B)
+    const int64_t GROUP = 8;
+    const int64_t per_group = GROUP * tiles_n;
+    const int64_t g = tile / per_group;
+    const int64_t first_m = g * GROUP;
+    const int64_t gsize = (tiles_m - first_m < GROUP) ? (tiles_m - first_m) : GROUP;
+    const int64_t tm = first_m + (tile % per_group) % gsize;
+    const int64_t tn = (tile % per_group) / gsize;
+    const int64_t m0 = tm * BM, n0 = tn * BN;
+
+    dgemm_tile<BM, BN, TA, TB>(M, N, K, alpha, A, lda, B, ldb, C, ldc, m0, n0);
+}
+
+// Grouped GEMM: up to XR_GROUP_PROBS independent products C_q = alpha
+// op(A_q) op(B_q) of uneven shapes in one launch.  The output tiles of all
+// problems form one grid (problem q owns tiles [first_q, first_q + tiles_q)),
+// so the launch fills the chip evenly where the problems alone would leave a
+// partly empty last round of workgroups.
+#define XR_GROUP_PROBS 24
+struct GemmProb {
+    const double* A;
+    const double* B;
+    double* C;
+    int64_t M, N, K, lda, ldb, ldc, tiles_m, tiles_n, first;
+};
+struct GemmGroup {
+    int n;
+    double alpha;
+    GemmProb p[XR_GROUP_PROBS];
+};
+
+template <int BM, int BN, bool TA, bool TB>
+__global__ __launch_bounds__(GEMM_THREADS) void k_dgemm_grouped(GemmGroup g) {
+    const int64_t tile = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+    int q = 0;
+    while (q + 1 < g.n && tile >= g.p[q + 1].first) q++;
+    const GemmProb& P = g.p[q];
+    const int64_t local = tile - P.first;
+    const int64_t GROUP = 8;
+    const int64_t per_group = GROUP * P.tiles_n;
+    const int64_t gi = local / per_group;
+    const int64_t first_m = gi * GROUP;
+    const int64_t gsize = (P.tiles_m - first_m < GROUP) ? (P.tiles_m - first_m) : GROUP;
+    const int64_t tm = first_m + (local % per_group) % gsize;
+    const int64_t tn = (local % per_group) / gsize;
+    dgemm_tile<BM, BN, TA, TB>(P.M, P.N, P.K, g.alpha, P.A, P.lda, P.B, P.ldb, P.C, P.ldc, tm * BM, tn * BN);
 }
 
 // C = alpha * sum_s P[s] + beta * C, summed in split order.
@@ -279,16 +324,25 @@ __global__ __launch_bounds__(256) void k_dgemm_tn_skinny(int64_t M, int64_t N, i
     }
 }
 
-__global__ void k_tn_reduce(const double* __restrict__ part, int64_t nblk, int64_t M, int64_t N, int nb,
-                            double alpha, double beta, double* __restrict__ C, int64_t ldc) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= M * N) return;
+// C[i, j] = alpha * sum_b part[b][i][j] + beta C: one workgroup per output
+// entry, the threads stride over the partial tiles and a fixed butterfly
+// adds their sums (same order on every run).
+__global__ __launch_bounds__(256) void k_tn_reduce(const double* __restrict__ part, int64_t nblk, int64_t M,
+                                                   int64_t N, int nb, double alpha, double beta,
+                                                   double* __restrict__ C, int64_t ldc) {
+    const int64_t t = blockIdx.x;
     const int64_t i = t / N, j = t - (t / N) * N;
     double s = 0.0;
-    for (int64_t b = 0; b < nblk; b++) s += part[(b * 64 + i) * nb + j];
-    double v = alpha * s;
-    if (beta != 0.0) v += beta * C[i * ldc + j];
-    C[i * ldc + j] = v;
+    for (int64_t b = threadIdx.x; b < nblk; b += 256) s += part[(b * 64 + i) * nb + j];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = alpha * ((red[0] + red[1]) + (red[2] + red[3]));
+        if (beta != 0.0) v += beta * C[i * ldc + j];
+        C[i * ldc + j] = v;
+    }
 }
 
 // Gram products with M <= 8 columns: the rows are spread over all 256
@@ -306,6 +360,7 @@ __global__ __launch_bounds__(256) void k_dgemm_tn_small(int64_t M, int64_t N, in
     for (int i = 0; i < MB; i++)
 #pragma unroll
         for (int j = 0; j < NB; j++) acc[i][j] = 0.0;
+#pragma unroll 4
     for (int64_t k = k0 + threadIdx.x; k < k1; k += 256) {
         double a[MB], b[NB];
 #pragma unroll
@@ -379,6 +434,43 @@ __global__ __launch_bounds__(256) void k_dgemm_rows_lds(int64_t M, int64_t N, in
                     if (beta != 0.0) v += beta * C[(r0 + row) * ldc + j];
                     C[(r0 + row) * ldc + j] = v;
                 }
+            }
+        }
+    }
+}
+
+// C = alpha A op(B) + beta C for tall A (M x K row-major) with K <= 4 and
+// N <= 8: one thread per output row (the K values of consecutive rows are
+// adjacent, so a wave reads one contiguous run).
+template <int NB, bool TB>
+__global__ __launch_bounds__(256) void k_dgemm_rows_small(int64_t M, int64_t N, int64_t K, double alpha,
+                                                          const double* __restrict__ A, int64_t lda,
+                                                          const double* __restrict__ B, int64_t ldb,
+                                                          double beta, double* __restrict__ C, int64_t ldc) {
+    double bs[4][NB];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            bs[k][j] = (k < K && j < N) ? (TB ? B[(int64_t)j * ldb + k] : B[(int64_t)k * ldb + j]) : 0.0;
+    XR_GRID_STRIDE(r, M) {
+        double acc[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++) acc[j] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (k < K) {
+                const double a = A[r * lda + k];
+#pragma unroll
+                for (int j = 0; j < NB; j++) acc[j] = fma(a, bs[k][j], acc[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            if (j < N) {
+                double v = alpha * acc[j];
+                if (beta != 0.0) v += beta * C[r * ldc + j];
+                C[r * ldc + j] = v;
             }
         }
     }
@@ -494,7 +586,7 @@ static int launch_tn_skinny(xr_ctx* c, int64_t M, int64_t N, int64_t K, double a
     else k_dgemm_tn_skinny<8><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
 #undef XR_TN_SMALL
     XR_LAUNCH_CHECK(c);
-    k_tn_reduce<<<(unsigned)ceil_div(M * N, 256), 256, 0, c->stream>>>(part, nblk, M, N, nb, alpha, beta, C, ldc);
+    k_tn_reduce<<<(unsigned)(M * N), 256, 0, c->stream>>>(part, nblk, M, N, nb, alpha, beta, C, ldc);
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }
@@ -515,7 +607,7 @@ static int launch_nn_gram(xr_ctx* c, int64_t M, int64_t N, int64_t K, double alp
     else if (nb == 4) k_dgemm_nn_gram<4><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
     else k_dgemm_nn_gram<8><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
     XR_LAUNCH_CHECK(c);
-    k_tn_reduce<<<(unsigned)ceil_div(M * N, 256), 256, 0, c->stream>>>(part, nblk, M, N, nb, alpha, beta, C, ldc);
+    k_tn_reduce<<<(unsigned)(M * N), 256, 0, c->stream>>>(part, nblk, M, N, nb, alpha, beta, C, ldc);
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }
@@ -623,6 +715,20 @@ int xr_dgemm_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, 
     }
     if (!ta && !tb && batch == 1 && N <= 8 && K >= 4096 && M * (N <= 1 ? 1 : (N <= 2 ? 2 : (N <= 4 ? 4 : 8))) <= 64)
         return launch_nn_gram(c, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    if (N <= 8 && !ta && batch == 1 && M >= 1024 && K <= 4) {
+        ProfScope prof(c, XR_K_GEMM_BW, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
+        prof.shape(M, N, K, batch, ta, tb);
+#define XR_ROWS_SMALL(NB_)                                                                              \
+        if (tb) k_dgemm_rows_small<NB_, true><<<xr_grid(M), 256, 0, c->stream>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc); \
+        else k_dgemm_rows_small<NB_, false><<<xr_grid(M), 256, 0, c->stream>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc)
+        if (N <= 1) { XR_ROWS_SMALL(1); }
+        else if (N <= 2) { XR_ROWS_SMALL(2); }
+        else if (N <= 4) { XR_ROWS_SMALL(4); }
+        else { XR_ROWS_SMALL(8); }
+#undef XR_ROWS_SMALL
+        XR_LAUNCH_CHECK(c);
+        return XR_OK;
+    }
     if (N <= 8 && !ta && batch == 1 && M >= 1024 && K <= 64) {
         ProfScope prof(c, XR_K_GEMM_BW, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
         prof.shape(M, N, K, batch, ta, tb);
@@ -679,6 +785,77 @@ extern "C" int xr_gemm_f64(xr_ctx* c, int transA, int transB, int64_t M, int64_t
     if ((transB ? ldb < K : ldb < N) && N > 0 && K > 0) return xr_fail(c, XR_ESHAPE, "ldb too small");
     return xr_dgemm_launch(c, transA, transB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc,
                            sC, batch);
+}
+
+// Launches the problems of one tile shape, XR_GROUP_PROBS per grid.
+template <int BM, int BN>
+static int launch_grouped(xr_ctx* c, const std::vector<int>& ids, int ta, int tb, const int64_t* M,
+                          const int64_t* N, const int64_t* K, double alpha, const double* const* A,
+                          const int64_t* lda, const double* const* B, const int64_t* ldb, double* const* C,
+                          const int64_t* ldc) {
+    for (size_t base = 0; base < ids.size(); base += XR_GROUP_PROBS) {
+        GemmGroup g;
+        g.n = 0;
+        g.alpha = alpha;
+        int64_t tiles = 0;
+        double flops = 0.0, bytes = 0.0;
+        for (size_t t = base; t < ids.size() && t < base + XR_GROUP_PROBS; t++) {
+            const int q = ids[t];
+            GemmProb& P = g.p[g.n++];
+            P = GemmProb{A[q], B[q], C[q], M[q], N[q], K[q], lda[q], ldb[q], ldc[q],
+                         ceil_div(M[q], BM), ceil_div(N[q], BN), tiles};
+            tiles += P.tiles_m * P.tiles_n;
+            flops += 2.0 * M[q] * N[q] * K[q];
+            bytes += 8.0 * (M[q] * K[q] + K[q] * N[q] + M[q] * N[q]);
+        }
+        if (tiles > 0x7fffffff) return xr_fail(c, XR_EUNSUPPORTED, "GEMM grid too large");
+        ProfScope prof(c, XR_K_GEMM_GROUPED, flops, bytes);
+        prof.shape(g.p[0].M, g.p[0].N, g.p[0].K, g.n, ta, tb);
+        const dim3 grid((unsigned)tiles);
+        if (ta && tb) k_dgemm_grouped<BM, BN, true, true><<<grid, GEMM_THREADS, 0, c->stream>>>(g);
+        else if (ta) k_dgemm_grouped<BM, BN, true, false><<<grid, GEMM_THREADS, 0, c->stream>>>(g);
+        else if (tb) k_dgemm_grouped<BM, BN, false, true><<<grid, GEMM_THREADS, 0, c->stream>>>(g);
+        else k_dgemm_grouped<BM, BN, false, false><<<grid, GEMM_THREADS, 0, c->stream>>>(g);
+        XR_LAUNCH_CHECK(c);
+    }
+    return XR_OK;
+}
+
+// C_q = alpha op(A_q) op(B_q) for n independent problems (row-major, the
+// same transposes).  Problems with N > 64 share 128 x 128 tile grids, those
+// with 8 < N <= 64 share 128 x 64 grids (both with M > 64); the rest (the
+// tall-skinny and small shapes) go to the single-product dispatch, whose
+// bandwidth kernels suit them.  No K split and no beta: every C_q is
+// overwritten.
+extern "C" int xr_gemm_grouped(xr_ctx* c, int n, int ta, int tb, const int64_t* M, const int64_t* N,
+                               const int64_t* K, double alpha, const double* const* A, const int64_t* lda,
+                               const double* const* B, const int64_t* ldb, double* const* C,
+                               const int64_t* ldc) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (n < 0) return xr_fail(c, XR_EINVAL, "negative problem count");
+    std::vector<int> wide, narrow;
+    for (int q = 0; q < n; q++) {
+        if (M[q] < 0 || N[q] < 0 || K[q] < 0) return xr_fail(c, XR_EINVAL, "negative GEMM size");
+        if (ldc[q] < N[q]) return xr_fail(c, XR_ESHAPE, "ldc < N");
+        if ((ta ? lda[q] < M[q] : lda[q] < K[q]) && M[q] > 0 && K[q] > 0)
+            return xr_fail(c, XR_ESHAPE, "lda too small");
+        if ((tb ? ldb[q] < K[q] : ldb[q] < N[q]) && N[q] > 0 && K[q] > 0)
+            return xr_fail(c, XR_ESHAPE, "ldb too small");
+    }
+    for (int q = 0; q < n; q++) {
+        if (M[q] == 0 || N[q] == 0) continue;
+        if (M[q] > 64 && N[q] > 64) wide.push_back(q);
+        else if (M[q] > 64 && N[q] > 8) narrow.push_back(q);
+        else {
+            const int rc = xr_dgemm_launch(c, ta, tb, M[q], N[q], K[q], alpha, A[q], lda[q], 0, B[q], ldb[q], 0,
+                                           0.0, C[q], ldc[q], 0, 1);
+            if (rc) return rc;
+        }
+    }
+    int rc = launch_grouped<128, 128>(c, wide, ta, tb, M, N, K, alpha, A, lda, B, ldb, C, ldc);
+    if (rc) return rc;
+    return launch_grouped<128, 64>(c, narrow, ta, tb, M, N, K, alpha, A, lda, B, ldb, C, ldc);
 }
 
 // ---------------------------------------------------------------------------

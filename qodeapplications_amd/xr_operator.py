@@ -164,6 +164,42 @@ class XROperator:
                       arr(P, [it[5].data_ptr() for it in items]), arr(P, [it[6].data_ptr() for it in items]),
                       arr(L, [int(it[7]) for it in items]), int(mode), float(alpha))
 
+    def _axis_multi(self, items, beta=0.0):
+        """items: [(U, src, dst, pre, n, post, trans)]: dst = op(U) along the
+        middle axis of the (pre, n, post) views; one launch per 16 items with
+        n <= 64, a GEMM per item otherwise."""
+        import ctypes
+        small = [it for it in items if it[4] <= 64]
+        for U, src, dst, pre, n, post, tr in items:
+            if n > 64:
+                _axis_apply(U, src, dst, pre, n, post, transA=tr, beta=beta)
+        if not small:
+            return
+        k = len(small)
+        I, L, P = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p
+        arr = lambda T, vals: (T * k)(*vals)
+        self.ctx.call("xr_axis_apply_multi", k, arr(P, [it[0].data_ptr() for it in small]),
+                      arr(I, [int(it[4]) for it in small]), arr(I, [int(it[6]) for it in small]),
+                      arr(L, [int(it[3]) for it in small]), arr(L, [int(it[5]) for it in small]),
+                      arr(P, [it[1].data_ptr() for it in small]), arr(P, [it[2].data_ptr() for it in small]),
+                      float(beta))
+
+    def _gemm_grouped(self, probs, alpha=1.0):
+        """C = alpha A B for every (A, B, C) of row-major matrices (unit
+        column stride), all in one tile grid per 24 products."""
+        import ctypes
+        k = len(probs)
+        if k == 0:
+            return
+        L, P = ctypes.c_int64, ctypes.c_void_p
+        arr = lambda T, vals: (T * k)(*vals)
+        self.ctx.call("xr_gemm_grouped", k, 0, 0, arr(L, [A.shape[0] for A, B, C in probs]),
+                      arr(L, [B.shape[1] for A, B, C in probs]), arr(L, [A.shape[1] for A, B, C in probs]),
+                      float(alpha), arr(P, [A.data_ptr() for A, B, C in probs]),
+                      arr(L, [A.stride(0) for A, B, C in probs]), arr(P, [B.data_ptr() for A, B, C in probs]),
+                      arr(L, [B.stride(0) for A, B, C in probs]), arr(P, [C.data_ptr() for A, B, C in probs]),
+                      arr(L, [C.stride(0) for A, B, C in probs]))
+
     def _pack(self, dims, ncol, f0, f1, rowmap, nrows, full, packed, mode, alpha=1.0, ld=0):
         import ctypes
         d = (ctypes.c_int64 * len(dims))(*[int(x) for x in dims])
@@ -412,27 +448,34 @@ class SectorSpace:
         op = self.op
         b = X.shape[1]
         nf = op.nf
-        for i, t in enumerate(self.sectors):
-            sh = self.shapes[i]
-            Xi, Yi = self._view(X, i), self._view(Y, i)
-            for m in range(nf):
-                _axis_apply(self.h1[m][t[m]], Xi, Yi, int(np.prod(sh[:m])), sh[m],
-                            int(np.prod(sh[m + 1:])) * b, beta=0.0 if m == 0 else 1.0)
+        for m in range(nf):
+            op._axis_multi([(self.h1[m][t[m]], self._view(X, i), self._view(Y, i),
+                             int(np.prod(self.shapes[i][:m])), self.shapes[i][m],
+                             int(np.prod(self.shapes[i][m + 1:])) * b, 0)
+                            for i, t in enumerate(self.sectors)], beta=0.0 if m == 0 else 1.0)
         # pair terms: the tuples sharing (m, n) charges share the stacked
-        # blocks, so their packed columns go side by side into one GEMM
+        # blocks, so their packed columns go side by side into one product;
+        # all packs first, then the products of every charge group in one
+        # grouped GEMM, then the unpacks group by group (targets of one
+        # group are distinct tuples; different groups add into the same ones)
+        packs, jobs = [], []
         for (m, n, cm, cn), members in self.groups.items():
             Hs, targets = self.stack[(m, n, cm, cn)]
             D = self.blocks_of[m][cm][1] * self.blocks_of[n][cn][1]
             widths = [self.sizes[i] // D * b for i in members]
             W = int(sum(widths))
             P = torch.empty(D, W, dtype=torch.float64, device=op.device)
-            items, col = [], 0
+            col = 0
             for i, w in zip(members, widths):
-                items.append((self.shapes[i], b, m, n, D, self._view(X, i), P[:, col:], W))
+                packs.append((self.shapes[i], b, m, n, D, self._view(X, i), P[:, col:], W))
                 col += w
-            op._pack_multi(items, 0)
-            T = mm(Hs, P)
+            T = torch.empty(Hs.shape[0], W, dtype=torch.float64, device=op.device)
+            jobs.append((m, n, members, widths, targets, Hs, P, T))
+        op._pack_multi(packs, 0)
+        op._gemm_grouped([(Hs, P, T) for m, n, members, widths, targets, Hs, P, T in jobs])
+        for m, n, members, widths, targets, Hs, P, T in jobs:
             items, col = [], 0
+            W = T.shape[1]
             for i, w in zip(members, widths):
                 t = self.sectors[i]
                 for tm, tn, off in targets:
@@ -453,11 +496,10 @@ class SectorSpace:
         cur = 0
         for m in range(self.op.nf):
             src, dst = bufs[cur], bufs[1 - cur]
-            for i, t in enumerate(self.sectors):
-                sh = self.shapes[i]
-                _axis_apply(self.eig[m][t[m]][1], self._view(src, i), self._view(dst, i),
-                            int(np.prod(sh[:m])), sh[m], int(np.prod(sh[m + 1:])) * b,
-                            transA=0 if inverse else 1)
+            self.op._axis_multi([(self.eig[m][t[m]][1], self._view(src, i), self._view(dst, i),
+                                  int(np.prod(self.shapes[i][:m])), self.shapes[i][m],
+                                  int(np.prod(self.shapes[i][m + 1:])) * b, 0 if inverse else 1)
+                                 for i, t in enumerate(self.sectors)])
             cur = 1 - cur
         return bufs[cur]
 

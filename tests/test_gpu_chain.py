@@ -68,6 +68,42 @@ def test_operator_block_columns():
     assert float((Y - M @ V).abs().max()) < 1e-12
 
 
+@pytest.mark.parametrize("beta", [0.0, 1.0])
+def test_axis_apply_multi_matches_einsum(beta):
+    """xr_axis_apply_multi (the monomer terms and eigenbasis changes of the
+    sector operator) against an FP64 einsum, over 20 items (two launches) of
+    every column layout: post 1 (contiguous tile), post < 64 (several p per
+    tile), post >= 64 with a ragged last tile; n from 1 to 64; U and U^T."""
+    import ctypes
+    from qodeapplications_amd._lib import context
+    g = torch.Generator(device="cuda").manual_seed(int(beta) + 7)
+    shapes = [(1, 32, 70000), (1024, 32, 32), (32768, 32, 1), (3, 64, 100), (5, 1, 7), (7, 3, 1),
+              (2, 17, 65), (9, 33, 20), (1, 64, 1), (4, 5, 129), (100, 8, 3), (1, 2, 64),
+              (13, 31, 1), (2, 64, 64), (6, 16, 63), (1, 4, 1000), (8, 40, 2), (1, 1, 1),
+              (3, 48, 5), (50, 12, 1)]
+    items = []
+    for q, (pre, n, post) in enumerate(shapes):
+        U = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+        src = torch.randn(pre * n * post, dtype=torch.float64, device="cuda", generator=g)
+        dst = torch.randn(pre * n * post, dtype=torch.float64, device="cuda", generator=g)
+        items.append((U, src, dst, pre, n, post, q % 2))
+    ref = []
+    for U, src, dst, pre, n, post, tr in items:
+        A = U.T if tr else U
+        ref.append(torch.einsum("ij,pjq->piq", A, src.view(pre, n, post)).reshape(-1) + beta * dst)
+    k = len(items)
+    I, L, P = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p
+    arr = lambda T, vals: (T * k)(*vals)
+    context(0).call("xr_axis_apply_multi", k, arr(P, [it[0].data_ptr() for it in items]),
+                    arr(I, [it[4] for it in items]), arr(I, [it[6] for it in items]),
+                    arr(L, [it[3] for it in items]), arr(L, [it[5] for it in items]),
+                    arr(P, [it[1].data_ptr() for it in items]), arr(P, [it[2].data_ptr() for it in items]),
+                    float(beta))
+    torch.cuda.synchronize()
+    for q, (it, r) in enumerate(zip(items, ref)):
+        assert float((it[2] - r).abs().max()) < 1e-12 * max(1.0, float(r.abs().max())), shapes[q]
+
+
 def test_davidson_root_matches_dense():
     X, _ = _system(42, (4, 4, 4), ns=7)
     H = X.blocks()

@@ -40,6 +40,37 @@ def test_tuned_gemm_matches_heuristic(ta, tb, M, N, K):
     ctx.call("xr_gemm_tune_reset")
 
 
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_grouped_gemm_matches_cpu(ta, tb):
+    """xr_gemm_grouped over 30 products of uneven shapes (both tile grids,
+    the single-product route, one empty product, edges that are not
+    multiples of the tiles)."""
+    ctx = context(0)
+    rng = np.random.default_rng(ta * 2 + tb)
+    shapes = [(int(rng.integers(1, 400)), int(rng.integers(1, 400)), int(rng.integers(1, 700)))
+              for _ in range(29)] + [(0, 5, 7)]
+    shapes[3] = (1024, 3072, 1024)
+    shapes[4:7] = [(5000, 4, 300), (300, 40, 200), (50, 300, 100)]     # skinny, 128 x 64, small M
+    probs = []
+    for M, N, K in shapes:
+        A = torch.randn(*((K, M) if ta else (M, K)), dtype=torch.float64)
+        B = torch.randn(*((N, K) if tb else (K, N)), dtype=torch.float64)
+        probs.append((A, B, torch.full((M, N), 7.0, dtype=torch.float64, device="cuda")))
+    dev = [(A.cuda(), B.cuda(), C) for A, B, C in probs]
+    k = len(dev)
+    L, P = ctypes.c_int64, ctypes.c_void_p
+    arr = lambda T, vals: (T * k)(*vals)
+    ctx.call("xr_gemm_grouped", k, ta, tb, arr(L, [s[0] for s in shapes]), arr(L, [s[1] for s in shapes]),
+             arr(L, [s[2] for s in shapes]), -0.5, arr(P, [A.data_ptr() for A, B, C in dev]),
+             arr(L, [A.stride(0) for A, B, C in dev]), arr(P, [B.data_ptr() for A, B, C in dev]),
+             arr(L, [B.stride(0) for A, B, C in dev]), arr(P, [C.data_ptr() for A, B, C in dev]),
+             arr(L, [C.stride(0) for A, B, C in dev]))
+    torch.cuda.synchronize()
+    for (M, N, K), (A, B, _), (_, _, C) in zip(shapes, probs, dev):
+        ref = -0.5 * ((A.T if ta else A) @ (B.T if tb else B))
+        np.testing.assert_allclose(C.cpu().numpy(), ref.numpy(), atol=1e-12 * max(1, K) ** 0.5, rtol=1e-12)
+
+
 def test_gemm_shapes_records_transposes():
     A = torch.randn(400, 300, dtype=torch.float64, device="cuda")
     B = torch.randn(500, 300, dtype=torch.float64, device="cuda")
