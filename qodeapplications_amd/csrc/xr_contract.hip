@@ -6,6 +6,7 @@
 // becomes two FP64 GEMMs:  G = R1 * W  (nA x k2), then  out = G * R2^T.
 #include "xr_common.h"
 #include "xr_gemm.h"
+#include <vector>
 
 // out[o0,o1,o2,o3] = in[...] with out axis j = in axis perm[j]
 __device__ __forceinline__ void k_permute4_elem(int64_t i, const double* __restrict__ in, int64_t d0, int64_t d1, int64_t d2,
@@ -356,6 +357,11 @@ extern "C" int xr_axes_pack(xr_ctx* c, int nf, const int64_t* dims, int64_t ncol
 // Several packs (or unpacks) of one mode in one launch: blockIdx.y picks the
 // item, the x dimension strides over its elements.  Items must not write
 // the same memory (unpacks of one launch target distinct elements).
+// When the front axes include the innermost fragment axis (stride ncol),
+// consecutive packed columns are far apart in the full tensor while
+// consecutive packed rows are adjacent: such an item ("tiled") is moved as
+// 64 x 64 tiles through LDS, read along rows on the full side and along
+// columns on the packed side, so both sides stay coalesced.
 #define XR_PACK_ITEMS 12
 struct AxesItem {
     AxesDesc ad;
@@ -363,6 +369,7 @@ struct AxesItem {
     double* full;
     double* packed;
     int64_t ldp;
+    int tiled;
 };
 struct AxesBatch {
     int n, mode;
@@ -370,10 +377,80 @@ struct AxesBatch {
     AxesItem it[XR_PACK_ITEMS];
 };
 
-__global__ void k_axes_pack_multi(AxesBatch bt) {
+// offset in the full tensor of packed column x (rest axes and column)
+__device__ __forceinline__ int64_t axes_rest_offset(const AxesDesc& ad, int64_t x) {
+    int64_t y = x / ad.ncol;
+    int64_t off = x - y * ad.ncol;
+    for (int k = ad.nrest - 1; k >= 0; k--) {
+        const int64_t q = y / ad.rest_dim[k];
+        off += (y - q * ad.rest_dim[k]) * ad.rest_stride[k];
+        y = q;
+    }
+    return off;
+}
+
+__global__ __launch_bounds__(256) void k_axes_pack_multi(AxesBatch bt) {
     const AxesItem& I = bt.it[blockIdx.y];
     const int64_t total = I.ad.nrows * I.ad.xsize;
     XR_GRID_STRIDE(i, total) axes_pack_elem(I.ad, I.rowmap, I.full, I.packed, I.ldp, bt.mode, bt.alpha, i);
+}
+
+__global__ __launch_bounds__(256) void k_axes_pack_tiled(AxesBatch bt) {
+    const AxesItem& I = bt.it[blockIdx.y];
+    const AxesDesc& ad = I.ad;
+    __shared__ double L[64][65];               // L[x - x0][r - r0]
+    __shared__ int64_t gx[64];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int64_t nr = ad.nrows, nx = ad.xsize;
+    const int64_t tx = (nx + 63) >> 6, tiles = ((nr + 63) >> 6) * tx;
+    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int64_t r0 = (t / tx) << 6, x0 = (t % tx) << 6;
+        const int64_t r = r0 + lane;
+        const bool rv = r < nr;
+        const int64_t fr = rv ? (r / ad.d1) * ad.s0 + (r % ad.d1) * ad.s1 : 0;
+        __syncthreads();                        // previous tile done with L and gx
+        if (tid < 64) gx[tid] = (x0 + tid < nx) ? axes_rest_offset(ad, x0 + tid) : 0;
+        if (bt.mode == 0) {
+            __syncthreads();
+            for (int xi = w; xi < 64; xi += 4)
+                if (x0 + xi < nx && rv) L[xi][lane] = I.full[fr + gx[xi]];
+            __syncthreads();
+            for (int ri = w; ri < 64; ri += 4)
+                if (r0 + ri < nr && x0 + lane < nx) I.packed[(r0 + ri) * I.ldp + x0 + lane] = L[lane][ri];
+        } else {
+            for (int ri = w; ri < 64; ri += 4)
+                if (r0 + ri < nr && x0 + lane < nx) L[lane][ri] = I.packed[(r0 + ri) * I.ldp + x0 + lane];
+            __syncthreads();
+            for (int xi = w; xi < 64; xi += 4)
+                if (x0 + xi < nx && rv) I.full[fr + gx[xi]] += bt.alpha * L[xi][lane];
+        }
+    }
+}
+
+// Launches the items of one kind (tiled or element-wise), XR_PACK_ITEMS per grid.
+static int pack_launch(xr_ctx* c, const std::vector<AxesItem>& items, int tiled, int mode, double alpha) {
+    for (size_t base = 0; base < items.size(); base += XR_PACK_ITEMS) {
+        AxesBatch bt;
+        bt.n = (int)((items.size() - base < XR_PACK_ITEMS) ? items.size() - base : XR_PACK_ITEMS);
+        bt.mode = mode;
+        bt.alpha = alpha;
+        int64_t most = 0, bytes = 0;
+        for (int t = 0; t < bt.n; t++) {
+            const AxesItem& I = bt.it[t] = items[base + t];
+            const int64_t tot = I.ad.nrows * I.ad.xsize;
+            const int64_t need = tiled ? ceil_div(I.ad.nrows, 64) * ceil_div(I.ad.xsize, 64) : (int64_t)xr_grid(tot);
+            most = need > most ? need : most;
+            bytes += tot;
+        }
+        if (bytes <= 0) continue;
+        ProfScope prof(c, XR_K_SCATTER, 0.0, (mode ? 24.0 : 16.0) * bytes);
+        dim3 grid((unsigned)(most < (1 << 20) ? most : (1 << 20)), (unsigned)bt.n);
+        if (tiled) k_axes_pack_tiled<<<grid, 256, 0, c->stream>>>(bt);
+        else k_axes_pack_multi<<<grid, 256, 0, c->stream>>>(bt);
+        XR_LAUNCH_CHECK(c);
+    }
+    return XR_OK;
 }
 
 extern "C" int xr_axes_pack_multi(xr_ctx* c, int n_items, const int* nf, const int64_t* dims,
@@ -384,34 +461,25 @@ extern "C" int xr_axes_pack_multi(xr_ctx* c, int n_items, const int* nf, const i
     XR_ENTRY(c);
     if (mode != 0 && mode != 1) return xr_fail(c, XR_EINVAL, "bad mode");
     if (n_items < 0) return xr_fail(c, XR_EINVAL, "negative item count");
-    for (int base = 0; base < n_items; base += XR_PACK_ITEMS) {
-        AxesBatch bt;
-        bt.n = (n_items - base < XR_PACK_ITEMS) ? n_items - base : XR_PACK_ITEMS;
-        bt.mode = mode;
-        bt.alpha = alpha;
-        int64_t most = 0, bytes = 0;
-        for (int t = 0; t < bt.n; t++) {
-            const int q = base + t;
-            AxesItem& I = bt.it[t];
-            int rc = axes_desc(c, nf[q], dims + (int64_t)q * XR_MAX_FRAG_AX, ncol[q], front0[q], front1[q],
-                               d_rowmaps ? d_rowmaps[q] : nullptr, nrows[q], &I.ad);
-            if (rc) return rc;
-            I.rowmap = d_rowmaps ? d_rowmaps[q] : nullptr;
-            I.full = d_full[q];
-            I.packed = d_packed[q];
-            I.ldp = ld_packed[q] ? ld_packed[q] : I.ad.xsize;
-            if (I.ldp < I.ad.xsize) return xr_fail(c, XR_ESHAPE, "ld_packed smaller than the packed row");
-            const int64_t tot = I.ad.nrows * I.ad.xsize;
-            most = tot > most ? tot : most;
-            bytes += tot;
-        }
-        if (most <= 0) continue;
-        ProfScope prof(c, XR_K_SCATTER, 0.0, (mode ? 24.0 : 16.0) * bytes);
-        dim3 grid(xr_grid(most), (unsigned)bt.n);
-        k_axes_pack_multi<<<grid, 256, 0, c->stream>>>(bt);
-        XR_LAUNCH_CHECK(c);
+    std::vector<AxesItem> plain, tiled;
+    for (int q = 0; q < n_items; q++) {
+        AxesItem I;
+        int rc = axes_desc(c, nf[q], dims + (int64_t)q * XR_MAX_FRAG_AX, ncol[q], front0[q], front1[q],
+                           d_rowmaps ? d_rowmaps[q] : nullptr, nrows[q], &I.ad);
+        if (rc) return rc;
+        I.rowmap = d_rowmaps ? d_rowmaps[q] : nullptr;
+        I.full = d_full[q];
+        I.packed = d_packed[q];
+        I.ldp = ld_packed[q] ? ld_packed[q] : I.ad.xsize;
+        if (I.ldp < I.ad.xsize) return xr_fail(c, XR_ESHAPE, "ld_packed smaller than the packed row");
+        // innermost front stride: s1 for two front axes, else s0
+        const int64_t inner = front1[q] >= 0 ? I.ad.s1 : I.ad.s0;
+        I.tiled = (!I.rowmap && inner == ncol[q] && I.ad.nrest > 0) ? 1 : 0;
+        (I.tiled ? tiled : plain).push_back(I);
     }
-    return XR_OK;
+    int rc = pack_launch(c, plain, 0, mode, alpha);
+    if (rc) return rc;
+    return pack_launch(c, tiled, 1, mode, alpha);
 }
 
 // dst[p, i, q] = sum_j op(U)[i, j] src[p, j, q] (+ beta dst) for many
@@ -438,17 +506,18 @@ struct AxisBatch {
 
 template <int RB>
 __global__ __launch_bounds__(256) void k_axis_apply_multi(AxisBatch bt) {
+    constexpr int NMAX = 4 * RB;              // largest axis length of this instance
     const AxisItem& I = bt.it[blockIdx.y];
-    __shared__ double Ut[64][64 + 1];         // Ut[j][i] = op(U)[i][j]
-    __shared__ double Xs[64][64 + 1];         // Xs[j][c]: operand tile, then output tile (post == 1)
+    __shared__ double Ut[NMAX][NMAX];         // Ut[j][i] = op(U)[i][j]
+    __shared__ double Xs[NMAX][64 + 1];       // Xs[j][c]: operand tile, then output tile (post == 1)
     const int n = I.n;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int64_t post = I.post, cols = I.pre * post;
     const int64_t tiles = (cols + 63) >> 6;
     const double beta = bt.beta;
-    for (int e = tid; e < 64 * 64; e += 256) {
-        const int i = e >> 6, j = e & 63;
+    for (int e = tid; e < NMAX * NMAX; e += 256) {
+        const int i = e / NMAX, j = e - (e / NMAX) * NMAX;
         double v = 0.0;
         if (i < n && j < n) v = I.trans ? I.U[j * n + i] : I.U[i * n + j];
         Ut[j][i] = v;

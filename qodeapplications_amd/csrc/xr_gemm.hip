@@ -516,12 +516,11 @@ __global__ __launch_bounds__(256) void k_dgemm_tn_tall(int64_t M, int64_t N, int
 // a contiguous run of k for all M x N entries (consecutive threads read
 // consecutive k of a row of A) into a partial tile; k_tn_reduce adds the
 // partials in block order.
-template <int NB>
+template <int MB, int NB>
 __global__ __launch_bounds__(256) void k_dgemm_nn_gram(int64_t M, int64_t N, int64_t K,
                                                        const double* __restrict__ A, int64_t lda,
                                                        const double* __restrict__ B, int64_t ldb,
                                                        int64_t rows, double* __restrict__ part) {
-    constexpr int MB = 64 / NB;
     const int64_t k0 = (int64_t)blockIdx.x * rows;
     const int64_t k1 = (K - k0 < rows) ? K : k0 + rows;
     double acc[MB][NB];
@@ -591,21 +590,63 @@ static int launch_tn_skinny(xr_ctx* c, int64_t M, int64_t N, int64_t K, double a
     return XR_OK;
 }
 
+// Workgroups of 256 threads of `kernel` resident on the device at once (CUs
+// x occupancy): a grid of that size runs in one round.
+static int64_t resident_blocks(const void* kernel) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1024;
+    }
+    return (int64_t)(cus > 0 ? cus : 256) * (per > 0 ? per : 1);
+}
+
+// Runs the Gram kernel with MB x NB accumulators per thread: one contiguous
+// run of k per workgroup, as many workgroups as are resident at once (no
+// partly filled second round); the partial tiles go to the workspace.
+template <int MB, int NB>
+static int nn_gram_go(xr_ctx* c, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                      const double* B, int64_t ldb, int64_t* nblk_out, double** part_out) {
+    static const int64_t res = resident_blocks(reinterpret_cast<const void*>(&k_dgemm_nn_gram<MB, NB>));
+    int64_t nblk = ceil_div(K, 2048);
+    if (nblk > res) nblk = res;
+    const int64_t rows = ceil_div(K, nblk);
+    nblk = ceil_div(K, rows);
+    double* part = (double*)xr_workspace(c, 52, nblk * 64 * NB * sizeof(double));
+    if (!part) return xr_fail(c, XR_ENOMEM, "workspace (Gram)");
+    k_dgemm_nn_gram<MB, NB><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
+    *nblk_out = nblk;
+    *part_out = part;
+    return XR_OK;
+}
+
 static int launch_nn_gram(xr_ctx* c, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
                           int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
     const int nb = N <= 1 ? 1 : (N <= 2 ? 2 : (N <= 4 ? 4 : 8));
-    int64_t nblk = ceil_div(K, 4096);
-    if (nblk > 1024) nblk = 1024;
-    const int64_t rows = ceil_div(K, nblk);
-    nblk = ceil_div(K, rows);
-    double* part = (double*)xr_workspace(c, 52, nblk * 64 * nb * sizeof(double));
-    if (!part) return xr_fail(c, XR_ENOMEM, "workspace (Gram)");
+    const int mb = M <= 8 ? 8 : (M <= 16 ? 16 : (M <= 32 ? 32 : 64));
     ProfScope prof(c, XR_K_GEMM_BW, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
     prof.shape(M, N, K, 1, 0, 0);
-    if (nb == 1) k_dgemm_nn_gram<1><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
-    else if (nb == 2) k_dgemm_nn_gram<2><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
-    else if (nb == 4) k_dgemm_nn_gram<4><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
-    else k_dgemm_nn_gram<8><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
+    int64_t nblk = 0;
+    double* part = nullptr;
+    int rc;
+    if (nb == 1) {
+        if (mb == 8) rc = nn_gram_go<8, 1>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
+        else if (mb == 16) rc = nn_gram_go<16, 1>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
+        else if (mb == 32) rc = nn_gram_go<32, 1>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
+        else rc = nn_gram_go<64, 1>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
+    } else if (nb == 2) {
+        if (mb == 8) rc = nn_gram_go<8, 2>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
+        else if (mb == 16) rc = nn_gram_go<16, 2>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
+        else rc = nn_gram_go<32, 2>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
+    } else if (nb == 4) {
+        if (mb == 8) rc = nn_gram_go<8, 4>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
+        else rc = nn_gram_go<16, 4>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
+    } else {
+        rc = nn_gram_go<8, 8>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
+    }
+    if (rc) return rc;
     XR_LAUNCH_CHECK(c);
     k_tn_reduce<<<(unsigned)(M * N), 256, 0, c->stream>>>(part, nblk, M, N, nb, alpha, beta, C, ldc);
     XR_LAUNCH_CHECK(c);
