@@ -85,28 +85,49 @@ def lowest_eigenpairs(H, k=1, tol=1e-9, max_iter=400, H1=None, block=None):
                     max_iter, block, H.device, precond=pre)
 
 
-def _orthonormalize(T, V=None, passes=2, Vt=None):
+def _orthonormalize(T, V=None, passes=2, Vt=None, extra=None):
     """Columns of T made orthonormal (and orthogonal to V's columns) by
     projected Cholesky QR; the tiny b x b factorisations run on the host.
     The projection is repeated only when it removed more than 3/4 of a
     column's squared norm (the Daniel-Gragg-Kaufman-Stewart test), since
     each pass reads all of V.  Vt: the basis stored one vector per row
     instead of V.  Returns the kept columns (rank-revealing through the
-    Cholesky pivots)."""
+    Cholesky pivots).  Each pass makes one device-to-host transfer; extra (a
+    device vector) rides along with the first, and (T, extra on the host)
+    is returned when it is given."""
     n0 = None
+    ex = None
+
+    def done(T):
+        return T if extra is None else (T, ex)
+
     for p in range(passes):
+        n0_t = None
         if (V is not None and V.shape[1]) or (Vt is not None and Vt.shape[0]):
             if p == 0:
-                n0 = np.diag(mm(T, T, transA=1).cpu().numpy()).copy()
+                n0_t = (T * T).sum(0)
             if Vt is not None:          # basis stored one vector per row
                 mm(Vt, mm(Vt, T), transA=1, alpha=-1.0, out=T, beta=1.0)
             else:
                 mm(V, mm(V, T, transA=1), alpha=-1.0, out=T, beta=1.0)
-        G = mm(T, T, transA=1).cpu().numpy()
+        nb = T.shape[1]
+        parts = [mm(T, T, transA=1).reshape(-1)]
+        if n0_t is not None:
+            parts.append(n0_t)
+        if p == 0 and extra is not None:
+            parts.append(extra.reshape(-1).to(T.dtype))
+        h = torch.cat(parts).cpu().numpy() if len(parts) > 1 else parts[0].cpu().numpy()
+        G = h[:nb * nb].reshape(nb, nb)
+        off = nb * nb
+        if n0_t is not None:
+            n0 = h[off:off + nb].copy()
+            off += nb
+        if p == 0 and extra is not None:
+            ex = h[off:].copy()
         d = np.sqrt(np.maximum(np.diag(G), 0.0))
         keep = d > 1e-10 * max(float(d.max()), 1e-300)
         if not keep.any():
-            return T[:, :0]
+            return done(T[:, :0])
         if not keep.all():
             T = T[:, torch.as_tensor(np.nonzero(keep)[0], device=T.device)].contiguous()
             G = G[np.ix_(keep, keep)]
@@ -125,7 +146,7 @@ def _orthonormalize(T, V=None, passes=2, Vt=None):
         T = mm(T, torch.as_tensor(Linv, device=T.device))
         if n0 is not None and np.all(np.diag(G) > 0.25 * n0):
             break
-    return T.contiguous()
+    return done(T.contiguous())
 
 
 def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=None,
@@ -185,17 +206,11 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         X = mm(Vt[:m], yb, transA=1)
         AX = mm(AVt[:m], yb, transA=1)
         R = AX - X * theta
-        rn = torch.linalg.norm(R[:, :k], dim=0).cpu().numpy()
+        # the residual norms reach the host with the orthonormalisation's
+        # first transfer (one synchronisation fewer per iteration); the
+        # correction computed meanwhile is dropped when they show convergence
+        rn_dev = torch.linalg.norm(R[:, :k], dim=0)
         target = tol * np.maximum(np.abs(w[:k]), 1.0)
-        if np.all(rn <= target):
-            keep_state()
-            return theta[:k].clone(), X[:, :k].contiguous()
-        # rounding floor: the residual is within 100x of the target and has
-        # stopped falling over the last 10 iterations
-        hist.append(float(np.max(rn / target)))
-        if len(hist) > 20 and hist[-1] < 100.0 and min(hist[-10:]) > 0.5 * min(hist[:-10]):
-            keep_state()
-            return theta[:k].clone(), X[:, :k].contiguous()
         if precond is not None:
             T = precond(R, theta)
             if olsen:
@@ -207,7 +222,8 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         else:
             T = R / torch.where((theta[None, :] - diag[:, None]).abs() < 1e-8, torch.full_like(R, 1e-8),
                                 theta[None, :] - diag[:, None])
-        if m + nb > max_space:              # thick restart on the lowest Ritz vectors
+        restart = m + nb > max_space
+        if restart:                         # thick restart on the lowest Ritz vectors
             nk = min(m, max(2 * nb, keep))
             yk = torch.as_tensor(np.ascontiguousarray(y[:, :nk]), device=device)
             Xk, AXk = mm(Vt[:m], yk, transA=1), mm(AVt[:m], yk, transA=1)
@@ -215,19 +231,28 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
             AVt[:nk] = AXk.T
             S = np.diag(w[:nk])
             m = nk
-        Q = _orthonormalize(T.contiguous(), Vt=Vt[:m])
+        Q, rn = _orthonormalize(T.contiguous(), Vt=Vt[:m], extra=rn_dev)
+        if np.all(rn <= target):
+            keep_state()
+            return theta[:k].clone(), X[:, :k].contiguous()
+        # rounding floor: the residual is within 100x of the target and has
+        # stopped falling over the last 10 iterations
+        hist.append(float(np.max(rn / target)))
+        if len(hist) > 20 and hist[-1] < 100.0 and min(hist[-10:]) > 0.5 * min(hist[:-10]):
+            keep_state()
+            return theta[:k].clone(), X[:, :k].contiguous()
         q = Q.shape[1]
         if q == 0:
             keep_state()
             return theta[:k].clone(), X[:, :k].contiguous()
         AQ = torch.empty(n, q, dtype=dt, device=device)
         matmul(Q, AQ)
-        # grow the projected matrix by the new rows / columns
-        VtAQ = mm(Vt[:m], AQ).cpu().numpy()
-        QtAQ = mm(Q, AQ, transA=1).cpu().numpy()
-        S = np.block([[S, VtAQ], [VtAQ.T, QtAQ]])
+        # grow the projected matrix by the new rows / columns: one product
+        # over the extended basis gives V^T A Q and Q^T A Q in one transfer
         Vt[m:m + q] = Q.T
         AVt[m:m + q] = AQ.T
+        VA = mm(Vt[:m + q], AQ).cpu().numpy()
+        S = np.block([[S, VA[:m]], [VA[:m].T, VA[m:]]])
         m += q
     if not strict:
         keep_state()

@@ -48,6 +48,15 @@ def _axis_apply(A, src, dst, pre, n, post, transA=0, beta=0.0):
         gemm_f64(A.unsqueeze(0), src.view(pre, n, post), dst.view(pre, n, post), transA=transA, beta=beta)
 
 
+def _arr(vals, dtype):
+    return np.ascontiguousarray(np.asarray(vals, dtype=dtype))
+
+
+def _ptr(a):
+    """Address of a numpy argument array (kept alive by the caller)."""
+    return a.ctypes.data
+
+
 def _i32(a, device):
     return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=device)
 
@@ -145,60 +154,6 @@ class XROperator:
                             torch.as_tensor(np.ascontiguousarray(U), device=self.device))
                 self._eig.append(d)
         return self._eig
-
-    def _pack_multi(self, items, mode, alpha=1.0):
-        """items: [(dims, ncol, f0, f1, nrows, full, packed, ld)], one launch."""
-        import ctypes
-        n = len(items)
-        if n == 0:
-            return
-        I, L, P = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p
-        dims = (L * (8 * n))()
-        for q, it in enumerate(items):
-            for a, d in enumerate(it[0]):
-                dims[8 * q + a] = int(d)
-        arr = lambda T, vals: (T * n)(*vals)
-        self.ctx.call("xr_axes_pack_multi", n, arr(I, [len(it[0]) for it in items]), dims,
-                      arr(L, [int(it[1]) for it in items]), arr(I, [int(it[2]) for it in items]),
-                      arr(I, [int(it[3]) for it in items]), None, arr(L, [int(it[4]) for it in items]),
-                      arr(P, [it[5].data_ptr() for it in items]), arr(P, [it[6].data_ptr() for it in items]),
-                      arr(L, [int(it[7]) for it in items]), int(mode), float(alpha))
-
-    def _axis_multi(self, items, beta=0.0):
-        """items: [(U, src, dst, pre, n, post, trans)]: dst = op(U) along the
-        middle axis of the (pre, n, post) views; one launch per 16 items with
-        n <= 64, a GEMM per item otherwise."""
-        import ctypes
-        small = [it for it in items if it[4] <= 64]
-        for U, src, dst, pre, n, post, tr in items:
-            if n > 64:
-                _axis_apply(U, src, dst, pre, n, post, transA=tr, beta=beta)
-        if not small:
-            return
-        k = len(small)
-        I, L, P = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p
-        arr = lambda T, vals: (T * k)(*vals)
-        self.ctx.call("xr_axis_apply_multi", k, arr(P, [it[0].data_ptr() for it in small]),
-                      arr(I, [int(it[4]) for it in small]), arr(I, [int(it[6]) for it in small]),
-                      arr(L, [int(it[3]) for it in small]), arr(L, [int(it[5]) for it in small]),
-                      arr(P, [it[1].data_ptr() for it in small]), arr(P, [it[2].data_ptr() for it in small]),
-                      float(beta))
-
-    def _gemm_grouped(self, probs, alpha=1.0):
-        """C = alpha A B for every (A, B, C) of row-major matrices (unit
-        column stride), all in one tile grid per 24 products."""
-        import ctypes
-        k = len(probs)
-        if k == 0:
-            return
-        L, P = ctypes.c_int64, ctypes.c_void_p
-        arr = lambda T, vals: (T * k)(*vals)
-        self.ctx.call("xr_gemm_grouped", k, 0, 0, arr(L, [A.shape[0] for A, B, C in probs]),
-                      arr(L, [B.shape[1] for A, B, C in probs]), arr(L, [A.shape[1] for A, B, C in probs]),
-                      float(alpha), arr(P, [A.data_ptr() for A, B, C in probs]),
-                      arr(L, [A.stride(0) for A, B, C in probs]), arr(P, [B.data_ptr() for A, B, C in probs]),
-                      arr(L, [B.stride(0) for A, B, C in probs]), arr(P, [C.data_ptr() for A, B, C in probs]),
-                      arr(L, [C.stride(0) for A, B, C in probs]))
 
     def _pack(self, dims, ncol, f0, f1, rowmap, nrows, full, packed, mode, alpha=1.0, ld=0):
         import ctypes
@@ -388,6 +343,7 @@ class SectorSpace:
         self.eig = op._monomer_eig(self.h1)
         # pair transitions: stacked target blocks per (pair, source charges),
         # shared by the sectors of every total charge
+        self._plans = {}
         self.stack = op._stacks
         for (m, n) in op.pairs:
             for cm in charges[m]:
@@ -443,63 +399,138 @@ class SectorSpace:
     def _view(self, X, i):
         return X[int(self.offsets[i]):int(self.offsets[i + 1])]
 
-    def matvec(self, X, Y):
-        """Y = H X for X, Y (size, b) in the sector layout."""
-        op = self.op
-        b = X.shape[1]
-        nf = op.nf
+    # -- launch plans ---------------------------------------------------------
+    def _axis_plan(self, mats, trans, b, off):
+        """Per fragment axis m: the arguments of xr_axis_apply_multi for every
+        tuple (byte offsets of the tuple's block, the same in source and
+        destination), plus the tuples whose axis is longer than 64."""
+        nf = self.op.nf
+        out = []
         for m in range(nf):
-            op._axis_multi([(self.h1[m][t[m]], self._view(X, i), self._view(Y, i),
-                             int(np.prod(self.shapes[i][:m])), self.shapes[i][m],
-                             int(np.prod(self.shapes[i][m + 1:])) * b, 0)
-                            for i, t in enumerate(self.sectors)], beta=0.0 if m == 0 else 1.0)
-        # pair terms: the tuples sharing (m, n) charges share the stacked
-        # blocks, so their packed columns go side by side into one product;
-        # all packs first, then the products of every charge group in one
-        # grouped GEMM, then the unpacks group by group (targets of one
-        # group are distinct tuples; different groups add into the same ones)
-        packs, jobs = [], []
+            fast = [i for i in range(len(self.sectors)) if self.shapes[i][m] <= 64]
+            out.append({
+                "U": _arr([mats[m][self.sectors[i][m]].data_ptr() for i in fast], np.uint64),
+                "n": _arr([self.shapes[i][m] for i in fast], np.int32),
+                "tr": _arr([trans] * len(fast), np.int32),
+                "pre": _arr([int(np.prod(self.shapes[i][:m])) for i in fast], np.int64),
+                "post": _arr([int(np.prod(self.shapes[i][m + 1:])) * b for i in fast], np.int64),
+                "off": _arr([off[i] for i in fast], np.int64),
+                "slow": [i for i in range(len(self.sectors)) if self.shapes[i][m] > 64]})
+        return out
+
+    def _pack_plan(self, items):
+        """items: [(shape, ncol, f0, f1, nrows, full byte offset, packed byte
+        offset, ld)] -> the arguments of xr_axes_pack_multi (offsets still
+        relative to the full / packed bases)."""
+        dims = np.zeros((len(items), 8), dtype=np.int64)
+        for q, it in enumerate(items):
+            dims[q, :len(it[0])] = it[0]
+        return {"k": len(items), "nf": _arr([len(it[0]) for it in items], np.int32), "dims": dims.reshape(-1),
+                "ncol": _arr([it[1] for it in items], np.int64), "f0": _arr([it[2] for it in items], np.int32),
+                "f1": _arr([it[3] for it in items], np.int32), "nrows": _arr([it[4] for it in items], np.int64),
+                "full": _arr([it[5] for it in items], np.int64), "packed": _arr([it[6] for it in items], np.int64),
+                "ld": _arr([it[7] for it in items], np.int64)}
+
+    def _plan(self, b):
+        """Every launch argument of one application to b columns, built once
+        per b: numpy arrays of shapes, fixed pointers (monomer matrices,
+        stacked pair blocks) and byte offsets relative to the vectors and to
+        the two scratch buffers (packed columns P, pair products T)."""
+        pl = self._plans.get(b)
+        if pl is not None:
+            return pl
+        off = [int(o) * b * 8 for o in self.offsets[:-1]]
+        eigU = [{c: e[1] for c, e in self.eig[m].items()} for m in range(self.op.nf)]
+        pl = {"mono": self._axis_plan(self.h1, 0, b, off), "fwd": self._axis_plan(eigU, 1, b, off),
+              "inv": self._axis_plan(eigU, 0, b, off)}
+        packs, grp, unpacks = [], {k: [] for k in ("M", "N", "K", "A", "lda", "B", "C")}, []
+        poff = toff = 0
         for (m, n, cm, cn), members in self.groups.items():
             Hs, targets = self.stack[(m, n, cm, cn)]
             D = self.blocks_of[m][cm][1] * self.blocks_of[n][cn][1]
             widths = [self.sizes[i] // D * b for i in members]
             W = int(sum(widths))
-            P = torch.empty(D, W, dtype=torch.float64, device=op.device)
-            col = 0
+            R = int(Hs.shape[0])
+            col, ups = 0, []
             for i, w in zip(members, widths):
-                packs.append((self.shapes[i], b, m, n, D, self._view(X, i), P[:, col:], W))
-                col += w
-            T = torch.empty(Hs.shape[0], W, dtype=torch.float64, device=op.device)
-            jobs.append((m, n, members, widths, targets, Hs, P, T))
-        op._pack_multi(packs, 0)
-        op._gemm_grouped([(Hs, P, T) for m, n, members, widths, targets, Hs, P, T in jobs])
-        for m, n, members, widths, targets, Hs, P, T in jobs:
-            items, col = [], 0
-            W = T.shape[1]
-            for i, w in zip(members, widths):
+                packs.append((self.shapes[i], b, m, n, D, off[i], poff + 8 * col, W))
                 t = self.sectors[i]
-                for tm, tn, off in targets:
+                for tm, tn, row in targets:
                     tt = list(t)
                     tt[m], tt[n] = tm, tn
                     j = self.index[tuple(tt)]
                     shj = self.shapes[j]
-                    items.append((shj, b, m, n, shj[m] * shj[n], self._view(Y, j), T[off:, col:], W))
+                    ups.append((shj, b, m, n, shj[m] * shj[n], off[j], toff + 8 * (row * W + col), W))
                 col += w
-            op._pack_multi(items, 1)
+            for key, v in (("M", R), ("N", W), ("K", D), ("A", Hs.data_ptr()), ("lda", Hs.stride(0)),
+                           ("B", poff), ("C", toff)):
+                grp[key].append(v)
+            unpacks.append(self._pack_plan(ups))
+            poff += 8 * D * W
+            toff += 8 * R * W
+        pl["pack"] = self._pack_plan(packs)
+        pl["grp"] = {k: _arr(v, np.uint64 if k == "A" else np.int64) for k, v in grp.items()}
+        pl["unpacks"] = unpacks
+        pl["P"], pl["T"] = poff // 8, toff // 8
+        self._plans[b] = pl
+        return pl
+
+    def _run_pack(self, pk, full, packed, mode):
+        if pk["k"]:
+            fp, pp = pk["full"] + full.data_ptr(), pk["packed"] + packed.data_ptr()   # alive through the call
+            self.op.ctx.call("xr_axes_pack_multi", pk["k"], _ptr(pk["nf"]), _ptr(pk["dims"]), _ptr(pk["ncol"]),
+                             _ptr(pk["f0"]), _ptr(pk["f1"]), None, _ptr(pk["nrows"]), _ptr(fp), _ptr(pp),
+                             _ptr(pk["ld"]), int(mode), 1.0)
+
+    def matvec(self, X, Y):
+        """Y = H X for X, Y (size, b) in the sector layout."""
+        op = self.op
+        b = X.shape[1]
+        pl = self._plan(b)
+        for m in range(op.nf):
+            self._axes_step(pl["mono"][m], self.h1, m, X, Y, b, 0, 0.0 if m == 0 else 1.0)
+        # pair terms: the tuples sharing (m, n) charges share the stacked
+        # blocks, so their packed columns go side by side into one product;
+        # all packs first, then the products of every charge group in one
+        # grouped GEMM, then the unpacks group by group (targets of one
+        # group are distinct tuples; different groups add into the same ones)
+        P = torch.empty(pl["P"], dtype=torch.float64, device=op.device)
+        T = torch.empty(pl["T"], dtype=torch.float64, device=op.device)
+        self._run_pack(pl["pack"], X, P, 0)
+        g = pl["grp"]
+        k = len(g["M"])
+        if k:
+            Bp, Cp = g["B"] + P.data_ptr(), g["C"] + T.data_ptr()
+            op.ctx.call("xr_gemm_grouped", k, 0, 0, _ptr(g["M"]), _ptr(g["N"]), _ptr(g["K"]), 1.0, _ptr(g["A"]),
+                        _ptr(g["lda"]), _ptr(Bp), _ptr(g["N"]), _ptr(Cp), _ptr(g["N"]))
+        for up in pl["unpacks"]:
+            self._run_pack(up, Y, T, 1)
         return Y
+
+    def _axes_step(self, ap, mats, m, X, Y, b, trans, beta):
+        """Applies mats[m][charge] (transposed when trans) along axis m of
+        every tuple, X -> Y (+ beta Y)."""
+        k = len(ap["n"])
+        if k:
+            src, dst = ap["off"] + X.data_ptr(), ap["off"] + Y.data_ptr()   # alive through the call
+            self.op.ctx.call("xr_axis_apply_multi", k, _ptr(ap["U"]), _ptr(ap["n"]), _ptr(ap["tr"]),
+                             _ptr(ap["pre"]), _ptr(ap["post"]), _ptr(src), _ptr(dst), float(beta))
+        for i in ap["slow"]:
+            sh = self.shapes[i]
+            _axis_apply(mats[m][self.sectors[i][m]], self._view(X, i), self._view(Y, i), int(np.prod(sh[:m])),
+                        sh[m], int(np.prod(sh[m + 1:])) * b, transA=trans, beta=beta)
 
     def _axes(self, X, inverse):
         """Applies U_m^T (or U_m when inverse) along every fragment axis of
         every tuple: the product eigenbasis of the monomer blocks."""
         b = X.shape[1]
+        pl = self._plan(b)
+        eigU = [{c: e[1] for c, e in self.eig[m].items()} for m in range(self.op.nf)]
         bufs = [X.clone(), torch.empty_like(X)]
         cur = 0
         for m in range(self.op.nf):
-            src, dst = bufs[cur], bufs[1 - cur]
-            self.op._axis_multi([(self.eig[m][t[m]][1], self._view(src, i), self._view(dst, i),
-                                  int(np.prod(self.shapes[i][:m])), self.shapes[i][m],
-                                  int(np.prod(self.shapes[i][m + 1:])) * b, 0 if inverse else 1)
-                                 for i, t in enumerate(self.sectors)])
+            ap = pl["inv" if inverse else "fwd"][m]
+            self._axes_step(ap, eigU, m, bufs[cur], bufs[1 - cur], b, 0 if inverse else 1, 0.0)
             cur = 1 - cur
         return bufs[cur]
 
