@@ -151,7 +151,7 @@ def _orthonormalize(T, V=None, passes=2, Vt=None, extra=None):
 
 def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=None,
              max_space=None, precond=None, start=None, keep=8, olsen=False, strict=True,
-             state=None):
+             state=None, stop_above=None, info=None):
     """Block Davidson.  matmul(X, Y) writes H X into Y for X of shape (n, b);
     precond(R, theta) approximates (H - theta)^-1 R (default: diagonal).
     Converged when every residual norm is below tol * max(1, |theta|), or
@@ -161,25 +161,31 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
     the current Ritz pairs after max_iter instead of raising.  state: a dict
     that receives the subspace on return and, when it already holds one (same
     n and block), continues from it instead of from the start vectors.
-    The subspace lives in preallocated device buffers; the Rayleigh-Ritz
-    eigenproblem and the b x b factorisations are solved on the host."""
+    stop_above: give up (info["abandoned"] = True) as soon as every wanted
+    Ritz value minus twice its residual norm lies above this energy, i.e.
+    when these roots can no longer be among the lowest ones of a larger
+    problem.  info: a dict that receives the iteration count, the last
+    residual norms and the abandoned / converged flags (neither set: the
+    call ran out of iterations).  The subspace lives in preallocated device
+    buffers; the Rayleigh-Ritz eigenproblem and the b x b factorisations are
+    solved on the host."""
     b = block or max(2 * k, 4)
     max_space = max_space or max(10 * b, 48)
     if start is not None:
         max_space = max(max_space, start.shape[1] + b)
     dt = torch.float64
-    if start is not None:
-        V0 = start
-    elif precond is not None:
-        V0 = precond.guess(b)
-    else:
-        idx = torch.argsort(diag)[:b]
-        V0 = torch.zeros(n, b, dtype=dt, device=device)
-        V0[idx, torch.arange(b, device=device)] = 1.0
     if state is not None and state.get("b") == b and state.get("n") == n:
         Vt, AVt, m, S = state["Vt"], state["AVt"], state["m"], state["S"]
         max_space = Vt.shape[0] - b
     else:
+        if start is not None:
+            V0 = start
+        elif precond is not None and hasattr(precond, "guess"):
+            V0 = precond.guess(b)
+        else:
+            idx = torch.argsort(diag)[:b]
+            V0 = torch.zeros(n, b, dtype=dt, device=device)
+            V0[idx, torch.arange(b, device=device)] = 1.0
         # basis and its image stored one vector per row: the tall-skinny
         # products then read contiguous vectors
         Vt = torch.empty(max_space + b, n, dtype=dt, device=device)
@@ -194,9 +200,15 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
 
     def keep_state():
         if state is not None:
-            state.update(Vt=Vt, AVt=AVt, m=m, S=S, b=b, n=n)
+            state.update(Vt=Vt, AVt=AVt, m=m, S=S, b=b, n=n, hist=hist)
 
-    hist = []
+    def converged():
+        if info is not None:
+            info["converged"] = True
+        keep_state()
+        return theta[:k].clone(), X[:, :k].contiguous()
+
+    hist = list(state.get("hist", [])) if state is not None and state.get("n") == n else []
     for it in range(max_iter):
         S = 0.5 * (S + S.T)
         w, y = np.linalg.eigh(S)
@@ -232,19 +244,23 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
             S = np.diag(w[:nk])
             m = nk
         Q, rn = _orthonormalize(T.contiguous(), Vt=Vt[:m], extra=rn_dev)
+        if info is not None:
+            info.update(iters=it + 1, rn=rn, abandoned=False, converged=False)
         if np.all(rn <= target):
+            return converged()
+        if stop_above is not None and np.all(w[:k] - 2.0 * rn > stop_above):
+            if info is not None:
+                info["abandoned"] = True
             keep_state()
             return theta[:k].clone(), X[:, :k].contiguous()
         # rounding floor: the residual is within 100x of the target and has
         # stopped falling over the last 10 iterations
         hist.append(float(np.max(rn / target)))
         if len(hist) > 20 and hist[-1] < 100.0 and min(hist[-10:]) > 0.5 * min(hist[:-10]):
-            keep_state()
-            return theta[:k].clone(), X[:, :k].contiguous()
+            return converged()
         q = Q.shape[1]
         if q == 0:
-            keep_state()
-            return theta[:k].clone(), X[:, :k].contiguous()
+            return converged()
         AQ = torch.empty(n, q, dtype=dt, device=device)
         matmul(Q, AQ)
         # grow the projected matrix by the new rows / columns: one product

@@ -29,7 +29,6 @@ from .gemm import gemm_f64, mm
 
 
 DENSE_SECTOR = 2048
-KEEP_BYTES = 48 << 30         # device memory for screened Davidson subspaces
 
 
 def _block_size(k, size):
@@ -76,7 +75,7 @@ class XROperator:
         self.charges = [np.asarray(c, dtype=np.int64) for c in charges]
         self.pairs = {}
         self._stacks, self._pair_diag = {}, {}
-        self.log = None          # set to a list to record the per-sector screening
+        self.log = None          # set to a list to record the per-sector solves
         for (m, n), H in H2.items():
             if H is None:
                 continue
@@ -203,19 +202,22 @@ class XROperator:
             tot = {t + x for t in tot for x in q}
         return sorted(tot)
 
-    def lowest(self, k=1, tol=1e-9, max_iter=400, block=None, charges=None, screen_tol=1e-3,
-               screen_iter=12):
+    def lowest(self, k=1, tol=1e-9, max_iter=400, block=None, charges=None, chunk=4):
         """(values[k], vectors[n, k]) of the lowest roots over all total
         charges (or the given ones), in the product layout.
 
         H conserves the total charge, so each charge sector is solved on its
-        own compact layout (block size k unless given).  Every sector is
-        first screened (residual screen_tol or screen_iter iterations); only
-        the sectors whose Ritz values can still be among the lowest k (within
-        twice their residual norms) are then refined to tol from their
-        screened vectors."""
-        cand = []
-        kept = 0                      # bytes of screened subspaces kept for the refinement
+        own compact layout (block size k unless given).  Small sectors are
+        diagonalised densely.  The others are iterated in turns of `chunk`
+        Davidson iterations, in the order of their lowest diagonal energy in
+        the monomer eigenbasis, each resuming its own subspace.  Every Ritz
+        value is an upper bound to an eigenvalue of its sector, so the k-th
+        lowest of all current Ritz values bounds the k-th lowest eigenvalue
+        from above; a sector whose Ritz values minus twice their residual
+        norms all lie above that bound cannot contribute and is dropped.  The
+        rest run until converged to tol."""
+        found = []                    # converged: (value, sector space, vector)
+        active = []
         for Q in (charges if charges is not None else self.total_charges()):
             sp = self.sector_space(Q)
             kk = min(k, sp.size)
@@ -223,43 +225,38 @@ class XROperator:
                 M = torch.empty(sp.size, sp.size, dtype=torch.float64, device=self.device)
                 sp.matvec(torch.eye(sp.size, dtype=torch.float64, device=self.device), M)
                 w, v = torch.linalg.eigh(0.5 * (M + M.T))
-                cand.append([sp, w[:kk], v[:, :kk].contiguous(), 0.0, True])
+                found += [(float(w[i]), sp, v[:, i]) for i in range(kk)]
             else:
-                b = block or _block_size(kk, sp.size)
-                need = 16 * sp.size * (max(10 * b, 48) + b)
-                st = {} if kept + need <= KEEP_BYTES else None     # subspace kept for the refinement
-                kept += need if st is not None else 0
-                w, v = davidson(sp.matvec, None, sp.size, kk, max(tol, screen_tol), screen_iter, b,
-                                self.device, precond=sp.precondition, start=sp.guess(b), strict=False,
-                                state=st)
-                sp.dav_state = st
-                # residual norm of the screened roots: an eigenvalue lies within it
-                r = torch.empty_like(v)
-                sp.matvec(v, r)
-                rn = float(torch.linalg.norm(r - v * w[None, :], dim=0).max())
-                cand.append([sp, w, v, 2.0 * rn, False])
-                if self.log is not None:
-                    self.log.append((sp.Q, sp.size, [float(x) for x in w], rn))
-        # k-th lowest upper bound over all sectors; a sector whose lowest Ritz
-        # value minus its residual margin lies above it cannot contribute
-        ups = sorted(float(x) for c in cand for x in c[1])
-        bound = ups[min(k, len(ups)) - 1]
-        for c in cand:                # free the subspaces of sectors that cannot contribute
-            if float(c[1][0]) - c[3] > bound:
-                c[0].dav_state = None
-        found = []
-        for c in cand:
-            sp, w, v, margin, exact = c
-            if float(w[0]) - margin > bound:
-                continue
-            if not exact and tol < screen_tol:
-                b = block or _block_size(v.shape[1], sp.size)
-                start = v if v.shape[1] >= b else torch.cat([v, sp.guess(b)[:, :b - v.shape[1]]], 1)
-                w, v = davidson(sp.matvec, None, sp.size, v.shape[1], tol, max_iter, b, self.device,
-                                precond=sp.precondition, start=start.contiguous(),
-                                state=getattr(sp, "dav_state", None))
-            for i in range(w.shape[0]):
-                found.append((float(w[i]), sp, v[:, i]))
+                active.append({"sp": sp, "kk": kk, "b": block or _block_size(kk, sp.size), "st": {},
+                               "w": None, "iters": 0, "lam": float(sp.lam.min())})
+        active.sort(key=lambda a: a["lam"])
+
+        def upper():
+            vals = [f[0] for f in found] + [float(x) for a in active if a["w"] is not None for x in a["w"]]
+            vals.sort()
+            return vals[k - 1] if len(vals) >= k else None
+
+        while active:
+            for a in list(active):
+                sp, inf = a["sp"], {}
+                w, v = davidson(sp.matvec, None, sp.size, a["kk"], tol, min(chunk, max_iter - a["iters"]), a["b"],
+                                self.device, precond=sp.precondition,
+                                start=sp.guess(a["b"]) if a["w"] is None else None, strict=False,
+                                state=a["st"], stop_above=upper(), info=inf)
+                a["w"] = [float(x) for x in w]
+                a["iters"] += inf.get("iters", 0)
+                done = inf.get("abandoned", False) or inf.get("converged", False)
+                if not done and a["iters"] >= max_iter:
+                    raise RuntimeError(f"Davidson did not converge in {max_iter} iterations in the sector of "
+                                       f"total charge {sp.Q} (residual {float(np.max(inf['rn'])):.2e})")
+                if done:
+                    active.remove(a)
+                    a["st"].clear()                 # free the subspace
+                    if inf.get("converged", False):
+                        found += [(float(w[i]), sp, v[:, i]) for i in range(w.shape[0])]
+                    if self.log is not None:
+                        self.log.append((sp.Q, sp.size, a["w"], float(np.max(inf.get("rn", [0.0]))), a["iters"],
+                                         inf.get("abandoned", False)))
         found.sort(key=lambda t: t[0])
         found = found[:k]
         vals = torch.tensor([f[0] for f in found], dtype=torch.float64, device=self.device)
