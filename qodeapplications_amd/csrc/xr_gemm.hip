@@ -13,6 +13,7 @@
 #include "xr_common.h"
 #include "xr_gemm.h"
 #include <vector>
+#include <algorithm>
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
@@ -189,16 +190,20 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm(
 }
 
 // Grouped GEMM: up to XR_GROUP_PROBS independent products C_q = alpha
-// op(A_q) op(B_q) of uneven shapes in one launch.  The output tiles of all
-// problems form one grid (problem q owns tiles [first_q, first_q + tiles_q)),
-// so the launch fills the chip evenly where the problems alone would leave a
-// partly empty last round of workgroups.
-#define XR_GROUP_PROBS 24
+// op(A_q) op(B_q) of uneven shapes in one launch, so the launch fills the
+// chip evenly where the problems alone would leave a partly empty last
+// round of workgroups.  Workgroups are handed to the 8 XCDs round-robin
+// (blockIdx % 8); every problem's tiles are split evenly over the XCDs, each
+// XCD taking a contiguous run of them (L2 reuse inside the run), and each
+// XCD works through the problems in order: per-tile work differs between
+// problems (K), and this keeps the XCDs' loads equal.  The host orders the
+// problems by decreasing K, so the longest tiles start first.
+#define XR_GROUP_PROBS 32
 struct GemmProb {
     const double* A;
     const double* B;
     double* C;
-    int64_t M, N, K, lda, ldb, ldc, tiles_m, tiles_n, first;
+    int64_t M, N, K, lda, ldb, ldc, tiles_m, tiles_n;
 };
 struct GemmGroup {
     int n;
@@ -208,11 +213,21 @@ struct GemmGroup {
 
 template <int BM, int BN, bool TA, bool TB>
 __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm_grouped(GemmGroup g) {
-    const int64_t tile = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+    const int x = (int)(blockIdx.x & 7);
+    int64_t j = blockIdx.x >> 3, local = -1;
     int q = 0;
-    while (q + 1 < g.n && tile >= g.p[q + 1].first) q++;
+    for (; q < g.n; q++) {                       // this XCD's share of each problem, in order
+        const int64_t T = g.p[q].tiles_m * g.p[q].tiles_n;
+        const int64_t base = T >> 3, r = T & 7;
+        const int64_t share = base + (x < r ? 1 : 0);
+        if (j < share) {
+            local = x * base + (x < r ? x : r) + j;
+            break;
+        }
+        j -= share;
+    }
+    if (local < 0) return;                       // padding workgroup (uneven XCD shares)
     const GemmProb& P = g.p[q];
-    const int64_t local = tile - P.first;
     const int64_t GROUP = 8;
     const int64_t per_group = GROUP * P.tiles_n;
     const int64_t gi = local / per_group;
@@ -512,50 +527,47 @@ __global__ __launch_bounds__(256) void k_dgemm_tn_tall(int64_t M, int64_t N, int
 }
 
 // C = alpha A B + beta C for A (M x K row-major) with M <= 64 / NB, N <= NB
-// and K large (Gram products of vectors stored one per row).  Each block sums
-// a contiguous run of k for all M x N entries (consecutive threads read
-// consecutive k of a row of A) into a partial tile; k_tn_reduce adds the
-// partials in block order.
-template <int MB, int NB>
+// and K large (Gram products of vectors stored one per row).  Each block
+// sums a contiguous run of k; wave w owns rows [w RB, w RB + RB) and its
+// lanes read consecutive k of those rows (few accumulators per thread, so
+// many waves stay resident to keep loads in flight).  The per-block sums go
+// to a partial tile; k_tn_reduce adds the partials in block order.
+template <int RB, int NB>
 __global__ __launch_bounds__(256) void k_dgemm_nn_gram(int64_t M, int64_t N, int64_t K,
                                                        const double* __restrict__ A, int64_t lda,
                                                        const double* __restrict__ B, int64_t ldb,
                                                        int64_t rows, double* __restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int i0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * RB;
     const int64_t k0 = (int64_t)blockIdx.x * rows;
     const int64_t k1 = (K - k0 < rows) ? K : k0 + rows;
-    double acc[MB][NB];
+    double acc[RB][NB];
 #pragma unroll
-    for (int i = 0; i < MB; i++)
+    for (int r = 0; r < RB; r++)
 #pragma unroll
-        for (int j = 0; j < NB; j++) acc[i][j] = 0.0;
-    for (int64_t k = k0 + threadIdx.x; k < k1; k += 256) {
+        for (int j = 0; j < NB; j++) acc[r][j] = 0.0;
+#pragma unroll 2
+    for (int64_t k = k0 + lane; k < k1; k += 64) {
         double b[NB];
 #pragma unroll
         for (int j = 0; j < NB; j++) b[j] = (j < N) ? B[k * ldb + j] : 0.0;
 #pragma unroll
-        for (int i = 0; i < MB; i++) {
-            if (i < M) {
-                const double a = A[(int64_t)i * lda + k];
+        for (int r = 0; r < RB; r++) {
+            if (i0 + r < M) {
+                const double a = A[(int64_t)(i0 + r) * lda + k];
 #pragma unroll
-                for (int j = 0; j < NB; j++) acc[i][j] = fma(a, b[j], acc[i][j]);
+                for (int j = 0; j < NB; j++) acc[r][j] = fma(a, b[j], acc[r][j]);
             }
         }
     }
-    __shared__ double red[4][MB * NB];
 #pragma unroll
-    for (int i = 0; i < MB; i++)
+    for (int r = 0; r < RB; r++)
 #pragma unroll
         for (int j = 0; j < NB; j++) {
-            double v = acc[i][j];
+            double v = acc[r][j];
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][i * NB + j] = v;
+            if (lane == 0 && i0 + r < M) part[((int64_t)blockIdx.x * 64 + i0 + r) * NB + j] = v;
         }
-    __syncthreads();
-    for (int e = threadIdx.x; e < MB * NB; e += 256) {
-        const int i = e / NB, j = e - (e / NB) * NB;
-        if (i < M)
-            part[((int64_t)blockIdx.x * 64 + i) * NB + j] = red[0][e] + red[1][e] + red[2][e] + red[3][e];
-    }
 }
 
 static int launch_tn_skinny(xr_ctx* c, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
@@ -603,20 +615,20 @@ static int64_t resident_blocks(const void* kernel) {
     return (int64_t)(cus > 0 ? cus : 256) * (per > 0 ? per : 1);
 }
 
-// Runs the Gram kernel with MB x NB accumulators per thread: one contiguous
+// Runs the Gram kernel with RB x NB accumulators per thread: one contiguous
 // run of k per workgroup, as many workgroups as are resident at once (no
 // partly filled second round); the partial tiles go to the workspace.
-template <int MB, int NB>
+template <int RB, int NB>
 static int nn_gram_go(xr_ctx* c, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
                       const double* B, int64_t ldb, int64_t* nblk_out, double** part_out) {
-    static const int64_t res = resident_blocks(reinterpret_cast<const void*>(&k_dgemm_nn_gram<MB, NB>));
+    static const int64_t res = resident_blocks(reinterpret_cast<const void*>(&k_dgemm_nn_gram<RB, NB>));
     int64_t nblk = ceil_div(K, 2048);
     if (nblk > res) nblk = res;
     const int64_t rows = ceil_div(K, nblk);
     nblk = ceil_div(K, rows);
     double* part = (double*)xr_workspace(c, 52, nblk * 64 * NB * sizeof(double));
     if (!part) return xr_fail(c, XR_ENOMEM, "workspace (Gram)");
-    k_dgemm_nn_gram<MB, NB><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
+    k_dgemm_nn_gram<RB, NB><<<(unsigned)nblk, 256, 0, c->stream>>>(M, N, K, A, lda, B, ldb, rows, part);
     *nblk_out = nblk;
     *part_out = part;
     return XR_OK;
@@ -625,27 +637,34 @@ static int nn_gram_go(xr_ctx* c, int64_t M, int64_t N, int64_t K, const double* 
 static int launch_nn_gram(xr_ctx* c, int64_t M, int64_t N, int64_t K, double alpha, const double* A,
                           int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
     const int nb = N <= 1 ? 1 : (N <= 2 ? 2 : (N <= 4 ? 4 : 8));
-    const int mb = M <= 8 ? 8 : (M <= 16 ? 16 : (M <= 32 ? 32 : 64));
+    const int64_t q = ceil_div(M, 4);             // rows per wave
+    const int rb = q <= 1 ? 1 : (q <= 2 ? 2 : (q <= 4 ? 4 : (q <= 8 ? 8 : 16)));
     ProfScope prof(c, XR_K_GEMM_BW, 2.0 * M * N * K, 8.0 * (M * K + K * N + M * N));
     prof.shape(M, N, K, 1, 0, 0);
     int64_t nblk = 0;
     double* part = nullptr;
     int rc;
+#define XR_GRAM_GO(RB_, NB_) rc = nn_gram_go<RB_, NB_>(c, M, N, K, A, lda, B, ldb, &nblk, &part)
     if (nb == 1) {
-        if (mb == 8) rc = nn_gram_go<8, 1>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
-        else if (mb == 16) rc = nn_gram_go<16, 1>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
-        else if (mb == 32) rc = nn_gram_go<32, 1>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
-        else rc = nn_gram_go<64, 1>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
+        if (rb == 1) XR_GRAM_GO(1, 1);
+        else if (rb == 2) XR_GRAM_GO(2, 1);
+        else if (rb == 4) XR_GRAM_GO(4, 1);
+        else if (rb == 8) XR_GRAM_GO(8, 1);
+        else XR_GRAM_GO(16, 1);
     } else if (nb == 2) {
-        if (mb == 8) rc = nn_gram_go<8, 2>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
-        else if (mb == 16) rc = nn_gram_go<16, 2>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
-        else rc = nn_gram_go<32, 2>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
+        if (rb == 1) XR_GRAM_GO(1, 2);
+        else if (rb == 2) XR_GRAM_GO(2, 2);
+        else if (rb == 4) XR_GRAM_GO(4, 2);
+        else XR_GRAM_GO(8, 2);
     } else if (nb == 4) {
-        if (mb == 8) rc = nn_gram_go<8, 4>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
-        else rc = nn_gram_go<16, 4>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
+        if (rb == 1) XR_GRAM_GO(1, 4);
+        else if (rb == 2) XR_GRAM_GO(2, 4);
+        else XR_GRAM_GO(4, 4);
     } else {
-        rc = nn_gram_go<8, 8>(c, M, N, K, A, lda, B, ldb, &nblk, &part);
+        if (rb == 1) XR_GRAM_GO(1, 8);
+        else XR_GRAM_GO(2, 8);
     }
+#undef XR_GRAM_GO
     if (rc) return rc;
     XR_LAUNCH_CHECK(c);
     k_tn_reduce<<<(unsigned)(M * N), 256, 0, c->stream>>>(part, nblk, M, N, nb, alpha, beta, C, ldc);
@@ -834,25 +853,29 @@ static int launch_grouped(xr_ctx* c, const std::vector<int>& ids, int ta, int tb
                           const int64_t* N, const int64_t* K, double alpha, const double* const* A,
                           const int64_t* lda, const double* const* B, const int64_t* ldb, double* const* C,
                           const int64_t* ldc) {
-    for (size_t base = 0; base < ids.size(); base += XR_GROUP_PROBS) {
+    // launches of equal problem counts, problems by decreasing K
+    std::vector<int> order(ids);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return K[a] > K[b]; });
+    const size_t nl = (order.size() + XR_GROUP_PROBS - 1) / XR_GROUP_PROBS;
+    for (size_t l = 0; l < nl; l++) {
         GemmGroup g;
         g.n = 0;
         g.alpha = alpha;
-        int64_t tiles = 0;
+        int64_t slots = 0;                         // workgroups per XCD
         double flops = 0.0, bytes = 0.0;
-        for (size_t t = base; t < ids.size() && t < base + XR_GROUP_PROBS; t++) {
-            const int q = ids[t];
+        for (size_t t = l; t < order.size(); t += nl) {
+            const int q = order[t];
             GemmProb& P = g.p[g.n++];
             P = GemmProb{A[q], B[q], C[q], M[q], N[q], K[q], lda[q], ldb[q], ldc[q],
-                         ceil_div(M[q], BM), ceil_div(N[q], BN), tiles};
-            tiles += P.tiles_m * P.tiles_n;
+                         ceil_div(M[q], BM), ceil_div(N[q], BN)};
+            slots += ceil_div(P.tiles_m * P.tiles_n, 8);
             flops += 2.0 * M[q] * N[q] * K[q];
             bytes += 8.0 * (M[q] * K[q] + K[q] * N[q] + M[q] * N[q]);
         }
-        if (tiles > 0x7fffffff) return xr_fail(c, XR_EUNSUPPORTED, "GEMM grid too large");
+        if (8 * slots > 0x7fffffff) return xr_fail(c, XR_EUNSUPPORTED, "GEMM grid too large");
         ProfScope prof(c, XR_K_GEMM_GROUPED, flops, bytes);
         prof.shape(g.p[0].M, g.p[0].N, g.p[0].K, g.n, ta, tb);
-        const dim3 grid((unsigned)tiles);
+        const dim3 grid((unsigned)(8 * slots));
         if (ta && tb) k_dgemm_grouped<BM, BN, true, true><<<grid, GEMM_THREADS, 0, c->stream>>>(g);
         else if (ta) k_dgemm_grouped<BM, BN, true, false><<<grid, GEMM_THREADS, 0, c->stream>>>(g);
         else if (tb) k_dgemm_grouped<BM, BN, false, true><<<grid, GEMM_THREADS, 0, c->stream>>>(g);

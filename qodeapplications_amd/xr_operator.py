@@ -74,7 +74,7 @@ class XROperator:
         self.ctx = context(self.device.index)
         self.charges = [np.asarray(c, dtype=np.int64) for c in charges]
         self.pairs = {}
-        self._stacks, self._pair_diag = {}, {}
+        self._sq, self._pair_diag = {}, {}
         self.log = None          # set to a list to record the per-sector solves
         for (m, n), H in H2.items():
             if H is None:
@@ -338,41 +338,38 @@ class SectorSpace:
         self.h1 = [{c: op.H1[m][st:st + n, st:st + n].contiguous() for c, (st, n) in self.blocks_of[m].items()}
                    for m in range(nf)]
         self.eig = op._monomer_eig(self.h1)
-        # pair transitions: stacked target blocks per (pair, source charges),
+        # pair terms: H2_mn conserves c_m + c_n, so for each pair and charge
+        # sum s the block between all (c_m, c_n) with that sum is one square
+        # matrix (rows and columns ordered by c_m, then by pair index); it is
         # shared by the sectors of every total charge
         self._plans = {}
-        self.stack = op._stacks
+        self.sq = op._sq
         for (m, n) in op.pairs:
-            for cm in charges[m]:
-                for cn in charges[n]:
-                    if (m, n, cm, cn) in self.stack:
-                        continue
-                    rows, tg = [], []
-                    for tm in charges[m]:
-                        tn = cm + cn - tm
-                        if tn in self.blocks_of[n]:
-                            tg.append((tm, tn, int(sum(r.size for r in rows))))
-                            rows.append(self._pair_index(m, n, tm, tn))
-                    cols = self._pair_index(m, n, cm, cn)
-                    R = np.concatenate(rows)
-                    self.stack[(m, n, cm, cn)] = (op.extract(m, n, R, cols),
-                                                  [(tm, tn, off) for tm, tn, off in tg])
-        self.groups = {}
-        for (m, n) in op.pairs:
-            for i, t in enumerate(self.sectors):
-                self.groups.setdefault((m, n, t[m], t[n]), []).append(i)
+            for s in sorted({cm + cn for cm in charges[m] for cn in charges[n]}):
+                if (m, n, s) in self.sq:
+                    continue
+                parts, idx, off = [], [], 0
+                for cm in charges[m]:
+                    cn = s - cm
+                    if cn in self.blocks_of[n]:
+                        D = self.blocks_of[m][cm][1] * self.blocks_of[n][cn][1]
+                        parts.append((cm, cn, off, D))
+                        idx.append(self._pair_index(m, n, cm, cn))
+                        off += D
+                R = np.concatenate(idx)
+                self.sq[(m, n, s)] = (op.extract(m, n, R, R), parts)
         # diagonal of each charge-conserving pair block in the monomer
         # eigenbasis: diag((Um x Un)^T H2[(cm,cn),(cm,cn)] (Um x Un))
         self.pair_diag = op._pair_diag
-        for (m, n, cm, cn), (Hs, targets) in self.stack.items():
-            if (m, n, cm, cn) in self.pair_diag:
-                continue
-            off = [o for tm, tn, o in targets if tm == cm and tn == cn][0]
-            Dm, Dn = self.blocks_of[m][cm][1], self.blocks_of[n][cn][1]
-            Hself = Hs[off:off + Dm * Dn]
-            K = torch.kron(self.eig[m][cm][1], self.eig[n][cn][1]).contiguous()
-            HK = mm(Hself, K)
-            self.pair_diag[(m, n, cm, cn)] = (K * HK).sum(0).reshape(Dm, Dn)
+        for (m, n, s), (Hss, parts) in self.sq.items():
+            for cm, cn, off, D in parts:
+                if (m, n, cm, cn) in self.pair_diag:
+                    continue
+                Dm, Dn = self.blocks_of[m][cm][1], self.blocks_of[n][cn][1]
+                Hself = Hss[off:off + D, off:off + D].contiguous()
+                K = torch.kron(self.eig[m][cm][1], self.eig[n][cn][1]).contiguous()
+                HK = mm(Hself, K)
+                self.pair_diag[(m, n, cm, cn)] = (K * HK).sum(0).reshape(Dm, Dn)
         lam = []
         for i, t in enumerate(self.sectors):
             l = torch.zeros(self.shapes[i], dtype=torch.float64, device=dev)
@@ -431,7 +428,7 @@ class SectorSpace:
     def _plan(self, b):
         """Every launch argument of one application to b columns, built once
         per b: numpy arrays of shapes, fixed pointers (monomer matrices,
-        stacked pair blocks) and byte offsets relative to the vectors and to
+        square pair blocks) and byte offsets relative to the vectors and to
         the two scratch buffers (packed columns P, pair products T)."""
         pl = self._plans.get(b)
         if pl is not None:
@@ -440,35 +437,45 @@ class SectorSpace:
         eigU = [{c: e[1] for c, e in self.eig[m].items()} for m in range(self.op.nf)]
         pl = {"mono": self._axis_plan(self.h1, 0, b, off), "fwd": self._axis_plan(eigU, 1, b, off),
               "inv": self._axis_plan(eigU, 0, b, off)}
+        # pair terms, per pair (m, n) and charge sum s: the sector's tuples
+        # with t_m + t_n = s are the row blocks (c_m, c_n) x column blocks
+        # (charges of the other fragments, sorted) of one matrix P_s; the
+        # product T_s = H_ss P_s has the same layout, so the unpack of every
+        # tuple of the pair reads its own block of T_s
+        nf = self.op.nf
         packs, grp, unpacks = [], {k: [] for k in ("M", "N", "K", "A", "lda", "B", "C")}, []
-        poff = toff = 0
-        for (m, n, cm, cn), members in self.groups.items():
-            Hs, targets = self.stack[(m, n, cm, cn)]
-            D = self.blocks_of[m][cm][1] * self.blocks_of[n][cn][1]
-            widths = [self.sizes[i] // D * b for i in members]
-            W = int(sum(widths))
-            R = int(Hs.shape[0])
-            col, ups = 0, []
-            for i, w in zip(members, widths):
-                packs.append((self.shapes[i], b, m, n, D, off[i], poff + 8 * col, W))
-                t = self.sectors[i]
-                for tm, tn, row in targets:
-                    tt = list(t)
-                    tt[m], tt[n] = tm, tn
-                    j = self.index[tuple(tt)]
-                    shj = self.shapes[j]
-                    ups.append((shj, b, m, n, shj[m] * shj[n], off[j], toff + 8 * (row * W + col), W))
-                col += w
-            for key, v in (("M", R), ("N", W), ("K", D), ("A", Hs.data_ptr()), ("lda", Hs.stride(0)),
-                           ("B", poff), ("C", toff)):
-                grp[key].append(v)
+        poff = 0
+        for (m, n) in self.op.pairs:
+            rest = [a for a in range(nf) if a not in (m, n)]
+            cols = {}
+            for i, t in enumerate(self.sectors):
+                cols.setdefault(t[m] + t[n], {})[tuple(t[a] for a in rest)] = i
+            base, ups = {}, []
+            for s_, L in sorted(cols.items()):
+                Hss, parts = self.sq[(m, n, s_)]
+                W = 0
+                for r in sorted(L):
+                    sh = self.shapes[L[r]]
+                    L[r] = W
+                    W += int(np.prod([sh[a] for a in rest])) * b
+                R = int(Hss.shape[0])
+                rowoff = {(cm, cn): off for cm, cn, off, D in parts}
+                base[s_] = (poff, W, rowoff, L)
+                for key, v in (("M", R), ("N", W), ("K", R), ("A", Hss.data_ptr()), ("lda", Hss.stride(0)),
+                               ("B", poff), ("C", poff)):
+                    grp[key].append(v)
+                poff += 8 * R * W
+            for i, t in enumerate(self.sectors):
+                bo, W, rowoff, L = base[t[m] + t[n]]
+                sh = self.shapes[i]
+                at = bo + 8 * (rowoff[(t[m], t[n])] * W + L[tuple(t[a] for a in rest)])
+                packs.append((sh, b, m, n, sh[m] * sh[n], off[i], at, W))
+                ups.append((sh, b, m, n, sh[m] * sh[n], off[i], at, W))
             unpacks.append(self._pack_plan(ups))
-            poff += 8 * D * W
-            toff += 8 * R * W
         pl["pack"] = self._pack_plan(packs)
         pl["grp"] = {k: _arr(v, np.uint64 if k == "A" else np.int64) for k, v in grp.items()}
         pl["unpacks"] = unpacks
-        pl["P"], pl["T"] = poff // 8, toff // 8
+        pl["P"] = pl["T"] = poff // 8
         self._plans[b] = pl
         return pl
 
@@ -486,11 +493,10 @@ class SectorSpace:
         pl = self._plan(b)
         for m in range(op.nf):
             self._axes_step(pl["mono"][m], self.h1, m, X, Y, b, 0, 0.0 if m == 0 else 1.0)
-        # pair terms: the tuples sharing (m, n) charges share the stacked
-        # blocks, so their packed columns go side by side into one product;
-        # all packs first, then the products of every charge group in one
-        # grouped GEMM, then the unpacks group by group (targets of one
-        # group are distinct tuples; different groups add into the same ones)
+        # pair terms: every pair's tuples packed into its P_s matrices (one
+        # launch set for all pairs), all products T_s = H_ss P_s in one
+        # grouped GEMM, then one unpack launch set per pair (the tuples of one
+        # pair are distinct targets; different pairs add into the same ones)
         P = torch.empty(pl["P"], dtype=torch.float64, device=op.device)
         T = torch.empty(pl["T"], dtype=torch.float64, device=op.device)
         self._run_pack(pl["pack"], X, P, 0)
