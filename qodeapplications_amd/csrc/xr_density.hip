@@ -133,26 +133,38 @@ __global__ void k_transpose(const double* __restrict__ z, int64_t nS, int64_t nC
     }
 }
 
-// Signed row gather.  layout 0: out[K][t][s]; layout 1: out[t][K][s].
-__device__ __forceinline__ void k_gather_elem(int64_t i, const int32_t* __restrict__ tab, int64_t nK, int64_t nT,
-                         const double* __restrict__ zT, int64_t nS, double* __restrict__ out,
-                         int layout) {
-    int64_t s = i % nS;
-    int64_t r = i / nS;
-    int64_t K = r / nT, t = r - K * nT;
-    int32_t e = tab[r];
-    double v = 0.0;
-    if (e > 0) v = zT[(int64_t)(e - 1) * nS + s];
-    else if (e < 0) v = -zT[(int64_t)(-e - 1) * nS + s];
-    if (layout == 0) out[i] = v;
-    else out[(t * nK + K) * nS + s] = v;
+// Signed row gather: row r = K * nT + t of the output is sign * row |e| - 1
+// of zT (nS states) for e = tab[r], zero for e = 0.  layout 0: out[K][t][s];
+// layout 1: out[t][K][s].  A wave copies 64 / SP rows at a time (SP = nS
+// rounded up to a power of two, at most 64; lanes (row, s)), so the
+// per-row index work is done once per row, not per element, and every row
+// is one contiguous read and one contiguous write.
+template <int SP>
+__global__ __launch_bounds__(256) void k_gather_rows(const int32_t* __restrict__ tab, int64_t nK, int64_t nT,
+                                                     const double* __restrict__ zT, int64_t nS,
+                                                     double* __restrict__ out, int layout) {
+    constexpr int RPW = 64 / SP;                 // rows per wave and step
+    const int lane = threadIdx.x & 63;
+    const int sub = lane / SP, s0 = lane % SP;
+    const int64_t rows = nK * nT;
+    const int64_t wave = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW;
+    const int64_t step = (int64_t)gridDim.x * 4 * RPW;
+    for (int64_t r0 = wave; r0 < rows; r0 += step) {
+        const int64_t r = r0 + sub;
+        if (r >= rows) continue;
+        const int32_t e = tab[r];
+        const int64_t K = r / nT, t = r - K * nT;
+        double* dst = out + (layout == 0 ? r : t * nK + K) * nS;
+        if (e == 0) {
+            for (int64_t s = s0; s < nS; s += SP) dst[s] = 0.0;
+        } else {
+            const double sg = e > 0 ? 1.0 : -1.0;
+            const double* src = zT + (int64_t)((e > 0 ? e : -e) - 1) * nS;
+            for (int64_t s = s0; s < nS; s += SP) dst[s] = sg * src[s];
+        }
+    }
 }
 
-__global__ void k_gather(const int32_t* __restrict__ tab, int64_t nK, int64_t nT,
-                         const double* __restrict__ zT, int64_t nS, double* __restrict__ out,
-                         int layout) {
-    XR_GRID_STRIDE(i, nK * nT * nS) k_gather_elem(i, tab, nK, nT, zT, nS, out, layout);
-}
 
 // For each flat index (d_0 ... d_{k-1}) in [0, dim^k): the lexical rank of
 // the sorted tuple and the parity of the sort, as sign * (rank + 1); 0 if an
@@ -293,7 +305,18 @@ int xr_gather(xr_ctx* c, const int32_t* tab, int64_t nK, int64_t nT, const doubl
     int64_t n = nK * nT * nS;
     if (n <= 0) return XR_OK;
     ProfScope prof(c, XR_K_GATHER, 0.0, 8.0 * n + 4.0 * nK * nT);
-    k_gather<<<xr_grid(n), 256, 0, c->stream>>>(tab, nK, nT, zT, nS, out, layout);
+    const int sp = nS <= 1 ? 1 : nS <= 2 ? 2 : nS <= 4 ? 4 : nS <= 8 ? 8 : nS <= 16 ? 16 : nS <= 32 ? 32 : 64;
+    const int64_t waves = ceil_div(nK * nT, 64 / sp);
+    const unsigned grid = xr_grid(waves * 64);
+    switch (sp) {
+    case 1: k_gather_rows<1><<<grid, 256, 0, c->stream>>>(tab, nK, nT, zT, nS, out, layout); break;
+    case 2: k_gather_rows<2><<<grid, 256, 0, c->stream>>>(tab, nK, nT, zT, nS, out, layout); break;
+    case 4: k_gather_rows<4><<<grid, 256, 0, c->stream>>>(tab, nK, nT, zT, nS, out, layout); break;
+    case 8: k_gather_rows<8><<<grid, 256, 0, c->stream>>>(tab, nK, nT, zT, nS, out, layout); break;
+    case 16: k_gather_rows<16><<<grid, 256, 0, c->stream>>>(tab, nK, nT, zT, nS, out, layout); break;
+    case 32: k_gather_rows<32><<<grid, 256, 0, c->stream>>>(tab, nK, nT, zT, nS, out, layout); break;
+    default: k_gather_rows<64><<<grid, 256, 0, c->stream>>>(tab, nK, nT, zT, nS, out, layout); break;
+    }
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }

@@ -33,19 +33,32 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t total) {
     return base + j;
 }
 
+// Staged operand tiles are k-major with rows of BM (BN) + 16 doubles, so
+// rows k and k + 1 fall in opposite halves of the 64 LDS banks and the MFMA
+// fragment reads (16 lanes over the column, rows k and k + 1 per half-wave)
+// touch 32 distinct 8-byte bank pairs.  An operand that arrives K-fastest
+// from global memory (A not transposed, B transposed) is stored with 16
+// lanes over k: there the column's bits 1..3 are XORed with (k >> 1) & 7,
+// which spreads those stores over distinct bank pairs too; the reads apply
+// the same XOR (its ks part is a compile-time constant of the unrolled loop).
+__device__ __forceinline__ int lds_swz(int k, int col) {
+    return col ^ (((k >> 1) & 7) << 1);
+}
+
 // One BM x BN output tile at (m0, n0): C = alpha op(A) op(B) over [0, K).
 template <int BM, int BN, bool TA, bool TB>
 __device__ __forceinline__ void dgemm_tile(int64_t M, int64_t N, int64_t K, double alpha,
                                            const double* __restrict__ A, int64_t lda,
                                            const double* __restrict__ B, int64_t ldb,
                                            double* __restrict__ C, int64_t ldc, int64_t m0, int64_t n0) {
-    constexpr int PAD = 4;
     constexpr int WM = BM / 2, WN = BN / 2;    // wave tile
     constexpr int FM = WM / 16, FN = WN / 16;  // MFMA tiles per wave
     constexpr int A_PER = BM * GEMM_BK / GEMM_THREADS;
     constexpr int B_PER = BN * GEMM_BK / GEMM_THREADS;
-    __shared__ double As[GEMM_BK][BM + PAD];
-    __shared__ double Bs[GEMM_BK][BN + PAD];
+    constexpr int SA = BM + 16, SB = BN + 16;  // LDS row strides (doubles)
+    constexpr bool XA = !TA, XB = TB;          // operands stored with the column XOR
+    __shared__ double As[GEMM_BK * SA];
+    __shared__ double Bs[GEMM_BK * SB];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -93,14 +106,14 @@ __device__ __forceinline__ void dgemm_tile(int64_t M, int64_t N, int64_t K, doub
             int mm, kk;
             if (TA) { mm = tid % BM; kk = tid / BM + i * (GEMM_THREADS / BM); }
             else { kk = tid % GEMM_BK; mm = tid / GEMM_BK + i * (GEMM_THREADS / GEMM_BK); }
-            As[kk][mm] = alpha * ra[i];
+            As[kk * SA + (XA ? lds_swz(kk, mm) : mm)] = alpha * ra[i];
         }
 #pragma unroll
         for (int i = 0; i < B_PER; i++) {
             int nn, kk;
             if (!TB) { nn = tid % BN; kk = tid / BN + i * (GEMM_THREADS / BN); }
             else { kk = tid % GEMM_BK; nn = tid / GEMM_BK + i * (GEMM_THREADS / GEMM_BK); }
-            Bs[kk][nn] = rb[i];
+            Bs[kk * SB + (XB ? lds_swz(kk, nn) : nn)] = rb[i];
         }
     };
 
@@ -119,6 +132,16 @@ __device__ __forceinline__ void dgemm_tile(int64_t M, int64_t N, int64_t K, doub
     }
     const int fr = lane & 15;   // row / column inside a 16-wide fragment
     const int fk = lane >> 4;   // k inside a 4-deep step
+    // fragment read offsets per 4-deep step ks: row ks*4 + fk, column
+    // (w*W + i*16 + fr) with the XOR of row (ks*4 + fk), whose ks part is
+    // 4 ks on the column's bits 2..3
+    int aoff[GEMM_BK / 4], boff[GEMM_BK / 4];
+#pragma unroll
+    for (int ks = 0; ks < GEMM_BK / 4; ks++) {
+        const int x = ((2 * ks + (fk >> 1)) & 7) << 1;
+        aoff[ks] = fk * SA + wm * WM + (XA ? (fr ^ x) : fr);
+        boff[ks] = fk * SB + wn * WN + (XB ? (fr ^ x) : fr);
+    }
     for (int64_t kt = 0; kt < nk; kt++) {
         if (kt + 1 < nk) {
             load_a((kt + 1) * GEMM_BK);
@@ -128,9 +151,9 @@ __device__ __forceinline__ void dgemm_tile(int64_t M, int64_t N, int64_t K, doub
         for (int ks = 0; ks < GEMM_BK / 4; ks++) {
             double af[FM], bf[FN];
 #pragma unroll
-            for (int i = 0; i < FM; i++) af[i] = As[ks * 4 + fk][wm * WM + i * 16 + fr];
+            for (int i = 0; i < FM; i++) af[i] = As[ks * 4 * SA + aoff[ks] + i * 16];
 #pragma unroll
-            for (int j = 0; j < FN; j++) bf[j] = Bs[ks * 4 + fk][wn * WN + j * 16 + fr];
+            for (int j = 0; j < FN; j++) bf[j] = Bs[ks * 4 * SB + boff[ks] + j * 16];
 #pragma unroll
             for (int i = 0; i < FM; i++)
 #pragma unroll
