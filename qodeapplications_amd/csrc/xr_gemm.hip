@@ -695,6 +695,36 @@ static int launch_nn_gram(xr_ctx* c, int64_t M, int64_t N, int64_t K, double alp
     return XR_OK;
 }
 
+// K split for `units` output tiles on `slots` resident workgroups:
+// returns the best score (ideal time / modelled time) and the split.  Rounds
+// of resident workgroups each take K/s + 32 (per-workgroup start-up and
+// store, in k units); the reduction reads and writes (s + 1) M N doubles
+// against the product's 2 M N K flops (~46 flops per byte at the kernel's
+// rate).
+static double split_choice(int64_t units, int64_t slots, int64_t K, int64_t* splits) {
+    auto score = [&](int64_t s) {
+        const double rounds = std::ceil((double)(units * s) / (double)slots);
+        const double ideal = (double)units * (double)K / (double)slots;
+        const double t = rounds * ((double)K / (double)s + 32.0);
+        return ideal / t / (1.0 + (s > 1 ? (double)(s + 1) * 46.0 / (double)K : 0.0));
+    };
+    double best = score(1);
+    *splits = 1;
+    if (K < 1024 || units >= 4 * slots) return best;
+    const int64_t smax = std::min<int64_t>(64, K / 512);
+    for (int64_t s = 2; s <= smax; s++) {
+        const double e = score(s);
+        if (e > best) { best = e; *splits = s; }
+    }
+    return best;
+}
+
+template <int BM, int BN>
+static int64_t tile_slots() {
+    static const int64_t r = resident_blocks(reinterpret_cast<const void*>(&k_dgemm<BM, BN, false, false>));
+    return r;
+}
+
 template <int BM, int BN>
 static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
                         const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
@@ -704,22 +734,21 @@ static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t
     int64_t tiles = tm * tn;
     if (tiles > 0x7fffffff || batch > 65535)
         return xr_fail(c, XR_EUNSUPPORTED, "GEMM grid too large");
-    // split K when the output tiles alone cannot fill the chip; the partial
-    // products go to scratch and are summed in a fixed order (deterministic)
+    // split K when the output tiles leave the last round of resident
+    // workgroups partly empty; the partial products go to scratch and are
+    // summed in a fixed order (deterministic)
     int64_t splits = 1, kchunk = K;
-    const int64_t target = 512;
     if (splits_req > 0) {                    // from the autotune table
         kchunk = ceil_div(ceil_div(K, splits_req), GEMM_BK) * GEMM_BK;
         if (kchunk < GEMM_BK) kchunk = GEMM_BK;
         splits = ceil_div(K, kchunk);
         if (splits < 1) { splits = 1; kchunk = K; }
-    } else if (tiles * batch < 256 && K >= 1024) {  // (splits may stay 1)
-        splits = ceil_div(target, tiles * batch);
-        int64_t max_split = K / 512;
-        if (splits > max_split) splits = max_split;
-        if (splits < 1) splits = 1;
-        kchunk = ceil_div(ceil_div(K, splits), GEMM_BK) * GEMM_BK;
-        splits = ceil_div(K, kchunk);
+    } else {
+        split_choice(tiles * batch, tile_slots<BM, BN>(), K, &splits);
+        if (splits > 1) {
+            kchunk = ceil_div(ceil_div(K, splits), GEMM_BK) * GEMM_BK;
+            splits = ceil_div(K, kchunk);
+        }
     }
     double* Cw = C;
     int64_t ldw = ldc, sCw = sC, sSplit = 0;
@@ -846,9 +875,12 @@ int xr_dgemm_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, 
     if (it != c->tune.end())
         return launch_cfg(c, it->second.first, it->second.second, ta, tb, M, N, K, alpha, A, lda, sA,
                           B, ldb, sB, beta, C, ldc, sC, batch);
-    const int64_t want = 512;  // workgroups that keep 256 CUs busy
-    int64_t big = ceil_div(M, 128) * ceil_div(N, 128) * batch;
-    if (big >= want) return launch_tiles<128, 128>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+    // 128 x 128 tiles when they (with a K split where K allows one) fill the
+    // rounds of resident workgroups well; smaller tiles for skinny or
+    // small problems
+    int64_t big = ceil_div(M, 128) * ceil_div(N, 128) * batch, s128 = 1;
+    if (big >= 512 || (M > 64 && N > 64 && split_choice(big, tile_slots<128, 128>(), K, &s128) >= 0.8))
+        return launch_tiles<128, 128>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
     if (N <= 64 && M > 64)
         return launch_tiles<128, 64>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
     if (M <= 64 && N > 64)
