@@ -170,7 +170,7 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
     buffers; the Rayleigh-Ritz eigenproblem and the b x b factorisations are
     solved on the host."""
     b = block or max(2 * k, 4)
-    max_space = max_space or max(10 * b, 48)
+    max_space = max_space or max(8 * b, 24)
     if start is not None:
         max_space = max(max_space, start.shape[1] + b)
     dt = torch.float64

@@ -529,13 +529,14 @@ class SectorSpace:
         b = X.shape[1]
         pl = self._plan(b)
         eigU = [{c: e[1] for c, e in self.eig[m].items()} for m in range(self.op.nf)]
-        bufs = [X.clone(), torch.empty_like(X)]
-        cur = 0
+        bufs = [torch.empty_like(X), torch.empty_like(X)]
+        src = X                               # read only; the steps ping-pong between bufs
         for m in range(self.op.nf):
             ap = pl["inv" if inverse else "fwd"][m]
-            self._axes_step(ap, eigU, m, bufs[cur], bufs[1 - cur], b, 0 if inverse else 1, 0.0)
-            cur = 1 - cur
-        return bufs[cur]
+            dst = bufs[m % 2]
+            self._axes_step(ap, eigU, m, src, dst, b, 0 if inverse else 1, 0.0)
+            src = dst
+        return src
 
     def precondition(self, R, theta):
         """(Hd - theta)^-1 R, Hd the diagonal of H in the product eigenbasis of
@@ -543,7 +544,7 @@ class SectorSpace:
         Y = self._axes(R.contiguous(), False)
         den = self.lam[:, None] - theta[None, :]
         den = torch.where(den.abs() < 1e-8, torch.full_like(den, 1e-8), den)
-        return self._axes((Y / den).contiguous(), True)
+        return self._axes(Y.div_(den), True)
 
     def guess(self, b):
         """The b product eigenstates with the lowest diagonal energy."""
