@@ -151,7 +151,7 @@ def _orthonormalize(T, V=None, passes=2, Vt=None, extra=None):
 
 def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=None,
              max_space=None, precond=None, start=None, keep=8, olsen=False, strict=True,
-             state=None, stop_above=None, info=None):
+             state=None, stop_above=None, stop_margin=4.0, info=None):
     """Block Davidson.  matmul(X, Y) writes H X into Y for X of shape (n, b);
     precond(R, theta) approximates (H - theta)^-1 R (default: diagonal).
     Converged when every residual norm is below tol * max(1, |theta|), or
@@ -162,9 +162,11 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
     that receives the subspace on return and, when it already holds one (same
     n and block), continues from it instead of from the start vectors.
     stop_above: give up (info["abandoned"] = True) as soon as every wanted
-    Ritz value minus twice its residual norm lies above this energy, i.e.
-    when these roots can no longer be among the lowest ones of a larger
-    problem.  info: a dict that receives the iteration count, the last
+    Ritz value minus stop_margin times its residual norm lies above this
+    energy, i.e. when these roots can no longer be among the lowest ones of
+    a larger problem (a heuristic: an eigenvalue lies within one residual
+    norm of each Ritz value, but early in the iteration not necessarily the
+    lowest one, hence the margin).  info: a dict that receives the iteration count, the last
     residual norms and the abandoned / converged flags (neither set: the
     call ran out of iterations).  The subspace lives in preallocated device
     buffers; the Rayleigh-Ritz eigenproblem and the b x b factorisations are
@@ -248,7 +250,7 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
             info.update(iters=it + 1, rn=rn, abandoned=False, converged=False)
         if np.all(rn <= target):
             return converged()
-        if stop_above is not None and np.all(w[:k] - 2.0 * rn > stop_above):
+        if stop_above is not None and np.all(w[:k] - stop_margin * rn > stop_above):
             if info is not None:
                 info["abandoned"] = True
             keep_state()
