@@ -710,8 +710,8 @@ static double split_choice(int64_t units, int64_t slots, int64_t K, int64_t* spl
     };
     double best = score(1);
     *splits = 1;
-    if (K < 1024 || units >= 4 * slots) return best;
-    const int64_t smax = std::min<int64_t>(64, K / 512);
+    if (K < 512 || units >= 4 * slots) return best;
+    const int64_t smax = std::min<int64_t>(64, K / 256);
     for (int64_t s = 2; s <= smax; s++) {
         const double e = score(s);
         if (e > best) { best = e; *splits = s; }
@@ -881,6 +881,8 @@ int xr_dgemm_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, 
     int64_t big = ceil_div(M, 128) * ceil_div(N, 128) * batch, s128 = 1;
     if (big >= 512 || (M > 64 && N > 64 && split_choice(big, tile_slots<128, 128>(), K, &s128) >= 0.8))
         return launch_tiles<128, 128>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+    if (big < 64)                  // small products: the most (and K-split) workgroups
+        return launch_tiles<64, 64>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
     if (N <= 64 && M > 64)
         return launch_tiles<128, 64>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
     if (M <= 64 && N > 64)

@@ -56,8 +56,24 @@ def _ptr(a):
     return a.ctypes.data
 
 
+_I32_CACHE = {}
+
+
 def _i32(a, device):
-    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=device)
+    """Device int32 copy of an index array.  The index arrays depend only on
+    the fragments' charge structure, so the copies are kept (keyed by
+    content) and reused by the operators of later iterations instead of
+    paying one synchronous host-to-device copy each."""
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    key = (str(device), a.size, hash(a.tobytes()))
+    hit = _I32_CACHE.get(key)
+    if hit is not None and np.array_equal(hit[0], a):
+        return hit[1]
+    t = torch.as_tensor(a, device=device)
+    if len(_I32_CACHE) > 4096:
+        _I32_CACHE.clear()
+    _I32_CACHE[key] = (a, t)
+    return t
 
 
 class XROperator:
