@@ -110,6 +110,8 @@ class XRAdjoint:
             # the gradient is first order in the eigenvector error: converge
             # the residual well below the energy tolerance
             w, v = op.lowest(k=1, tol=GRAD_TOL)
+        # the root in its charge sector's compact layout (pair densities)
+        self.sector = op.found[0] if op.n > DENSE_MAX else None
         self.E = float(w[0])
         self.v = v[:, 0].contiguous()
         return self.E
@@ -161,7 +163,7 @@ class XRAdjoint:
 
         # ---- pair blocks
         for (m1, m2) in xop.pairs:
-            Gb = xop.pair_density(v, m1, m2)
+            Gb = xop.pair_density(v, m1, m2, sector=self.sector)
             for k1, k2 in X.pair_charge_combos(m1, m2):
                 self._pair_adjoint(m1, m2, k1, k2, Gb, rbar, ib, h_bar)
             del Gb

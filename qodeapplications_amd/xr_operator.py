@@ -90,7 +90,8 @@ class XROperator:
         self.ctx = context(self.device.index)
         self.charges = [np.asarray(c, dtype=np.int64) for c in charges]
         self.pairs = {}
-        self._sq, self._pair_diag = {}, {}
+        self._sq, self._pair_diag, self._sqidx = {}, {}, {}
+        self.found = None
         self.log = None          # set to a list to record the per-sector solves
         for (m, n), H in H2.items():
             if H is None:
@@ -132,7 +133,7 @@ class XROperator:
                 blk = torch.add(blk, blk.T).mul_(0.5)
             blocks.append(blk)
         return {"order": _i32(order, self.device), "inv": _i32(inv, self.device), "segs": segs,
-                "segq": segq, "blocks": blocks, "inv_np": inv}
+                "segq": segq, "blocks": blocks, "inv_np": inv, "order_np": order}
 
     def extract(self, m, n, rows, cols):
         """H2_mn[rows][:, cols] for pair indices a * d_n + b of one charge sum."""
@@ -320,6 +321,7 @@ class XROperator:
                                          inf.get("abandoned", False)))
         found.sort(key=lambda t: t[0])
         found = found[:k]
+        self.found = [(sp, x) for _, sp, x in found]     # the roots in their sector layouts
         vals = torch.tensor([f[0] for f in found], dtype=torch.float64, device=self.device)
         vecs = torch.zeros(self.n, len(found), dtype=torch.float64, device=self.device)
         for i, (_, sp, x) in enumerate(found):
@@ -334,9 +336,13 @@ class XROperator:
         self._pack(self.dims, 1, m, -1, None, d, v, P, 0)
         return mm(P, P, transB=1)
 
-    def pair_density(self, v, m, n):
+    def pair_density(self, v, m, n, sector=None):
         """Diagonal blocks of G_mn = sum over the other fragments of v v^T on
-        the (m, n) pair index, in the charge-sorted order of the pair blocks."""
+        the (m, n) pair index, in the charge-sorted order of the pair blocks.
+        sector = (sector space, v in its layout): the products then run over
+        the sector's tuples only (v vanishes outside its charge sector)."""
+        if sector is not None:
+            return sector[0].pair_density(sector[1], m, n)
         P = self.pairs[(m, n)]
         D = self.dims[m] * self.dims[n]
         Vp = torch.empty(D, self.n // D, dtype=torch.float64, device=self.device)
@@ -419,6 +425,7 @@ class SectorSpace:
                         off += D
                 R = np.concatenate(idx)
                 self.sq[(m, n, s)] = (op.extract(m, n, R, R), parts)
+                op._sqidx[(m, n, s)] = R
         # preconditioner basis: monomer blocks dressed with the partners' mean
         # field; then the diagonal of each charge-conserving pair block in it,
         # diag((Um x Un)^T H2[(cm,cn),(cm,cn)] (Um x Un))
@@ -506,14 +513,14 @@ class SectorSpace:
         # product T_s = H_ss P_s has the same layout, so the unpack of every
         # tuple of the pair reads its own block of T_s
         nf = self.op.nf
-        packs, grp, unpacks = [], {k: [] for k in ("M", "N", "K", "A", "lda", "B", "C")}, []
+        packs, grp, unpacks, pair_sums = [], {k: [] for k in ("M", "N", "K", "A", "lda", "B", "C")}, [], []
         poff = 0
         for (m, n) in self.op.pairs:
             rest = [a for a in range(nf) if a not in (m, n)]
             cols = {}
             for i, t in enumerate(self.sectors):
                 cols.setdefault(t[m] + t[n], {})[tuple(t[a] for a in rest)] = i
-            base, ups = {}, []
+            base, ups, sums = {}, [], []
             for s_, L in sorted(cols.items()):
                 Hss, parts = self.sq[(m, n, s_)]
                 W = 0
@@ -524,6 +531,7 @@ class SectorSpace:
                 R = int(Hss.shape[0])
                 rowoff = {(cm, cn): off for cm, cn, off, D in parts}
                 base[s_] = (poff, W, rowoff, L)
+                sums.append((s_, poff, R, W))
                 for key, v in (("M", R), ("N", W), ("K", R), ("A", Hss.data_ptr()), ("lda", Hss.stride(0)),
                                ("B", poff), ("C", poff)):
                     grp[key].append(v)
@@ -535,7 +543,9 @@ class SectorSpace:
                 packs.append((sh, b, m, n, sh[m] * sh[n], off[i], at, W))
                 ups.append((sh, b, m, n, sh[m] * sh[n], off[i], at, W))
             unpacks.append(self._pack_plan(ups))
+            pair_sums.append(sums)
         pl["pack"] = self._pack_plan(packs)
+        pl["pair_sums"] = pair_sums
         pl["grp"] = {k: _arr(v, np.uint64 if k == "A" else np.int64) for k, v in grp.items()}
         pl["unpacks"] = unpacks
         pl["P"] = pl["T"] = poff // 8
@@ -548,6 +558,36 @@ class SectorSpace:
             self.op.ctx.call("xr_axes_pack_multi", pk["k"], _ptr(pk["nf"]), _ptr(pk["dims"]), _ptr(pk["ncol"]),
                              _ptr(pk["f0"]), _ptr(pk["f1"]), None, _ptr(pk["nrows"]), _ptr(fp), _ptr(pp),
                              _ptr(pk["ld"]), int(mode), 1.0)
+
+    def pair_density(self, x, m, n):
+        """op.pair_density for a vector x of this sector: G_s = P_s P_s^T per
+        charge sum s of the pair (P_s the packed tuples, as in matvec), then
+        reordered from the square-block order into the charge-sorted order of
+        the operator's pair blocks; sums without tuples here give zeros."""
+        op = self.op
+        pl = self._plan(1)
+        pi = list(op.pairs).index((m, n))
+        P = torch.empty(pl["P"], dtype=torch.float64, device=op.device)
+        self._run_pack(pl["unpacks"][pi], x.reshape(-1, 1), P, 0)
+        G = {}
+        for s, boff, R, W in pl["pair_sums"][pi]:
+            Ps = P[boff // 8: boff // 8 + R * W].view(R, W)
+            G[s] = mm(Ps, Ps, transB=1)
+        Pp = op.pairs[(m, n)]
+        blocks = []
+        for (start, cnt), q in zip(Pp["segs"], Pp["segq"]):
+            if q not in G:
+                blocks.append(torch.zeros(cnt, cnt, dtype=torch.float64, device=op.device))
+                continue
+            pos = getattr(self, "_segpos", {}).get((m, n, q))
+            if pos is None:
+                sq = op._sqidx[(m, n, q)]
+                where = np.empty(op.dims[m] * op.dims[n], dtype=np.int64)
+                where[sq] = np.arange(sq.size)
+                pos = torch.as_tensor(where[Pp["order_np"][start:start + cnt]], device=op.device)
+                self.__dict__.setdefault("_segpos", {})[(m, n, q)] = pos
+            blocks.append(G[q].index_select(0, pos).index_select(1, pos))
+        return blocks
 
     def matvec(self, X, Y):
         """Y = H X for X, Y (size, b) in the sector layout."""
