@@ -336,6 +336,9 @@ static int axes_desc(xr_ctx* c, int nf, const int64_t* dims, int64_t ncol, int f
     return XR_OK;
 }
 
+static int axes_pack_tiled_one(xr_ctx* c, const AxesDesc& ad, const int32_t* rowmap, double* full,
+                               double* packed, int64_t ldp, int mode, double alpha);
+
 extern "C" int xr_axes_pack(xr_ctx* c, int nf, const int64_t* dims, int64_t ncol, int front0, int front1,
                             const int32_t* d_rowmap, int64_t nrows, double* d_full, double* d_packed,
                             int64_t ld_packed, int mode, double alpha) {
@@ -348,6 +351,9 @@ extern "C" int xr_axes_pack(xr_ctx* c, int nf, const int64_t* dims, int64_t ncol
     if (ld_packed < ad.xsize) return xr_fail(c, XR_ESHAPE, "ld_packed smaller than the packed row");
     const int64_t total = nrows * ad.xsize;
     if (total <= 0) return XR_OK;
+    const int64_t inner = front1 >= 0 ? ad.s1 : ad.s0;
+    if (inner == ncol && ad.nrest > 0)          // transposing: LDS tiles (pack_launch below)
+        return axes_pack_tiled_one(c, ad, d_rowmap, d_full, d_packed, ld_packed, mode, alpha);
     ProfScope prof(c, XR_K_SCATTER, 0.0, (mode ? 24.0 : 16.0) * total);
     k_axes_pack<<<xr_grid(total), 256, 0, c->stream>>>(ad, d_rowmap, d_full, d_packed, ld_packed, mode, alpha);
     XR_LAUNCH_CHECK(c);
@@ -400,6 +406,7 @@ __global__ __launch_bounds__(256) void k_axes_pack_tiled(AxesBatch bt) {
     const AxesDesc& ad = I.ad;
     __shared__ double L[64][65];               // L[x - x0][r - r0]
     __shared__ int64_t gx[64];
+    __shared__ int rok[64];                    // packed row r0 + i maps to a front index
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int64_t nr = ad.nrows, nx = ad.xsize;
@@ -407,17 +414,22 @@ __global__ __launch_bounds__(256) void k_axes_pack_tiled(AxesBatch bt) {
     for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const int64_t r0 = (t / tx) << 6, x0 = (t % tx) << 6;
         const int64_t r = r0 + lane;
-        const bool rv = r < nr;
-        const int64_t fr = rv ? (r / ad.d1) * ad.s0 + (r % ad.d1) * ad.s1 : 0;
-        __syncthreads();                        // previous tile done with L and gx
-        if (tid < 64) gx[tid] = (x0 + tid < nx) ? axes_rest_offset(ad, x0 + tid) : 0;
+        const int64_t f = r < nr ? (I.rowmap ? (int64_t)I.rowmap[r] : r) : -1;
+        const bool rv = f >= 0 && f < ad.nfront;    // rows mapped outside are left alone
+        const int64_t fr = rv ? (f / ad.d1) * ad.s0 + (f % ad.d1) * ad.s1 : 0;
+        __syncthreads();                        // previous tile done with L, gx and rok
+        if (tid < 64) {
+            gx[tid] = (x0 + tid < nx) ? axes_rest_offset(ad, x0 + tid) : 0;
+            rok[tid] = rv ? 1 : 0;
+        }
         if (bt.mode == 0) {
             __syncthreads();
             for (int xi = w; xi < 64; xi += 4)
                 if (x0 + xi < nx && rv) L[xi][lane] = I.full[fr + gx[xi]];
             __syncthreads();
             for (int ri = w; ri < 64; ri += 4)
-                if (r0 + ri < nr && x0 + lane < nx) I.packed[(r0 + ri) * I.ldp + x0 + lane] = L[lane][ri];
+                if (r0 + ri < nr && x0 + lane < nx && rok[ri])
+                    I.packed[(r0 + ri) * I.ldp + x0 + lane] = L[lane][ri];
         } else {
             for (int ri = w; ri < 64; ri += 4)
                 if (r0 + ri < nr && x0 + lane < nx) L[lane][ri] = I.packed[(r0 + ri) * I.ldp + x0 + lane];
@@ -453,6 +465,18 @@ static int pack_launch(xr_ctx* c, const std::vector<AxesItem>& items, int tiled,
     return XR_OK;
 }
 
+static int axes_pack_tiled_one(xr_ctx* c, const AxesDesc& ad, const int32_t* rowmap, double* full,
+                               double* packed, int64_t ldp, int mode, double alpha) {
+    AxesItem I;
+    I.ad = ad;
+    I.rowmap = rowmap;
+    I.full = full;
+    I.packed = packed;
+    I.ldp = ldp;
+    I.tiled = 1;
+    return pack_launch(c, std::vector<AxesItem>{I}, 1, mode, alpha);
+}
+
 extern "C" int xr_axes_pack_multi(xr_ctx* c, int n_items, const int* nf, const int64_t* dims,
                                   const int64_t* ncol, const int* front0, const int* front1,
                                   const int32_t* const* d_rowmaps, const int64_t* nrows,
@@ -474,7 +498,7 @@ extern "C" int xr_axes_pack_multi(xr_ctx* c, int n_items, const int* nf, const i
         if (I.ldp < I.ad.xsize) return xr_fail(c, XR_ESHAPE, "ld_packed smaller than the packed row");
         // innermost front stride: s1 for two front axes, else s0
         const int64_t inner = front1[q] >= 0 ? I.ad.s1 : I.ad.s0;
-        I.tiled = (!I.rowmap && inner == ncol[q] && I.ad.nrest > 0) ? 1 : 0;
+        I.tiled = (inner == ncol[q] && I.ad.nrest > 0) ? 1 : 0;
         (I.tiled ? tiled : plain).push_back(I);
     }
     int rc = pack_launch(c, plain, 0, mode, alpha);

@@ -101,7 +101,7 @@ def _diagram(name, A, B, rA, rB, kA, kB, ints, n_j0):
         nIB = rB["ca"][kB].shape[0]
         e = precontract(R, ints.U(B, A, A).reshape(-1, 1)).reshape(R.shape[0], R.shape[1])
         eye = torch.eye(nIB, dtype=torch.float64, device=R.device)
-        return torch.einsum("ab,cd->acbd", e, eye)
+        return e.view(e.shape[0], 1, e.shape[1], 1) * eye.view(1, nIB, 1, nIB)   # [a, c, b, d]
     if name == "v0101":
         X, Y, s = rA["ca"][kA], rB["ca"][kB], 4.0
         W = ints.V(A, B, A, B).permute(0, 2, 1, 3).reshape(dA * dA, dB * dB)

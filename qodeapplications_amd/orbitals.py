@@ -16,6 +16,41 @@ import itertools
 import numpy as np
 import torch
 
+from .gemm import mm
+
+
+class _LeadingAxisProduct(torch.autograd.Function):
+    """Y[i..., p] = sum_a X[a, i...] R[a, p] for X viewed as (n, rest): the
+    leading axis is contracted with R and the new axis appended last, so k
+    applications transform all k axes of a tensor (R^T A R for k = 2, the
+    4-index transform of V for k = 4).  Forward and backward run on the
+    library's FP64 GEMM (dX = R dY^T, dR = X dY)."""
+
+    @staticmethod
+    def forward(ctx, X, R):
+        n = R.shape[0]
+        Xm = X.reshape(n, -1).contiguous()
+        ctx.save_for_backward(Xm, R)
+        ctx.shape = X.shape
+        Y = mm(Xm, R.contiguous(), transA=1)                 # (rest, n)
+        return Y.view(*X.shape[1:], R.shape[1])
+
+    @staticmethod
+    def backward(ctx, dY):
+        Xm, R = ctx.saved_tensors
+        dYm = dY.reshape(-1, R.shape[1]).contiguous()
+        dX = mm(R.contiguous(), dYm, transB=1).view(ctx.shape) if ctx.needs_input_grad[0] else None
+        dR = mm(Xm, dYm) if ctx.needs_input_grad[1] else None
+        return dX, dR
+
+
+def transform(A, R):
+    """A with every axis transformed by R: sum A[a, b, ...] R[a, p] R[b, q] ..."""
+    Y = A
+    for _ in range(A.dim()):
+        Y = _LeadingAxisProduct.apply(Y, R)
+    return Y
+
 
 class OrbitalRotation:
     def __init__(self, frag_orbs, h_sp, V_sp, U_sp=None, device=0):
@@ -40,9 +75,9 @@ class OrbitalRotation:
             parts.append(X - X.T)
         K = torch.block_diag(*parts)
         R = torch.linalg.matrix_exp(K)
-        h = R.T @ self.h @ R
-        V = torch.einsum("abcd,ap,bq,cr,ds->pqrs", self.V, R, R, R, R)
-        U = None if self.U is None else torch.stack([R.T @ u @ R for u in self.U])
+        h = transform(self.h, R)
+        V = transform(self.V, R)
+        U = None if self.U is None else torch.stack([transform(u, R) for u in self.U])
         return h, V, U
 
     def _sl(self, m):

@@ -203,8 +203,13 @@ class XROperator:
                     o, D = [(o, D) for c1, c2, o, D in parts if (c1, c2) == (ca, cb)][0]
                     da, db = sp.blocks_of[a][ca][1], sp.blocks_of[b][cb][1]
                     blk = Hss[o:o + D, o:o + D].reshape(da, db, da, db)
-                    ht = ht + (torch.einsum("ibjd,db->ij", blk, rho) if a == m else
-                               torch.einsum("aibj,ba->ij", blk, rho))
+                    # sum over the partner's bra / ket states weighted by rho[ket, bra]
+                    if a == m:         # blk[i, b, j, d] rho[d, b]
+                        lhs, nd = blk.permute(0, 2, 1, 3), db
+                    else:              # blk[b, i, d, j] rho[d, b]
+                        lhs, nd = blk.permute(1, 3, 0, 2), da
+                    ht = ht + mm(lhs.contiguous().view(h.shape[0] * h.shape[0], nd * nd),
+                                 rho.T.contiguous().view(nd * nd, 1)).view(h.shape)
                 hh = ht.cpu().numpy()
                 _, U = np.linalg.eigh(0.5 * (hh + hh.T))
                 U = torch.as_tensor(np.ascontiguousarray(U), device=self.device)
