@@ -45,12 +45,15 @@ __device__ __forceinline__ int lds_swz(int k, int col) {
     return col ^ (((k >> 1) & 7) << 1);
 }
 
-// One BM x BN output tile at (m0, n0): C = alpha op(A) op(B) over [0, K).
-template <int BM, int BN, bool TA, bool TB>
+// One BM x BN output tile at (m0, n0): C = alpha op(A) op(B) over [0, K),
+// or C = alpha op(A) op(B) + beta C with ACC (an epilogue that reads C costs
+// the accumulators' AGPR residency, so only the accumulating variant has it).
+template <int BM, int BN, bool TA, bool TB, bool ACC = false>
 __device__ __forceinline__ void dgemm_tile(int64_t M, int64_t N, int64_t K, double alpha,
                                            const double* __restrict__ A, int64_t lda,
                                            const double* __restrict__ B, int64_t ldb,
-                                           double* __restrict__ C, int64_t ldc, int64_t m0, int64_t n0) {
+                                           double* __restrict__ C, int64_t ldc, int64_t m0, int64_t n0,
+                                           double beta = 0.0) {
     constexpr int WM = BM / 2, WN = BN / 2;    // wave tile
     constexpr int FM = WM / 16, FN = WN / 16;  // MFMA tiles per wave
     constexpr int A_PER = BM * GEMM_BK / GEMM_THREADS;
@@ -177,7 +180,10 @@ __device__ __forceinline__ void dgemm_tile(int64_t M, int64_t N, int64_t K, doub
             for (int r = 0; r < 4; r++) {
                 int64_t gm = m0 + wm * WM + i * 16 + fk + 4 * r;
                 int64_t gn = n0 + wn * WN + j * 16 + fr;
-                if (gm < M && gn < N) C[gm * ldc + gn] = acc[i][j][r];
+                if (gm < M && gn < N) {
+                    if (ACC) C[gm * ldc + gn] = acc[i][j][r] + beta * C[gm * ldc + gn];
+                    else C[gm * ldc + gn] = acc[i][j][r];
+                }
             }
 }
 
@@ -210,6 +216,28 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm(
     const int64_t m0 = tm * BM, n0 = tn * BN;
 
     dgemm_tile<BM, BN, TA, TB>(M, N, K, alpha, A, lda, B, ldb, C, ldc, m0, n0);
+}
+
+// C_b = alpha op(A_b) op(B_b) + beta C_b without a K split, accumulating in
+// the epilogue (saves the workspace pass of the staged path for beta != 0).
+template <int BM, int BN, bool TA, bool TB>
+__global__ __launch_bounds__(GEMM_THREADS) void k_dgemm_acc(
+    int64_t M, int64_t N, int64_t K, double alpha, const double* __restrict__ A, int64_t lda,
+    int64_t sA, const double* __restrict__ B, int64_t ldb, int64_t sB, double beta,
+    double* __restrict__ C, int64_t ldc, int64_t sC, int64_t tiles_m, int64_t tiles_n) {
+    const int64_t batch = blockIdx.z;
+    A += batch * sA;
+    B += batch * sB;
+    C += batch * sC;
+    const int64_t tile = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+    const int64_t GROUP = 8;
+    const int64_t per_group = GROUP * tiles_n;
+    const int64_t g = tile / per_group;
+    const int64_t first_m = g * GROUP;
+    const int64_t gsize = (tiles_m - first_m < GROUP) ? (tiles_m - first_m) : GROUP;
+    const int64_t tm = first_m + (tile % per_group) % gsize;
+    const int64_t tn = (tile % per_group) / gsize;
+    dgemm_tile<BM, BN, TA, TB, true>(M, N, K, alpha, A, lda, B, ldb, C, ldc, tm * BM, tn * BN, beta);
 }
 
 // Grouped GEMM: up to XR_GROUP_PROBS independent products C_q = alpha
@@ -749,6 +777,24 @@ static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t
             kchunk = ceil_div(ceil_div(K, splits), GEMM_BK) * GEMM_BK;
             splits = ceil_div(K, kchunk);
         }
+    }
+    // accumulate in the epilogue when the tiles fit in one round of half the
+    // resident workgroups (the variant's lower occupancy then costs nothing)
+    if (splits == 1 && beta != 0.0 && 2 * tiles * batch <= tile_slots<BM, BN>()) {
+        dim3 grid((unsigned)tiles, 1, (unsigned)batch);
+        ProfScope prof(c, XR_K_GEMM_F64, 2.0 * M * N * K * batch, 8.0 * (M * K + K * N + 2 * M * N) * batch);
+        prof.shape(M, N, K, batch, ta, tb);
+#define XR_GEMM_ACC(TA_, TB_)                                                                    \
+    k_dgemm_acc<BM, BN, TA_, TB_><<<grid, GEMM_THREADS, 0, c->stream>>>(M, N, K, alpha, A, lda, sA, \
+                                                                        B, ldb, sB, beta, C, ldc,  \
+                                                                        sC, tm, tn)
+        if (ta && tb) XR_GEMM_ACC(true, true);
+        else if (ta) XR_GEMM_ACC(true, false);
+        else if (tb) XR_GEMM_ACC(false, true);
+        else XR_GEMM_ACC(false, false);
+#undef XR_GEMM_ACC
+        XR_LAUNCH_CHECK(c);
+        return XR_OK;
     }
     double* Cw = C;
     int64_t ldw = ldc, sCw = sC, sSplit = 0;
