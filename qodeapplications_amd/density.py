@@ -134,16 +134,40 @@ def charge_blocks(z_lists, n_orbs, n_core, device=0):
     return {chg: ChargeBlock.from_states(st, n_orbs, n_core, device) for chg, st in z_lists.items()}
 
 
-def build_density_arrays(blocks, ops_table=OPS_BY_CHG_DIFF_FULL, skip=("ccaa",)):
+def conjugate_string(op):
+    """The string of op^+: reversed, with c and a swapped."""
+    return op[::-1].translate(str.maketrans("ca", "ac"))
+
+
+def conjugate(rho):
+    """<J| op^+ |I>[reversed indices] from <I| op |J>[indices] for real
+    states: (nI, nJ, d, ..., d) -> (nJ, nI, d, ..., d) with the index axes
+    reversed.  An involution, so it also maps the adjoint back."""
+    return rho.permute(1, 0, *range(rho.dim() - 1, 1, -1)).contiguous()
+
+
+def build_density_arrays(blocks, ops_table=OPS_BY_CHG_DIFF_FULL, skip=("ccaa",), derived=None):
     """All densities of one fragment on the device:
     rho[op][(bra_chg, ket_chg)] = tensor (nI, nJ, dim, ...).  Operator strings
     in `skip` are not built (the monomer 4-index term goes through
-    monomer_2e instead)."""
+    monomer_2e instead).  A density whose conjugate string between the
+    swapped charges is in the table is not built again but taken as the
+    conjugate of that one (<I|op|J> = <J|op^+|I> for real states: half the
+    GEMM work of the charge-changing strings); derived, if given, receives
+    {(op, (bra, ket)): (op^+, (ket, bra))} for them."""
     rho = {}
-    for bra_chg, bb in blocks.items():
-        for ket_chg, kb in blocks.items():
-            for op in ops_table.get(bra_chg - ket_chg, ()):
-                if op in skip:
-                    continue
-                rho.setdefault(op, {})[(bra_chg, ket_chg)] = density(op, bb, kb)
+    pairs = [(b, k) for b in blocks for k in blocks]
+    pairs.sort(key=lambda bk: bk[1] - bk[0])        # bra - ket >= 0 first: the conjugate sources
+    for bra_chg, ket_chg in pairs:
+        for op in ops_table.get(bra_chg - ket_chg, ()):
+            if op in skip:
+                continue
+            cop = conjugate_string(op)
+            src = rho.get(cop, {}).get((ket_chg, bra_chg)) if cop != op else None
+            if src is not None:
+                rho.setdefault(op, {})[(bra_chg, ket_chg)] = conjugate(src)
+                if derived is not None:
+                    derived[(op, (bra_chg, ket_chg))] = (cop, (ket_chg, bra_chg))
+            else:
+                rho.setdefault(op, {})[(bra_chg, ket_chg)] = density(op, blocks[bra_chg], blocks[ket_chg])
     return rho

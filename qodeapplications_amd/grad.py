@@ -30,6 +30,7 @@ gradients follow by ordinary autograd through the integral transformation
 import torch
 
 from ._lib import context, ptr
+from .density import conjugate
 from .eigen import DENSE_MAX
 from .gemm import mm as _mm
 from .xr import precision
@@ -170,6 +171,11 @@ class XRAdjoint:
 
         # ---- densities -> coefficients
         for m, F in enumerate(X.frags):
+            # a density taken as the conjugate of another passes its adjoint
+            # back through the same (self-inverse) conjugation
+            for (op, key) in [k for k in rbar[m].d if k in F.derived]:
+                cop, ckey = F.derived[(op, key)]
+                rbar[m].get((cop, ckey), F.rho[cop][ckey].shape).add_(conjugate(rbar[m].d.pop((op, key))))
             for (op, key), Rb in rbar[m].d.items():
                 bb, kb = F.blocks[key[0]], F.blocks[key[1]]
                 ctx.call("xr_density_adjoint", op.encode(), kb.n_orbs, kb.n_core, kb.n_elec,

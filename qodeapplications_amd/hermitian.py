@@ -16,20 +16,9 @@ compression (thresh, natural orbitals, Qode field_op) is not applied.
 import numpy as np
 import torch
 
-from .density import ChargeBlock, density
+from .density import ChargeBlock, conjugate, conjugate_string, density
 
 OPS_XR0 = {2: ("aa",), 1: ("a", "caa"), 0: ("ca", "ccaa")}
-
-
-def conjugate_string(op):
-    """The string of op^+: reversed, with c and a swapped."""
-    return op[::-1].translate(str.maketrans("ca", "ac"))
-
-
-def conjugate(rho):
-    """<j| op^+ |i>[rev idx] from <i| op |j>[idx]: tensor (nI, nJ, d, ..., d)
-    -> (nJ, nI, d, ..., d) with the index axes reversed."""
-    return rho.permute(1, 0, *range(rho.dim() - 1, 1, -1)).contiguous()
 
 
 def build_tensors(states, n_orbs, n_elec_0, n_core=0, device=0, op_strings=OPS_XR0):

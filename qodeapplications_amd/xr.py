@@ -55,12 +55,14 @@ class XRFragment:
             self.pos[chg] = torch.tensor(idx, dtype=torch.int64, device=device)
         self.n_states = {chg: b.n_states for chg, b in self.blocks.items()}
         self.rho = None
+        self.derived = {}
         self.E2 = None
 
     def build(self, V_mm):
         """Densities for all charge pairs (not the 4-index one) and the
         monomer two-electron matrices."""
-        self.rho = build_density_arrays(self.blocks, OPS_BY_CHG_DIFF, skip=("ccaa",))
+        self.derived = {}
+        self.rho = build_density_arrays(self.blocks, OPS_BY_CHG_DIFF, skip=("ccaa",), derived=self.derived)
         self.E2 = {chg: monomer_2e(b, b, V_mm) for chg, b in self.blocks.items()}
 
     def parity(self):

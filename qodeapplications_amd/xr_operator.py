@@ -657,7 +657,8 @@ class SectorSpace:
     def guess(self, b):
         """The b product eigenstates with the lowest diagonal energy."""
         b = min(b, self.size)
-        idx = torch.topk(self.lam, b, largest=False).indices
+        # (a reduction for one vector; a device top-k sorts all entries)
+        idx = torch.argmin(self.lam).view(1) if b == 1 else torch.topk(self.lam, b, largest=False).indices
         E = torch.zeros(self.size, b, dtype=torch.float64, device=self.op.device)
         E[idx, torch.arange(b, device=E.device)] = 1.0
         return self._axes(E, True)

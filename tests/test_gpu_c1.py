@@ -119,8 +119,16 @@ def test_c1_traces(c1):
 
 
 def test_c1_conjugates_and_antisymmetry(c1):
-    """<I|X|J> = <J|X^+|I>: X^+ reverses the string and swaps c <-> a."""
-    _, _, rho = c1
+    """<I|X|J> = <J|X^+|I>: X^+ reverses the string and swaps c <-> a.
+    The build takes the lowering strings (bra charge below ket charge) as
+    conjugates of the raising ones, so those are also built directly here
+    and compared."""
+    _, blocks, rho = c1
+    for op in ("c", "cca", "ccca", "cc"):
+        for (bc, kc), R in rho[op].items():
+            direct = D.density(op, blocks[bc], blocks[kc])
+            assert float((R - direct).abs().max()) < 1e-12, (op, bc, kc)
+            del direct
     pairs = [("a", "c", (1, 0, 2)), ("caa", "cca", (1, 0, 4, 3, 2)),
              ("caaa", "ccca", (1, 0, 5, 4, 3, 2)), ("ccaa", "ccaa", (1, 0, 5, 4, 3, 2)),
              ("ca", "ca", (1, 0, 3, 2))]
