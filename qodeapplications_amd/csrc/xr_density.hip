@@ -350,7 +350,7 @@ static int check_spaces(xr_ctx* c, const OpPlan& pl, int n_orbs, int n_core, int
 // entry points
 // ---------------------------------------------------------------------------
 enum { WS_ZTB = 0, WS_ZTK, WS_TABL, WS_TABR, WS_X, WS_Y, WS_C, WS_IDXL, WS_IDXR, WS_VE, WS_W };
-enum { WS_GC = 40, WS_XB, WS_YB, WS_ZBT, WS_WB, WS_VEB };
+enum { WS_GC = 40, WS_XB, WS_YB, WS_ZBT, WS_WB, WS_VEB, WS_GS };
 
 // The two gathered factors of a density: X (nK x nLt*nI) from the bra
 // coefficients and Y (nK x nT*nJ) from the ket coefficients, over the K
@@ -655,6 +655,23 @@ __global__ void k_add_transpose(const double* __restrict__ zT, int64_t nS, int64
     }
 }
 
+// S = G + G^T for a square n x n matrix (32 x 32 tiles, the transposed tile
+// through LDS).
+__global__ void k_sym_sum(const double* __restrict__ G, int64_t n, double* __restrict__ S) {
+    __shared__ double tile[32][33];
+    const int64_t c0 = (int64_t)blockIdx.x * 32, r0 = (int64_t)blockIdx.y * 32;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    for (int j = ty; j < 32; j += 8) {            // tile[j][tx] = G[c0 + j][r0 + tx]
+        const int64_t r = c0 + j, col = r0 + tx;
+        if (r < n && col < n) tile[j][tx] = G[r * n + col];
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {            // S[r0 + j][c0 + tx] = G[r0 + j][c0 + tx] + G[c0 + tx][r0 + j]
+        const int64_t r = r0 + j, col = c0 + tx;
+        if (r < n && col < n) S[r * n + col] = G[r * n + col] + tile[tx][j];
+    }
+}
+
 static int side_adjoint(xr_ctx* c, const SidePlan& sd, int n_orbs, int n_core, int kspace_core,
                         int n_elec_src, int64_t nK, int64_t nT, const double* Ybar, int64_t nS,
                         int layout, double* zbar, double* tmpT) {
@@ -809,6 +826,24 @@ extern "C" int xr_density_adjoint(xr_ctx* c, const char* ops, int n_orbs, int n_
         k_compress<<<xr_grid(n), 256, 0, c->stream>>>(Rbar, nI, nJ, pl.L.k, pl.R.k, dim,
                                                             f.nLt, f.nT, Gc, Ng);
         XR_LAUNCH_CHECK(c);
+    }
+    // One-body string 'ca' between a block and itself, adjoints into one
+    // buffer: X = Y (both sides remove one electron from the same states,
+    // through the same table) and the two side adjoints are the same map, so
+    // side(Y Gc^T) + side(X Gc) = side(X (Gc + Gc^T)): one GEMM, one side
+    // adjoint.
+    const bool mirror = ops[0] == 'c' && ops[1] == 'a' && ops[2] == 0 && zbra == zket && nI == nJ &&
+                        nP == nQ && Mg == Ng && zbar_bra == zbar_ket && zbar_ket && pl.R.k;
+    if (mirror) {
+        double* Gs = (double*)xr_workspace(c, WS_GS, Mg * Ng * sizeof(double));
+        if (!Gs) return xr_fail(c, XR_ENOMEM, "workspace (adjoint)");
+        dim3 grid((unsigned)ceil_div(Mg, 32), (unsigned)ceil_div(Mg, 32));
+        k_sym_sum<<<grid, dim3(32, 8), 0, c->stream>>>(Gc, Mg, Gs);   // Gs = Gc + Gc^T
+        XR_LAUNCH_CHECK(c);
+        if ((rc = xr_dgemm_launch(c, 0, 0, f.nK, Ng, Mg, 1.0, f.X, Mg, 0, Gs, Ng, 0, 0.0, Ybar, Ng, 0, 1)))
+            return rc;
+        return side_adjoint(c, pl.R, n_orbs, n_core, f.kspace_core, n_elec_ket, f.nK, f.nT, Ybar, nJ, 0,
+                            zbar_ket, zbT);
     }
     // Ybar = X Gc  (nK x Ng);  Xbar = Y Gc^T  (nK x Mg)
     if ((rc = xr_dgemm_launch(c, 0, 0, f.nK, Ng, Mg, 1.0, f.X, Mg, 0, Gc, Ng, 0, 0.0, Ybar, Ng, 0, 1)))
