@@ -309,6 +309,12 @@ int xr_axis_apply_multi(xr_ctx *ctx, int n_items, const double *const *d_U, cons
                         const int *trans, const int64_t *pre, const int64_t *post,
                         const double *const *d_src, double *const *d_dst, double beta);
 
+/* A = A + A^T in place for a square n x n row-major matrix (the XR pair
+ * matrix is symmetric for real states: the blocks of one charge combination
+ * and of its transpose are built once and completed by this; no reference
+ * counterpart, build_H.py:56-102 builds both). */
+int xr_symmetrize(xr_ctx *ctx, double *d_A, int64_t n);
+
 /* Pair-block adjoint input (the pair-outer layout of xr_pair_outer) read from
  * a reduced pair matrix G2 = sum over the other fragments of v v^T, whose
  * rows and columns run over the pair index a*d2 + b, reordered through

@@ -639,6 +639,40 @@ extern "C" int xr_axis_apply_multi(xr_ctx* c, int n_items, const double* const* 
     return XR_OK;
 }
 
+// A = A + A^T in place for a square n x n matrix: workgroup (a, b) with a <= b
+// holds the 32 x 32 tiles (a, b) and (b, a) in LDS and writes both sums (the
+// workgroups with a > b have nothing to do).
+__global__ __launch_bounds__(256) void k_symmetrize(double* __restrict__ A, int64_t n) {
+    const int64_t a = blockIdx.y, b = blockIdx.x;
+    if (a > b) return;
+    __shared__ double P[32][33], Q[32][33];       // P = tile (a, b), Q = tile (b, a)
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int64_t ra = a * 32, rb = b * 32;
+    for (int j = ty; j < 32; j += 8) {
+        const int64_t r1 = ra + j, c1 = rb + tx, r2 = rb + j, c2 = ra + tx;
+        P[j][tx] = (r1 < n && c1 < n) ? A[r1 * n + c1] : 0.0;
+        Q[j][tx] = (r2 < n && c2 < n) ? A[r2 * n + c2] : 0.0;
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {
+        const int64_t r1 = ra + j, c1 = rb + tx, r2 = rb + j, c2 = ra + tx;
+        if (r1 < n && c1 < n) A[r1 * n + c1] = P[j][tx] + Q[tx][j];
+        if (a != b && r2 < n && c2 < n) A[r2 * n + c2] = Q[j][tx] + P[tx][j];
+    }
+}
+
+extern "C" int xr_symmetrize(xr_ctx* c, double* A, int64_t n) {
+    XR_ENTRY(c);
+    if (n < 0) return xr_fail(c, XR_EINVAL, "negative size");
+    if (n == 0) return XR_OK;
+    const int64_t t = ceil_div(n, 32);
+    if (t > 65535) return xr_fail(c, XR_EUNSUPPORTED, "matrix too large for the symmetrize grid");
+    ProfScope prof(c, XR_K_SCATTER, 0.0, 16.0 * n * n);
+    k_symmetrize<<<dim3((unsigned)t, (unsigned)t), dim3(32, 8), 0, c->stream>>>(A, n);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
 // Pair-block adjoint input from a reduced pair matrix G2 (rows and columns
 // over the pair index a * d2 + b, reordered through inv when given):
 //   out[(I1*nJ1+J1)*nB + I2*nJ2+J2] = G2[row(r1[I1], r2[I2]), row(c1[J1], c2[J2])]

@@ -165,8 +165,13 @@ class XRAdjoint:
         # ---- pair blocks
         for (m1, m2) in xop.pairs:
             Gb = xop.pair_density(v, m1, m2, sector=self.sector)
+            # the pair matrix is built as S + S^T from the 'source' blocks and
+            # half of the 'self' ones; Gb is symmetric, so a source block's
+            # adjoint counts twice and the transposed combinations none
             for k1, k2 in X.pair_charge_combos(m1, m2):
-                self._pair_adjoint(m1, m2, k1, k2, Gb, rbar, ib, h_bar)
+                role = X.combo_role(k1, k2)
+                if role != "derived":
+                    self._pair_adjoint(m1, m2, k1, k2, Gb, rbar, ib, h_bar, 1.0 if role == "self" else 2.0)
             del Gb
 
         # ---- densities -> coefficients
@@ -190,7 +195,7 @@ class XRAdjoint:
         self.D = D
         return zbar, ibar.d
 
-    def _pair_adjoint(self, m1, m2, k1, k2, Gb, rbar, ib, h_bar):
+    def _pair_adjoint(self, m1, m2, k1, k2, Gb, rbar, ib, h_bar, factor=1.0):
         X = self.X
         F1, F2 = X.frags[m1], X.frags[m2]
         (i1c, j1c), (i2c, j2c) = k1, k2
@@ -201,7 +206,8 @@ class XRAdjoint:
         sign2 = float((-1) ** ((F2.n_elec_ref - i2c) % 2))
 
         def G(swapped):
-            return self.op.pair_gather(Gb, m1, m2, (F1.pos, F2.pos), k1, k2, swapped)
+            g = self.op.pair_gather(Gb, m1, m2, (F1.pos, F2.pos), k1, k2, swapped)
+            return g.mul_(factor) if factor != 1.0 else g
 
         R1 = lambda op: F1.rho[op][k1]
         R2 = lambda op: F2.rho[op][k2]

@@ -125,6 +125,19 @@ class XRSystem:
                     if c2k in F2.charges:
                         yield (c1b, c1k), (c2b, c2k)
 
+    @staticmethod
+    def combo_role(k1, k2):
+        """How the pair block of charges (k1, k2) is built: 'self' when it is
+        its own transpose (both fragments keep their charges; built at half
+        weight and completed by A + A^T), 'source' when it is built and its
+        transposed combination ((k1[1], k1[0]), (k2[1], k2[0])) is not,
+        'derived' for that transposed one (the XR pair matrix is symmetric
+        for real states)."""
+        kt = ((k1[1], k1[0]), (k2[1], k2[0]))
+        if kt == (k1, k2):
+            return "self"
+        return "source" if (k1, k2) < kt else "derived"
+
     def pair_matrix(self, m1, m2):
         """H2[m1][m2] over the product basis [(a, b) for a in states(m1) for b
         in states(m2)] = build_matrix_elements.dimer (build_H.py:56-102)."""
@@ -134,6 +147,10 @@ class XRSystem:
         out = torch.zeros(D, D, dtype=torch.float64, device=self.device)
         dims = [F1.d, F2.d]
         for k1, k2 in self.pair_charge_combos(m1, m2):
+            role = self.combo_role(k1, k2)
+            if role == "derived":
+                continue
+            half = 0.5 if role == "self" else 1.0
             nI1, nJ1 = F1.n_states[k1[0]], F1.n_states[k1[1]]
             nI2, nJ2 = F2.n_states[k2[0]], F2.n_states[k2[1]]
             rows = [F1.pos[k1[0]], F2.pos[k2[0]]]
@@ -143,14 +160,15 @@ class XRSystem:
             nA, nB = nI1 * nJ1, nI2 * nJ2
             for Hb, swapped, scale in terms:
                 strides = [1, nA] if swapped else [nB, 1]
-                B.scatter(out, D, fr, rows, cols, dims, Hb, strides, scale=scale)
+                B.scatter(out, D, fr, rows, cols, dims, Hb, strides, scale=half * scale)
             for kind, v in extras:
                 if kind == "e2":   # where I1 == J1
-                    B.scatter(out, D, fr, rows, cols, dims, v, [0, 1], diag=[1, 0])
+                    B.scatter(out, D, fr, rows, cols, dims, v, [0, 1], diag=[1, 0], scale=half)
                 else:              # "e1", where I2 == J2
-                    B.scatter(out, D, fr, rows, cols, dims, v, [1, 0], diag=[0, 1])
+                    B.scatter(out, D, fr, rows, cols, dims, v, [1, 0], diag=[0, 1], scale=half)
             if k1[0] == k1[1] and k2[0] == k2[1] and self.nuc[m1, m2] != 0.0:
-                B.scatter(out, D, fr, rows, cols, dims, None, [0, 0], diag=[1, 1], scale=self.nuc[m1, m2])
+                B.scatter(out, D, fr, rows, cols, dims, None, [0, 0], diag=[1, 1], scale=half * self.nuc[m1, m2])
+        context(self.device.index).call("xr_symmetrize", ptr(out), D)
         return out
 
     def trimer_matrix(self, m1, m2, m3):
@@ -224,7 +242,8 @@ class XRSystem:
             raise ValueError("the matrix-free operator carries monomer and pair terms only")
         nf = len(self.frags)
         pairs = {(m, n): H2[m][n] for m in range(nf) for n in range(m + 1, nf)}
-        return XROperator(H1, pairs, self.state_charges(), symmetrize=self.precision != "f64")
+        # pair_matrix returns exactly symmetric matrices in every precision
+        return XROperator(H1, pairs, self.state_charges())
 
     def xr_matrix(self, with_trimers=False, spectator_sign=False, H=None):
         """Tensor-product XR matrix (all fragments need the same number of
