@@ -208,6 +208,15 @@ int xr_dimer_1min1pls_2e(xr_ctx *ctx, int64_t n1, int64_t n2, int64_t nA, const 
                          const double *d_Rcaa2, const double *d_V1112, const double *d_V1222,
                          double *d_out);
 
+/* xr_dimer_1min1pls_1e + xr_dimer_1min1pls_2e in one pass (dimer_1min1pls_1e
+ * :116 plus dimer_1min1pls_2e :129, the sum the reference forms in
+ * build_H.py:56-102): out = [T | Rc1] [Ra2 | S]^T with T = Rc1 h + 2 Rcca1 W1
+ * and S = 2 Rcaa2 W2^T; the output is written once. */
+int xr_dimer_1min1pls(xr_ctx *ctx, int64_t n1, int64_t n2, int64_t nA, const double *d_Rc1,
+                      const double *d_Rcca1, int64_t nB, const double *d_Ra2,
+                      const double *d_Rcaa2, const double *d_h, const double *d_V1112,
+                      const double *d_V1222, double *d_out);
+
 /* 4 sum V[p1,q2,r1,s2] Rca1[a,p1,r1] Rca2[b,q2,s2]   (dimer_ExEx, :163) */
 int xr_dimer_ExEx(xr_ctx *ctx, int64_t n1, int64_t n2, int64_t nA, const double *d_Rca1,
                   int64_t nB, const double *d_Rca2, const double *d_V, double *d_out);

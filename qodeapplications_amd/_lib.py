@@ -57,6 +57,7 @@ SIGNATURES = {
     "xr_dimer_2min2pls": (_I, [_P, _L, _L, _L, _P, _L, _P, _P, _P]),
     "xr_dimer_1min1pls_1e": (_I, [_P, _L, _L, _L, _P, _L, _P, _P, _P]),
     "xr_dimer_1min1pls_2e": (_I, [_P, _L, _L, _L, _P, _P, _L, _P, _P, _P, _P, _P]),
+    "xr_dimer_1min1pls": (_I, [_P, _L, _L, _L, _P, _P, _L, _P, _P, _P, _P, _P, _P]),
     "xr_trimer_2min1pls1pls": (_I, [_P, _L, _L, _L, _L, _P, _L, _P, _L, _P, _P, _P]),
     "xr_trimer_2pls1min1min": (_I, [_P, _L, _L, _L, _L, _P, _L, _P, _L, _P, _P, _P]),
     "xr_trimer_Ex1min1pls": (_I, [_P, _L, _L, _L, _L, _P, _L, _P, _L, _P, _P, _P]),

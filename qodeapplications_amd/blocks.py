@@ -111,11 +111,9 @@ def dimer_block(m1, m2, n_elec_ref2, rho1, rho2, keys1, keys2, ints):
         nA, nB = Rc1.shape[0] * Rc1.shape[1], Ra2.shape[0] * Ra2.shape[1]
         h = ints.h_sum((m1, m2, "1min"), [("T", (m1, m2)), ("U", (m1, m1, m2)), ("U", (m2, m1, m2))])
         out = torch.empty(nA, nB, dtype=torch.float64, device=dev)
-        tmp = torch.empty(nA, nB, dtype=torch.float64, device=dev)
-        L("xr_dimer_1min1pls_1e", n1, n2, nA, ptr(_flat(Rc1)), nB, ptr(_flat(Ra2)), ptr(h), ptr(out))
-        L("xr_dimer_1min1pls_2e", n1, n2, nA, ptr(_flat(Rc1)), ptr(_flat(Rcca1)), nB, ptr(_flat(Ra2)),
-          ptr(_flat(Rcaa2)), ptr(ints.V(m1, m1, m1, m2)), ptr(ints.V(m1, m2, m2, m2)), ptr(tmp))
-        return [(out, False, float(sign2)), (tmp, False, float(sign2))], []
+        L("xr_dimer_1min1pls", n1, n2, nA, ptr(_flat(Rc1)), ptr(_flat(Rcca1)), nB, ptr(_flat(Ra2)),
+          ptr(_flat(Rcaa2)), ptr(h), ptr(ints.V(m1, m1, m1, m2)), ptr(ints.V(m1, m2, m2, m2)), ptr(out))
+        return [(out, False, float(sign2))], []
     if d == +1:
         Rc2, Rcca2 = rho2["c"][keys2], rho2["cca"][keys2]
         Ra1, Rcaa1 = rho1["a"][keys1], rho1["caa"][keys1]
@@ -123,11 +121,9 @@ def dimer_block(m1, m2, n_elec_ref2, rho1, rho2, keys1, keys2, ints):
         nA, nB = Ra1.shape[0] * Ra1.shape[1], Rc2.shape[0] * Rc2.shape[1]
         h = ints.h_sum((m2, m1, "1pls"), [("T", (m2, m1)), ("U", (m1, m2, m1)), ("U", (m2, m2, m1))])
         out = torch.empty(nB, nA, dtype=torch.float64, device=dev)
-        tmp = torch.empty(nB, nA, dtype=torch.float64, device=dev)
-        L("xr_dimer_1min1pls_1e", n2, n1, nB, ptr(_flat(Rc2)), nA, ptr(_flat(Ra1)), ptr(h), ptr(out))
-        L("xr_dimer_1min1pls_2e", n2, n1, nB, ptr(_flat(Rc2)), ptr(_flat(Rcca2)), nA, ptr(_flat(Ra1)),
-          ptr(_flat(Rcaa1)), ptr(ints.V(m2, m2, m2, m1)), ptr(ints.V(m2, m1, m1, m1)), ptr(tmp))
-        return [(out, True, float(-sign2)), (tmp, True, float(-sign2))], []
+        L("xr_dimer_1min1pls", n2, n1, nB, ptr(_flat(Rc2)), ptr(_flat(Rcca2)), nA, ptr(_flat(Ra1)),
+          ptr(_flat(Rcaa1)), ptr(h), ptr(ints.V(m2, m2, m2, m1)), ptr(ints.V(m2, m1, m1, m1)), ptr(out))
+        return [(out, True, float(-sign2))], []
     # d == 0: exciton-exciton coupling plus the spectator terms
     Rca1, Rca2 = rho1["ca"][keys1], rho2["ca"][keys2]
     n1, n2 = Rca1.shape[2], Rca2.shape[2]

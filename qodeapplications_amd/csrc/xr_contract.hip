@@ -128,6 +128,48 @@ extern "C" int xr_dimer_1min1pls_2e(xr_ctx* c, int64_t n1, int64_t n2, int64_t n
     return xr_gemm_2e(c, 0, 1, nA, nB, n1, 2.0, Rc1, n1, 0, S, n1, 0, 1.0, out, nB, 0, 1);
 }
 
+// dst[r * ldd + j] = src[r * lds + j] for r < nr, j < nc
+__global__ void k_copy_cols(const double* __restrict__ src, int64_t lds, int64_t nr, int64_t nc,
+                            double* __restrict__ dst, int64_t ldd) {
+    XR_GRID_STRIDE(i, nr * nc) {
+        const int64_t r = i / nc, j = i - r * nc;
+        dst[r * ldd + j] = src[r * lds + j];
+    }
+}
+
+// The one-electron-transfer block (fragment 1 gains the electron) with its
+// one- and two-electron parts in one product:
+//   out = [T | Rc1] [Ra2 | S]^T,   T = Rc1 h + 2 Rcca1 W1,   S = 2 Rcaa2 W2^T
+// (xr_dimer_1min1pls_1e + xr_dimer_1min1pls_2e, each nA x nB output written
+// once instead of three times).
+extern "C" int xr_dimer_1min1pls(xr_ctx* c, int64_t n1, int64_t n2, int64_t nA, const double* Rc1,
+                                 const double* Rcca1, int64_t nB, const double* Ra2,
+                                 const double* Rcaa2, const double* h, const double* V1112,
+                                 const double* V1222, double* out) {
+    XR_ENTRY(c);
+    if (n1 <= 0 || n2 <= 0 || nA < 0 || nB < 0) return xr_fail(c, XR_EINVAL, "bad sizes");
+    if (nA == 0 || nB == 0) return XR_OK;
+    const int64_t n1c = n1 * n1 * n1, n2c = n2 * n2 * n2, Kc = n1 + n2;
+    double* W1 = (double*)xr_workspace(c, WS_CW1, n1c * n2 * sizeof(double));
+    double* W2 = (double*)xr_workspace(c, WS_CW2, n1 * n2c * sizeof(double));
+    double* Ac = (double*)xr_workspace(c, WS_CG1, nA * Kc * sizeof(double));
+    double* Bc = (double*)xr_workspace(c, WS_CG2, nB * Kc * sizeof(double));
+    if (!W1 || !W2 || !Ac || !Bc) return xr_fail(c, XR_ENOMEM, "workspace");
+    int rc;
+    if ((rc = permute4(c, V1112, n1, n1, n1, n2, 1, 0, 2, 3, W1))) return rc;
+    if ((rc = permute4(c, V1222, n1, n2, n2, n2, 0, 1, 3, 2, W2))) return rc;
+    // Ac[:, :n2] = 2 Rcca1 W1 + Rc1 h;  Ac[:, n2:] = Rc1
+    if ((rc = xr_gemm_2e(c, 0, 0, nA, n2, n1c, 2.0, Rcca1, n1c, 0, W1, n2, 0, 0.0, Ac, Kc, 0, 1))) return rc;
+    if ((rc = xr_dgemm_launch(c, 0, 0, nA, n2, n1, 1.0, Rc1, n1, 0, h, n2, 0, 1.0, Ac, Kc, 0, 1))) return rc;
+    k_copy_cols<<<xr_grid(nA * n1), 256, 0, c->stream>>>(Rc1, n1, nA, n1, Ac + n2, Kc);
+    XR_LAUNCH_CHECK(c);
+    // Bc[:, :n2] = Ra2;  Bc[:, n2:] = 2 Rcaa2 W2^T
+    k_copy_cols<<<xr_grid(nB * n2), 256, 0, c->stream>>>(Ra2, n2, nB, n2, Bc, Kc);
+    XR_LAUNCH_CHECK(c);
+    if ((rc = xr_gemm_2e(c, 0, 1, nB, n1, n2c, 2.0, Rcaa2, n2c, 0, W2, n2c, 0, 0.0, Bc + n2, Kc, 0, 1))) return rc;
+    return xr_gemm_2e(c, 0, 1, nA, nB, Kc, 1.0, Ac, Kc, 0, Bc, Kc, 0, 0.0, out, nB, 0, 1);
+}
+
 // out[a, b, c] = f * sum_{x y r s} W[(x y), r, s] R1[a, (x y)] R2[b, r] R3[c, s]
 // with W the permuted integral block; the three trimer families differ only
 // in the permutation, the prefactor and which densities they take.

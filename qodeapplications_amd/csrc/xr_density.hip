@@ -505,62 +505,72 @@ extern "C" int xr_monomer_2e(xr_ctx* c, int n_orbs, int n_core, int n_elec, int6
 // ---------------------------------------------------------------------------
 // adjoints
 // ---------------------------------------------------------------------------
-// Transpose of k_expand: Gc[(l*nI + I) * ldc + t*nJ + J] = sum over the index
-// orderings of l and t of sign * Rbar[I, J, iL, iR].
-__device__ __forceinline__ void k_compress_elem(int64_t i, const double* __restrict__ Rbar, int64_t nI, int64_t nJ, int kL, int kR,
-                           int dim, int64_t nLt, int64_t nT, double* __restrict__ Gc, int64_t ldc) {
-    int64_t col = i % (nT * nJ), row = i / (nT * nJ);
-    int64_t l = row / nI, I = row - l * nI;
-    int64_t t = col / nJ, J = col - t * nJ;
-    int lo[3] = {0, 0, 0}, ro[3] = {0, 0, 0};
-    if (kL) {
-        uint64_t m = d_lex_unrank(l, dim, kL);
-        for (int j = 0; j < kL; j++) { lo[j] = __ffsll((unsigned long long)m) - 1; m &= m - 1; }
-    }
-    if (kR) {
-        uint64_t m = d_lex_unrank(t, dim, kR);
-        for (int j = 0; j < kR; j++) { ro[j] = __ffsll((unsigned long long)m) - 1; m &= m - 1; }
-    }
-    int64_t nIR = 1, nIL = 1;
-    for (int j = 0; j < kR; j++) nIR *= dim;
-    for (int j = 0; j < kL; j++) nIL *= dim;
-    // permutations of up to 3 items with their parities
+// Orbitals of the lexicographic k-tuple of rank r (ascending).
+__device__ __forceinline__ void tuple_orbs(int64_t r, int dim, int k, int* o) {
+    if (!k) return;
+    uint64_t m = d_lex_unrank(r, dim, k);
+    for (int j = 0; j < k; j++) { o[j] = __ffsll((unsigned long long)m) - 1; m &= m - 1; }
+}
+
+// Sum over the index orderings of the sorted tuples lo (kL) and ro (kR) of
+// sign * plane[iL * nIR + iR].
+__device__ __forceinline__ double compress_sum(const double* __restrict__ plane, const int* lo, const int* ro,
+                                               int kL, int kR, int dim, int64_t nIR) {
     const int P3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
     const int S3[6] = {1, -1, -1, 1, 1, -1};
     const int P2[2][3] = {{0, 1, 0}, {1, 0, 0}};
     const int S2[2] = {1, -1};
-    int nPL = kL == 3 ? 6 : (kL == 2 ? 2 : 1);
-    int nPR = kR == 3 ? 6 : (kR == 2 ? 2 : 1);
-    const double* base = Rbar + (I * nJ + J) * nIL * nIR;
+    const int nPL = kL == 3 ? 6 : (kL == 2 ? 2 : 1);
+    const int nPR = kR == 3 ? 6 : (kR == 2 ? 2 : 1);
     double acc = 0.0;
     for (int a = 0; a < nPL; a++) {
         int64_t iL = 0;
-        int sL = 1;
-        for (int j = 0; j < kL; j++) {
-            int src = kL == 3 ? P3[a][j] : (kL == 2 ? P2[a][j] : 0);
-            iL = iL * dim + lo[src];
-        }
-        if (kL == 3) sL = S3[a];
-        else if (kL == 2) sL = S2[a];
+        for (int j = 0; j < kL; j++) iL = iL * dim + lo[kL == 3 ? P3[a][j] : (kL == 2 ? P2[a][j] : 0)];
+        const int sL = kL == 3 ? S3[a] : (kL == 2 ? S2[a] : 1);
         for (int b = 0; b < nPR; b++) {
             int64_t iR = 0;
-            int sR = 1;
-            for (int j = 0; j < kR; j++) {
-                int src = kR == 3 ? P3[b][j] : (kR == 2 ? P2[b][j] : 0);
-                iR = iR * dim + ro[src];
-            }
-            if (kR == 3) sR = S3[b];
-            else if (kR == 2) sR = S2[b];
-            acc += (double)(sL * sR) * base[iL * nIR + iR];
+            for (int j = 0; j < kR; j++) iR = iR * dim + ro[kR == 3 ? P3[b][j] : (kR == 2 ? P2[b][j] : 0)];
+            const int sR = kR == 3 ? S3[b] : (kR == 2 ? S2[b] : 1);
+            acc += (double)(sL * sR) * plane[iL * nIR + iR];
         }
     }
-    Gc[row * ldc + col] = acc;
+    return acc;
 }
 
-__global__ void k_compress(const double* __restrict__ Rbar, int64_t nI, int64_t nJ, int kL, int kR,
-                           int dim, int64_t nLt, int64_t nT, double* __restrict__ Gc, int64_t ldc) {
-    int64_t total = nLt * nI * nT * nJ;
-    XR_GRID_STRIDE(i, total) k_compress_elem(i, Rbar, nI, nJ, kL, kR, dim, nLt, nT, Gc, ldc);
+// Transpose of k_expand: Gc[(l*nI + I) * ldc + t*nJ + J] = sum over the index
+// orderings of l and t of sign * Rbar[I, J, iL, iR].  Rbar is read along t
+// inside one (I, J) plane and Gc written along J: a workgroup takes 32
+// consecutive J by 32 consecutive t for one (l, I) and turns the tile
+// through LDS.  blockIdx.x = (I, J tile), blockIdx.y = l, blockIdx.z = t tile.
+__global__ __launch_bounds__(256) void k_compress(
+    const double* __restrict__ Rbar, int64_t nI, int64_t nJ, int kL, int kR, int dim, int64_t nT,
+    double* __restrict__ Gc, int64_t ldc) {
+    __shared__ double tile[32][33];
+    const int64_t jt = (nJ + 31) / 32;
+    const int64_t I = blockIdx.x / jt, J0 = (blockIdx.x - I * jt) * 32;
+    const int64_t l = blockIdx.y, t0 = (int64_t)blockIdx.z * 32;
+    int64_t nIL = 1, nIR = 1;
+    for (int j = 0; j < kL; j++) nIL *= dim;
+    for (int j = 0; j < kR; j++) nIR *= dim;
+    int lo[3] = {0, 0, 0}, ro[3] = {0, 0, 0};
+    tuple_orbs(l, dim, kL, lo);
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int64_t t = t0 + tx;
+    if (t < nT) tuple_orbs(t, dim, kR, ro);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int jj = ty + 8 * q;
+        const int64_t J = J0 + jj;
+        if (t < nT && J < nJ)
+            tile[jj][tx] = compress_sum(Rbar + (I * nJ + J) * nIL * nIR, lo, ro, kL, kR, dim, nIR);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int tt = ty + 8 * q;
+        const int64_t J = J0 + tx;
+        if (t0 + tt < nT && J < nJ) Gc[(l * nI + I) * ldc + (t0 + tt) * nJ + J] = tile[tx][tt];
+    }
 }
 
 // Transpose of k_gather through the side table: for each source determinant
@@ -823,8 +833,10 @@ extern "C" int xr_density_adjoint(xr_ctx* c, const char* ops, int n_orbs, int n_
     {
         int64_t n = Mg * Ng;
         ProfScope prof(c, XR_K_EXPAND, 0.0, 8.0 * n * (pl.k > 2 ? 4 : 2));
-        k_compress<<<xr_grid(n), 256, 0, c->stream>>>(Rbar, nI, nJ, pl.L.k, pl.R.k, dim,
-                                                            f.nLt, f.nT, Gc, Ng);
+        if (nI * ceil_div(nJ, 32) > 0x7fffffffLL || f.nLt > 65535 || ceil_div(f.nT, 32) > 65535)
+            return xr_fail(c, XR_EUNSUPPORTED, "density adjoint grid too large");
+        dim3 grid((unsigned)(nI * ceil_div(nJ, 32)), (unsigned)f.nLt, (unsigned)ceil_div(f.nT, 32));
+        k_compress<<<grid, 256, 0, c->stream>>>(Rbar, nI, nJ, pl.L.k, pl.R.k, dim, f.nT, Gc, Ng);
         XR_LAUNCH_CHECK(c);
     }
     // One-body string 'ca' between a block and itself, adjoints into one

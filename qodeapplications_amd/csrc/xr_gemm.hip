@@ -1139,18 +1139,19 @@ __device__ __forceinline__ int xr_group8_max(int e) {
     return e;
 }
 
-#define SPLIT_THREADS 512
-
-template <bool TA, bool TB>
-__global__ __launch_bounds__(SPLIT_THREADS) void k_dgemm_split(
+// BM x BM tiles on 4 * BM threads (4 elements of each operand per thread
+// and 16-deep k step): 128 x 128 as 8 waves of 64 x 32, 64 x 64 as 4 waves
+// of 32 x 32.
+template <bool TA, bool TB, int BM>
+__global__ __launch_bounds__(4 * BM) void k_dgemm_split(
     int64_t M, int64_t N, int64_t K, const double* __restrict__ A, int64_t lda, int64_t sA,
     const double* __restrict__ B, int64_t ldb, int64_t sB, double* __restrict__ C, int64_t ldc,
     int64_t sC, int64_t tiles_m, int64_t tiles_n, int64_t kchunk, int64_t sSplit) {
-    // 128x128 tile, 8 waves as 2 x 4, each wave 64 x 32 (4 x 2 MFMA blocks).
-    constexpr int BM = 128, BN = 128, PAD = 4;
-    constexpr int WM = 64, WN = 32, FM = WM / 16, FN = WN / 16;
-    constexpr int PER = BM * GEMM_BK / SPLIT_THREADS;
-    static_assert(PER == 4 && BM == BN, "4 elements per thread and operand");
+    constexpr int THREADS = 4 * BM, BN = BM, PAD = 4;
+    constexpr int WGN = (BM == 128) ? 4 : 2, WGM = THREADS / 64 / WGN;
+    constexpr int WM = BM / WGM, WN = BN / WGN, FM = WM / 16, FN = WN / 16;
+    constexpr int PER = BM * GEMM_BK / THREADS;
+    static_assert(PER == 4 && (BM == 128 || BM == 64), "4 elements per thread and operand");
     __shared__ float Ah[GEMM_BK][BM + PAD], Al[GEMM_BK][BM + PAD];
     __shared__ float Bh[GEMM_BK][BN + PAD], Bl[GEMM_BK][BN + PAD];
     const int64_t batch = blockIdx.z;
@@ -1166,14 +1167,14 @@ __global__ __launch_bounds__(SPLIT_THREADS) void k_dgemm_split(
     const int64_t gsize = (tiles_m - first_m < GROUP) ? (tiles_m - first_m) : GROUP;
     const int64_t tm = first_m + (tile % per_group) % gsize, tn = (tile % per_group) / gsize;
     const int64_t m0 = tm * BM, n0 = tn * BN;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wm = wave >> 2, wn = wave & 3;
-    // Element i of a thread as (row, k) inside a 128 x 16 tile.  Operands
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wm = wave / WGN, wn = wave % WGN;
+    // Element i of a thread as (row, k) inside a BM x 16 tile.  Operands
     // stored k-major (A transposed, B plain) give lanes t and t^1 the two
     // halves of one row's 8-deep group; row-major ones give 8 aligned lanes
     // one k each of a row's group.
     auto pos = [&](bool kmajor, int i, int& row, int& kk) {
-        if (kmajor) { row = (tid >> 1) & 127; kk = (tid >> 8) * 8 + (tid & 1) * 4 + i; }
-        else { kk = tid % GEMM_BK; row = tid / GEMM_BK + i * (SPLIT_THREADS / GEMM_BK); }
+        if (kmajor) { row = (tid >> 1) & (BM - 1); kk = (tid / (2 * BM)) * 8 + (tid & 1) * 4 + i; }
+        else { kk = tid % GEMM_BK; row = tid / GEMM_BK + i * (THREADS / GEMM_BK); }
     };
     auto group_max = [&](bool kmajor, int* e) {
         if (kmajor) {
@@ -1278,19 +1279,35 @@ __global__ __launch_bounds__(SPLIT_THREADS) void k_dgemm_split(
             }
 }
 
+template <int BM>
+static void split_go(xr_ctx* c, int ta, int tb, dim3 grid, int64_t M, int64_t N, int64_t K, const double* A,
+                     int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB, double* Cw,
+                     int64_t tm, int64_t tn, int64_t kchunk, int64_t batch) {
+    const int nt = 4 * BM;
+    if (ta && tb) k_dgemm_split<true, true, BM><<<grid, nt, 0, c->stream>>>(M, N, K, A, lda, sA, B, ldb, sB, Cw, N, M * N, tm, tn, kchunk, batch * M * N);
+    else if (ta) k_dgemm_split<true, false, BM><<<grid, nt, 0, c->stream>>>(M, N, K, A, lda, sA, B, ldb, sB, Cw, N, M * N, tm, tn, kchunk, batch * M * N);
+    else if (tb) k_dgemm_split<false, true, BM><<<grid, nt, 0, c->stream>>>(M, N, K, A, lda, sA, B, ldb, sB, Cw, N, M * N, tm, tn, kchunk, batch * M * N);
+    else k_dgemm_split<false, false, BM><<<grid, nt, 0, c->stream>>>(M, N, K, A, lda, sA, B, ldb, sB, Cw, N, M * N, tm, tn, kchunk, batch * M * N);
+}
+
 // Split-FP32 product (always staged: partial tiles go to a workspace and the
 // fixed-order reduction applies beta).
 int xr_dgemm_split_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
                           const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
                           int64_t sB, double beta, double* C, int64_t ldc, int64_t sC, int64_t batch) {
     if (M <= 0 || N <= 0 || batch <= 0) return XR_OK;
-    int64_t tm = ceil_div(M, 128), tn = ceil_div(N, 128), tiles = tm * tn;
+    // 128 x 128 tiles when they alone give two workgroups per CU, else
+    // 64 x 64 tiles and a K split (chunks of at least 256) toward 1024
+    // workgroups
+    const int bm = (ceil_div(M, 128) * ceil_div(N, 128) * batch >= 512) ? 128 : 64;
+    int64_t tm = ceil_div(M, bm), tn = ceil_div(N, bm), tiles = tm * tn;
     if (tiles > 0x7fffffff || batch > 65535) return xr_fail(c, XR_EUNSUPPORTED, "GEMM grid too large");
     int64_t splits = 1, kchunk = K;
-    if (tiles * batch < 256 && K >= 1024) {
-        splits = ceil_div(512, tiles * batch);
-        int64_t max_split = K / 512;
+    if (tiles * batch < 512 && K >= 512) {
+        splits = ceil_div(1024, tiles * batch);
+        int64_t max_split = K / 256;
         if (splits > max_split) splits = max_split;
+        if (splits > 128) splits = 128;
         if (splits < 1) splits = 1;
         kchunk = ceil_div(ceil_div(K, splits), GEMM_BK) * GEMM_BK;
         splits = ceil_div(K, kchunk);
@@ -1301,10 +1318,8 @@ int xr_dgemm_split_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64
     {
         ProfScope prof(c, XR_K_GEMM_SPLIT, 2.0 * M * N * K * batch, 8.0 * (M * K + K * N + M * N) * batch);
         prof.shape(M, N, K, batch, ta, tb);
-        if (ta && tb) k_dgemm_split<true, true><<<grid, SPLIT_THREADS, 0, c->stream>>>(M, N, K, A, lda, sA, B, ldb, sB, Cw, N, M * N, tm, tn, kchunk, batch * M * N);
-        else if (ta) k_dgemm_split<true, false><<<grid, SPLIT_THREADS, 0, c->stream>>>(M, N, K, A, lda, sA, B, ldb, sB, Cw, N, M * N, tm, tn, kchunk, batch * M * N);
-        else if (tb) k_dgemm_split<false, true><<<grid, SPLIT_THREADS, 0, c->stream>>>(M, N, K, A, lda, sA, B, ldb, sB, Cw, N, M * N, tm, tn, kchunk, batch * M * N);
-        else k_dgemm_split<false, false><<<grid, SPLIT_THREADS, 0, c->stream>>>(M, N, K, A, lda, sA, B, ldb, sB, Cw, N, M * N, tm, tn, kchunk, batch * M * N);
+        if (bm == 128) split_go<128>(c, ta, tb, grid, M, N, K, A, lda, sA, B, ldb, sB, Cw, tm, tn, kchunk, batch);
+        else split_go<64>(c, ta, tb, grid, M, N, K, A, lda, sA, B, ldb, sB, Cw, tm, tn, kchunk, batch);
         XR_LAUNCH_CHECK(c);
     }
     int64_t n = batch * M * N;

@@ -101,8 +101,7 @@ class XRAdjoint:
     def forward(self):
         X = self.X
         X.build_densities()
-        H1, H2, _ = X.blocks()
-        self.op = op = X.operator(H=(H1, H2, None))
+        self.op = op = X.operator()
         if op.n <= DENSE_MAX:
             M = op.dense()
             w, v = torch.linalg.eigh(0.5 * (M + M.T))

@@ -86,6 +86,7 @@ class XRSystem:
         self.ints = B.IntegralCache(integrals, self.device)
         self.nuc = np.asarray(nuc, dtype=np.float64)
         self.built = False
+        self._sector_idx = {}    # per pair: fixed-charge block layout and scatter indices
 
     def build_densities(self):
         with precision(self.precision, self.device.index):
@@ -138,38 +139,95 @@ class XRSystem:
             return "self"
         return "source" if (k1, k2) < kt else "derived"
 
-    def pair_matrix(self, m1, m2):
-        """H2[m1][m2] over the product basis [(a, b) for a in states(m1) for b
-        in states(m2)] = build_matrix_elements.dimer (build_H.py:56-102)."""
+    def _pair_terms(self, m1, m2, place):
+        """Adds the pair terms of build_matrix_elements.dimer (build_H.py:56-102)
+        into the matrices chosen by place(k1, k2) -> (out, ldo, rows, cols, d)
+        (the arguments of blocks.scatter for that charge combination).  Only
+        the 'self' (at half weight) and 'source' combinations are built: the
+        caller completes each matrix as A + A^T."""
         self._need()
         F1, F2 = self.frags[m1], self.frags[m2]
-        D = F1.d * F2.d
-        out = torch.zeros(D, D, dtype=torch.float64, device=self.device)
-        dims = [F1.d, F2.d]
         for k1, k2 in self.pair_charge_combos(m1, m2):
             role = self.combo_role(k1, k2)
             if role == "derived":
                 continue
             half = 0.5 if role == "self" else 1.0
+            out, ldo, rows, cols, dims = place(k1, k2)
             nI1, nJ1 = F1.n_states[k1[0]], F1.n_states[k1[1]]
             nI2, nJ2 = F2.n_states[k2[0]], F2.n_states[k2[1]]
-            rows = [F1.pos[k1[0]], F2.pos[k2[0]]]
-            cols = [F1.pos[k1[1]], F2.pos[k2[1]]]
             fr = [(nI1, nJ1), (nI2, nJ2)]
             terms, extras = B.dimer_block(m1, m2, F2.n_elec_ref, F1.rho, F2.rho, k1, k2, self.ints)
             nA, nB = nI1 * nJ1, nI2 * nJ2
             for Hb, swapped, scale in terms:
                 strides = [1, nA] if swapped else [nB, 1]
-                B.scatter(out, D, fr, rows, cols, dims, Hb, strides, scale=half * scale)
+                B.scatter(out, ldo, fr, rows, cols, dims, Hb, strides, scale=half * scale)
             for kind, v in extras:
                 if kind == "e2":   # where I1 == J1
-                    B.scatter(out, D, fr, rows, cols, dims, v, [0, 1], diag=[1, 0], scale=half)
+                    B.scatter(out, ldo, fr, rows, cols, dims, v, [0, 1], diag=[1, 0], scale=half)
                 else:              # "e1", where I2 == J2
-                    B.scatter(out, D, fr, rows, cols, dims, v, [1, 0], diag=[0, 1], scale=half)
+                    B.scatter(out, ldo, fr, rows, cols, dims, v, [1, 0], diag=[0, 1], scale=half)
             if k1[0] == k1[1] and k2[0] == k2[1] and self.nuc[m1, m2] != 0.0:
-                B.scatter(out, D, fr, rows, cols, dims, None, [0, 0], diag=[1, 1], scale=half * self.nuc[m1, m2])
+                B.scatter(out, ldo, fr, rows, cols, dims, None, [0, 0], diag=[1, 1], scale=half * self.nuc[m1, m2])
+
+    def pair_matrix(self, m1, m2):
+        """H2[m1][m2] over the product basis [(a, b) for a in states(m1) for b
+        in states(m2)] = build_matrix_elements.dimer (build_H.py:56-102)."""
+        F1, F2 = self.frags[m1], self.frags[m2]
+        D = F1.d * F2.d
+        out = torch.zeros(D, D, dtype=torch.float64, device=self.device)
+        dims = [F1.d, F2.d]
+        self._pair_terms(m1, m2, lambda k1, k2: (out, D, [F1.pos[k1[0]], F2.pos[k2[0]]],
+                                                 [F1.pos[k1[1]], F2.pos[k2[1]]], dims))
         context(self.device.index).call("xr_symmetrize", ptr(out), D)
         return out
+
+    def pair_sector_blocks(self, m1, m2, segs, segq):
+        """H2[m1][m2] as its square blocks of fixed pair charge, without the
+        dense pair matrix: block i spans the pairs (a, b) with charge(a) +
+        charge(b) = segq[i] in increasing a * d2 + b (segs[i] = (start, count)
+        in that ordering of all pairs).  Pair (a, b) is row off_q[a] + r(b) of
+        its block, r(b) being b's rank among the states of its charge and
+        off_q[a] the number of pairs of charge q with a smaller first state,
+        so the scatter indices stay one per fragment (added, not multiplied)."""
+        F1, F2 = self.frags[m1], self.frags[m2]
+        cache = self._sector_idx.setdefault((m1, m2), {})
+        if "layout" not in cache:
+            ch1 = np.array([c for c, _ in F1.state_indices])
+            ch2 = np.array([c for c, _ in F2.state_indices])
+            rank2 = np.zeros(F2.d, dtype=np.int64)
+            n2 = {}
+            for c in F2.charges:
+                idx = np.nonzero(ch2 == c)[0]
+                rank2[idx] = np.arange(idx.size)
+                n2[c] = idx.size
+            off = {}
+            for q in segq:
+                sizes = np.array([n2.get(q - c, 0) for c in ch1], dtype=np.int64)
+                off[q] = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+            pos1 = {c: F1.pos[c].cpu().numpy() for c in F1.charges}
+            pos2 = {c: F2.pos[c].cpu().numpy() for c in F2.charges}
+            cache["layout"] = (off, rank2, pos1, pos2)
+        off, rank2, pos1, pos2 = cache["layout"]
+        seg_of = {q: i for i, q in enumerate(segq)}
+        blocks = [torch.zeros(cnt, cnt, dtype=torch.float64, device=self.device) for _, cnt in segs]
+        dev = self.device
+
+        def place(k1, k2):
+            q = k1[0] + k2[0]
+            i = seg_of[q]
+            key = (k1, k2)
+            if key not in cache:
+                t = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.int64), device=dev)
+                cache[key] = ([t(off[q][pos1[k1[0]]]), t(rank2[pos2[k2[0]]])],
+                              [t(off[q][pos1[k1[1]]]), t(rank2[pos2[k2[1]]])])
+            rows, cols = cache[key]
+            return blocks[i], segs[i][1], rows, cols, [1, 1]
+
+        self._pair_terms(m1, m2, place)
+        ctx = context(self.device.index)
+        for blk, (_, cnt) in zip(blocks, segs):
+            ctx.call("xr_symmetrize", ptr(blk), cnt)
+        return blocks
 
     def trimer_matrix(self, m1, m2, m3):
         """H3[m1][m2][m3] = build_matrix_elements.trimer (build_H.py:103-188)."""
@@ -235,9 +293,17 @@ class XRSystem:
         return [np.array([chg for chg, _ in F.state_indices], dtype=np.int64) for F in self.frags]
 
     def operator(self, H=None):
-        """Matrix-free XR operator (monomer and pair terms) over the blocks H."""
+        """Matrix-free XR operator (monomer and pair terms) over the blocks H,
+        or, without H, with the pair terms built straight into the operator's
+        fixed-charge blocks (no dense pair matrices)."""
         from .xr_operator import XROperator
-        H1, H2, H3 = H if H is not None else self.blocks()
+        if H is None:
+            nf = len(self.frags)
+            with precision(self.precision, self.device.index):
+                H1 = [self.monomer_matrix(m) for m in range(nf)]
+                return XROperator(H1, {(m, n): None for m in range(nf) for n in range(m + 1, nf)},
+                                  self.state_charges(), pair_builder=self.pair_sector_blocks)
+        H1, H2, H3 = H
         if H3 is not None:
             raise ValueError("the matrix-free operator carries monomer and pair terms only")
         nf = len(self.frags)

@@ -81,7 +81,7 @@ class XROperator:
     matrix (d_m d_n x d_m d_n)} for m < n; charges: per fragment, the charge
     of each state in state order."""
 
-    def __init__(self, H1, H2, charges, symmetrize=False):
+    def __init__(self, H1, H2, charges, symmetrize=False, pair_builder=None):
         self.H1 = [h.contiguous() for h in H1]
         self.dims = [int(h.shape[0]) for h in H1]
         self.nf = len(self.dims)
@@ -94,9 +94,13 @@ class XROperator:
         self.found = None
         self.log = None          # set to a list to record the per-sector solves
         for (m, n), H in H2.items():
-            if H is None:
-                continue
-            self.pairs[(m, n)] = self._pair_blocks(m, n, H, symmetrize)
+            if H is not None:
+                self.pairs[(m, n)] = self._pair_blocks(m, n, H, symmetrize)
+            elif pair_builder is not None:
+                # pair_builder(m, n, segs, segq) returns the fixed-charge
+                # blocks in the layout _pair_blocks cuts out of a dense matrix
+                order, inv, segs, segq = self._order(m, n)
+                self.pairs[(m, n)] = self._pair_meta(order, inv, segs, segq, pair_builder(m, n, segs, segq))
 
     # -- setup ----------------------------------------------------------------
     def _order(self, m, n):
@@ -132,6 +136,9 @@ class XROperator:
             if symmetrize:
                 blk = torch.add(blk, blk.T).mul_(0.5)
             blocks.append(blk)
+        return self._pair_meta(order, inv, segs, segq, blocks)
+
+    def _pair_meta(self, order, inv, segs, segq, blocks):
         return {"order": _i32(order, self.device), "inv": _i32(inv, self.device), "segs": segs,
                 "segq": segq, "blocks": blocks, "inv_np": inv, "order_np": order}
 

@@ -47,6 +47,26 @@ def test_operator_equals_assembled_matrix(orbs):
     assert float((A - M).abs().max()) < 1e-13
 
 
+@pytest.mark.parametrize("orbs,precision", [((2, 3), "f64"), ((2, 2, 2), "f64"), ((3, 2, 3), "split_f32")])
+def test_sector_blocks_match_dense_pair_matrices(orbs, precision):
+    """Pair terms built straight into the operator's fixed-charge blocks equal
+    the same blocks cut out of the dense pair matrices (same sums, same
+    order), and the operator built that way equals the assembled matrix."""
+    X, _ = _system(42, orbs, ns=3, precision=precision)
+    H = X.blocks()
+    ref = X.operator(H=H)
+    op = X.operator()
+    assert set(op.pairs) == set(ref.pairs)
+    for key, P in op.pairs.items():
+        R = ref.pairs[key]
+        assert P["segs"] == R["segs"] and P["segq"] == R["segq"]
+        for a, b in zip(P["blocks"], R["blocks"]):
+            assert torch.equal(a, b.to(a.dtype))
+            assert torch.equal(a, a.T)
+    if precision == "f64":
+        assert float((op.dense() - X.xr_matrix(H=H)).abs().max()) < 1e-13
+
+
 def test_operator_equals_assembled_matrix_c0():
     w = syn.config0(seed=0)
     X = XRSystem(w.fragments, w.integrals, w.nuc)

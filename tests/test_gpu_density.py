@@ -177,7 +177,8 @@ def _group_max(X):
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
 @pytest.mark.parametrize("M,N,K,wide", [(300, 260, 700, False), (64, 64, 40000, False),
-                                        (1000, 1200, 333, False), (200, 300, 517, True)])
+                                        (1000, 1200, 333, False), (200, 300, 517, True),
+                                        (2900, 2900, 640, False), (32, 40, 70000, True)])
 def test_gemm_split_f32_accuracy(ta, tb, M, N, K, wide):
     """Split-FP32 product with FP64 accumulation.  hi*hi sums are exact (common
     power-of-two grid per 8-deep group, 11-bit hi); the hi*lo terms and the

@@ -165,3 +165,33 @@ def test_split_precision_xr_energy():
     assert float((M - M.T).abs().max()) == 0.0
     ev = torch.linalg.eigvalsh(M)[:10].cpu().numpy()
     assert float(np.abs(ev - ref["evals"]).max()) < 1e-9
+
+
+@pytest.mark.parametrize("n1,n2,nA,nB,precision", [(4, 6, 30, 45, 0), (6, 4, 70, 20, 0), (5, 5, 40, 40, 1)])
+def test_fused_one_electron_transfer_block(n1, n2, nA, nB, precision):
+    """xr_dimer_1min1pls (one product over K = n1 + n2) equals the sum of the
+    separate one- and two-electron blocks; in split precision both sides run
+    their V products in split FP32 and agree to its error."""
+    from qodeapplications_amd._lib import context, ptr
+    g = torch.Generator().manual_seed(n1 * 100 + n2 + nA)
+    r = lambda *s: torch.randn(*s, dtype=torch.float64, generator=g).cuda()
+    Rc1, Rcca1, Ra2, Rcaa2 = r(nA, n1), r(nA, n1 ** 3), r(nB, n2), r(nB, n2 ** 3)
+    h, V1, V2 = r(n1, n2), r(n1, n1, n1, n2), r(n1, n2, n2, n2)
+    ctx = context(0)
+    prev = getattr(ctx, "precision", 0)
+    ctx.set_precision(precision)
+    try:
+        one = torch.empty(nA, nB, dtype=torch.float64, device="cuda")
+        two = torch.empty(nA, nB, dtype=torch.float64, device="cuda")
+        fused = torch.empty(nA, nB, dtype=torch.float64, device="cuda")
+        ctx.call("xr_dimer_1min1pls_1e", n1, n2, nA, ptr(Rc1), nB, ptr(Ra2), ptr(h), ptr(one))
+        ctx.call("xr_dimer_1min1pls_2e", n1, n2, nA, ptr(Rc1), ptr(Rcca1), nB, ptr(Ra2), ptr(Rcaa2),
+                 ptr(V1), ptr(V2), ptr(two))
+        ctx.call("xr_dimer_1min1pls", n1, n2, nA, ptr(Rc1), ptr(Rcca1), nB, ptr(Ra2), ptr(Rcaa2),
+                 ptr(h), ptr(V1), ptr(V2), ptr(fused))
+        torch.cuda.synchronize()
+    finally:
+        ctx.set_precision(prev)
+    ref = one + two
+    tol = (1e-12 if precision == 0 else 1e-8) * float(ref.abs().max())
+    assert float((fused - ref).abs().max()) < tol
