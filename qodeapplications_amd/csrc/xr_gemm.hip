@@ -289,21 +289,52 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm_grouped(GemmGroup g) {
     dgemm_tile<BM, BN, TA, TB>(P.M, P.N, P.K, g.alpha, P.A, P.lda, P.B, P.ldb, P.C, P.ldc, tm * BM, tn * BN);
 }
 
-// C = alpha * sum_s P[s] + beta * C, summed in split order.
+// C = alpha * sum_s P[s * step] + beta * C, summed in split order.
 __global__ void k_splitk_reduce(const double* __restrict__ P, int64_t splits, int64_t batch,
                                 int64_t M, int64_t N, double alpha, double beta,
-                                double* __restrict__ C, int64_t ldc, int64_t sC) {
+                                double* __restrict__ C, int64_t ldc, int64_t sC, int64_t step = 1) {
     const int64_t n = batch * M * N;
     XR_GRID_STRIDE(i, n) {
         int64_t b = i / (M * N), r = i - b * M * N;
         int64_t m = r / N, col = r - m * N;
         double acc = 0.0;
-        for (int64_t s = 0; s < splits; s++) acc += P[s * n + i];
+        for (int64_t s = 0; s < splits; s += step) acc += P[s * n + i];
         double* cp = C + b * sC + m * ldc + col;
         double v = alpha * acc;
         if (beta != 0.0) v += beta * (*cp);
         *cp = v;
     }
+}
+
+// First stage of a reduction over many splits on a small output: each
+// group of SPLIT_FOLD consecutive partials is summed (in order) into its
+// first one, so the final pass reads every SPLIT_FOLD-th partial.
+#define SPLIT_FOLD 32
+__global__ void k_splitk_fold(double* __restrict__ P, int64_t splits, int64_t n) {
+    const int64_t groups = (splits + SPLIT_FOLD - 1) / SPLIT_FOLD;
+    XR_GRID_STRIDE(i, groups * n) {
+        const int64_t g = i / n, e = i - g * n;
+        const int64_t s0 = g * SPLIT_FOLD, s1 = s0 + SPLIT_FOLD < splits ? s0 + SPLIT_FOLD : splits;
+        double acc = 0.0;
+        for (int64_t s = s0; s < s1; s++) acc += P[s * n + e];
+        P[s0 * n + e] = acc;
+    }
+}
+
+// C = alpha * (sum of the split partials in Cw) + beta * C: outputs that
+// leave most of the chip idle fold groups of partials first.
+static int splitk_reduce(xr_ctx* c, double* Cw, int64_t splits, int64_t batch, int64_t M, int64_t N,
+                         double alpha, double beta, double* C, int64_t ldc, int64_t sC) {
+    const int64_t n = batch * M * N;
+    int64_t step = 1;
+    if (splits > 2 * SPLIT_FOLD && n < (1 << 18)) {
+        k_splitk_fold<<<xr_grid(ceil_div(splits, SPLIT_FOLD) * n), 256, 0, c->stream>>>(Cw, splits, n);
+        XR_LAUNCH_CHECK(c);
+        step = SPLIT_FOLD;
+    }
+    k_splitk_reduce<<<xr_grid(n), 256, 0, c->stream>>>(Cw, splits, batch, M, N, alpha, beta, C, ldc, sC, step);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
 }
 
 // Tall-skinny product for N <= 8 with A row-major (M x K, the Davidson
@@ -739,7 +770,9 @@ static double split_choice(int64_t units, int64_t slots, int64_t K, int64_t* spl
     double best = score(1);
     *splits = 1;
     if (K < 512 || units >= 4 * slots) return best;
-    const int64_t smax = std::min<int64_t>(64, K / 256);
+    // up to two rounds of resident workgroups (at most 512 splits): a few
+    // output tiles over a long K are bound by how many CUs stream operands
+    const int64_t smax = std::min<int64_t>(std::min<int64_t>(512, std::max<int64_t>(64, 2 * slots / units)), K / 256);
     for (int64_t s = 2; s <= smax; s++) {
         const double e = score(s);
         if (e > best) { best = e; *splits = s; }
@@ -826,10 +859,8 @@ static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t
     }
     if (staged) {
         ProfScope prof(c, XR_K_OTHER, 0.0, 8.0 * (splits + 2) * batch * M * N);
-        int64_t n = batch * M * N;
-        k_splitk_reduce<<<xr_grid(n), 256, 0, c->stream>>>(Cw, splits, batch, M, N, alpha, beta, C, ldc,
-                                                            sC);
-        XR_LAUNCH_CHECK(c);
+        int rc = splitk_reduce(c, Cw, splits, batch, M, N, alpha, beta, C, ldc, sC);
+        if (rc) return rc;
     }
     return XR_OK;
 }
@@ -1307,7 +1338,7 @@ int xr_dgemm_split_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64
         splits = ceil_div(1024, tiles * batch);
         int64_t max_split = K / 256;
         if (splits > max_split) splits = max_split;
-        if (splits > 128) splits = 128;
+        if (splits > 512) splits = 512;
         if (splits < 1) splits = 1;
         kchunk = ceil_div(ceil_div(K, splits), GEMM_BK) * GEMM_BK;
         splits = ceil_div(K, kchunk);
@@ -1322,10 +1353,7 @@ int xr_dgemm_split_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64
         else split_go<64>(c, ta, tb, grid, M, N, K, A, lda, sA, B, ldb, sB, Cw, tm, tn, kchunk, batch);
         XR_LAUNCH_CHECK(c);
     }
-    int64_t n = batch * M * N;
-    k_splitk_reduce<<<xr_grid(n), 256, 0, c->stream>>>(Cw, splits, batch, M, N, alpha, beta, C, ldc, sC);
-    XR_LAUNCH_CHECK(c);
-    return XR_OK;
+    return splitk_reduce(c, Cw, splits, batch, M, N, alpha, beta, C, ldc, sC);
 }
 
 // The contraction GEMMs of the two-electron terms go through here: FP64 or
