@@ -113,6 +113,9 @@ struct xr_ctx {
     std::vector<size_t> ws_size;
     // cache of K-space occupation masks keyed by (dim, n_elec)
     std::map<std::pair<int, int>, std::pair<uint64_t*, int64_t>> kspace;
+    // side-adjoint index tables (one int32 code per source determinant and
+    // tuple), keyed by the side string and the spaces it maps between
+    std::map<std::array<int64_t, 12>, std::pair<int32_t*, int64_t>> sidetab;
     // GEMM autotune table: (ta, tb, M, N, K, batch) -> (tile, splits)
     std::map<std::array<int64_t, 6>, std::pair<int, int>> tune;
     std::mutex mu;

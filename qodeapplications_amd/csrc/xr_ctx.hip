@@ -268,6 +268,7 @@ extern "C" int xr_destroy(xr_ctx* c) {
     for (void* p : c->ws)
         if (p) (void)hipFree(p);
     for (auto& kv : c->kspace) (void)hipFree(kv.second.first);
+    for (auto& kv : c->sidetab) (void)hipFree(kv.second.first);
     for (auto& r : c->recs) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);

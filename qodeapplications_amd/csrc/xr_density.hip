@@ -581,16 +581,17 @@ __global__ __launch_bounds__(256) void k_compress(
 // output is owned by one thread: deterministic, no atomics.
 // One (tuple of K ascending orbitals o[0] < o[1] < o[2]) term of the side
 // adjoint: sign of applying the string (slot order) to src, K's rank, t's rank.
+// Encoded as sign * (row + 1) with row the Ybar row (K, t) in the given
+// layout, 0 where K falls outside the K space.
 template <int K>
-__device__ __forceinline__ double side_term(const SidePlan& sd, uint64_t src, const int* o, int n_orbs,
+__device__ __forceinline__ int64_t side_code(const SidePlan& sd, uint64_t src, const int* o, int n_orbs,
                                             uint64_t kcore, int kspace_core, int nK_val, int dim,
-                                            int64_t nK, int64_t nT, const double* __restrict__ Ybar,
-                                            int64_t nS, int64_t s, int layout) {
+                                            int64_t nK, int64_t nT, int layout) {
     uint64_t tm = 0;
 #pragma unroll
     for (int j = 0; j < K; j++) tm |= 1ull << o[j];
     const uint64_t km = sd.is_c ? (src | tm) : (src & ~tm);
-    if ((km & kcore) != kcore) return 0.0;
+    if ((km & kcore) != kcore) return 0;
     uint64_t cur = src;
     int sg = 1;
 #pragma unroll
@@ -602,8 +603,79 @@ __device__ __forceinline__ double side_term(const SidePlan& sd, uint64_t src, co
     }
     const int64_t Kr = d_lex_rank(d_to_valence(km, n_orbs, kspace_core), nK_val);
     const int64_t t = d_lex_rank(tm, dim);
-    const double y = layout == 0 ? Ybar[(Kr * nT + t) * nS + s] : Ybar[(t * nK + Kr) * nS + s];
-    return sg * y;
+    const int64_t row = layout == 0 ? Kr * nT + t : t * nK + Kr;
+    return sg * (row + 1);
+}
+
+template <int K>
+__device__ __forceinline__ double side_term(const SidePlan& sd, uint64_t src, const int* o, int n_orbs,
+                                            uint64_t kcore, int kspace_core, int nK_val, int dim,
+                                            int64_t nK, int64_t nT, const double* __restrict__ Ybar,
+                                            int64_t nS, int64_t s, int layout) {
+    const int64_t e = side_code<K>(sd, src, o, n_orbs, kcore, kspace_core, nK_val, dim, nK, nT, layout);
+    if (!e) return 0.0;
+    const double y = Ybar[((e > 0 ? e : -e) - 1) * nS + s];
+    return e > 0 ? y : -y;
+}
+
+// The codes of side_code for every source determinant S and tuple j (in the
+// enumeration order of k_side_adjoint): tab[S * nTup + j].  One thread per S.
+template <int K>
+__global__ void k_side_index(SidePlan sd, const uint64_t* __restrict__ Sm, int64_t nS_cfg, int n_orbs,
+                             int kspace_core, int64_t nK, int64_t nT, int layout, int64_t nTup,
+                             int32_t* __restrict__ tab) {
+    XR_GRID_STRIDE(S, nS_cfg) {
+        const int dim = 2 * n_orbs;
+        const uint64_t src = Sm[S];
+        const uint64_t full = (dim >= 64) ? ~0ull : ((1ull << dim) - 1);
+        const uint64_t pool = sd.is_c ? (~src & full) : src;
+        const uint64_t kcore = d_core_mask(n_orbs, kspace_core);
+        const int nK_val = dim - 2 * kspace_core;
+        int32_t* out = tab + S * nTup;
+        int64_t j = 0;
+        int o[3] = {0, 0, 0};
+        for (uint64_t p1 = pool; p1; p1 &= p1 - 1) {
+            o[0] = __ffsll((unsigned long long)p1) - 1;
+            if (K == 1) {
+                if (j < nTup) out[j++] = (int32_t)side_code<K>(sd, src, o, n_orbs, kcore, kspace_core, nK_val, dim, nK, nT, layout);
+                continue;
+            }
+            for (uint64_t p2 = p1 & (p1 - 1); p2; p2 &= p2 - 1) {
+                o[1] = __ffsll((unsigned long long)p2) - 1;
+                if (K == 2) {
+                    if (j < nTup) out[j++] = (int32_t)side_code<K>(sd, src, o, n_orbs, kcore, kspace_core, nK_val, dim, nK, nT, layout);
+                    continue;
+                }
+                for (uint64_t p3 = p2 & (p2 - 1); p3; p3 &= p3 - 1) {
+                    o[2] = __ffsll((unsigned long long)p3) - 1;
+                    if (j < nTup) out[j++] = (int32_t)side_code<K>(sd, src, o, n_orbs, kcore, kspace_core, nK_val, dim, nK, nT, layout);
+                }
+            }
+        }
+        for (; j < nTup; j++) out[j] = 0;
+    }
+}
+
+// zbarT[S, s] = sum_j sign_j Ybar[row_j, s] over the tuples of S, in the
+// tuple order of k_side_adjoint (same sums, same order); the codes of one S
+// are shared by the lanes of its states.
+__global__ __launch_bounds__(256) void k_side_adjoint_tab(const int32_t* __restrict__ tab, int64_t nS_cfg,
+                                                          int64_t nTup, const double* __restrict__ Ybar,
+                                                          int64_t nS, double* __restrict__ zbarT) {
+    XR_GRID_STRIDE(i, nS_cfg * nS) {
+        const int64_t S = i / nS, s = i - S * nS;
+        const int32_t* e = tab + S * nTup;
+        double acc = 0.0;
+#pragma unroll 8
+        for (int64_t j = 0; j < nTup; j++) {
+            const int32_t v = e[j];
+            if (v) {
+                const double y = Ybar[(int64_t)((v > 0 ? v : -v) - 1) * nS + s];
+                acc += v > 0 ? y : -y;
+            }
+        }
+        zbarT[i] = acc;
+    }
 }
 
 // Each output (S, s) is owned by one thread (deterministic, no atomics); the
@@ -691,7 +763,34 @@ static int side_adjoint(xr_ctx* c, const SidePlan& sd, int n_orbs, int n_core, i
     if (sd.k < 1 || sd.k > 3) return xr_fail(c, XR_EINVAL, "side string of 1 to 3 operators expected");
     if ((rc = xr_space_masks(c, n_orbs, n_core, n_elec_src, &Sm, &nS_cfg))) return rc;
     int64_t n = nS_cfg * nS;
-    {
+    // index table of the (S, tuple) terms: built once per side string and
+    // spaces, then every call is a signed gather-sum
+    const int dim = 2 * n_orbs;
+    const int64_t pool = sd.is_c ? dim - n_elec_src : n_elec_src;
+    const int64_t nTup = h_binom((int)pool, sd.k);
+    const std::array<int64_t, 12> key{sd.k, sd.is_c, sd.slot[0], sd.slot[1], sd.slot[2], n_orbs, n_core,
+                                      kspace_core, n_elec_src, layout, nK, nT};
+    int32_t* tab = nullptr;
+    auto it = c->sidetab.find(key);
+    if (it != c->sidetab.end()) {
+        tab = it->second.first;
+    } else if (nK * nT < 0x7fffffffLL && nS_cfg * nTup > 0 && nS_cfg * nTup <= (1LL << 28)) {
+        XR_HIP(c, hipMalloc(&tab, nS_cfg * nTup * sizeof(int32_t)));
+        c->sidetab[key] = std::make_pair(tab, nS_cfg * nTup);
+        ProfScope prof(c, XR_K_TABLE, 0.0, 4.0 * nS_cfg * nTup);
+        if (sd.k == 1)
+            k_side_index<1><<<xr_grid(nS_cfg), 256, 0, c->stream>>>(sd, Sm, nS_cfg, n_orbs, kspace_core, nK, nT, layout, nTup, tab);
+        else if (sd.k == 2)
+            k_side_index<2><<<xr_grid(nS_cfg), 256, 0, c->stream>>>(sd, Sm, nS_cfg, n_orbs, kspace_core, nK, nT, layout, nTup, tab);
+        else
+            k_side_index<3><<<xr_grid(nS_cfg), 256, 0, c->stream>>>(sd, Sm, nS_cfg, n_orbs, kspace_core, nK, nT, layout, nTup, tab);
+        XR_LAUNCH_CHECK(c);
+    }
+    if (tab) {
+        ProfScope prof(c, XR_K_GATHER, 0.0, 8.0 * n + 4.0 * nS_cfg * nTup);
+        k_side_adjoint_tab<<<xr_grid(n), 256, 0, c->stream>>>(tab, nS_cfg, nTup, Ybar, nS, tmpT);
+        XR_LAUNCH_CHECK(c);
+    } else {
         ProfScope prof(c, XR_K_GATHER, 0.0, 8.0 * n);
         if (sd.k == 1)
             k_side_adjoint<1><<<xr_grid(n), 256, 0, c->stream>>>(sd, Sm, nS_cfg, n_orbs, n_core, kspace_core,
