@@ -27,7 +27,7 @@ int xr_space_masks(xr_ctx* c, int n_orbs, int n_core, int n_elec, const uint64_t
                    int64_t* n);
 int xr_transpose(xr_ctx* c, const double* z, int64_t nS, int64_t nC, double* zT);
 int xr_side_table(xr_ctx* c, const SidePlan& sd, const uint64_t* Km, int64_t nK, int n_orbs,
-                  int n_core, int32_t* tab);
+                  int n_core, int32_t** tab);
 int xr_gather(xr_ctx* c, const int32_t* tab, int64_t nK, int64_t nT, const double* zT,
               int64_t nS, double* out, int layout);
 int xr_index_table(xr_ctx* c, int k, int dim, int32_t* tab);

@@ -288,15 +288,29 @@ int xr_transpose(xr_ctx* c, const double* z, int64_t nS, int64_t nC, double* zT)
     return XR_OK;
 }
 
+// The side table of (sd, K space) depends on the determinant spaces only:
+// built on first use and kept for the context's lifetime (*tab; nullptr
+// for an empty table).
 int xr_side_table(xr_ctx* c, const SidePlan& sd, const uint64_t* Km, int64_t nK, int n_orbs,
-                  int n_core, int32_t* tab) {
+                  int n_core, int32_t** tab) {
     int64_t nT = h_binom(2 * n_orbs, sd.k);
     int64_t n = nK * nT;
+    *tab = nullptr;
     if (n <= 0) return XR_OK;
+    const std::array<int64_t, 12> key{sd.k, sd.is_c, sd.slot[0], sd.slot[1], sd.slot[2],
+                                      (int64_t)(uintptr_t)Km, nK, n_orbs, n_core, -1, 0, 0};
+    auto it = c->sidetab.find(key);
+    if (it != c->sidetab.end()) {
+        *tab = it->second.first;
+        return XR_OK;
+    }
+    int32_t* p = nullptr;
+    XR_HIP(c, hipMalloc(&p, n * sizeof(int32_t)));
+    c->sidetab[key] = std::make_pair(p, n);
     ProfScope prof(c, XR_K_TABLE, 0.0, 4.0 * n);
-    k_side_table<<<xr_grid(n), 256, 0, c->stream>>>(sd, Km, nK, 2 * n_orbs, n_orbs, n_core,
-                                                           nT, tab);
+    k_side_table<<<xr_grid(n), 256, 0, c->stream>>>(sd, Km, nK, 2 * n_orbs, n_orbs, n_core, nT, p);
     XR_LAUNCH_CHECK(c);
+    *tab = p;
     return XR_OK;
 }
 
@@ -390,18 +404,18 @@ static int prepare_factors(xr_ctx* c, const OpPlan& pl, int n_orbs, int n_core, 
         f->kspace_core = 0;
     }
     if (pl.L.k) {
-        int32_t* tabL = (int32_t*)xr_workspace(c, WS_TABL, f->nK * f->nLt * sizeof(int32_t));
+        int32_t* tabL = nullptr;
         double* Xb = (double*)xr_workspace(c, WS_X, f->nK * f->nLt * nI * sizeof(double));
-        if (!tabL || !Xb) return xr_fail(c, XR_ENOMEM, "workspace (bra gather)");
-        if ((rc = xr_side_table(c, pl.L, f->Km, f->nK, n_orbs, n_core, tabL))) return rc;
+        if (!Xb) return xr_fail(c, XR_ENOMEM, "workspace (bra gather)");
+        if ((rc = xr_side_table(c, pl.L, f->Km, f->nK, n_orbs, n_core, &tabL))) return rc;
         if ((rc = xr_gather(c, tabL, f->nK, f->nLt, zTb, nI, Xb, 0))) return rc;
         f->X = Xb;
     }
     if (pl.R.k) {
-        int32_t* tabR = (int32_t*)xr_workspace(c, WS_TABR, f->nK * f->nT * sizeof(int32_t));
+        int32_t* tabR = nullptr;
         double* Yb = (double*)xr_workspace(c, WS_Y, f->nK * f->nT * nJ * sizeof(double));
-        if (!tabR || !Yb) return xr_fail(c, XR_ENOMEM, "workspace (ket gather)");
-        if ((rc = xr_side_table(c, pl.R, f->Km, f->nK, n_orbs, n_core, tabR))) return rc;
+        if (!Yb) return xr_fail(c, XR_ENOMEM, "workspace (ket gather)");
+        if ((rc = xr_side_table(c, pl.R, f->Km, f->nK, n_orbs, n_core, &tabR))) return rc;
         if ((rc = xr_gather(c, tabR, f->nK, f->nT, zTk, nJ, Yb, 0))) return rc;
         f->Y = Yb;
     }
@@ -481,15 +495,15 @@ extern "C" int xr_monomer_2e(xr_ctx* c, int n_orbs, int n_core, int n_elec, int6
     int64_t nK = 0;
     if ((rc = xr_space_masks(c, n_orbs, 0, n_elec - 2, &Km, &nK))) return rc;
     const int64_t nU = h_binom(dim, 2);
-    int32_t* tabL = (int32_t*)xr_workspace(c, WS_TABL, nK * nU * sizeof(int32_t));
-    int32_t* tabR = (int32_t*)xr_workspace(c, WS_TABR, nK * nU * sizeof(int32_t));
+    int32_t* tabL = nullptr;
+    int32_t* tabR = nullptr;
     double* X = (double*)xr_workspace(c, WS_X, nU * nK * nI * sizeof(double));
     double* Y = (double*)xr_workspace(c, WS_Y, nU * nK * nJ * sizeof(double));
     double* W = (double*)xr_workspace(c, WS_W, nU * nK * nJ * sizeof(double));
     double* Ve = (double*)xr_workspace(c, WS_VE, nU * nU * sizeof(double));
-    if (!tabL || !tabR || !X || !Y || !W || !Ve) return xr_fail(c, XR_ENOMEM, "workspace (sigma)");
-    if ((rc = xr_side_table(c, pl.L, Km, nK, n_orbs, n_core, tabL))) return rc;
-    if ((rc = xr_side_table(c, pl.R, Km, nK, n_orbs, n_core, tabR))) return rc;
+    if (!X || !Y || !W || !Ve) return xr_fail(c, XR_ENOMEM, "workspace (sigma)");
+    if ((rc = xr_side_table(c, pl.L, Km, nK, n_orbs, n_core, &tabL))) return rc;
+    if ((rc = xr_side_table(c, pl.R, Km, nK, n_orbs, n_core, &tabR))) return rc;
     if ((rc = xr_gather(c, tabL, nK, nU, zTb, nI, X, 1))) return rc;   // X[u][K][I]
     if ((rc = xr_gather(c, tabR, nK, nU, zTk, nJ, Y, 1))) return rc;   // Y[w][K][J]
     k_veff<<<xr_grid(nU * nU), 256, 0, c->stream>>>(V, dim, nU, Ve);
@@ -857,8 +871,8 @@ extern "C" int xr_monomer_2e_adjoint(xr_ctx* c, int n_orbs, int n_core, int n_el
     int64_t nK = 0;
     if ((rc = xr_space_masks(c, n_orbs, 0, n_elec - 2, &Km, &nK))) return rc;
     const int64_t nU = h_binom(dim, 2);
-    int32_t* tabL = (int32_t*)xr_workspace(c, WS_TABL, nK * nU * sizeof(int32_t));
-    int32_t* tabR = (int32_t*)xr_workspace(c, WS_TABR, nK * nU * sizeof(int32_t));
+    int32_t* tabL = nullptr;
+    int32_t* tabR = nullptr;
     double* X = (double*)xr_workspace(c, WS_X, nU * nK * nI * sizeof(double));
     double* Y = (double*)xr_workspace(c, WS_Y, nU * nK * nJ * sizeof(double));
     double* W = (double*)xr_workspace(c, WS_W, nU * nK * nJ * sizeof(double));
@@ -869,10 +883,10 @@ extern "C" int xr_monomer_2e_adjoint(xr_ctx* c, int n_orbs, int n_core, int n_el
     double* Veb = (double*)xr_workspace(c, WS_VEB, nU * nU * sizeof(double));
     int64_t nmax = nP * nI > nQ * nJ ? nP * nI : nQ * nJ;
     double* zbT = (double*)xr_workspace(c, WS_ZBT, nmax * sizeof(double));
-    if (!tabL || !tabR || !X || !Y || !W || !Ve || !Xbar || !Ybar || !Wbar || !Veb || !zbT)
+    if (!X || !Y || !W || !Ve || !Xbar || !Ybar || !Wbar || !Veb || !zbT)
         return xr_fail(c, XR_ENOMEM, "workspace (sigma adjoint)");
-    if ((rc = xr_side_table(c, pl.L, Km, nK, n_orbs, n_core, tabL))) return rc;
-    if ((rc = xr_side_table(c, pl.R, Km, nK, n_orbs, n_core, tabR))) return rc;
+    if ((rc = xr_side_table(c, pl.L, Km, nK, n_orbs, n_core, &tabL))) return rc;
+    if ((rc = xr_side_table(c, pl.R, Km, nK, n_orbs, n_core, &tabR))) return rc;
     if ((rc = xr_gather(c, tabL, nK, nU, zTb, nI, X, 1))) return rc;   // X[u][K][I]
     if ((rc = xr_gather(c, tabR, nK, nU, zTk, nJ, Y, 1))) return rc;   // Y[w][K][J]
     k_veff<<<xr_grid(nU * nU), 256, 0, c->stream>>>(V, dim, nU, Ve);
