@@ -324,6 +324,14 @@ int xr_axis_apply_multi(xr_ctx *ctx, int n_items, const double *const *d_U, cons
  * counterpart, build_H.py:56-102 builds both). */
 int xr_symmetrize(xr_ctx *ctx, double *d_A, int64_t n);
 
+/* dst[J, I, x_k .. x_1] = src[I, J, x_1 .. x_k] (accumulate: +=) for a
+ * density of k <= 4 spin-orbital indices over d <= 64 spin orbitals: the
+ * density of the conjugate string between the swapped charges for real
+ * states, <I|op|J> = <J|op^+|I> (as Be-states/densities.py:192-206 takes
+ * the conjugates by transposition). */
+int xr_conjugate(xr_ctx *ctx, const double *d_src, int64_t nI, int64_t nJ, int d, int k, double *d_dst,
+                 int accumulate);
+
 /* Pair-block adjoint input (the pair-outer layout of xr_pair_outer) read from
  * a reduced pair matrix G2 = sum over the other fragments of v v^T, whose
  * rows and columns run over the pair index a*d2 + b, reordered through

@@ -45,6 +45,7 @@ SIGNATURES = {
     "xr_gemm_tune": (_I, [_P, _I, _I, _L, _L, _L, _L, _I, _P, _P, _P]),
     "xr_gemm_tune_reset": (_I, [_P]),
     "xr_symmetrize": (_I, [_P, _P, _L]),
+    "xr_conjugate": (_I, [_P, _P, _L, _L, _I, _I, _P, _I]),
     "xr_gemm_grouped": (_I, [_P, _I, _I, _I, _P, _P, _P, _D, _P, _P, _P, _P, _P, _P]),
     "xr_axes_pack": (_I, [_P, _I, _P, _L, _I, _I, _P, _L, _P, _P, _L, _I, _D]),
     "xr_axis_apply_multi": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _D]),

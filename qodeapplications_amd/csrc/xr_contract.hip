@@ -715,6 +715,61 @@ extern "C" int xr_symmetrize(xr_ctx* c, double* A, int64_t n) {
     return XR_OK;
 }
 
+// dst[J, I, rev(x)] = src[I, J, x] (or +=) for a density (nI, nJ, d^k): the
+// state axes swapped and the index digits reversed (the density of the
+// conjugate string between the swapped charges, for real states).  A
+// workgroup takes the d x d (first digit, last digit) tile of one (I, J)
+// plane for one string of middle digits and turns it through LDS; the tile's
+// rows are runs of d contiguous doubles in src and in dst.  k <= 1: plane
+// copies.
+__global__ __launch_bounds__(256) void k_conjugate(const double* __restrict__ src, int64_t nI, int64_t nJ,
+                                                   int d, int k, int64_t P, int64_t nmid,
+                                                   double* __restrict__ dst, int acc) {
+    __shared__ double T[64][65];
+    const int64_t blk = blockIdx.x;
+    const int64_t pl = blk / nmid, mid = blk - pl * nmid;
+    const int64_t I = pl / nJ, J = pl - I * nJ;
+    const double* s = src + pl * P;
+    double* o = dst + (J * nI + I) * P;
+    if (k <= 1) {
+        for (int64_t t = threadIdx.x; t < P; t += blockDim.x) o[t] = acc ? o[t] + s[t] : s[t];
+        return;
+    }
+    int64_t md = 0;
+    for (int64_t x = mid, j = 0; j < k - 2; j++, x /= d) md = md * d + x % d;
+    const int64_t hi = P / d;
+    const int dd = d * d;
+    for (int t = threadIdx.x; t < dd; t += blockDim.x) {
+        const int p = t / d, q = t - (t / d) * d;
+        T[p][q] = s[p * hi + mid * d + q];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < dd; t += blockDim.x) {
+        const int q = t / d, p = t - (t / d) * d;
+        const int64_t idx = q * hi + md * d + p;
+        o[idx] = acc ? o[idx] + T[p][q] : T[p][q];
+    }
+}
+
+extern "C" int xr_conjugate(xr_ctx* c, const double* src, int64_t nI, int64_t nJ, int d, int k, double* dst,
+                            int accumulate) {
+    XR_ENTRY(c);
+    if (nI < 0 || nJ < 0 || d < 1 || k < 0 || k > 4) return xr_fail(c, XR_EINVAL, "bad density shape");
+    if (d > 64) return xr_fail(c, XR_EUNSUPPORTED, "more than 64 spin orbitals");
+    if (src == dst && nI * nJ > 0) return xr_fail(c, XR_EINVAL, "xr_conjugate cannot work in place");
+    int64_t P = 1;
+    for (int j = 0; j < k; j++) P *= d;
+    int64_t nmid = 1;
+    for (int j = 2; j < k; j++) nmid *= d;
+    const int64_t blocks = nI * nJ * nmid;
+    if (blocks == 0) return XR_OK;
+    if (blocks > 0x7fffffffLL) return xr_fail(c, XR_EUNSUPPORTED, "density too large for the conjugate grid");
+    ProfScope prof(c, XR_K_SCATTER, 0.0, (accumulate ? 24.0 : 16.0) * nI * nJ * P);
+    k_conjugate<<<(unsigned)blocks, 256, 0, c->stream>>>(src, nI, nJ, d, k, P, nmid, dst, accumulate ? 1 : 0);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
 // Pair-block adjoint input from a reduced pair matrix G2 (rows and columns
 // over the pair index a * d2 + b, reordered through inv when given):
 //   out[(I1*nJ1+J1)*nB + I2*nJ2+J2] = G2[row(r1[I1], r2[I2]), row(c1[J1], c2[J2])]

@@ -139,11 +139,21 @@ def conjugate_string(op):
     return op[::-1].translate(str.maketrans("ca", "ac"))
 
 
-def conjugate(rho):
+def conjugate(rho, out=None, accumulate=False):
     """<J| op^+ |I>[reversed indices] from <I| op |J>[indices] for real
     states: (nI, nJ, d, ..., d) -> (nJ, nI, d, ..., d) with the index axes
-    reversed.  An involution, so it also maps the adjoint back."""
-    return rho.permute(1, 0, *range(rho.dim() - 1, 1, -1)).contiguous()
+    reversed (xr_conjugate; into out, added to it with accumulate).  An
+    involution, so it also maps the adjoint back."""
+    rho = rho.contiguous()
+    k = rho.dim() - 2
+    nI, nJ = int(rho.shape[0]), int(rho.shape[1])
+    d = int(rho.shape[2]) if k else 1
+    if out is None:
+        out = torch.empty((nJ, nI) + tuple(rho.shape[2:]), dtype=rho.dtype, device=rho.device)
+    if tuple(out.shape) != (nJ, nI) + tuple(rho.shape[2:]) or not out.is_contiguous():
+        raise ValueError("conjugate: out must be a contiguous (nJ, nI, d, ...) tensor")
+    context(rho.device.index).call("xr_conjugate", ptr(rho), nI, nJ, d, k, ptr(out), 1 if accumulate else 0)
+    return out
 
 
 def build_density_arrays(blocks, ops_table=OPS_BY_CHG_DIFF_FULL, skip=("ccaa",), derived=None):

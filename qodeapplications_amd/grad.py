@@ -179,7 +179,8 @@ class XRAdjoint:
             # back through the same (self-inverse) conjugation
             for (op, key) in [k for k in rbar[m].d if k in F.derived]:
                 cop, ckey = F.derived[(op, key)]
-                rbar[m].get((cop, ckey), F.rho[cop][ckey].shape).add_(conjugate(rbar[m].d.pop((op, key))))
+                conjugate(rbar[m].d.pop((op, key)), out=rbar[m].get((cop, ckey), F.rho[cop][ckey].shape),
+                          accumulate=True)
             for (op, key), Rb in rbar[m].d.items():
                 bb, kb = F.blocks[key[0]], F.blocks[key[1]]
                 ctx.call("xr_density_adjoint", op.encode(), kb.n_orbs, kb.n_core, kb.n_elec,
