@@ -15,6 +15,16 @@ from .gemm import gemm_f64, mm
 DENSE_MAX = 6144
 
 
+def upload(a, device):
+    """Host array -> device tensor without waiting for the work already
+    queued on the stream (a pageable copy would): staged in pinned memory and
+    copied on the stream, which keeps the staging block until the copy ran."""
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if t.dtype == torch.float32:
+        t = t.double()
+    return t.pin_memory().to(device, non_blocking=True)
+
+
 class KroneckerSumPreconditioner:
     """(H0 - theta)^-1 for H0 = H1_1 (x) 1 + 1 (x) H1_2 (+ ...), applied in the
     product eigenbasis of the monomer blocks.  With the states' charges
@@ -139,11 +149,11 @@ def _orthonormalize(T, V=None, passes=2, Vt=None, extra=None):
             w, U = np.linalg.eigh(G)
             good = w > 1e-14 * w.max()
             Minv = U[:, good] / np.sqrt(w[good])
-            T = mm(T, torch.as_tensor(np.ascontiguousarray(Minv), device=T.device))
+            T = mm(T, upload(Minv, T.device))
             n0 = None
             continue
         Linv = np.ascontiguousarray(np.linalg.inv(L).T)
-        T = mm(T, torch.as_tensor(Linv, device=T.device))
+        T = mm(T, upload(Linv, T.device))
         if n0 is not None and np.all(np.diag(G) > 0.25 * n0):
             break
     return done(T.contiguous())
@@ -215,8 +225,8 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         S = 0.5 * (S + S.T)
         w, y = np.linalg.eigh(S)
         nb = min(b, m)
-        yb = torch.as_tensor(np.ascontiguousarray(y[:, :nb]), device=device)
-        theta = torch.as_tensor(w[:nb], device=device)
+        yb = upload(y[:, :nb], device)
+        theta = upload(w[:nb], device)
         X = mm(Vt[:m], yb, transA=1)
         AX = mm(AVt[:m], yb, transA=1)
         R = AX - X * theta
@@ -239,7 +249,7 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         restart = m + nb > max_space
         if restart:                         # thick restart on the lowest Ritz vectors
             nk = min(m, max(2 * nb, keep))
-            yk = torch.as_tensor(np.ascontiguousarray(y[:, :nk]), device=device)
+            yk = upload(y[:, :nk], device)
             Xk, AXk = mm(Vt[:m], yk, transA=1), mm(AVt[:m], yk, transA=1)
             Vt[:nk] = Xk.T
             AVt[:nk] = AXk.T

@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from ._lib import context, ptr
-from .eigen import davidson
+from .eigen import davidson, upload
 from .gemm import gemm_f64, mm
 
 
@@ -69,7 +69,7 @@ def _i32(a, device):
     hit = _I32_CACHE.get(key)
     if hit is not None and np.array_equal(hit[0], a):
         return hit[1]
-    t = torch.as_tensor(a, device=device)
+    t = upload(a, device)
     if len(_I32_CACHE) > 4096:
         _I32_CACHE.clear()
     _I32_CACHE[key] = (a, t)
@@ -167,15 +167,19 @@ class XROperator:
         per operator on the host (blocks of at most a few hundred states; a
         device solver would launch thousands of tiny kernels)."""
         if getattr(self, "_eig", None) is None:
-            self._eig = []
-            for blocks in h1:
-                d = {}
-                for c, h in blocks.items():
-                    hh = h.cpu().numpy()
-                    w, U = np.linalg.eigh(0.5 * (hh + hh.T))
-                    d[c] = (torch.as_tensor(w, device=self.device),
-                            torch.as_tensor(np.ascontiguousarray(U), device=self.device))
-                self._eig.append(d)
+            # all blocks in one transfer
+            keys = [(m, c) for m, blocks in enumerate(h1) for c in blocks]
+            flat = torch.cat([h1[m][c].reshape(-1) for m, c in keys]).cpu().numpy()
+            self._eig = [{} for _ in h1]
+            self._eig_low = {}         # (m, c) -> lowest eigenvalue, on the host
+            off = 0
+            for m, c in keys:
+                n = h1[m][c].shape[0]
+                hh = flat[off:off + n * n].reshape(n, n)
+                off += n * n
+                w, U = np.linalg.eigh(0.5 * (hh + hh.T))
+                self._eig[m][c] = (upload(w, self.device), upload(U, self.device))
+                self._eig_low[(m, c)] = float(w[0])
         return self._eig
 
     def _dressed_eig(self, sp, eig0):
@@ -192,12 +196,13 @@ class XROperator:
         nf = self.nf
         ground = []                   # (charge, |0_n><0_n|) of each fragment's lowest monomer state
         for n in range(nf):
-            c0 = min(eig0[n], key=lambda c: float(eig0[n][c][0][0]))
+            c0 = min(eig0[n], key=lambda c: self._eig_low[(n, c)])
             u = eig0[n][c0][1][:, 0]
             ground.append((c0, torch.outer(u, u)))
-        self._deig = []
+        # the dressed blocks of every (m, c) are formed on the device and
+        # reach the host in one transfer
+        dressed = []
         for m in range(nf):
-            d = {}
             for c, h in sp.h1[m].items():
                 ht = h.clone()
                 for (a, b) in self.pairs:
@@ -217,11 +222,17 @@ class XROperator:
                         lhs, nd = blk.permute(1, 3, 0, 2), da
                     ht = ht + mm(lhs.contiguous().view(h.shape[0] * h.shape[0], nd * nd),
                                  rho.T.contiguous().view(nd * nd, 1)).view(h.shape)
-                hh = ht.cpu().numpy()
-                _, U = np.linalg.eigh(0.5 * (hh + hh.T))
-                U = torch.as_tensor(np.ascontiguousarray(U), device=self.device)
-                d[c] = ((U * mm(h, U)).sum(0), U)
-            self._deig.append(d)
+                dressed.append((m, c, h, ht))
+        flat = torch.cat([ht.reshape(-1) for _, _, _, ht in dressed]).cpu().numpy()
+        self._deig = [{} for _ in range(nf)]
+        off = 0
+        for m, c, h, ht in dressed:
+            n = ht.shape[0]
+            hh = flat[off:off + n * n].reshape(n, n)
+            off += n * n
+            _, U = np.linalg.eigh(0.5 * (hh + hh.T))
+            U = upload(U, self.device)
+            self._deig[m][c] = ((U * mm(h, U)).sum(0), U)
         return self._deig
 
     def _pack(self, dims, ncol, f0, f1, rowmap, nrows, full, packed, mode, alpha=1.0, ld=0):
@@ -292,6 +303,7 @@ class XROperator:
         rest run until converged to tol."""
         found = []                    # converged: (value, sector space, vector)
         active = []
+        iterative = []
         for Q in (charges if charges is not None else self.total_charges()):
             sp = self.sector_space(Q)
             kk = min(k, sp.size)
@@ -301,8 +313,12 @@ class XROperator:
                 w, v = torch.linalg.eigh(0.5 * (M + M.T))
                 found += [(float(w[i]), sp, v[:, i]) for i in range(kk)]
             else:
+                iterative.append((sp, kk))
+        if iterative:                 # the sectors' lowest diagonal energies in one transfer
+            lams = torch.stack([sp.lam.min() for sp, _ in iterative]).cpu().numpy()
+            for (sp, kk), lam in zip(iterative, lams):
                 active.append({"sp": sp, "kk": kk, "b": block or _block_size(kk, sp.size), "st": {},
-                               "w": None, "iters": 0, "lam": float(sp.lam.min())})
+                               "w": None, "iters": 0, "lam": float(lam)})
         active.sort(key=lambda a: a["lam"])
 
         def upper():
@@ -596,7 +612,7 @@ class SectorSpace:
                 sq = op._sqidx[(m, n, q)]
                 where = np.empty(op.dims[m] * op.dims[n], dtype=np.int64)
                 where[sq] = np.arange(sq.size)
-                pos = torch.as_tensor(where[Pp["order_np"][start:start + cnt]], device=op.device)
+                pos = _i32(where[Pp["order_np"][start:start + cnt]], op.device)
                 self.__dict__.setdefault("_segpos", {})[(m, n, q)] = pos
             blocks.append(G[q].index_select(0, pos).index_select(1, pos))
         return blocks
