@@ -452,7 +452,16 @@ class SectorSpace:
                         idx.append(self._pair_index(m, n, cm, cn))
                         off += D
                 R = np.concatenate(idx)
-                self.sq[(m, n, s)] = (op.extract(m, n, R, R), parts)
+                # states grouped by ascending charge: the fixed-charge pair
+                # block is already in this order and is used as it is
+                Pp = op.pairs[(m, n)]
+                blk = None
+                if s in Pp["segq"]:
+                    i = Pp["segq"].index(s)
+                    start, cnt = Pp["segs"][i]
+                    if cnt == R.size and np.array_equal(R, Pp["order_np"][start:start + cnt]):
+                        blk = Pp["blocks"][i]
+                self.sq[(m, n, s)] = (op.extract(m, n, R, R) if blk is None else blk, parts)
                 op._sqidx[(m, n, s)] = R
         # preconditioner basis: monomer blocks dressed with the partners' mean
         # field; then the diagonal of each charge-conserving pair block in it,
@@ -468,19 +477,30 @@ class SectorSpace:
                 K = torch.kron(self.eig[m][cm][1], self.eig[n][cn][1]).contiguous()
                 HK = mm(Hself, K)
                 self.pair_diag[(m, n, cm, cn)] = (K * HK).sum(0).reshape(Dm, Dn)
-        lam = []
-        for i, t in enumerate(self.sectors):
-            l = torch.zeros(self.shapes[i], dtype=torch.float64, device=dev)
+        # lam over the tuples of one shape at once (stacked per fragment and
+        # pair: a few launches per shape instead of a few per tuple); the sums
+        # run in the same order as tuple by tuple
+        groups = {}
+        for i in range(len(self.sectors)):
+            groups.setdefault(tuple(self.shapes[i]), []).append(i)
+        self.lam = torch.empty(self.size, dtype=torch.float64, device=dev)
+        for sh, idx in groups.items():
+            ts = [self.sectors[i] for i in idx]
+            l = torch.zeros((len(idx),) + sh, dtype=torch.float64, device=dev)
             for m in range(nf):
-                sh = [1] * nf
-                sh[m] = self.shapes[i][m]
-                l = l + self.eig[m][t[m]][0].reshape(sh)
+                vs = [len(idx)] + [1] * nf
+                vs[m + 1] = sh[m]
+                l = l + torch.stack([self.eig[m][t[m]][0] for t in ts]).reshape(vs)
             for (m, n) in op.pairs:
-                sh = [1] * nf
-                sh[m], sh[n] = self.shapes[i][m], self.shapes[i][n]
-                l = l + self.pair_diag[(m, n, t[m], t[n])].reshape(sh)
-            lam.append(l.reshape(-1))
-        self.lam = torch.cat(lam) if lam else torch.zeros(0, dtype=torch.float64, device=dev)
+                vs = [len(idx)] + [1] * nf
+                vs[m + 1], vs[n + 1] = sh[m], sh[n]
+                l = l + torch.stack([self.pair_diag[(m, n, t[m], t[n])] for t in ts]).reshape(vs)
+            l = l.reshape(len(idx), -1)
+            if len(groups) == 1:
+                self.lam = l.reshape(-1)
+            else:
+                for j, i in enumerate(idx):
+                    self.lam[int(self.offsets[i]):int(self.offsets[i + 1])] = l[j]
 
     def _pair_index(self, m, n, cm, cn):
         """Pair indices a * d_n + b of the states with charges (cm, cn)."""
