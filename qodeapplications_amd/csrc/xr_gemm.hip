@@ -1327,10 +1327,11 @@ int xr_dgemm_split_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64
                           const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
                           int64_t sB, double beta, double* C, int64_t ldc, int64_t sC, int64_t batch) {
     if (M <= 0 || N <= 0 || batch <= 0) return XR_OK;
-    // 128 x 128 tiles when they alone give two workgroups per CU, else
-    // 64 x 64 tiles and a K split (chunks of at least 256) toward 1024
-    // workgroups
-    const int bm = (ceil_div(M, 128) * ceil_div(N, 128) * batch >= 512) ? 128 : 64;
+    // 64 x 64 tiles, with a K split (chunks of at least 256) toward 1024
+    // workgroups when the tiles are few.  The 128 x 128 variant needs more
+    // than the 256 registers a lane has at two waves per SIMD (it spilled 27)
+    // and ran the c3 split iteration 6% slower.
+    const int bm = 64;
     int64_t tm = ceil_div(M, bm), tn = ceil_div(N, bm), tiles = tm * tn;
     if (tiles > 0x7fffffff || batch > 65535) return xr_fail(c, XR_EUNSUPPORTED, "GEMM grid too large");
     int64_t splits = 1, kchunk = K;
@@ -1349,8 +1350,7 @@ int xr_dgemm_split_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64
     {
         ProfScope prof(c, XR_K_GEMM_SPLIT, 2.0 * M * N * K * batch, 8.0 * (M * K + K * N + M * N) * batch);
         prof.shape(M, N, K, batch, ta, tb);
-        if (bm == 128) split_go<128>(c, ta, tb, grid, M, N, K, A, lda, sA, B, ldb, sB, Cw, tm, tn, kchunk, batch);
-        else split_go<64>(c, ta, tb, grid, M, N, K, A, lda, sA, B, ldb, sB, Cw, tm, tn, kchunk, batch);
+        split_go<64>(c, ta, tb, grid, M, N, K, A, lda, sA, B, ldb, sB, Cw, tm, tn, kchunk, batch);
         XR_LAUNCH_CHECK(c);
     }
     return splitk_reduce(c, Cw, splits, batch, M, N, alpha, beta, C, ldc, sC);
