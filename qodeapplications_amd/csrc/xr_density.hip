@@ -887,13 +887,44 @@ extern "C" int xr_monomer_2e_adjoint(xr_ctx* c, int n_orbs, int n_core, int n_el
         return xr_fail(c, XR_ENOMEM, "workspace (sigma adjoint)");
     if ((rc = xr_side_table(c, pl.L, Km, nK, n_orbs, n_core, &tabL))) return rc;
     if ((rc = xr_side_table(c, pl.R, Km, nK, n_orbs, n_core, &tabR))) return rc;
-    if ((rc = xr_gather(c, tabL, nK, nU, zTb, nI, X, 1))) return rc;   // X[u][K][I]
+    // A block with itself into one adjoint buffer: both sides remove two
+    // electrons from the same states, and their tables differ only by the
+    // order of the two annihilators (sg = -1 when the slots are swapped), so
+    // X = sg Y.  Ve is symmetric (the two-body part of H is), hence
+    // Ybar = Ve^T Wbar = sg W Ebar with W = Ve Y, side_R = sg side_L, and the
+    // two side adjoints add up to side_L(W (Ebar + Ebar^T)): one gather, no
+    // Ve^T Wbar product, one side adjoint.
+    const double sg = (pl.L.slot[0] == pl.R.slot[0]) ? 1.0 : -1.0;
+    const bool mirror = zbra == zket && nI == nJ && nP == nQ && zbar_bra == zbar_ket && zbar_ket &&
+                        pl.L.k == 2 && pl.R.k == 2 && pl.L.is_c == pl.R.is_c &&
+                        pl.L.slot[0] != pl.L.slot[1] && pl.R.slot[0] != pl.R.slot[1];
+    if (!mirror && (rc = xr_gather(c, tabL, nK, nU, zTb, nI, X, 1))) return rc;   // X[u][K][I]
     if ((rc = xr_gather(c, tabR, nK, nU, zTk, nJ, Y, 1))) return rc;   // Y[w][K][J]
     k_veff<<<xr_grid(nU * nU), 256, 0, c->stream>>>(V, dim, nU, Ve);
     XR_LAUNCH_CHECK(c);
     const int64_t KJ = nK * nJ;
     // W = Ve Y (nU x KJ)
     if ((rc = xr_gemm_2e(c, 0, 0, nU, KJ, nU, 1.0, Ve, nU, 0, Y, KJ, 0, 0.0, W, KJ, 0, 1))) return rc;
+    if (mirror) {
+        double* Es = (double*)xr_workspace(c, WS_GS, nI * nI * sizeof(double));
+        if (!Es) return xr_fail(c, XR_ENOMEM, "workspace (sigma adjoint)");
+        dim3 grid((unsigned)ceil_div(nI, 32), (unsigned)ceil_div(nI, 32));
+        k_sym_sum<<<grid, dim3(32, 8), 0, c->stream>>>(Ebar, nI, Es);   // Es = Ebar + Ebar^T
+        XR_LAUNCH_CHECK(c);
+        // Xbar[(u,K), I] = sum_J W[(u,K), J] Es[I, J]
+        if ((rc = xr_gemm_2e(c, 0, 1, nU * nK, nI, nJ, 1.0, W, nJ, 0, Es, nJ, 0, 0.0, Xbar, nI, 0, 1)))
+            return rc;
+        if (Vbar) {
+            // Wbar = X Ebar = sg Y Ebar, Veb = Wbar Y^T, then scatter into Vbar
+            if ((rc = xr_gemm_2e(c, 0, 0, nU * nK, nJ, nI, sg, Y, nI, 0, Ebar, nJ, 0, 0.0, Wbar, nJ, 0, 1)))
+                return rc;
+            if ((rc = xr_gemm_2e(c, 0, 1, nU, nU, KJ, 1.0, Wbar, KJ, 0, Y, KJ, 0, 0.0, Veb, nU, 0, 1)))
+                return rc;
+            k_veff_adjoint<<<xr_grid(nU * nU), 256, 0, c->stream>>>(Veb, dim, nU, Vbar);
+            XR_LAUNCH_CHECK(c);
+        }
+        return side_adjoint(c, pl.L, n_orbs, n_core, 0, n_elec, nK, nU, Xbar, nI, 1, zbar_bra, zbT);
+    }
     // Xbar[(u,K), I] = sum_J W[(u,K), J] Ebar[I, J]    ((nU nK) x nI)
     if ((rc = xr_gemm_2e(c, 0, 1, nU * nK, nI, nJ, 1.0, W, nJ, 0, Ebar, nJ, 0, 0.0, Xbar, nI, 0, 1)))
         return rc;
