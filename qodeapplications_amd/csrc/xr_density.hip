@@ -504,7 +504,12 @@ extern "C" int xr_monomer_2e(xr_ctx* c, int n_orbs, int n_core, int n_elec, int6
     if (!X || !Y || !W || !Ve) return xr_fail(c, XR_ENOMEM, "workspace (sigma)");
     if ((rc = xr_side_table(c, pl.L, Km, nK, n_orbs, n_core, &tabL))) return rc;
     if ((rc = xr_side_table(c, pl.R, Km, nK, n_orbs, n_core, &tabR))) return rc;
-    if ((rc = xr_gather(c, tabL, nK, nU, zTb, nI, X, 1))) return rc;   // X[u][K][I]
+    // a block with itself: the two sides' tables differ only by the order of
+    // the two annihilators, so X = sg Y and one gather serves both
+    const double sg = (pl.L.slot[0] == pl.R.slot[0]) ? 1.0 : -1.0;
+    const bool same = zbra == zket && nI == nJ && nP == nQ && pl.L.k == 2 && pl.R.k == 2 &&
+                      pl.L.is_c == pl.R.is_c && pl.L.slot[0] != pl.L.slot[1] && pl.R.slot[0] != pl.R.slot[1];
+    if (!same && (rc = xr_gather(c, tabL, nK, nU, zTb, nI, X, 1))) return rc;   // X[u][K][I]
     if ((rc = xr_gather(c, tabR, nK, nU, zTk, nJ, Y, 1))) return rc;   // Y[w][K][J]
     k_veff<<<xr_grid(nU * nU), 256, 0, c->stream>>>(V, dim, nU, Ve);
     XR_LAUNCH_CHECK(c);
@@ -513,6 +518,7 @@ extern "C" int xr_monomer_2e(xr_ctx* c, int n_orbs, int n_core, int n_elec, int6
                               nK * nJ, 0, 1)))
         return rc;
     // E[I,J] = sum_{(u,K)} X[(u,K), I] W[(u,K), J]
+    if (same) return xr_gemm_2e(c, 1, 0, nI, nJ, nU * nK, sg, Y, nI, 0, W, nJ, 0, 0.0, out, nJ, 0, 1);
     return xr_gemm_2e(c, 1, 0, nI, nJ, nU * nK, 1.0, X, nI, 0, W, nJ, 0, 0.0, out, nJ, 0, 1);
 }
 
