@@ -403,8 +403,8 @@ static int prepare_factors(xr_ctx* c, const OpPlan& pl, int n_orbs, int n_core, 
         if ((rc = xr_space_masks(c, n_orbs, 0, n_elec_ket - pl.R.k, &f->Km, &f->nK))) return rc;
         f->kspace_core = 0;
     }
+    int32_t* tabL = nullptr;
     if (pl.L.k) {
-        int32_t* tabL = nullptr;
         double* Xb = (double*)xr_workspace(c, WS_X, f->nK * f->nLt * nI * sizeof(double));
         if (!Xb) return xr_fail(c, XR_ENOMEM, "workspace (bra gather)");
         if ((rc = xr_side_table(c, pl.L, f->Km, f->nK, n_orbs, n_core, &tabL))) return rc;
@@ -413,11 +413,15 @@ static int prepare_factors(xr_ctx* c, const OpPlan& pl, int n_orbs, int n_core, 
     }
     if (pl.R.k) {
         int32_t* tabR = nullptr;
-        double* Yb = (double*)xr_workspace(c, WS_Y, f->nK * f->nT * nJ * sizeof(double));
-        if (!Yb) return xr_fail(c, XR_ENOMEM, "workspace (ket gather)");
         if ((rc = xr_side_table(c, pl.R, f->Km, f->nK, n_orbs, n_core, &tabR))) return rc;
-        if ((rc = xr_gather(c, tabR, f->nK, f->nT, zTk, nJ, Yb, 0))) return rc;
-        f->Y = Yb;
+        if (pl.L.k && tabR == tabL && zbra == zket && nI == nJ && nP == nQ) {
+            f->Y = f->X;                    // a block with itself through one table: the same factor
+        } else {
+            double* Yb = (double*)xr_workspace(c, WS_Y, f->nK * f->nT * nJ * sizeof(double));
+            if (!Yb) return xr_fail(c, XR_ENOMEM, "workspace (ket gather)");
+            if ((rc = xr_gather(c, tabR, f->nK, f->nT, zTk, nJ, Yb, 0))) return rc;
+            f->Y = Yb;
+        }
     }
     return XR_OK;
 }
@@ -442,8 +446,12 @@ int xr_density_impl(xr_ctx* c, const char* ops, int n_orbs, int n_core, int n_el
     const int64_t Mg = f.nLt * nI, Ng = f.nT * nJ;
     double* Cc = (double*)xr_workspace(c, WS_C, Mg * Ng * sizeof(double));
     if (!Cc) return xr_fail(c, XR_ENOMEM, "workspace (compressed density)");
-    if ((rc = xr_dgemm_launch(c, 1, 0, Mg, Ng, f.nK, 1.0, f.X, Mg, 0, f.Y, Ng, 0, 0.0, Cc, Ng, 0, 1)))
+    if (f.X == f.Y && Mg == Ng) {
+        // a block with itself through one factor: Cc = X^T X is symmetric
+        if ((rc = xr_dgemm_syrk_tn(c, Mg, f.nK, 1.0, f.X, Mg, Cc, Ng))) return rc;
+    } else if ((rc = xr_dgemm_launch(c, 1, 0, Mg, Ng, f.nK, 1.0, f.X, Mg, 0, f.Y, Ng, 0, 0.0, Cc, Ng, 0, 1))) {
         return rc;
+    }
     int64_t nIL = 1, nIR = 1;
     for (int j = 0; j < pl.L.k; j++) nIL *= dim;
     for (int j = 0; j < pl.R.k; j++) nIR *= dim;

@@ -8,6 +8,10 @@ int xr_dgemm_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, 
                     const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
                     int64_t sB, double beta, double* C, int64_t ldc, int64_t sC, int64_t batch);
 
+// C = alpha A^T A (A: K x n, C: n x n): upper tiles, then the mirror.
+int xr_dgemm_syrk_tn(xr_ctx* c, int64_t n, int64_t K, double alpha, const double* A, int64_t lda, double* C,
+                     int64_t ldc);
+
 // Split-FP32 product (hi/lo FP32 MFMA, FP64 accumulation) and the
 // precision-switched GEMM used by the two-electron contractions.
 int xr_dgemm_split_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,

@@ -193,7 +193,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm(
     int64_t M, int64_t N, int64_t K, double alpha, const double* __restrict__ A, int64_t lda,
     int64_t sA, const double* __restrict__ B, int64_t ldb, int64_t sB, double beta,
     double* __restrict__ C, int64_t ldc, int64_t sC, int64_t tiles_m, int64_t tiles_n,
-    int64_t kchunk, int64_t sSplit) {
+    int64_t kchunk, int64_t sSplit, int upper = 0) {
     (void)beta;
     const int64_t batch = blockIdx.z;
     A += batch * sA;
@@ -213,6 +213,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm(
     const int64_t gsize = (tiles_m - first_m < GROUP) ? (tiles_m - first_m) : GROUP;
     const int64_t tm = first_m + (tile % per_group) % gsize;
     const int64_t tn = (tile % per_group) / gsize;
+    if (upper && tm > tn) return;              // symmetric product: the upper tiles only
     const int64_t m0 = tm * BM, n0 = tn * BN;
 
     dgemm_tile<BM, BN, TA, TB>(M, N, K, alpha, A, lda, B, ldb, C, ldc, m0, n0);
@@ -790,7 +791,7 @@ template <int BM, int BN>
 static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
                         const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
                         int64_t sB, double beta, double* C, int64_t ldc, int64_t sC,
-                        int64_t batch, int splits_req = 0) {
+                        int64_t batch, int splits_req = 0, bool upper = false) {
     int64_t tm = ceil_div(M, BM), tn = ceil_div(N, BN);
     int64_t tiles = tm * tn;
     if (tiles > 0x7fffffff || batch > 65535)
@@ -813,7 +814,7 @@ static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t
     }
     // accumulate in the epilogue when the tiles fit in one round of half the
     // resident workgroups (the variant's lower occupancy then costs nothing)
-    if (splits == 1 && beta != 0.0 && 2 * tiles * batch <= tile_slots<BM, BN>()) {
+    if (!upper && splits == 1 && beta != 0.0 && 2 * tiles * batch <= tile_slots<BM, BN>()) {
         dim3 grid((unsigned)tiles, 1, (unsigned)batch);
         ProfScope prof(c, XR_K_GEMM_F64, 2.0 * M * N * K * batch, 8.0 * (M * K + K * N + 2 * M * N) * batch);
         prof.shape(M, N, K, batch, ta, tb);
@@ -849,7 +850,7 @@ static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t
 #define XR_GEMM_LAUNCH(TA_, TB_)                                                                 \
     k_dgemm<BM, BN, TA_, TB_><<<grid, GEMM_THREADS, 0, c->stream>>>(M, N, K, aw, A, lda, sA, B,   \
                                                                      ldb, sB, bw, Cw, ldw, sCw,   \
-                                                                     tm, tn, kchunk, sSplit)
+                                                                     tm, tn, kchunk, sSplit, upper ? 1 : 0)
         if (ta && tb) XR_GEMM_LAUNCH(true, true);
         else if (ta) XR_GEMM_LAUNCH(true, false);
         else if (tb) XR_GEMM_LAUNCH(false, true);
@@ -862,6 +863,40 @@ static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t
         int rc = splitk_reduce(c, Cw, splits, batch, M, N, alpha, beta, C, ldc, sC);
         if (rc) return rc;
     }
+    return XR_OK;
+}
+
+// C[i][j] = C[j][i] for i > j: the lower triangle from the upper one (32 x 32
+// tiles through LDS; a workgroup writes one lower tile and reads its mirror,
+// which no workgroup writes).
+__global__ __launch_bounds__(256) void k_mirror_upper(double* __restrict__ C, int64_t n, int64_t ldc) {
+    const int64_t a = blockIdx.y, b = blockIdx.x;     // target tile rows a, columns b
+    if (a < b) return;
+    __shared__ double T[32][33];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    for (int j = ty; j < 32; j += 8) {                // T[j][tx] = C[b*32 + j][a*32 + tx]
+        const int64_t r = b * 32 + j, col = a * 32 + tx;
+        if (r < n && col < n) T[j][tx] = C[r * ldc + col];
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {                // C[a*32 + j][b*32 + tx] = C[b*32 + tx][a*32 + j]
+        const int64_t r = a * 32 + j, col = b * 32 + tx;
+        if (r < n && col < n && r > col) C[r * ldc + col] = T[tx][j];
+    }
+}
+
+// C = alpha A^T A for A (K x n, leading dimension lda): the tile GEMM on the
+// upper tiles only, then the lower triangle mirrored.
+int xr_dgemm_syrk_tn(xr_ctx* c, int64_t n, int64_t K, double alpha, const double* A, int64_t lda, double* C,
+                     int64_t ldc) {
+    if (n <= 0) return XR_OK;
+    int rc = launch_tiles<128, 128>(c, 1, 0, n, n, K, alpha, A, lda, 0, A, lda, 0, 0.0, C, ldc, 0, 1, 0, true);
+    if (rc) return rc;
+    const int64_t t = ceil_div(n, 32);
+    if (t > 65535) return xr_fail(c, XR_EUNSUPPORTED, "matrix too large for the mirror grid");
+    ProfScope prof(c, XR_K_OTHER, 0.0, 8.0 * n * n);
+    k_mirror_upper<<<dim3((unsigned)t, (unsigned)t), dim3(32, 8), 0, c->stream>>>(C, n, ldc);
+    XR_LAUNCH_CHECK(c);
     return XR_OK;
 }
 
