@@ -145,6 +145,23 @@ int xr_monomer_2e_adjoint(xr_ctx *ctx, int n_orbs, int n_core, int n_elec,
                           const double *d_V, const double *d_Ebar, double *d_zbar_bra,
                           double *d_zbar_ket, double *d_Vbar);
 
+/* xr_monomer_2e that also leaves its intermediate W = Veff Y (Veff: V
+ * antisymmetrised on sorted spin-orbital pairs, Y: the ket gathered over
+ * the pairs it loses) in d_W, w_len = C(dim,2) * C(dim, n_elec-2) * nJ
+ * doubles, and the adjoint that takes it instead of recomputing it (same V
+ * and ket; for the gradient step that follows the energy).  The same
+ * references as xr_monomer_2e (build_density_tensors.py:122-133,
+ * H_contractions.c:61-78). */
+int xr_monomer_2e_keep(xr_ctx *ctx, int n_orbs, int n_core, int n_elec,
+                       int64_t nI, int64_t nP, const double *d_zbra,
+                       int64_t nJ, int64_t nQ, const double *d_zket,
+                       const double *d_V, double *d_out, double *d_W, int64_t w_len);
+int xr_monomer_2e_adjoint_keep(xr_ctx *ctx, int n_orbs, int n_core, int n_elec,
+                               int64_t nI, int64_t nP, const double *d_zbra,
+                               int64_t nJ, int64_t nQ, const double *d_zket,
+                               const double *d_V, const double *d_Ebar, double *d_zbar_bra,
+                               double *d_zbar_ket, double *d_Vbar, const double *d_W, int64_t w_len);
+
 /* Strided-batched FP64 GEMM on the matrix cores:
  *   C_b = alpha * op(A_b) * op(B_b) + beta * C_b,  b < batch.
  * transA = 0: A is M x K row-major (A[m*lda+k]); 1: A is K x M (A[k*lda+m]).

@@ -41,6 +41,8 @@ SIGNATURES = {
     "xr_monomer_2e": (_I, [_P, _I, _I, _I, _L, _L, _P, _L, _L, _P, _P, _P]),
     "xr_density_adjoint": (_I, [_P, _S, _I, _I, _I, _L, _L, _P, _L, _L, _P, _P, _P, _P]),
     "xr_monomer_2e_adjoint": (_I, [_P, _I, _I, _I, _L, _L, _P, _L, _L, _P, _P, _P, _P, _P, _P]),
+    "xr_monomer_2e_keep": (_I, [_P, _I, _I, _I, _L, _L, _P, _L, _L, _P, _P, _P, _P, _L]),
+    "xr_monomer_2e_adjoint_keep": (_I, [_P, _I, _I, _I, _L, _L, _P, _L, _L, _P, _P, _P, _P, _P, _P, _P, _L]),
     "xr_pair_outer": (_I, [_P, _P, _L, _P, _P, _P, _P, _L, _L, _L, _L, _I, _P]),
     "xr_gemm_tune": (_I, [_P, _I, _I, _L, _L, _L, _L, _I, _P, _P, _P]),
     "xr_gemm_tune_reset": (_I, [_P]),

@@ -481,11 +481,11 @@ extern "C" int xr_density(xr_ctx* c, const char* ops, int n_orbs, int n_core, in
     return xr_density_impl(c, ops, n_orbs, n_core, n_elec_ket, nI, nP, zbra, nJ, nQ, zket, out);
 }
 
-extern "C" int xr_monomer_2e(xr_ctx* c, int n_orbs, int n_core, int n_elec, int64_t nI, int64_t nP,
-                             const double* zbra, int64_t nJ, int64_t nQ, const double* zket,
-                             const double* V, double* out) {
-    if (!c) return XR_EINVAL;
-    std::lock_guard<std::mutex> g(c->mu);
+// Wkeep: caller's buffer for W = Veff Y (nU * nK * nJ doubles, w_len), which
+// the adjoint can take instead of recomputing it; nullptr: a workspace.
+static int monomer_2e_impl(xr_ctx* c, int n_orbs, int n_core, int n_elec, int64_t nI, int64_t nP,
+                           const double* zbra, int64_t nJ, int64_t nQ, const double* zket,
+                           const double* V, double* out, double* Wkeep, int64_t w_len) {
     OpPlan pl;
     xr_plan_op("ccaa", &pl);
     int nb = 0;
@@ -507,7 +507,8 @@ extern "C" int xr_monomer_2e(xr_ctx* c, int n_orbs, int n_core, int n_elec, int6
     int32_t* tabR = nullptr;
     double* X = (double*)xr_workspace(c, WS_X, nU * nK * nI * sizeof(double));
     double* Y = (double*)xr_workspace(c, WS_Y, nU * nK * nJ * sizeof(double));
-    double* W = (double*)xr_workspace(c, WS_W, nU * nK * nJ * sizeof(double));
+    if (Wkeep && w_len != nU * nK * nJ) return xr_fail(c, XR_ESHAPE, "sigma factor buffer size");
+    double* W = Wkeep ? Wkeep : (double*)xr_workspace(c, WS_W, nU * nK * nJ * sizeof(double));
     double* Ve = (double*)xr_workspace(c, WS_VE, nU * nU * sizeof(double));
     if (!X || !Y || !W || !Ve) return xr_fail(c, XR_ENOMEM, "workspace (sigma)");
     if ((rc = xr_side_table(c, pl.L, Km, nK, n_orbs, n_core, &tabL))) return rc;
@@ -528,6 +529,23 @@ extern "C" int xr_monomer_2e(xr_ctx* c, int n_orbs, int n_core, int n_elec, int6
     // E[I,J] = sum_{(u,K)} X[(u,K), I] W[(u,K), J]
     if (same) return xr_gemm_2e(c, 1, 0, nI, nJ, nU * nK, sg, Y, nI, 0, W, nJ, 0, 0.0, out, nJ, 0, 1);
     return xr_gemm_2e(c, 1, 0, nI, nJ, nU * nK, 1.0, X, nI, 0, W, nJ, 0, 0.0, out, nJ, 0, 1);
+}
+
+extern "C" int xr_monomer_2e(xr_ctx* c, int n_orbs, int n_core, int n_elec, int64_t nI, int64_t nP,
+                             const double* zbra, int64_t nJ, int64_t nQ, const double* zket,
+                             const double* V, double* out) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    return monomer_2e_impl(c, n_orbs, n_core, n_elec, nI, nP, zbra, nJ, nQ, zket, V, out, nullptr, 0);
+}
+
+extern "C" int xr_monomer_2e_keep(xr_ctx* c, int n_orbs, int n_core, int n_elec, int64_t nI, int64_t nP,
+                                  const double* zbra, int64_t nJ, int64_t nQ, const double* zket,
+                                  const double* V, double* out, double* W, int64_t w_len) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!W) return xr_fail(c, XR_EINVAL, "null sigma factor buffer");
+    return monomer_2e_impl(c, n_orbs, n_core, n_elec, nI, nP, zbra, nJ, nQ, zket, V, out, W, w_len);
 }
 
 // ---------------------------------------------------------------------------
@@ -863,12 +881,13 @@ __global__ void k_veff_adjoint(const double* __restrict__ Veb, int dim, int64_t 
     XR_GRID_STRIDE(i, nU * nU) k_veff_adjoint_elem(i, Veb, dim, nU, Vbar);
 }
 
-extern "C" int xr_monomer_2e_adjoint(xr_ctx* c, int n_orbs, int n_core, int n_elec, int64_t nI,
-                                     int64_t nP, const double* zbra, int64_t nJ, int64_t nQ,
-                                     const double* zket, const double* V, const double* Ebar,
-                                     double* zbar_bra, double* zbar_ket, double* Vbar) {
-    if (!c) return XR_EINVAL;
-    std::lock_guard<std::mutex> g(c->mu);
+// Wgiven: W = Veff Y as xr_monomer_2e_keep left it (same V and ket), used
+// instead of recomputing the product; nullptr: recomputed.
+static int monomer_2e_adjoint_impl(xr_ctx* c, int n_orbs, int n_core, int n_elec, int64_t nI,
+                                   int64_t nP, const double* zbra, int64_t nJ, int64_t nQ,
+                                   const double* zket, const double* V, const double* Ebar,
+                                   double* zbar_bra, double* zbar_ket, double* Vbar,
+                                   const double* Wgiven, int64_t w_len) {
     OpPlan pl;
     xr_plan_op("ccaa", &pl);
     int nb = 0;
@@ -889,7 +908,8 @@ extern "C" int xr_monomer_2e_adjoint(xr_ctx* c, int n_orbs, int n_core, int n_el
     int32_t* tabR = nullptr;
     double* X = (double*)xr_workspace(c, WS_X, nU * nK * nI * sizeof(double));
     double* Y = (double*)xr_workspace(c, WS_Y, nU * nK * nJ * sizeof(double));
-    double* W = (double*)xr_workspace(c, WS_W, nU * nK * nJ * sizeof(double));
+    if (Wgiven && w_len != nU * nK * nJ) return xr_fail(c, XR_ESHAPE, "sigma factor buffer size");
+    double* W = Wgiven ? const_cast<double*>(Wgiven) : (double*)xr_workspace(c, WS_W, nU * nK * nJ * sizeof(double));
     double* Ve = (double*)xr_workspace(c, WS_VE, nU * nU * sizeof(double));
     double* Xbar = (double*)xr_workspace(c, WS_XB, nU * nK * nI * sizeof(double));
     double* Ybar = (double*)xr_workspace(c, WS_YB, nU * nK * nJ * sizeof(double));
@@ -918,7 +938,7 @@ extern "C" int xr_monomer_2e_adjoint(xr_ctx* c, int n_orbs, int n_core, int n_el
     XR_LAUNCH_CHECK(c);
     const int64_t KJ = nK * nJ;
     // W = Ve Y (nU x KJ)
-    if ((rc = xr_gemm_2e(c, 0, 0, nU, KJ, nU, 1.0, Ve, nU, 0, Y, KJ, 0, 0.0, W, KJ, 0, 1))) return rc;
+    if (!Wgiven && (rc = xr_gemm_2e(c, 0, 0, nU, KJ, nU, 1.0, Ve, nU, 0, Y, KJ, 0, 0.0, W, KJ, 0, 1))) return rc;
     if (mirror) {
         double* Es = (double*)xr_workspace(c, WS_GS, nI * nI * sizeof(double));
         if (!Es) return xr_fail(c, XR_ENOMEM, "workspace (sigma adjoint)");
@@ -962,6 +982,28 @@ extern "C" int xr_monomer_2e_adjoint(xr_ctx* c, int n_orbs, int n_core, int n_el
         if ((rc = side_adjoint(c, pl.L, n_orbs, n_core, 0, n_elec, nK, nU, Xbar, nI, 1, zbar_bra, zbT)))
             return rc;
     return XR_OK;
+}
+
+extern "C" int xr_monomer_2e_adjoint(xr_ctx* c, int n_orbs, int n_core, int n_elec, int64_t nI,
+                                     int64_t nP, const double* zbra, int64_t nJ, int64_t nQ,
+                                     const double* zket, const double* V, const double* Ebar,
+                                     double* zbar_bra, double* zbar_ket, double* Vbar) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    return monomer_2e_adjoint_impl(c, n_orbs, n_core, n_elec, nI, nP, zbra, nJ, nQ, zket, V, Ebar, zbar_bra,
+                                   zbar_ket, Vbar, nullptr, 0);
+}
+
+extern "C" int xr_monomer_2e_adjoint_keep(xr_ctx* c, int n_orbs, int n_core, int n_elec, int64_t nI,
+                                          int64_t nP, const double* zbra, int64_t nJ, int64_t nQ,
+                                          const double* zket, const double* V, const double* Ebar,
+                                          double* zbar_bra, double* zbar_ket, double* Vbar,
+                                          const double* W, int64_t w_len) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!W) return xr_fail(c, XR_EINVAL, "null sigma factor buffer");
+    return monomer_2e_adjoint_impl(c, n_orbs, n_core, n_elec, nI, nP, zbra, nJ, nQ, zket, V, Ebar, zbar_bra,
+                                   zbar_ket, Vbar, W, w_len);
 }
 
 extern "C" int xr_density_adjoint(xr_ctx* c, const char* ops, int n_orbs, int n_core,

@@ -5,6 +5,8 @@ HBM as torch tensors (the container), every density, excitation list and
 monomer energy is computed by the HIP kernels behind the C ABI.
 """
 
+import math
+
 import numpy as np
 import torch
 
@@ -87,8 +89,10 @@ def density(op, bra, ket, out=None):
     return out
 
 
-def monomer_2e(bra, ket, V):
-    """E[I,J] = sum V[p,q,r,s] <I|p+ q+ s r|J> (nI x nJ), V as (dim,)*4."""
+def monomer_2e(bra, ket, V, keep=False):
+    """E[I,J] = sum V[p,q,r,s] <I|p+ q+ s r|J> (nI x nJ), V as (dim,)*4.
+    keep: also return the intermediate W = Veff Y (or None) that
+    monomer_2e_adjoint can take instead of recomputing it."""
     if bra.n_elec != ket.n_elec:
         raise ValueError("monomer_2e needs one charge")
     V = torch.as_tensor(V, dtype=torch.float64, device=ket.device).contiguous()
@@ -96,9 +100,15 @@ def monomer_2e(bra, ket, V):
         raise ValueError("V must be (dim, dim, dim, dim)")
     out = torch.empty((bra.n_states, ket.n_states), dtype=torch.float64, device=ket.device)
     ctx = context(ket.device.index)
-    ctx.call("xr_monomer_2e", ket.n_orbs, ket.n_core, ket.n_elec, bra.n_states, bra.n_configs,
-             ptr(bra.coeffs), ket.n_states, ket.n_configs, ptr(ket.coeffs), ptr(V), ptr(out))
-    return out
+    if not keep or ket.n_elec < 2:
+        ctx.call("xr_monomer_2e", ket.n_orbs, ket.n_core, ket.n_elec, bra.n_states, bra.n_configs,
+                 ptr(bra.coeffs), ket.n_states, ket.n_configs, ptr(ket.coeffs), ptr(V), ptr(out))
+        return (out, None) if keep else out
+    w_len = math.comb(ket.dim, 2) * math.comb(ket.dim, ket.n_elec - 2) * ket.n_states
+    W = torch.empty(w_len, dtype=torch.float64, device=ket.device)
+    ctx.call("xr_monomer_2e_keep", ket.n_orbs, ket.n_core, ket.n_elec, bra.n_states, bra.n_configs,
+             ptr(bra.coeffs), ket.n_states, ket.n_configs, ptr(ket.coeffs), ptr(V), ptr(out), ptr(W), w_len)
+    return out, W
 
 
 def excitation_list(op, ket, q0=0, q1=None):

@@ -100,7 +100,7 @@ class XRAdjoint:
 
     def forward(self):
         X = self.X
-        X.build_densities()
+        X.build_densities(keep_sigma=True)
         self.op = op = X.operator()
         if op.n <= DENSE_MAX:
             M = op.dense()
@@ -156,9 +156,14 @@ class XRAdjoint:
                 mm(_flat(Rca), G1.view(-1, 1), transA=1, out=hb.view(-1, 1), beta=1.0)
                 blk = F.blocks[c]
                 Vb = ib("V", m, m, m, m)
-                ctx.call("xr_monomer_2e_adjoint", blk.n_orbs, blk.n_core, blk.n_elec, blk.n_states,
-                         blk.n_configs, ptr(blk.coeffs), blk.n_states, blk.n_configs, ptr(blk.coeffs),
-                         ptr(X.ints.V(m, m, m, m)), ptr(G1), ptr(zbar[m][c]), ptr(zbar[m][c]), ptr(Vb))
+                W = F.E2W.get(c)
+                args = (blk.n_orbs, blk.n_core, blk.n_elec, blk.n_states, blk.n_configs, ptr(blk.coeffs),
+                        blk.n_states, blk.n_configs, ptr(blk.coeffs), ptr(X.ints.V(m, m, m, m)), ptr(G1),
+                        ptr(zbar[m][c]), ptr(zbar[m][c]), ptr(Vb))
+                if W is not None:       # the forward's W = Veff Y (same V and coefficients)
+                    ctx.call("xr_monomer_2e_adjoint_keep", *args, ptr(W), W.numel())
+                else:
+                    ctx.call("xr_monomer_2e_adjoint", *args)
             h_bar(parts, hb)
 
         # ---- pair blocks

@@ -57,13 +57,20 @@ class XRFragment:
         self.rho = None
         self.derived = {}
         self.E2 = None
+        self.E2W = {}
 
-    def build(self, V_mm):
+    def build(self, V_mm, keep_sigma=False):
         """Densities for all charge pairs (not the 4-index one) and the
-        monomer two-electron matrices."""
+        monomer two-electron matrices; keep_sigma keeps their intermediate
+        W = Veff Y per charge (E2W) for the gradient step."""
         self.derived = {}
         self.rho = build_density_arrays(self.blocks, OPS_BY_CHG_DIFF, skip=("ccaa",), derived=self.derived)
-        self.E2 = {chg: monomer_2e(b, b, V_mm) for chg, b in self.blocks.items()}
+        self.E2, self.E2W = {}, {}
+        for chg, b in self.blocks.items():
+            if keep_sigma:
+                self.E2[chg], self.E2W[chg] = monomer_2e(b, b, V_mm, keep=True)
+            else:
+                self.E2[chg] = monomer_2e(b, b, V_mm)
 
     def parity(self):
         return torch.tensor([(self.n_elec_ref - chg) % 2 for chg, _ in self.state_indices],
@@ -88,10 +95,10 @@ class XRSystem:
         self.built = False
         self._sector_idx = {}    # per pair: fixed-charge block layout and scatter indices
 
-    def build_densities(self):
+    def build_densities(self, keep_sigma=False):
         with precision(self.precision, self.device.index):
             for m, F in enumerate(self.frags):
-                F.build(self.ints.V(m, m, m, m))
+                F.build(self.ints.V(m, m, m, m), keep_sigma)
         self.built = True
 
     def _need(self):

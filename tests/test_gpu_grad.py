@@ -77,6 +77,39 @@ def test_monomer_2e_adjoint_is_exact(n_orbs, n_core, n_elec):
     assert abs(f(bb, kb, V + dV) - f0 - float((Vb * dV).sum())) < 1e-10 * scale
 
 
+@pytest.mark.parametrize("same", [True, False])
+@pytest.mark.parametrize("n_orbs,n_core,n_elec", [(4, 0, 3), (5, 1, 5)])
+def test_monomer_2e_keep_matches_recompute(n_orbs, n_core, n_elec, same):
+    """The sigma intermediate kept by xr_monomer_2e_keep and handed to
+    xr_monomer_2e_adjoint_keep gives the same energy matrix and the same
+    adjoints, bit for bit, as recomputing it (a block with itself takes the
+    one-gather path, two blocks the general one)."""
+    rng = np.random.default_rng(33)
+    kb = _blk(rng, n_orbs, n_elec, 3, n_core)
+    bb = kb if same else _blk(rng, n_orbs, n_elec, 4, n_core)
+    ints = syn.FragmentBlocked([n_orbs], syn.spatial_h(n_orbs, rng), syn.spatial_V(n_orbs, rng))
+    V = torch.as_tensor(ints.V[0, 0, 0, 0], device="cuda")
+    E0 = D.monomer_2e(bb, kb, V)
+    E1, W = D.monomer_2e(bb, kb, V, keep=True)
+    assert torch.equal(E0, E1)
+    nI = bb.coeffs.shape[0]
+    Eb = torch.randn(nI, 3, dtype=torch.float64, device="cuda")
+    outs = []
+    for keep in (False, True):
+        zb, zk, Vb = torch.zeros_like(bb.coeffs), torch.zeros_like(kb.coeffs), torch.zeros_like(V)
+        if same:
+            zk = zb
+        args = (n_orbs, n_core, n_elec, nI, bb.n_configs, ptr(bb.coeffs), 3, kb.n_configs, ptr(kb.coeffs),
+                ptr(V), ptr(Eb), ptr(zb), ptr(zk), ptr(Vb))
+        if keep:
+            context(0).call("xr_monomer_2e_adjoint_keep", *args, ptr(W), W.numel())
+        else:
+            context(0).call("xr_monomer_2e_adjoint", *args)
+        outs.append((zb, zk, Vb))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
+
+
 def _dimer(seed, orbs=(3, 3), elec=(1, 2, 3), ns=3):
     rng = np.random.default_rng(seed)
     frags = [syn.make_fragment(n, list(elec), ns, 2, rng) for n in orbs]
