@@ -77,6 +77,25 @@ class XRFragment:
                             dtype=torch.int8, device=self.device)
 
 
+def sector_layout(ch1, ch2, segq):
+    """Row of pair (a, b) inside its fixed-charge block q = ch1[a] + ch2[b],
+    for blocks ordered by increasing a * len(ch2) + b: off[q][a] + rank2[b],
+    with rank2[b] the rank of b among the states of its charge and off[q][a]
+    the number of charge-q pairs whose first state comes before a."""
+    ch1, ch2 = np.asarray(ch1), np.asarray(ch2)
+    rank2 = np.zeros(ch2.size, dtype=np.int64)
+    n2 = {}
+    for c in np.unique(ch2):
+        idx = np.nonzero(ch2 == c)[0]
+        rank2[idx] = np.arange(idx.size)
+        n2[int(c)] = idx.size
+    off = {}
+    for q in segq:
+        sizes = np.array([n2.get(int(q) - int(c), 0) for c in ch1], dtype=np.int64)
+        off[q] = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    return off, rank2
+
+
 class XRSystem:
     """Device-resident XR model of a supersystem (fragments with .n_orbs,
     .n_core, .n_elec_ref, .states, .state_indices; fragment-blocked
@@ -201,16 +220,7 @@ class XRSystem:
         if "layout" not in cache:
             ch1 = np.array([c for c, _ in F1.state_indices])
             ch2 = np.array([c for c, _ in F2.state_indices])
-            rank2 = np.zeros(F2.d, dtype=np.int64)
-            n2 = {}
-            for c in F2.charges:
-                idx = np.nonzero(ch2 == c)[0]
-                rank2[idx] = np.arange(idx.size)
-                n2[c] = idx.size
-            off = {}
-            for q in segq:
-                sizes = np.array([n2.get(q - c, 0) for c in ch1], dtype=np.int64)
-                off[q] = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+            off, rank2 = sector_layout(ch1, ch2, segq)
             pos1 = {c: F1.pos[c].cpu().numpy() for c in F1.charges}
             pos2 = {c: F2.pos[c].cpu().numpy() for c in F2.charges}
             cache["layout"] = (off, rank2, pos1, pos2)
