@@ -1,19 +1,29 @@
 """XR H-build + gradient benchmark on one MI355X.
 
-One step = one XR iteration on the split-precision dimer workload (two
-fragments of 18 spatial orbitals, 3/4/5 electrons, 64 states per charge
-block, seeded synthetic states and integrals): build every transition
-density the XR blocks need, the monomer two-electron sigma terms, all
-monomer and pair blocks, assemble the 36864 x 36864 XR matrix, solve for the
-lowest eigenpair, and run the adjoint for dE/dz and dE/dkappa.  Inputs are
-resident in HBM before the timed region.
+One step = one XR iteration on the differentiation workload (default): a
+4-fragment linear chain of 13 spatial orbitals each, 3/4/5 electrons, 32
+seeded orthonormal states per charge block, seeded h and V with
+inter-fragment decay exp(-d/2).  An iteration builds every transition
+density the XR blocks need, the monomer two-electron sigma terms, the
+monomer blocks and every pair's fixed-charge blocks, solves for the lowest
+root of the 84,934,656-state XR Hamiltonian (matrix-free, per charge
+sector), and runs the hand-written adjoint for dE/dz and dE/dkappa.
+Inputs are resident in HBM before the timed region.
 
-Prints one JSON line (see the keys below).  --gpus N > 1 is refused: this
-build targets one GPU.
+Legs: FP64 (the value), split-FP32 two-electron contractions (|dE| and
+gradient error against FP64 beside it) and the plain FP64 route (unbatched:
+no conjugate densities, dense pair matrices, one GEMM launch per operator
+product, every sector converged), plus the CPU baseline (oracle/timing.py,
+in a child process).  --config c3 / c0 run the same legs on the
+2 x 18-orbital and 2 x 9-orbital dimers; c1 and c2 time the density build
+and the six-fragment block build.
+
+Prints one JSON line.  --gpus N > 1 is refused: this build targets one GPU.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -24,28 +34,38 @@ sys.path.insert(0, ROOT)
 
 MFMA_F64_PEAK = 78.6e12     # MI355X dense FP64 matrix, FLOP/s
 MFMA_F32_PEAK = 157.3e12    # MI355X dense FP32 matrix, FLOP/s
+HBM_PEAK = 8.0e12           # MI355X HBM3E, B/s
+
+WORKLOADS = {
+    "c0": ("dimer 2x9 spatial orbitals, 3/4/5 e, 8 states/block", "dimer_9x9_e345_s8"),
+    "c3": ("dimer 2x18 spatial orbitals, 3/4/5 e, 64 states/block", "dimer_18x18_e345_s64"),
+    "c4": ("chain 4x13 spatial orbitals, 3/4/5 e, 32 states/block, inter-fragment decay exp(-d/2)",
+           "chain4_13_e345_s32"),
+}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c3", choices=["c0", "c1", "c2", "c3", "c4"],
-                    help="c3: XR iteration on the 2x18-orbital dimer (the headline line); "
-                         "c0: the same on the 2x9-orbital dimer; c1: density build of one "
-                         "13-orbital fragment, 2-6 electrons, 32 states; c2: all blocks of six "
-                         "uneven fragments, eager vs autotuned graph replay (separate metrics); "
-                         "c4: XR iteration on the 4x13-orbital chain (no CPU baseline)")
-    ap.add_argument("--precision", default="both", choices=["f64", "split_f32", "both"],
-                    help="two-electron contraction precision; 'both' times FP64 (the value) "
-                         "and split-FP32 (reported beside it with its |dE|)")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c4", choices=["c0", "c1", "c2", "c3", "c4"],
+                    help="c4: XR iteration (densities, blocks, lowest root, dE/dz and dE/dkappa) on the "
+                         "4 x 13-orbital chain (the headline line); c3 / c0: the same on the 2 x 18 / 2 x 9 "
+                         "orbital dimers; c1: density build of one 13-orbital fragment, 2-6 electrons, 32 "
+                         "states; c2: all blocks of six uneven fragments, eager vs graph replay")
+    ap.add_argument("--legs", default="f64,split_f32,plain_f64",
+                    help="comma list of f64 (the value), split_f32, plain_f64")
+    ap.add_argument("--side-steps", type=int, default=3,
+                    help="timed steps of the split_f32 and plain_f64 legs (the f64 leg times --steps)")
     ap.add_argument("--no-grad", action="store_true", help="skip the adjoint (diagnostics only)")
     ap.add_argument("--tune", action="store_true",
-                    help="autotune the GEMM tiles of one iteration before the warm-up (about 2%% faster; "
-                         "the tuning launches then also appear in a profiler's kernel statistics)")
+                    help="autotune the GEMM tiles of one iteration before the warm-up (the tuning "
+                         "launches then also appear in a profiler's kernel statistics)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="sampling budget of the CPU projection")
+    ap.add_argument("--cpu-procs", type=int, default=0, help="worker processes of the CPU pool leg "
+                    "(default min(nproc, 16))")
     ap.add_argument("--dump", default=None, help="write every timed launch as CSV")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run one device-to-device copy of this many bytes (calibrates "
@@ -55,82 +75,13 @@ def parse():
 
 def workload(name):
     from qodeapplications_amd import synthetic as syn
-    if name == "c0":
-        return syn.config0(seed=0), "dimer 2x9 spatial orbitals, 3/4/5 e, 8 states/block"
-    if name == "c4":
-        return syn.config4(seed=4), "chain 4x13 spatial orbitals, 3/4/5 e, 32 states/block"
-    return syn.config3(seed=3), "dimer 2x18 spatial orbitals, 3/4/5 e, 64 states/block"
-
-
-def cpu_baseline(w, budget_s):
-    """Times the CPU oracle (C restatement of density_tensors.c /
-    H_contractions.c, one thread) on bounded samples of the same workload
-    and scales by the counted work of one iteration."""
-    from oracle import oracle as orc
-    from qodeapplications_amd.density import OPS_BY_CHG_DIFF
-    f = w.fragments[0]
-    n_orbs = f.n_orbs
-    dim = 2 * n_orbs
-    # (1) density build: time the ccaa-free densities of one charge pair on a
-    # slice of ket determinants, as rank-1 updates per second
-    ket_chg = min(f.states, key=lambda c: -f.states[c].configs.shape[1])  # most electrons
-    bra_chg = ket_chg + 1
-    sk, sb = f.states[ket_chg], f.states[bra_chg]
-    nq = 1
-    t_used = 0.0
-    units = 0
-    while t_used < 0.4 * budget_s and nq <= sk.configs.shape[0]:
-        cfg = sk.configs[:nq]
-        zk = np.ascontiguousarray(sk.coeffs[:, :nq])
-        t0 = time.perf_counter()
-        orc.density("caa", n_orbs, 0, sb.coeffs, zk, cfg)
-        dt = time.perf_counter() - t0
-        q, _, _, _ = orc.excitations("caa", n_orbs, 0, cfg)
-        units = len(q) * sb.coeffs.shape[0] * zk.shape[0]
-        t_used += dt
-        rate_dens = units / dt
-        nq *= 2
-    # total rank-1 state-pair updates of one iteration (both fragments, every
-    # density the blocks need, ccaa for the monomer term included as the
-    # reference builds it before contracting, build_density_tensors.py:122-133)
-    total_units = 0
-    for fr in w.fragments:
-        for bc, bs in fr.states.items():
-            for kc, ks in fr.states.items():
-                for op in OPS_BY_CHG_DIFF.get(bc - kc, ()):
-                    n_ket = ks.configs.shape[1]
-                    per_q = _paths_per_config(op, n_ket, dim)
-                    total_units += per_q * ks.configs.shape[0] * bs.coeffs.shape[0] * ks.coeffs.shape[0]
-    t_dens = total_units / rate_dens
-    # (2) matrix elements: time dimer_ExEx calls (the dominant family)
-    rng = np.random.default_rng(0)
-    R1 = rng.standard_normal((dim, dim))
-    R2 = rng.standard_normal((dim, dim))
-    V = np.ascontiguousarray(w.integrals.V[0, 1, 0, 1])
-    L = orc.lib()
-    import ctypes
-    P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-    n_calls = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < 0.4 * budget_s:
-        L.orc_dimer_ExEx(ctypes.c_int64(dim), ctypes.c_int64(dim), P(R1), P(R2), P(V))
-        n_calls += 1
-    t_elem = (time.perf_counter() - t0) / n_calls
-    spf = sum(s.coeffs.shape[0] for s in f.states.values())
-    n_elem = (spf * spf) ** 2
-    # every charge-conserving element costs at least one O(dim^4) call
-    frac = _conserving_fraction(w)
-    t_H = n_elem * frac * t_elem
-    total = t_dens + t_H
-    sample = (f"oracle caa density on {min(nq // 2, sk.configs.shape[0])} ket determinants "
-              f"({rate_dens:.3g} rank-1 updates/s) scaled to {total_units:.3g} updates; "
-              f"{n_calls} dimer_ExEx element calls ({t_elem * 1e3:.2f} ms each) scaled to "
-              f"{n_elem * frac:.3g} charge-conserving pair elements; gradient not included")
-    return total, sample
+    gen = {"c0": lambda: syn.config0(seed=0), "c3": lambda: syn.config3(seed=3),
+           "c4": lambda: syn.config4(seed=4)}[name]
+    return gen(), WORKLOADS[name][0]
 
 
 def _paths_per_config(op, n_ket, dim):
-    """Surviving (operator-index tuple) paths per ket determinant."""
+    """Surviving (operator index tuple) paths per ket determinant."""
     occ, paths = n_ket, 1
     for o in reversed(op):
         if o == "a":
@@ -142,19 +93,39 @@ def _paths_per_config(op, n_ket, dim):
     return paths
 
 
-def _conserving_fraction(w):
-    f1, f2 = w.fragments[0], w.fragments[1]
-    n1 = {c: s.coeffs.shape[0] for c, s in f1.states.items()}
-    n2 = {c: s.coeffs.shape[0] for c, s in f2.states.items()}
-    tot = sum(n1.values()) * sum(n2.values())
-    good = 0
-    for a in n1:
-        for b in n2:
-            for c in n1:
-                for d in n2:
-                    if a + b == c + d and abs(a - c) <= 2:
-                        good += n1[a] * n2[b] * n1[c] * n2[d]
-    return good / float(tot * tot)
+def cpu_baseline(args):
+    """The oracle timed on this host in a child process (oracle/timing.py):
+    c0 measured end to end on one core and on a process pool, and the
+    bench workload projected from rates sampled on its own fragments."""
+    procs = args.cpu_procs or min(os.cpu_count() or 1, 16)
+    cmd = [sys.executable, "-m", "oracle.timing", "--workload", args.config, "--procs", str(procs),
+           "--budget", str(args.cpu_seconds)]
+    print(f"bench: cpu baseline: {' '.join(cmd[1:])}", file=sys.stderr, flush=True)
+    r = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU baseline failed ({r.returncode}): {r.stderr[-2000:]}")
+    t = json.loads(r.stdout.strip().splitlines()[-1])
+    c0 = t["c0_measured"]
+    if args.config == "c0":
+        value = c0["single"]["total_s"]
+        sample = ("oracle c0 iteration measured end to end on one core (densities incl. the ccaa monomer "
+                  "terms, every H1/H2 element, fci); gradient not included")
+    else:
+        pr = t["projected"]
+        value = pr["total_s"]
+        sample = (f"projection: oracle density rates per op string sampled on ket-determinant slices of "
+                  f"the largest fragment and H_contractions call times per family, times this workload's "
+                  f"counts ({pr['charge_conserving_pair_elements']:.3g} charge-conserving pair elements); "
+                  f"densities + matrix elements only (the dense XR eigensolve is infeasible here, "
+                  f"the gradient is not included); the same method projects c0 at "
+                  f"{t['c0_projection_over_measured']:.2f}x its measured time")
+    return {"value": value, "unit": "s/iter", "cores": 1, "kind": "port", "sample": sample,
+            "cpu_model": t["cpu_model"], "nproc": t["nproc"],
+            "c0_measured_1core_s": c0["single"]["total_s"],
+            "c0_measured_pool_s": c0["pool"]["total_s"], "c0_pool_processes": c0["pool"]["processes"],
+            "c0_measured": c0, "c0_projected_s": t["c0_projected"]["total_s"],
+            "c0_projection_over_measured": t["c0_projection_over_measured"],
+            "projected": t.get("projected")}
 
 
 def main():
@@ -187,10 +158,11 @@ def main():
     rot = OrbitalRotation(I.frag_orbs, I.h_sp, I.V_sp, I.U_sp)
     ctx = _lib.context(0)
 
-    def measure(prec):
-        """Warm-up, then the timed steps of one precision path; returns the
-        per-step time, energy, gradient and the profiler's kernel stats."""
-        X = XRSystem(w.fragments, w.integrals, w.nuc, precision=prec)
+    def measure(leg, steps):
+        """Warm-up, then the timed steps of one leg; returns the per-step
+        time, energy, gradient and the library's kernel statistics."""
+        prec = "split_f32" if leg == "split_f32" else "f64"
+        X = XRSystem(w.fragments, w.integrals, w.nuc, precision=prec, plain=(leg == "plain_f64"))
 
         def step():
             if args.no_grad:
@@ -203,28 +175,26 @@ def main():
             g = torch.cat([x.reshape(-1) for x in gz.values()] + [x.reshape(-1) for x in gk])
             return E, g
 
-        print(f"bench: workload {wdesc}; precision {prec}; warmup {args.warmup}", file=sys.stderr, flush=True)
+        print(f"bench: workload {wdesc}; leg {leg}; warmup {args.warmup}", file=sys.stderr, flush=True)
         if args.tune:
-            # tile / K-split choice per GEMM shape of one iteration (outside the timed steps)
             from qodeapplications_amd.graph import autotune
             tuned = autotune(step)
             print(f"bench: autotuned {len(tuned)} GEMM shapes", file=sys.stderr, flush=True)
         for i in range(args.warmup):
             E, g = step()
             torch.cuda.synchronize()
-            print(f"bench: warmup {i} E={E:.12f}", file=sys.stderr, flush=True)
+            print(f"bench: {leg} warmup {i} E={E:.12f}", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         ctx.profile(True)
         t0 = time.perf_counter()
-        for i in range(args.steps):
+        for i in range(steps):
             E, g = step()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         ctx.profile(False)
         if args.dump:
-            ctx.profile_dump(args.dump.replace(".csv", f"_{prec}.csv"))
-        ms = (t1 - t0) * 1e3 / args.steps
-        kind = _lib.K_GEMM_F64 if prec == "f64" else _lib.K_GEMM_SPLIT
+            ctx.profile_dump(args.dump.replace(".csv", f"_{leg}.csv"))
+        ms = (t1 - t0) * 1e3 / steps
         stats = {}
         for name, k in (("gemm_f64", _lib.K_GEMM_F64), ("gemm_split_f32", _lib.K_GEMM_SPLIT),
                         ("gemm_grouped", _lib.K_GEMM_GROUPED), ("gemm_bandwidth", _lib.K_GEMM_BW),
@@ -233,47 +203,53 @@ def main():
                         ("excitation_table", _lib.K_TABLE), ("block_scatter", _lib.K_SCATTER),
                         ("braket", _lib.K_BRAKET)):
             ln, kms, fl, by = ctx.profile_stats(k)
-            stats[name] = {"launches": ln, "ms_per_step": round(kms / args.steps, 3),
+            stats[name] = {"launches": ln, "ms_per_step": round(kms / steps, 3),
                            "tflops": round(fl / (kms * 1e-3) / 1e12, 2) if kms > 0 and fl > 0 else None}
         # the dominant FP64 GEMM kernel: plain tile GEMM or grouped tile GEMM
         gemm = max((ctx.profile_stats(k) + (name,) for k, name in
                     ((_lib.K_GEMM_F64, "k_dgemm"), (_lib.K_GEMM_GROUPED, "k_dgemm_grouped"))),
                    key=lambda r: r[1])
-        print(f"bench: {prec} E={E:.12f} ms/step={ms:.1f} kernels={stats}", file=sys.stderr, flush=True)
+        print(f"bench: {leg} E={E:.12f} ms/step={ms:.1f} kernels={stats}", file=sys.stderr, flush=True)
         del X
         torch.cuda.empty_cache()
-        return {"ms": ms, "E": E, "g": g, "stats": stats, "gemm": gemm}
+        return {"ms": ms, "E": E, "g": g, "stats": stats, "gemm": gemm, "steps": steps}
 
-    precs = ["f64", "split_f32"] if args.precision == "both" else [args.precision]
-    res = {p: measure(p) for p in precs}
-    main_p = precs[0]
-    r0 = res[main_p]
+    legs = [x for x in args.legs.split(",") if x]
+    if not legs or legs[0] != "f64":
+        raise SystemExit("the first leg must be f64 (it gives the value)")
+    res = {leg: measure(leg, args.steps if leg == "f64" else args.side_steps) for leg in legs}
+    r0 = res["f64"]
     ms_per_step, E = r0["ms"], r0["E"]
     n, gms, gfl, gby, gname = r0["gemm"]
     avg_ms = gms / max(n, 1)
     achieved = gfl / (gms * 1e-3) if gms > 0 else 0.0
-    stats = r0["stats"]
     gn = None if r0["g"] is None else float(torch.linalg.norm(r0["g"]))
-    split = None
-    if "split_f32" in res and main_p == "f64":
-        rs = res["split_f32"]
-        split = {"s_per_iter": rs["ms"] / 1e3, "E0": rs["E"], "abs_dE_vs_f64": abs(rs["E"] - E),
-                 "speedup_vs_f64": ms_per_step / rs["ms"], "kernels": rs["stats"]}
+
+    def side(leg):
+        if leg not in res:
+            return None
+        rs = res[leg]
+        out = {"s_per_iter": rs["ms"] / 1e3, "steps": rs["steps"], "E0": rs["E"], "abs_dE_vs_f64": abs(rs["E"] - E),
+               "f64_speedup_over_this": rs["ms"] / ms_per_step, "kernels": rs["stats"]}
         if rs["g"] is not None and r0["g"] is not None:
-            split["grad_max_abs_err_vs_f64"] = float((rs["g"] - r0["g"]).abs().max())
-            split["grad_rel_err_vs_f64"] = float(torch.linalg.norm(rs["g"] - r0["g"]) / torch.linalg.norm(r0["g"]))
+            out["grad_max_abs_err_vs_f64"] = float((rs["g"] - r0["g"]).abs().max())
+            out["grad_rel_err_vs_f64"] = float(torch.linalg.norm(rs["g"] - r0["g"]) / torch.linalg.norm(r0["g"]))
+        return out
 
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "c3_dimer_traffic.json")
-    if args.config == "c3" and os.path.exists(tfile):
-        # HBM bytes per FP64 GEMM launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
-        # on this workload (the profile summary names the commands)
-        traffic = json.load(open(tfile))["traffic_bytes_per_launch"].get(gname)
+    split, plain = side("split_f32"), side("plain_f64")
+    if split is not None and plain is not None:
+        split["speedup_vs_plain_f64"] = plain["s_per_iter"] / split["s_per_iter"]
 
-    cpu = None
-    if not args.no_cpu and len(w.fragments) == 2:
-        t_cpu, sample = cpu_baseline(w, args.cpu_seconds)
-        cpu = {"value": t_cpu, "unit": "s/iter", "cores": 1, "kind": "port", "sample": sample}
+    traffic, tsrc = None, None
+    tfile = os.path.join(ROOT, "profiles", f"{WORKLOADS[args.config][1]}_traffic.json")
+    if os.path.exists(tfile):
+        # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE /
+        # WRITE_SIZE passes over this workload (a profile constant, not measured here)
+        tj = json.load(open(tfile))
+        traffic = tj["traffic_bytes_per_launch"].get(gname)
+        tsrc = tj.get("source")
+
+    cpu = None if args.no_cpu else cpu_baseline(args)
 
     out = {
         "metric": "XR H-build+grad s/iter & % roofline: FP64 plain vs split-FP32, |dE|<=1e-9 Eh",
@@ -288,15 +264,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded random orthonormal CI vectors, 8-fold symmetric V)",
-        "config": {"workload": wdesc, "precision_path": main_p, "gradient": not args.no_grad,
+        "config": {"workload": wdesc, "precision_path": "f64", "gradient": not args.no_grad,
                    "E0": E, "grad_norm": gn},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": MFMA_F64_PEAK / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / MFMA_F64_PEAK,
                      "kernel": f"{gname} (FP64 MFMA GEMM)", "launches_per_step": n / args.steps,
-                     "avg_launch_ms": avg_ms, "traffic": traffic,
+                     "avg_launch_ms": avg_ms, "algorithmic_flops_per_launch": gfl / max(n, 1),
+                     "traffic": traffic, "traffic_source": tsrc,
                      "algorithmic_bytes_per_launch": gby / max(n, 1)},
-        "kernels": stats,
+        "kernels": r0["stats"],
         "split_fp32": split,
+        "plain_fp64": plain,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -166,7 +166,8 @@ def conjugate(rho, out=None, accumulate=False):
     return out
 
 
-def build_density_arrays(blocks, ops_table=OPS_BY_CHG_DIFF_FULL, skip=("ccaa",), derived=None):
+def build_density_arrays(blocks, ops_table=OPS_BY_CHG_DIFF_FULL, skip=("ccaa",), derived=None,
+                         conjugates=True):
     """All densities of one fragment on the device:
     rho[op][(bra_chg, ket_chg)] = tensor (nI, nJ, dim, ...).  Operator strings
     in `skip` are not built (the monomer 4-index term goes through
@@ -174,7 +175,8 @@ def build_density_arrays(blocks, ops_table=OPS_BY_CHG_DIFF_FULL, skip=("ccaa",),
     swapped charges is in the table is not built again but taken as the
     conjugate of that one (<I|op|J> = <J|op^+|I> for real states: half the
     GEMM work of the charge-changing strings); derived, if given, receives
-    {(op, (bra, ket)): (op^+, (ket, bra))} for them."""
+    {(op, (bra, ket)): (op^+, (ket, bra))} for them.  conjugates=False builds
+    every string directly."""
     rho = {}
     pairs = [(b, k) for b in blocks for k in blocks]
     pairs.sort(key=lambda bk: bk[1] - bk[0])        # bra - ket >= 0 first: the conjugate sources
@@ -183,7 +185,7 @@ def build_density_arrays(blocks, ops_table=OPS_BY_CHG_DIFF_FULL, skip=("ccaa",),
             if op in skip:
                 continue
             cop = conjugate_string(op)
-            src = rho.get(cop, {}).get((ket_chg, bra_chg)) if cop != op else None
+            src = rho.get(cop, {}).get((ket_chg, bra_chg)) if cop != op and conjugates else None
             if src is not None:
                 rho.setdefault(op, {})[(bra_chg, ket_chg)] = conjugate(src)
                 if derived is not None:

@@ -109,7 +109,7 @@ class XRAdjoint:
         else:
             # the gradient is first order in the eigenvector error: converge
             # the residual well below the energy tolerance
-            w, v = op.lowest(k=1, tol=GRAD_TOL)
+            w, v = op.lowest(k=1, tol=GRAD_TOL, screen=not X.plain)
         # the root in its charge sector's compact layout (pair densities)
         self.sector = op.found[0] if op.n > DENSE_MAX else None
         self.E = float(w[0])

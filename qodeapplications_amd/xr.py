@@ -59,12 +59,14 @@ class XRFragment:
         self.E2 = None
         self.E2W = {}
 
-    def build(self, V_mm, keep_sigma=False):
+    def build(self, V_mm, keep_sigma=False, conjugates=True):
         """Densities for all charge pairs (not the 4-index one) and the
         monomer two-electron matrices; keep_sigma keeps their intermediate
-        W = Veff Y per charge (E2W) for the gradient step."""
+        W = Veff Y per charge (E2W) for the gradient step; conjugates=False
+        builds the lowering strings directly instead of as conjugates."""
         self.derived = {}
-        self.rho = build_density_arrays(self.blocks, OPS_BY_CHG_DIFF, skip=("ccaa",), derived=self.derived)
+        self.rho = build_density_arrays(self.blocks, OPS_BY_CHG_DIFF, skip=("ccaa",), derived=self.derived,
+                                        conjugates=conjugates)
         self.E2, self.E2W = {}, {}
         for chg, b in self.blocks.items():
             if keep_sigma:
@@ -101,12 +103,19 @@ class XRSystem:
     .n_core, .n_elec_ref, .states, .state_indices; fragment-blocked
     integrals with T, U, V; nuclear repulsion matrix)."""
 
-    def __init__(self, fragments, integrals, nuc, device=0, precision="f64"):
+    def __init__(self, fragments, integrals, nuc, device=0, precision="f64", plain=False):
         """precision: 'f64', or 'split_f32' for the two-electron contraction
-        GEMMs (hi/lo FP32 matrix-core products accumulated in FP64)."""
+        GEMMs (hi/lo FP32 matrix-core products accumulated in FP64).
+
+        plain=True is the unbatched FP64 route kept as a comparison point:
+        every density string built directly (no conjugates), the monomer
+        sigma recomputed in the adjoint, dense pair matrices cut into the
+        operator's blocks, one GEMM launch per pair product of the operator
+        (no grouped GEMM) and every charge sector converged (no screening)."""
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         self.precision = precision
+        self.plain = bool(plain)
         self.device = torch.device("cuda", device)
         self.frags = [XRFragment(f, self.device) for f in fragments]
         self.ints = B.IntegralCache(integrals, self.device)
@@ -117,7 +126,7 @@ class XRSystem:
     def build_densities(self, keep_sigma=False):
         with precision(self.precision, self.device.index):
             for m, F in enumerate(self.frags):
-                F.build(self.ints.V(m, m, m, m), keep_sigma)
+                F.build(self.ints.V(m, m, m, m), keep_sigma and not self.plain, conjugates=not self.plain)
         self.built = True
 
     def _need(self):
@@ -202,8 +211,9 @@ class XRSystem:
         D = F1.d * F2.d
         out = torch.zeros(D, D, dtype=torch.float64, device=self.device)
         dims = [F1.d, F2.d]
-        self._pair_terms(m1, m2, lambda k1, k2: (out, D, [F1.pos[k1[0]], F2.pos[k2[0]]],
-                                                 [F1.pos[k1[1]], F2.pos[k2[1]]], dims))
+        with precision(self.precision, self.device.index):
+            self._pair_terms(m1, m2, lambda k1, k2: (out, D, [F1.pos[k1[0]], F2.pos[k2[0]]],
+                                                     [F1.pos[k1[1]], F2.pos[k2[1]]], dims))
         context(self.device.index).call("xr_symmetrize", ptr(out), D)
         return out
 
@@ -240,7 +250,8 @@ class XRSystem:
             rows, cols = cache[key]
             return blocks[i], segs[i][1], rows, cols, [1, 1]
 
-        self._pair_terms(m1, m2, place)
+        with precision(self.precision, self.device.index):
+            self._pair_terms(m1, m2, place)
         ctx = context(self.device.index)
         for blk, (_, cnt) in zip(blocks, segs):
             ctx.call("xr_symmetrize", ptr(blk), cnt)
@@ -314,6 +325,10 @@ class XRSystem:
         or, without H, with the pair terms built straight into the operator's
         fixed-charge blocks (no dense pair matrices)."""
         from .xr_operator import XROperator
+        if H is None and self.plain:
+            op = self.operator(H=self.blocks())
+            op.grouped = False
+            return op
         if H is None:
             nf = len(self.frags)
             with precision(self.precision, self.device.index):

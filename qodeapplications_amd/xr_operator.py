@@ -93,6 +93,7 @@ class XROperator:
         self._sq, self._pair_diag, self._sqidx = {}, {}, {}
         self.found = None
         self.log = None          # set to a list to record the per-sector solves
+        self.grouped = True      # pair products of a sector in one grouped GEMM (else one launch each)
         for (m, n), H in H2.items():
             if H is not None:
                 self.pairs[(m, n)] = self._pair_blocks(m, n, H, symmetrize)
@@ -283,7 +284,8 @@ class XROperator:
             tot = {t + x for t in tot for x in q}
         return sorted(tot)
 
-    def lowest(self, k=1, tol=1e-9, max_iter=400, block=None, charges=None, chunk=4, margin=4.0):
+    def lowest(self, k=1, tol=1e-9, max_iter=400, block=None, charges=None, chunk=4, margin=4.0,
+               screen=True, verify=True, verify_tol=1e-5):
         """(values[k], vectors[n, k]) of the lowest roots over all total
         charges (or the given ones), in the product layout.
 
@@ -291,19 +293,36 @@ class XROperator:
         own compact layout (block size k unless given).  Small sectors are
         diagonalised densely.  The others are iterated in turns of `chunk`
         Davidson iterations, in the order of their lowest diagonal energy in
-        the monomer eigenbasis, each resuming its own subspace.  Every Ritz
-        value is an upper bound to an eigenvalue of its sector, so the k-th
-        lowest of all current Ritz values bounds the k-th lowest eigenvalue
-        from above; a sector whose Ritz values minus `margin` times their
-        residual norms all lie above that bound is taken to be unable to
-        contribute and is dropped (early Ritz values of a strongly
-        correlated sector can lie well above its lowest eigenvalue: a margin
-        of 2 dropped the true ground-state sector of a 3-fragment test chain
-        after 3 iterations at residual 0.55 and 2.1 Eh above the root).  The
-        rest run until converged to tol."""
+        the monomer eigenbasis, each resuming its own subspace.
+
+        screen: every Ritz value is an upper bound to an eigenvalue of its
+        sector, so the k-th lowest of all current Ritz values bounds the k-th
+        lowest eigenvalue from above; a sector whose Ritz values minus
+        `margin` times their residual norms all lie above that bound is set
+        aside early (early Ritz values of a strongly correlated sector can
+        lie well above its lowest eigenvalue: a margin of 2 set aside the
+        true ground-state sector of a 3-fragment test chain after 3
+        iterations at residual 0.55 and 2.1 Eh above the root).  Such a test
+        bounds *some* eigenvalue of the sector, not its lowest one, so it
+        only orders the work:
+
+        verify (default): once the other sectors are converged, the lowest
+        root of every set-aside sector is converged (resuming its subspace)
+        to a residual of verify_tol * max(1, |value|); a sector whose
+        converged lowest Ritz value minus its residual norm is not above the
+        k-th root found so far is then converged to `tol` and competes like
+        the others.  Every sector's lowest root is thus a converged Davidson
+        root, the same standard as the root returned.  verify=False keeps
+        the early drop final (faster, heuristic).  screen=False converges
+        every sector to `tol` without setting any aside.
+
+        self.log receives (Q, size, Ritz values, residual, iterations, how)
+        per sector when it is a list; how is 'converged', 'dense',
+        'verified-above' or 'dropped'."""
         found = []                    # converged: (value, sector space, vector)
         active = []
         iterative = []
+        set_aside = []
         for Q in (charges if charges is not None else self.total_charges()):
             sp = self.sector_space(Q)
             kk = min(k, sp.size)
@@ -312,6 +331,8 @@ class XROperator:
                 sp.matvec(torch.eye(sp.size, dtype=torch.float64, device=self.device), M)
                 w, v = torch.linalg.eigh(0.5 * (M + M.T))
                 found += [(float(w[i]), sp, v[:, i]) for i in range(kk)]
+                if self.log is not None:
+                    self.log.append((sp.Q, sp.size, [float(w[i]) for i in range(kk)], 0.0, 0, "dense"))
             else:
                 iterative.append((sp, kk))
         if iterative:                 # the sectors' lowest diagonal energies in one transfer
@@ -321,10 +342,17 @@ class XROperator:
                                "w": None, "iters": 0, "lam": float(lam)})
         active.sort(key=lambda a: a["lam"])
 
-        def upper():
-            vals = [f[0] for f in found] + [float(x) for a in active if a["w"] is not None for x in a["w"]]
+        def upper(pending=True):
+            vals = [f[0] for f in found]
+            if pending:
+                vals += [float(x) for a in active if a["w"] is not None for x in a["w"]]
             vals.sort()
             return vals[k - 1] if len(vals) >= k else None
+
+        def record(a, inf, how):
+            if self.log is not None:
+                self.log.append((a["sp"].Q, a["sp"].size, a["w"], float(np.max(inf.get("rn", [0.0]))),
+                                 a["iters"], how))
 
         while active:
             for a in list(active):
@@ -332,7 +360,8 @@ class XROperator:
                 w, v = davidson(sp.matvec, None, sp.size, a["kk"], tol, min(chunk, max_iter - a["iters"]), a["b"],
                                 self.device, precond=sp.precondition,
                                 start=sp.guess(a["b"]) if a["w"] is None else None, strict=False,
-                                state=a["st"], stop_above=upper(), stop_margin=margin, info=inf)
+                                state=a["st"], stop_above=upper() if screen else None, stop_margin=margin,
+                                info=inf)
                 a["w"] = [float(x) for x in w]
                 a["iters"] += inf.get("iters", 0)
                 done = inf.get("abandoned", False) or inf.get("converged", False)
@@ -341,12 +370,33 @@ class XROperator:
                                        f"total charge {sp.Q} (residual {float(np.max(inf['rn'])):.2e})")
                 if done:
                     active.remove(a)
-                    a["st"].clear()                 # free the subspace
                     if inf.get("converged", False):
+                        a["st"].clear()             # free the subspace
                         found += [(float(w[i]), sp, v[:, i]) for i in range(w.shape[0])]
-                    if self.log is not None:
-                        self.log.append((sp.Q, sp.size, a["w"], float(np.max(inf.get("rn", [0.0]))), a["iters"],
-                                         inf.get("abandoned", False)))
+                        record(a, inf, "converged")
+                    elif verify:
+                        set_aside.append(a)         # keeps its subspace for the verification
+                    else:
+                        a["st"].clear()
+                        record(a, inf, "dropped")
+        for a in set_aside:
+            sp, inf = a["sp"], {}
+            w, v = davidson(sp.matvec, None, sp.size, 1, verify_tol, max(max_iter - a["iters"], 1), a["b"],
+                            self.device, precond=sp.precondition, state=a["st"], info=inf)
+            a["iters"] += inf.get("iters", 0)
+            a["w"] = [float(x) for x in w]
+            bound = upper(pending=False)
+            if bound is None or float(w[0]) - float(np.max(inf.get("rn", [0.0]))) <= bound:
+                inf = {}
+                w, v = davidson(sp.matvec, None, sp.size, a["kk"], tol, max(max_iter - a["iters"], 1), a["b"],
+                                self.device, precond=sp.precondition, state=a["st"], info=inf)
+                a["iters"] += inf.get("iters", 0)
+                a["w"] = [float(x) for x in w]
+                found += [(float(w[i]), sp, v[:, i]) for i in range(w.shape[0])]
+                record(a, inf, "converged")
+            else:
+                record(a, inf, "verified-above")
+            a["st"].clear()
         found.sort(key=lambda t: t[0])
         found = found[:k]
         self.found = [(sp, x) for _, sp, x in found]     # the roots in their sector layouts
@@ -653,10 +703,15 @@ class SectorSpace:
         self._run_pack(pl["pack"], X, P, 0)
         g = pl["grp"]
         k = len(g["M"])
-        if k:
+        if k and op.grouped:
             Bp, Cp = g["B"] + P.data_ptr(), g["C"] + T.data_ptr()
             op.ctx.call("xr_gemm_grouped", k, 0, 0, _ptr(g["M"]), _ptr(g["N"]), _ptr(g["K"]), 1.0, _ptr(g["A"]),
                         _ptr(g["lda"]), _ptr(Bp), _ptr(g["N"]), _ptr(Cp), _ptr(g["N"]))
+        elif k:
+            for (m, n), sums in zip(op.pairs, pl["pair_sums"]):
+                for s_, boff, R, W in sums:
+                    o = boff // 8
+                    mm(self.sq[(m, n, s_)][0], P[o:o + R * W].view(R, W), out=T[o:o + R * W].view(R, W))
         for up in pl["unpacks"]:
             self._run_pack(up, Y, T, 1)
         return Y
