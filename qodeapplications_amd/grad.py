@@ -336,3 +336,26 @@ def energy_and_gradient(system, lagrangian=True):
 
 
 DimerAdjoint = XRAdjoint
+
+
+def herm_state_gradient(zbar, system):
+    """The state gradient in the definition of the reference's "herm" level
+    (StateSpaceOptimizer/state_gradients.py:121-244, xr_order 0):
+    g[m][c][k, p] = sum_l d[k, l] <p l| H - E |Psi>, half the derivative of
+    the variational XR energy (every product state with its overlaps), for
+    fragment m's states k and determinants p of charge c.
+
+    The adjoint's dE/dz differentiates the XR matrix as assembled (the
+    identity on the spectator fragments, general-XRCC/hamiltonian.py:21-83).
+    For orthonormal states the two agree on every direction orthogonal to
+    each charge block's state span, and the variational energy is
+    invariant within the span, so g = (1/2) zbar (1 - z^T z) for zbar
+    from either lagrangian setting (the overlap term -2 E D z lies in the
+    span).  Returns {(m, c): g} as device tensors."""
+    out = {}
+    for m, F in enumerate(system.frags):
+        for c in F.charges:
+            z = F.blocks[c].coeffs
+            zb = zbar[m][c]
+            out[(m, c)] = 0.5 * (zb - mm(mm(zb, z, transB=1), z))
+    return out

@@ -67,14 +67,19 @@ class OrbitalRotation:
         return [torch.zeros(n, n, dtype=torch.float64, device=self.device, requires_grad=True)
                 for n in self.frag_orbs]
 
-    def rotated(self, kappas):
+    def rotated(self, kappas, at_zero=False):
+        """Rotated h, V, U.  at_zero: the caller guarantees every kappa is
+        zero (the gradient point), where exp(K) = 1 + K holds exactly in
+        value and first derivative, so the matrix exponential is skipped."""
         n = int(self.off[-1])
-        K = torch.zeros(n, n, dtype=torch.float64, device=self.device)
         parts = []
         for m, X in enumerate(kappas):
             parts.append(X - X.T)
         K = torch.block_diag(*parts)
-        R = torch.linalg.matrix_exp(K)
+        if at_zero:
+            R = K + torch.eye(n, dtype=torch.float64, device=self.device)
+        else:
+            R = torch.linalg.matrix_exp(K)
         h = transform(self.h, R)
         V = transform(self.V, R)
         U = None if self.U is None else torch.stack([transform(u, R) for u in self.U])
@@ -110,10 +115,10 @@ class OrbitalRotation:
             return torch.cat([cat(level + 1, prefix + (b,)) for b in (0, 1)], dim=level)
         return cat(0, ())
 
-    def blocks(self, kappas, keys):
+    def blocks(self, kappas, keys, at_zero=False):
         """Differentiable blocks for IntegralCache keys ('T', m, n), ('U', a, m, n),
         ('V', m, n, o, p)."""
-        h, V, U = self.rotated(kappas)
+        h, V, U = self.rotated(kappas, at_zero)
         out = {}
         for key in keys:
             if key[0] == "T":
@@ -155,11 +160,11 @@ def energy_and_gradients(system, rotation, lagrangian=True):
     from .grad import XREnergy
     kappas = rotation.zero_kappas()
     keys = integral_keys(len(system.frags))
-    blocks = rotation.blocks(kappas, keys)
+    blocks = rotation.blocks(kappas, keys, at_zero=True)
     z_keys = [(m, c) for m, F in enumerate(system.frags) for c in F.charges]
     zs = [system.frags[m].blocks[c].coeffs.clone().requires_grad_(True) for m, c in z_keys]
     E = XREnergy.apply(system, lagrangian, z_keys, keys, *zs, *[blocks[k] for k in keys])
     gz_and_k = torch.autograd.grad(E, zs + kappas)
     gz = {k: g for k, g in zip(z_keys, gz_and_k[:len(zs)])}
     gk = list(gz_and_k[len(zs):])
-    return float(E), gz, gk
+    return float(E.detach()), gz, gk

@@ -285,7 +285,7 @@ class XROperator:
         return sorted(tot)
 
     def lowest(self, k=1, tol=1e-9, max_iter=400, block=None, charges=None, chunk=4, margin=4.0,
-               screen=True, verify=True, verify_tol=1e-5):
+               screen=True, verify=True, verify_tol=1e-4):
         """(values[k], vectors[n, k]) of the lowest roots over all total
         charges (or the given ones), in the product layout.
 
