@@ -41,7 +41,8 @@ enum {
     XR_EPAULI = 4,               /* a config repeats a spin orbital         */
     XR_ENOMEM = 5,               /* device allocation failed                */
     XR_EHIP = 6,                 /* HIP runtime error                       */
-    XR_EUNSUPPORTED = 7          /* operator string / size not supported    */
+    XR_EUNSUPPORTED = 7,         /* operator string / size not supported    */
+    XR_EBUSY = 8                 /* a pinned workspace would have to grow   */
 };
 
 typedef struct xr_ctx xr_ctx;
@@ -57,6 +58,20 @@ int xr_destroy(xr_ctx *ctx);
 int xr_set_stream(xr_ctx *ctx, void *stream);
 const char *xr_last_error(xr_ctx *ctx);
 int xr_synchronize(xr_ctx *ctx);
+
+/* Pins (delta = +1) or releases (delta = -1) the context's workspaces.  A
+ * captured HIP graph replays the device addresses the calls had when it was
+ * captured; while the pin count is positive no workspace slot is
+ * reallocated, and a call that would need a larger one fails with XR_EBUSY
+ * instead of freeing memory the graph still reads.  No reference
+ * counterpart (the reference has no captured launches). */
+int xr_pin(xr_ctx *ctx, int delta);
+
+/* on != 0: the caller promises that every V passed to the monomer
+ * two-electron adjoints on this context satisfies V[pqrs] = V[rspq] (the
+ * two-body part of a Hermitian Hamiltonian), which enables their one-gather
+ * path for a block with itself.  Default off.  No reference counterpart. */
+int xr_assume_hermitian_v(xr_ctx *ctx, int on);
 
 /* Kernel timing with HIP events on the context stream.  xr_profile(ctx, 1)
  * clears the records and starts bracketing every launch; xr_profile(ctx, 0)
@@ -138,7 +153,11 @@ int xr_monomer_2e(xr_ctx *ctx, int n_orbs, int n_core, int n_elec,
                   const double *d_V, double *d_out);
 
 /* Adjoint of xr_monomer_2e for Ebar = dE/dE2 (nI x nJ): ADDS dE/dzbra,
- * dE/dzket and dE/dV (dim^4) into the given buffers (any may be NULL). */
+ * dE/dzket and dE/dV (dim^4) into the given buffers (any may be NULL).
+ * For a block with itself (d_zbra == d_zket) into one shared coefficient
+ * adjoint it takes a one-gather path when the context was told that V is
+ * Hermitian, V[pqrs] = V[rspq] (xr_assume_hermitian_v); without that
+ * promise, or when it does not hold, the general two-sided path runs. */
 int xr_monomer_2e_adjoint(xr_ctx *ctx, int n_orbs, int n_core, int n_elec,
                           int64_t nI, int64_t nP, const double *d_zbra,
                           int64_t nJ, int64_t nQ, const double *d_zket,
@@ -171,6 +190,7 @@ int xr_gemm_f64(xr_ctx *ctx, int transA, int transB, int64_t M, int64_t N, int64
                 double alpha, const double *d_A, int64_t lda, int64_t strideA,
                 const double *d_B, int64_t ldb, int64_t strideB,
                 double beta, double *d_C, int64_t ldc, int64_t strideC, int64_t batch);
+
 
 /* n independent products C_q = alpha op(A_q) op(B_q) of uneven shapes
  * (host arrays of n shapes, leading dimensions and device pointers; the same

@@ -14,9 +14,10 @@ LIB_PATH = os.path.join(_HERE, "libxrhip.so")
 PREC_F64, PREC_SPLIT_F32 = 0, 1
 (K_GEMM_F64, K_GATHER, K_EXPAND, K_TABLE, K_SCATTER, K_BRAKET, K_OTHER, K_GEMM_SPLIT, K_GEMM_BW,
  K_GEMM_GROUPED) = range(10)
-XR_OK, XR_EINVAL, XR_ESHAPE, XR_EINVAL_CONFIG_ORDER, XR_EPAULI, XR_ENOMEM, XR_EHIP, XR_EUNSUPPORTED = range(8)
+XR_OK, XR_EINVAL, XR_ESHAPE, XR_EINVAL_CONFIG_ORDER, XR_EPAULI, XR_ENOMEM, XR_EHIP, XR_EUNSUPPORTED, \
+    XR_EBUSY = range(9)
 _CODE_NAMES = {1: "XR_EINVAL", 2: "XR_ESHAPE", 3: "XR_EINVAL_CONFIG_ORDER", 4: "XR_EPAULI",
-               5: "XR_ENOMEM", 6: "XR_EHIP", 7: "XR_EUNSUPPORTED"}
+               5: "XR_ENOMEM", 6: "XR_EHIP", 7: "XR_EUNSUPPORTED", 8: "XR_EBUSY"}
 
 # every symbol include/xr_hip.h declares, with its ctypes signature
 _P = ctypes.c_void_p
@@ -30,6 +31,8 @@ SIGNATURES = {
     "xr_set_stream": (_I, [_P, _P]),
     "xr_last_error": (_S, [_P]),
     "xr_synchronize": (_I, [_P]),
+    "xr_pin": (_I, [_P, _I]),
+    "xr_assume_hermitian_v": (_I, [_P, _I]),
     "xr_profile": (_I, [_P, _I]),
     "xr_profile_stats": (_I, [_P, _I, ctypes.POINTER(_L), ctypes.POINTER(_D), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "xr_profile_dump": (_I, [_P, _S]),

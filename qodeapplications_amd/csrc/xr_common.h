@@ -118,6 +118,13 @@ struct xr_ctx {
     std::map<std::array<int64_t, 12>, std::pair<int32_t*, int64_t>> sidetab;
     // GEMM autotune table: (ta, tb, M, N, K, batch) -> (tile, splits)
     std::map<std::array<int64_t, 6>, std::pair<int, int>> tune;
+    // captured graphs holding the workspace addresses (xr_pin): while > 0 a
+    // workspace slot may not be reallocated
+    int pins = 0;
+    bool pin_blocked = false;   // the last xr_workspace refused to grow a pinned slot
+    // the caller guarantees V[pqrs] = V[rspq] for the monomer sigma adjoint
+    // (xr_assume_hermitian_v): enables its one-gather path for a block with itself
+    bool hermitian_v = false;
     std::mutex mu;
 };
 

@@ -5,6 +5,13 @@
 __constant__ uint64_t c_binom[65 * 65];
 
 int xr_fail(xr_ctx* c, int code, const std::string& msg) {
+    if (c && c->pin_blocked && code == XR_ENOMEM) {
+        // the allocation failed because a captured graph pins the workspaces
+        c->pin_blocked = false;
+        c->err = msg + ": a workspace would have to grow while a captured graph holds its address "
+                       "(xr_pin); release the capture or capture after the largest call";
+        return XR_EBUSY;
+    }
     if (c) c->err = msg;
     return code;
 }
@@ -22,6 +29,10 @@ void* xr_workspace(xr_ctx* c, int slot, size_t bytes) {
         c->ws_size.resize(slot + 1, 0);
     }
     if (c->ws_size[slot] >= bytes && c->ws[slot]) return c->ws[slot];
+    if (c->pins > 0) {          // a captured graph reads the current address
+        c->pin_blocked = true;
+        return nullptr;
+    }
     if (c->ws[slot]) {
         (void)hipStreamSynchronize(c->stream);
         (void)hipFree(c->ws[slot]);
@@ -37,6 +48,21 @@ void* xr_workspace(xr_ctx* c, int slot, size_t bytes) {
     c->ws[slot] = p;
     c->ws_size[slot] = sz;
     return p;
+}
+
+extern "C" int xr_assume_hermitian_v(xr_ctx* c, int on) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    c->hermitian_v = on != 0;
+    return XR_OK;
+}
+
+extern "C" int xr_pin(xr_ctx* c, int delta) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (c->pins + delta < 0) return xr_fail(c, XR_EINVAL, "xr_pin: more releases than pins");
+    c->pins += delta;
+    return XR_OK;
 }
 
 // ---------------------------------------------------------------------------

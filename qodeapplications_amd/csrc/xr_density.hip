@@ -924,12 +924,13 @@ static int monomer_2e_adjoint_impl(xr_ctx* c, int n_orbs, int n_core, int n_elec
     // A block with itself into one adjoint buffer: both sides remove two
     // electrons from the same states, and their tables differ only by the
     // order of the two annihilators (sg = -1 when the slots are swapped), so
-    // X = sg Y.  Ve is symmetric (the two-body part of H is), hence
+    // X = sg Y.  When the caller guarantees a Hermitian two-body part
+    // (V[pqrs] = V[rspq], xr_assume_hermitian_v), Ve is symmetric, hence
     // Ybar = Ve^T Wbar = sg W Ebar with W = Ve Y, side_R = sg side_L, and the
     // two side adjoints add up to side_L(W (Ebar + Ebar^T)): one gather, no
     // Ve^T Wbar product, one side adjoint.
     const double sg = (pl.L.slot[0] == pl.R.slot[0]) ? 1.0 : -1.0;
-    const bool mirror = zbra == zket && nI == nJ && nP == nQ && zbar_bra == zbar_ket && zbar_ket &&
+    const bool mirror = c->hermitian_v && zbra == zket && nI == nJ && nP == nQ && zbar_bra == zbar_ket && zbar_ket &&
                         pl.L.k == 2 && pl.R.k == 2 && pl.L.is_c == pl.R.is_c &&
                         pl.L.slot[0] != pl.L.slot[1] && pl.R.slot[0] != pl.R.slot[1];
     if (!mirror && (rc = xr_gather(c, tabL, nK, nU, zTb, nI, X, 1))) return rc;   // X[u][K][I]

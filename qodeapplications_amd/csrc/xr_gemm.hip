@@ -1421,3 +1421,4 @@ extern "C" int xr_gemm_split_f32(xr_ctx* c, int transA, int transB, int64_t M, i
     return xr_dgemm_split_launch(c, transA, transB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc,
                                  sC, batch);
 }
+

@@ -101,6 +101,7 @@ class XRAdjoint:
     def forward(self):
         X = self.X
         X.build_densities(keep_sigma=True)
+        self.build_count = X.build_count
         self.op = op = X.operator()
         if op.n <= DENSE_MAX:
             M = op.dense()
@@ -120,8 +121,18 @@ class XRAdjoint:
         """Returns (zbar, ibar): zbar[m][chg] = dE/dz (n_states x n_configs),
         ibar[key] = dE/d(integral block) for keys ('T', m, n), ('U', a, m, n),
         ('V', m, n, o, p)."""
-        with precision(self.X.precision, self.X.device.index):
-            return self._backward(lagrangian)
+        if self.X.build_count != self.build_count:
+            raise RuntimeError("the XRSystem was rebuilt after this forward; run forward again "
+                               "before its backward")
+        ctx = context(self.X.device.index)
+        # the integral blocks of an XR Hamiltonian are Hermitian, which the
+        # monomer sigma adjoint's one-gather path needs
+        ctx.call("xr_assume_hermitian_v", 1)
+        try:
+            with precision(self.X.precision, self.X.device.index):
+                return self._backward(lagrangian)
+        finally:
+            ctx.call("xr_assume_hermitian_v", 0)
 
     def _backward(self, lagrangian):
         X, E, v, xop = self.X, self.E, self.v, self.op

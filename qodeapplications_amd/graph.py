@@ -91,6 +91,7 @@ class CapturedBuild:
         build()                                   # final workspace sizes, warm caches
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
+        self._pinned = False
         try:
             with torch.cuda.graph(self.graph):
                 self.ctx.sync_stream()            # launch on the capture stream
@@ -98,6 +99,23 @@ class CapturedBuild:
         finally:
             self.ctx.sync_stream()                # back to the ambient stream
         torch.cuda.synchronize()
+        # the graph replays the workspace addresses it was captured with: no
+        # call on this context may reallocate them while it exists
+        self.ctx.call("xr_pin", 1)
+        self._pinned = True
+
+    def release(self):
+        """Drops the graph and lets the context's workspaces grow again."""
+        if getattr(self, "_pinned", False):
+            self._pinned = False
+            self.graph = None
+            self.ctx.call("xr_pin", -1)
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
     def set_coeffs(self, coeffs):
         """coeffs: {(fragment, charge): (n_states x n_configs) tensor}; copied
@@ -108,5 +126,7 @@ class CapturedBuild:
     def replay(self):
         """Recomputes every density and block; returns (H1, H2, H3) (the same
         tensors on every replay)."""
+        if self.graph is None:
+            raise RuntimeError("the captured build was released")
         self.graph.replay()
         return self.H

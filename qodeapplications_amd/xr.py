@@ -121,6 +121,7 @@ class XRSystem:
         self.ints = B.IntegralCache(integrals, self.device)
         self.nuc = np.asarray(nuc, dtype=np.float64)
         self.built = False
+        self.build_count = 0     # densities builds so far (a backward checks it matches its forward)
         self._sector_idx = {}    # per pair: fixed-charge block layout and scatter indices
 
     def build_densities(self, keep_sigma=False):
@@ -128,6 +129,7 @@ class XRSystem:
             for m, F in enumerate(self.frags):
                 F.build(self.ints.V(m, m, m, m), keep_sigma and not self.plain, conjugates=not self.plain)
         self.built = True
+        self.build_count += 1
 
     def _need(self):
         if not self.built:

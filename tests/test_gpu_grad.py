@@ -110,6 +110,46 @@ def test_monomer_2e_keep_matches_recompute(n_orbs, n_core, n_elec, same):
         assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("hermitian", [False, True])
+def test_monomer_2e_adjoint_shared_buffer(hermitian):
+    """A block with itself into one shared coefficient adjoint equals the sum
+    of the two separate adjoints: with a V lacking the symmetry
+    V[pqrs] = V[rspq] the general path runs (no promise made); with a
+    Hermitian V and xr_assume_hermitian_v the one-gather path does."""
+    rng = np.random.default_rng(34)
+    n_orbs, n_core, n_elec = 4, 0, 3
+    kb = _blk(rng, n_orbs, n_elec, 3, n_core)
+    dim = 2 * n_orbs
+    if hermitian:
+        ints = syn.FragmentBlocked([n_orbs], syn.spatial_h(n_orbs, rng), syn.spatial_V(n_orbs, rng))
+        V = torch.as_tensor(ints.V[0, 0, 0, 0], device="cuda")
+    else:
+        V = torch.as_tensor(rng.standard_normal((dim,) * 4), device="cuda")
+        assert float((V - V.permute(2, 3, 0, 1)).abs().max()) > 0.1
+    Eb = torch.randn(3, 3, dtype=torch.float64, device="cuda")
+    ctx = context(0)
+    args = lambda zb, zk, Vb: (n_orbs, n_core, n_elec, 3, kb.n_configs, ptr(kb.coeffs), 3, kb.n_configs,
+                               ptr(kb.coeffs), ptr(V), ptr(Eb), ptr(zb), ptr(zk), ptr(Vb))
+    zb, zk, Vb2 = torch.zeros_like(kb.coeffs), torch.zeros_like(kb.coeffs), torch.zeros_like(V)
+    ctx.call("xr_monomer_2e_adjoint", *args(zb, zk, Vb2))
+    zs, Vb1 = torch.zeros_like(kb.coeffs), torch.zeros_like(V)
+    ctx.call("xr_assume_hermitian_v", 1 if hermitian else 0)
+    try:
+        ctx.call("xr_monomer_2e_adjoint", *args(zs, zs, Vb1))
+    finally:
+        ctx.call("xr_assume_hermitian_v", 0)
+    scale = float((zb + zk).abs().max())
+    assert float((zs - (zb + zk)).abs().max()) <= 1e-12 * max(1.0, scale)
+    assert float((Vb1 - Vb2).abs().max()) <= 1e-12 * max(1.0, float(Vb2.abs().max()))
+    # and it is the derivative of sum(Eb * E2) with bra and ket moving together
+    f = lambda z: float((Eb * D.monomer_2e(kb.with_coeffs(z), kb.with_coeffs(z), V)).sum())
+    # f is quadratic in z: the central difference is exact up to rounding
+    dz = torch.randn_like(kb.coeffs) * 1e-3
+    lin = float((zs * dz).sum())
+    cd = 0.5 * (f(kb.coeffs + dz) - f(kb.coeffs - dz))
+    assert abs(cd - lin) <= 1e-10 * max(abs(lin), 1.0)
+
+
 def _dimer(seed, orbs=(3, 3), elec=(1, 2, 3), ns=3):
     rng = np.random.default_rng(seed)
     frags = [syn.make_fragment(n, list(elec), ns, 2, rng) for n in orbs]

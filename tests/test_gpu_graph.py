@@ -125,3 +125,29 @@ def test_captured_build_replays_match_eager(cfg):
             for n in range(m + 1, nf):
                 assert float((H2[m][n] - R2[m][n]).abs().max()) < 1e-12, (rep, m, n)
     context(0).call("xr_gemm_tune_reset")
+
+
+def test_capture_pins_the_workspaces():
+    """While a captured build exists, a call that would reallocate one of the
+    context's workspaces fails with XR_EBUSY instead of freeing memory the
+    graph replays; after release() the same call runs."""
+    from qodeapplications_amd._lib import XR_EBUSY, XRError
+    w = syn.config0(seed=0)
+    X = XRSystem(w.fragments, w.integrals, w.nuc)
+    cap = CapturedBuild(X, tune=False)
+    # a product far larger than any of the build's: its split-K staging
+    # needs a larger workspace slot than the capture saw
+    A = torch.randn(4096, 20000, dtype=torch.float64, device="cuda")
+    B = torch.randn(20000, 4096, dtype=torch.float64, device="cuda")
+    C = torch.zeros(4096, 4096, dtype=torch.float64, device="cuda")
+    with pytest.raises(XRError) as e:
+        mm(A, B, out=C, beta=1.0)
+    assert e.value.code == XR_EBUSY
+    H1, H2, _ = cap.replay()          # the graph still replays on its own memory
+    torch.cuda.synchronize()
+    cap.release()
+    mm(A, B, out=C, beta=1.0)
+    torch.cuda.synchronize()
+    assert float((C - A @ B).abs().max()) < 1e-9 * float((A.abs() @ B.abs()).max())
+    with pytest.raises(RuntimeError):
+        cap.replay()
