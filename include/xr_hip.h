@@ -191,6 +191,23 @@ int xr_gemm_f64(xr_ctx *ctx, int transA, int transB, int64_t M, int64_t N, int64
                 const double *d_B, int64_t ldb, int64_t strideB,
                 double beta, double *d_C, int64_t ldc, int64_t strideC, int64_t batch);
 
+/* xr_gemm_f64 that allocates nothing: no reference counterpart; it is for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * caller passes A, B and C as for xr_gemm_f64; the split-K partial tiles
+ * and the tall-skinny partial sums go to a static 64 MiB device array of
+ * the library (one per GPU, kept until the process ends) instead of the
+ * context's workspaces.  A call whose staging would exceed 64 MiB fails
+ * with XR_EINVAL (the allocating entry point handles it).  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other contexts, other threads) race.  Results equal
+ * xr_gemm_f64's (the same kernels and tile / K-split choices).  Launch
+ * timing (xr_profile) is suspended for the call. */
+int xr_gemm_f64_noalloc(xr_ctx *ctx, int transA, int transB, int64_t M, int64_t N, int64_t K,
+                        double alpha, const double *d_A, int64_t lda, int64_t strideA,
+                        const double *d_B, int64_t ldb, int64_t strideB,
+                        double beta, double *d_C, int64_t ldc, int64_t strideC, int64_t batch);
+
 
 /* n independent products C_q = alpha op(A_q) op(B_q) of uneven shapes
  * (host arrays of n shapes, leading dimensions and device pointers; the same

@@ -125,6 +125,14 @@ struct xr_ctx {
     // the caller guarantees V[pqrs] = V[rspq] for the monomer sigma adjoint
     // (xr_assume_hermitian_v): enables its one-gather path for a block with itself
     bool hermitian_v = false;
+    // no-allocation calls (xr_gemm_f64_noalloc): the device address of the
+    // library's static scratch array on this context's device, its size, and
+    // whether a call is running in that mode (xr_workspace then hands out
+    // the scratch array instead of allocating)
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    bool noalloc = false;
+    bool noalloc_blocked = false;
     std::mutex mu;
 };
 
@@ -141,6 +149,7 @@ struct ProfScope {
 };
 int xr_hip_check(xr_ctx* c, hipError_t e, const char* what);
 void* xr_workspace(xr_ctx* c, int slot, size_t bytes);
+int xr_noalloc_init(xr_ctx* c);
 int xr_fill(xr_ctx* c, double* p, int64_t n, double v);
 
 #define XR_HIP(c, expr)                                         \

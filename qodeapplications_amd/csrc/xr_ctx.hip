@@ -5,6 +5,12 @@
 __constant__ uint64_t c_binom[65 * 65];
 
 int xr_fail(xr_ctx* c, int code, const std::string& msg) {
+    if (c && c->noalloc_blocked && code == XR_ENOMEM) {
+        c->noalloc_blocked = false;
+        c->err = msg + ": the call needs more staging memory than the no-allocation scratch holds (" +
+                 std::to_string(c->scratch_bytes >> 20) + " MiB); use the allocating entry point";
+        return XR_EINVAL;
+    }
     if (c && c->pin_blocked && code == XR_ENOMEM) {
         // the allocation failed because a captured graph pins the workspaces
         c->pin_blocked = false;
@@ -24,6 +30,11 @@ int xr_hip_check(xr_ctx* c, hipError_t e, const char* what) {
 // Returns a device buffer of at least `bytes` for scratch slot `slot`,
 // growing it when needed (the old contents are not kept).
 void* xr_workspace(xr_ctx* c, int slot, size_t bytes) {
+    if (c->noalloc) {           // a no-allocation call: the static scratch array or nothing
+        if (c->scratch && bytes <= c->scratch_bytes) return c->scratch;
+        c->noalloc_blocked = true;
+        return nullptr;
+    }
     if ((int)c->ws.size() <= slot) {
         c->ws.resize(slot + 1, nullptr);
         c->ws_size.resize(slot + 1, 0);
@@ -280,6 +291,10 @@ extern "C" int xr_create(int device, void* stream, xr_ctx** out) {
             else tbl[n * 65 + k] = tbl[(n - 1) * 65 + k - 1] + tbl[(n - 1) * 65 + k];
         }
     if (hipMemcpyToSymbol(HIP_SYMBOL(c_binom), tbl, sizeof(tbl)) != hipSuccess) {
+        delete c;
+        return XR_EHIP;
+    }
+    if (xr_noalloc_init(c) != XR_OK) {     // the device address of the scratch array
         delete c;
         return XR_EHIP;
     }

@@ -1422,3 +1422,54 @@ extern "C" int xr_gemm_split_f32(xr_ctx* c, int transA, int transB, int64_t M, i
                                  sC, batch);
 }
 
+// ---------------------------------------------------------------------------
+// No-allocation FP64 GEMM
+// ---------------------------------------------------------------------------
+// Staging memory of xr_gemm_f64_noalloc (the split-K partial tiles and the
+// partial sums of the tall-skinny kernels): one 64 MiB array per device,
+// held until the process ends and shared by every such call on that device.
+// Host code never passes the array's name to a kernel (in host code it is a
+// host-side stand-in); it uses the device address xr_noalloc_init stores in
+// the context.
+#define XR_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+static __device__ __align__(256) double scratch[XR_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+// Called by xr_create with the context's device current: the device address
+// of this device's copy of `scratch`.
+int xr_noalloc_init(xr_ctx* c) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return xr_fail(c, XR_EHIP, "hipGetSymbolAddress(scratch) failed");
+    }
+    c->scratch = p;
+    c->scratch_bytes = XR_NOALLOC_SCRATCH_BYTES;
+    return XR_OK;
+}
+
+extern "C" int xr_gemm_f64_noalloc(xr_ctx* c, int transA, int transB, int64_t M, int64_t N, int64_t K,
+                                   double alpha, const double* A, int64_t lda, int64_t sA,
+                                   const double* B, int64_t ldb, int64_t sB, double beta, double* C,
+                                   int64_t ldc, int64_t sC, int64_t batch) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (M < 0 || N < 0 || K < 0 || batch < 0) return xr_fail(c, XR_EINVAL, "negative GEMM size");
+    if (ldc < N) return xr_fail(c, XR_ESHAPE, "ldc < N");
+    if ((transA ? lda < M : lda < K) && M > 0 && K > 0) return xr_fail(c, XR_ESHAPE, "lda too small");
+    if ((transB ? ldb < K : ldb < N) && N > 0 && K > 0) return xr_fail(c, XR_ESHAPE, "ldb too small");
+    if (!c->scratch) return xr_fail(c, XR_EHIP, "no scratch address for this device");
+    // the same kernels and choices as xr_gemm_f64, with every staging buffer
+    // taken from the scratch array; the launch timers are off for the call
+    // (they would create events)
+    const bool prof = c->prof;
+    c->prof = false;
+    c->noalloc = true;
+    c->noalloc_blocked = false;
+    int rc = xr_dgemm_launch(c, transA, transB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC,
+                             batch);
+    c->noalloc = false;
+    c->noalloc_blocked = false;
+    c->prof = prof;
+    return rc;
+}

@@ -1,0 +1,63 @@
+"""xr_gemm_f64_noalloc, the FP64 GEMM entry point that allocates nothing
+(its staging lives in the library's static per-device scratch array),
+against xr_gemm_f64 on the same inputs, through the C ABI: two calls in a
+row per case, the second with a smaller input, covering the split-K tile
+path (partial tiles in the scratch), the accumulating beta != 0 path, and
+the tall-skinny reductions; a call whose staging would exceed the scratch
+is refused with XR_EINVAL before anything is launched."""
+import pytest
+import torch
+
+from qodeapplications_amd._lib import XR_EINVAL, XRError, context, ptr
+
+pytestmark = pytest.mark.gpu
+
+
+def _gemm(name, ta, tb, A, B, C, alpha, beta):
+    M, N = C.shape
+    K = A.shape[0] if ta else A.shape[1]
+    context(0).call(name, ta, tb, M, N, K, alpha, ptr(A), A.stride(0), 0, ptr(B), B.stride(0), 0, beta,
+                    ptr(C), C.stride(0), 0, 1)
+
+
+def _inputs(ta, tb, M, N, K, seed):
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    A = torch.randn(*((K, M) if ta else (M, K)), dtype=torch.float64, device="cuda", generator=g)
+    B = torch.randn(*((N, K) if tb else (K, N)), dtype=torch.float64, device="cuda", generator=g)
+    C0 = torch.randn(M, N, dtype=torch.float64, device="cuda", generator=g)
+    return A, B, C0
+
+
+# (transA, transB, M, N, K, beta): split-K tiles, split-K with beta, the
+# transposed tall-skinny (Gram) reduction, the row-major Gram reduction,
+# a large tile grid
+CASES = [(0, 0, 300, 200, 5000, 0.0), (1, 0, 64, 96, 40000, 0.5), (1, 0, 48, 4, 100000, 0.0),
+         (0, 0, 16, 4, 50000, 1.0), (0, 1, 1500, 700, 300, 0.0)]
+
+
+@pytest.mark.parametrize("ta,tb,M,N,K,beta", CASES)
+def test_noalloc_gemm_equals_allocating_gemm(ta, tb, M, N, K, beta):
+    for rep, (m, n, k) in enumerate(((M, N, K), (max(1, M // 2), max(1, N // 2), max(1, K // 3)))):
+        A, B, C0 = _inputs(ta, tb, m, n, k, 1000 * rep + M + N + K)
+        ref = C0.clone()
+        got = C0.clone()
+        _gemm("xr_gemm_f64", ta, tb, A, B, ref, 0.75, beta)
+        _gemm("xr_gemm_f64_noalloc", ta, tb, A, B, got, 0.75, beta)
+        torch.cuda.synchronize()
+        assert torch.equal(got, ref), (rep, float((got - ref).abs().max()))
+
+
+def test_noalloc_gemm_refuses_staging_beyond_the_scratch():
+    # beta != 0 over a 6000 x 6000 output stages 288 MB, more than the 64 MiB array
+    A, B, C = _inputs(0, 0, 6000, 6000, 64, 7)
+    before = C.clone()
+    with pytest.raises(XRError) as e:
+        _gemm("xr_gemm_f64_noalloc", 0, 0, A, B, C, 1.0, 1.0)
+    assert e.value.code == XR_EINVAL
+    assert "scratch" in str(e.value)
+    torch.cuda.synchronize()
+    assert torch.equal(C, before)
+    # the allocating entry point still serves the same call
+    _gemm("xr_gemm_f64", 0, 0, A, B, C, 1.0, 1.0)
+    torch.cuda.synchronize()
+    assert float((C - (before + A @ B)).abs().max()) < 1e-10
