@@ -67,6 +67,11 @@ int xr_synchronize(xr_ctx *ctx);
  * counterpart (the reference has no captured launches). */
 int xr_pin(xr_ctx *ctx, int delta);
 
+/* Frees the context's workspaces (they grow again on demand); XR_EBUSY while
+ * pinned.  The cached excitation and side tables stay.  No reference
+ * counterpart. */
+int xr_trim(xr_ctx *ctx);
+
 /* on != 0: the caller promises that every V passed to the monomer
  * two-electron adjoints on this context satisfies V[pqrs] = V[rspq] (the
  * two-body part of a Hermitian Hamiltonian), which enables their one-gather

@@ -68,6 +68,19 @@ extern "C" int xr_assume_hermitian_v(xr_ctx* c, int on) {
     return XR_OK;
 }
 
+extern "C" int xr_trim(xr_ctx* c) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (c->pins > 0) return xr_fail(c, XR_EBUSY, "xr_trim: a captured graph holds the workspaces (xr_pin)");
+    XR_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < c->ws.size(); i++) {
+        if (c->ws[i]) XR_HIP(c, hipFree(c->ws[i]));
+        c->ws[i] = nullptr;
+        c->ws_size[i] = 0;
+    }
+    return XR_OK;
+}
+
 extern "C" int xr_pin(xr_ctx* c, int delta) {
     if (!c) return XR_EINVAL;
     std::lock_guard<std::mutex> g(c->mu);

@@ -1,0 +1,81 @@
+"""The Be 6-31G fragment states the reference keeps as test data
+(tests/ref_data/check_{configs,states}_{m}_{chg}.pkl, read by
+hermitian-XRCC/mains/workflow.py:80-85; 9 spatial orbitals, the 1s core
+frozen, charges +1 / 0 / -1 with 2 / 10 / 10 states), converted from their
+bit-packed colex layout to the lexical one (tests/golden/be631g_states.npz),
+through the GPU path: every density of both fragments and the monomer
+sigma against the oracle, and the XR dimer with seeded synthetic integrals
+(no Be integrals exist here without psi4) against the oracle's monomer
+blocks, 4000 sampled pair elements and lowest eigenvalues."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle.oracle as orc
+from qodeapplications_amd import density as D
+from qodeapplications_amd import synthetic as syn
+from qodeapplications_amd.xr import XRSystem
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "be631g_states.npz")
+N_ORBS, N_CORE = 9, 1
+
+
+def _fragments():
+    g = np.load(GOLD)
+    frags = []
+    for m in (0, 1):
+        states = {}
+        for chg, key in ((0, "p0"), (1, "p1"), (-1, "m1")):
+            states[chg] = syn.ChargeStates(configs=g[f"configs_{m}_{key}"], coeffs=g[f"coeffs_{m}_{key}"])
+        frags.append(syn.Fragment(n_orbs=N_ORBS, n_core=N_CORE, n_elec_ref=4, states=states))
+    return frags, g
+
+
+def _integrals():
+    rng = np.random.default_rng(631)
+    h = syn.spatial_h(2 * N_ORBS, rng)
+    V = syn.spatial_V(2 * N_ORBS, rng, np.concatenate([np.arange(N_ORBS)] * 2))
+    return syn.FragmentBlocked([N_ORBS, N_ORBS], h, V)
+
+
+@pytest.mark.parametrize("m", [0, 1])
+def test_be631g_densities_match_oracle(m):
+    frags, _ = _fragments()
+    f = frags[m]
+    blocks = D.charge_blocks(f.states, N_ORBS, N_CORE)
+    n = 0
+    for bc in f.states:
+        for kc in f.states:
+            for op in D.OPS_BY_CHG_DIFF_FULL.get(bc - kc, ()):
+                got = D.density(op, blocks[bc], blocks[kc]).cpu().numpy()
+                ref = orc.density(op, N_ORBS, N_CORE, f.states[bc].coeffs, f.states[kc].coeffs, f.states[kc].configs)
+                assert float(np.abs(got - ref).max()) <= 1e-12, (op, bc, kc)
+                n += 1
+    assert n == 17
+    V = torch.as_tensor(_integrals().V[m, m, m, m], device="cuda")
+    for c, b in blocks.items():
+        R = orc.density("ccaa", N_ORBS, N_CORE, f.states[c].coeffs, f.states[c].coeffs, f.states[c].configs)
+        Vn = _integrals().V[m, m, m, m]
+        ref = np.array([[orc.monomer_2e(2 * N_ORBS, R[i, j], Vn) for j in range(R.shape[1])] for i in range(R.shape[0])])
+        got = D.monomer_2e(b, b, V).cpu().numpy()
+        assert float(np.abs(got - ref).max()) <= 1e-12, c
+
+
+def test_be631g_dimer_matches_oracle():
+    frags, g = _fragments()
+    X = XRSystem(frags, _integrals(), np.zeros((2, 2)))
+    H1, H2, _ = X.blocks()
+    for m in (0, 1):
+        assert float(np.abs(H1[m].cpu().numpy() - g[f"H1_{m}"]).max()) <= 1e-12
+    idx = torch.as_tensor(g["H2_idx"], device="cuda")
+    got = H2[0][1][idx[:, 0], idx[:, 1]].cpu().numpy()
+    assert float(np.abs(got - g["H2_val"]).max()) <= 1e-12
+    w = torch.linalg.eigvalsh(X.xr_matrix(H=(H1, H2, None)))[:5].cpu().numpy()
+    assert float(np.abs(w - g["evals"]).max()) <= 1e-9
+    op = X.operator()
+    E, _ = op.lowest(k=1, tol=1e-10)
+    assert abs(float(E[0]) - float(g["evals"][0])) <= 1e-9

@@ -134,9 +134,13 @@ def test_capture_pins_the_workspaces():
     from qodeapplications_amd._lib import XR_EBUSY, XRError
     w = syn.config0(seed=0)
     X = XRSystem(w.fragments, w.integrals, w.nuc)
+    context(0).call("xr_trim")        # workspaces sized by this build only
     cap = CapturedBuild(X, tune=False)
-    # a product far larger than any of the build's: its split-K staging
-    # needs a larger workspace slot than the capture saw
+    with pytest.raises(XRError) as e:
+        context(0).call("xr_trim")    # pinned
+    assert e.value.code == XR_EBUSY
+    # a product far larger than any of the build's: its staging needs a
+    # larger workspace slot than the capture saw
     A = torch.randn(4096, 20000, dtype=torch.float64, device="cuda")
     B = torch.randn(20000, 4096, dtype=torch.float64, device="cuda")
     C = torch.zeros(4096, 4096, dtype=torch.float64, device="cuda")
