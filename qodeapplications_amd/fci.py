@@ -88,8 +88,9 @@ class Hamiltonian:
             occ = self.configs
             d = h[occ, occ].sum(axis=1)
             if self.n_elec >= 2:
-                # <P| p+ q+ s r |P> for p != q occupied: +1 for (s, r) = (q, p), -1 for (p, q)
-                J = np.einsum("pqqp->pq", V) - np.einsum("pqpq->pq", V)
+                # <P| p+ q+ s r |P> for p != q occupied: p+ q+ q p = n_p n_q (r = p,
+                # s = q: V[p, q, p, q]) and p+ q+ p q = -n_p n_q (V[p, q, q, p])
+                J = np.einsum("pqpq->pq", V) - np.einsum("pqqp->pq", V)
                 np.fill_diagonal(J, 0.0)
                 d = d + J[occ[:, :, None], occ[:, None, :]].sum(axis=(1, 2))
             self._diag = upload(d, self.device)

@@ -55,7 +55,7 @@ def test_be631g_densities_match_oracle(m):
                 ref = orc.density(op, N_ORBS, N_CORE, f.states[bc].coeffs, f.states[kc].coeffs, f.states[kc].configs)
                 assert float(np.abs(got - ref).max()) <= 1e-12, (op, bc, kc)
                 n += 1
-    assert n == 17
+    assert n == 18
     V = torch.as_tensor(_integrals().V[m, m, m, m], device="cuda")
     for c, b in blocks.items():
         R = orc.density("ccaa", N_ORBS, N_CORE, f.states[c].coeffs, f.states[c].coeffs, f.states[c].configs)
