@@ -449,3 +449,93 @@ def compute_H_blocks(supersystem, integrals, nuc, with_trimers=False):
                             h3[i, j] = bme.trimer((M, N, O), I, J)
                     H3[M][N][O] = h3
     return H1, H2, H3
+
+
+# ---------------------------------------------------------------------------
+# XR-CCSD by brute force (general-XRCC/excitonic.py:28-53)
+# ---------------------------------------------------------------------------
+
+class _NoTrimers:
+    """H3 of a Hamiltonian without three-fragment terms: every element 0."""
+
+    def __getitem__(self, k):
+        return 0.0 if isinstance(k, tuple) else self
+
+
+def xr_ccsd(H1, H2, thresh=1e-11):
+    """Restates the XR-CCSD of general-XRCC/excitonic.py:28-53 (Qode's CCSD
+    on H_012_operator) by brute force: the dense XR matrix from braket_loops,
+    T (singles t_m[i] |i><0|_m, doubles t_mn[i, j] |i><0|_m |j><0|_n, i, j != 0)
+    applied with numpy tensor contractions, exp(+-T) by the finite series,
+    the residuals <mu| e^-T H e^T |0> for every single and double mu, solved
+    by scipy.optimize.root from T = 0 (not the Jacobi / DIIS of the GPU
+    path).  H1: monomer matrices; H2[m][n]: pair matrices (m < n).  Returns
+    (E, singles, doubles)."""
+    from scipy.optimize import root
+    nf = len(H1)
+    dims = [h.shape[0] for h in H1]
+    n = int(np.prod(dims))
+    spf = dims[0]
+    if any(d != spf for d in dims):
+        raise ValueError("braket_loops needs equal state counts")
+    H = np.zeros((n, n))
+    braket_loops(H, nf, spf, (H1, H2, _NoTrimers()))
+    pairs = [(m, k) for m in range(nf) for k in range(m + 1, nf)]
+
+    def unpack(x):
+        t1, o = [], 0
+        for d in dims:
+            t = np.zeros(d)
+            t[1:] = x[o:o + d - 1]
+            o += d - 1
+            t1.append(t)
+        t2 = {}
+        for m, k in pairs:
+            t = np.zeros((dims[m], dims[k]))
+            c = (dims[m] - 1) * (dims[k] - 1)
+            t[1:, 1:] = x[o:o + c].reshape(dims[m] - 1, dims[k] - 1)
+            o += c
+            t2[(m, k)] = t
+        return t1, t2
+
+    def apply_T(t1, t2, x, sign):
+        X = x.reshape(dims)
+        Y = np.zeros_like(X)
+        for m in range(nf):
+            A = np.zeros((dims[m], dims[m]))
+            A[:, 0] = t1[m]
+            Y += np.moveaxis(np.tensordot(A, X, axes=(1, m)), 0, m)
+        for (m, k), t in t2.items():
+            S = np.take(np.take(X, 0, axis=k), 0, axis=m)      # fragments m, k in state 0
+            full = np.multiply.outer(t, S)                      # axes (m, k, others)
+            Y += np.moveaxis(full, [0, 1], [m, k])
+        return sign * Y.reshape(-1)
+
+    def exp_T(t1, t2, x, sign):
+        out, term = x.copy(), x
+        for j in range(1, nf + 1):
+            term = apply_T(t1, t2, term, sign) / j
+            out = out + term
+        return out
+
+    ref = np.zeros(n)
+    ref[0] = 1.0
+    strides = [int(np.prod(dims[m + 1:])) for m in range(nf)]
+
+    def residual(x):
+        t1, t2 = unpack(x)
+        W = exp_T(t1, t2, H @ exp_T(t1, t2, ref, 1.0), -1.0)
+        r = [W[np.arange(1, d) * strides[m]] for m, d in enumerate(dims)]
+        for m, k in pairs:
+            i = np.arange(1, dims[m]) * strides[m]
+            j = np.arange(1, dims[k]) * strides[k]
+            r.append(W[i[:, None] + j[None, :]].reshape(-1))
+        return np.concatenate(r), W[0]
+
+    n_amp = sum(d - 1 for d in dims) + sum((dims[m] - 1) * (dims[k] - 1) for m, k in pairs)
+    sol = root(lambda x: residual(x)[0], np.zeros(n_amp), method="hybr", tol=thresh)
+    r, E = residual(sol.x)
+    if float(np.abs(r).max()) > 1e3 * thresh:
+        raise RuntimeError(f"oracle XR-CCSD did not converge (max residual {np.abs(r).max():.2e})")
+    t1, t2 = unpack(sol.x)
+    return float(E), t1, t2

@@ -44,7 +44,10 @@ def _axis_apply(A, src, dst, pre, n, post, transA=0, beta=0.0):
     if post == 1:
         mm(src.view(pre, n), A, transB=0 if transA else 1, out=dst.view(pre, n), beta=beta)
     else:
-        gemm_f64(A.unsqueeze(0), src.view(pre, n, post), dst.view(pre, n, post), transA=transA, beta=beta)
+        S, D = src.view(pre, n, post), dst.view(pre, n, post)
+        for b0 in range(0, pre, 65535):        # the batch is a grid dimension of at most 65535
+            b1 = min(pre, b0 + 65535)
+            gemm_f64(A.unsqueeze(0), S[b0:b1], D[b0:b1], transA=transA, beta=beta)
 
 
 def _arr(vals, dtype):
