@@ -336,11 +336,12 @@ int xr_pair_outer(xr_ctx *ctx, const double *d_v, int64_t d2, const int64_t *d_r
 /* GEMM autotuner (no counterpart in the reference, whose contractions are
  * fixed C loops, general-XRCC/H_contractions.c): times the four tile shapes
  * of xr_gemm_f64 with K splits 1..64 on scratch operands of the given shape
- * (reps timed launches each) and records the fastest in the context; later
- * xr_gemm_f64 calls of exactly that shape use it.  Outputs the chosen tile
- * (0 128x128, 1 128x64, 2 64x128, 3 64x64; -1 = the shape takes the skinny
- * kernel and is not tuned), splits and mean time.  Synchronises the stream:
- * call it outside graph capture. */
+ * (reps timed launches each) against the untuned dispatch and records the
+ * fastest in the context when it beats the untuned one; later xr_gemm_f64
+ * calls of exactly that shape use it.  Outputs the chosen tile (0 128x128,
+ * 1 128x64, 2 64x128, 3 64x64; -1 = the untuned dispatch is kept, e.g. the
+ * shape takes a skinny kernel), splits and mean time.  Synchronises the
+ * stream: call it outside graph capture. */
 int xr_gemm_tune(xr_ctx *ctx, int transA, int transB, int64_t M, int64_t N, int64_t K,
                  int64_t batch, int reps, int *tile, int *splits, double *ms);
 int xr_gemm_tune_reset(xr_ctx *ctx);
@@ -402,6 +403,87 @@ int xr_pair_gather(xr_ctx *ctx, const double *d_G2, int64_t ldg, int64_t d2, int
                    const int64_t *d_r1, const int64_t *d_c1, const int64_t *d_r2,
                    const int64_t *d_c2, int64_t nI1, int64_t nJ1, int64_t nI2, int64_t nJ2,
                    int swapped, double *d_out);
+
+/* ---------------------------------------------------------------------------
+ * Batched pair blocks (general-XRCC/build_H.py:56-102 build_matrix_elements
+ * .dimer over every state pair of every charge combination of every
+ * fragment pair, through the H_contractions.c:95-180 families).
+ *
+ * One job is one charge combination of one fragment pair.  Its family is
+ * named from the charge transfer of the pair's first fragment; the family's
+ * "fragment 1" is the pair's first fragment unless `transposed` (then it is
+ * the second one and the block is built as [b, a]):
+ *   XR_PAIR_2MIN2PLS  (two electrons move)   R1 = {cc}, R2 = {aa},
+ *                     V = {V[1,1,2,2]}                     (dimer_2min2pls)
+ *   XR_PAIR_1MIN1PLS  (one electron moves)   R1 = {c, cca}, R2 = {a, caa},
+ *                     V = {V[1,1,1,2], V[1,2,2,2]}, h = {T + U sums [1,2]}
+ *                                            (dimer_1min1pls_1e + _2e)
+ *   XR_PAIR_EXEX      (no transfer)          R1 = {ca}, R2 = {ca},
+ *                     V = {V[1,2,1,2]}, h = {U[1,2,2], U[2,1,1]}
+ *                     (dimer_ExEx + the extPot spectator terms and `nuc`
+ *                     where both fragments are diagonal; never transposed)
+ * Densities are (nI nJ) x d^k row-major as xr_density writes them, over the
+ * family's n1 / n2 spin orbitals.  Element (I1 J1, I2 J2) of the block (state
+ * pairs of the PAIR's fragments 1 and 2), times `scale`, is written to
+ *   out[(rows1[I1] * mul + rows2[I2]) * ldo + cols1[J1] * mul + cols2[J2]]
+ * (index arrays on the device).  The outputs listed in the call are zeroed
+ * first and symmetrised (A + A^T) last, so the jobs are the 'self' (half
+ * scale) and 'source' combinations only; two jobs may not place the same
+ * element.  Inputs are read-only and must stay valid while a plan lives. */
+enum { XR_PAIR_2MIN2PLS = 0, XR_PAIR_1MIN1PLS = 1, XR_PAIR_EXEX = 2 };
+
+typedef struct xr_pair_job {
+    int32_t kind;
+    int32_t transposed;
+    int64_t n1, n2;                 /* spin orbitals of the family's fragments 1, 2 */
+    int64_t nI1, nJ1, nI2, nJ2;     /* bra / ket states of the pair's fragments 1, 2 */
+    const double *R1[2];
+    const double *R2[2];
+    const double *V[2];
+    const double *h[2];
+    double scale;
+    double nuc;
+    double *out;
+    int64_t ldo;
+    int64_t mul;
+    const int64_t *rows1, *cols1, *rows2, *cols2;
+} xr_pair_job;
+
+typedef struct xr_pair_plan xr_pair_plan;
+
+/* Runs the jobs once on the context's stream (intermediates in the context's
+ * workspaces).  outs[i] is an out_n[i] x out_n[i] matrix with leading
+ * dimension out_ld[i]; mode XR_PREC_F64 or XR_PREC_SPLIT_F32 (the products
+ * with two-electron integrals as split-FP32 GEMMs, one launch each). */
+int xr_pair_blocks(xr_ctx *ctx, const xr_pair_job *jobs, int n_jobs, double *const *outs,
+                   const int64_t *out_n, const int64_t *out_ld, int n_outs, int mode);
+
+/* The same program as a plan that owns its intermediates and device
+ * descriptor tables: xr_pair_plan_run allocates nothing and launches the
+ * same kernels on the same addresses every time (capturable, below). */
+int xr_pair_plan_create(xr_ctx *ctx, const xr_pair_job *jobs, int n_jobs, double *const *outs,
+                        const int64_t *out_n, const int64_t *out_ld, int n_outs, int mode,
+                        xr_pair_plan **plan);
+int xr_pair_plan_run(xr_pair_plan *plan);
+/* launches, workgroups and executed FP64 flops of one run (any may be NULL) */
+int xr_pair_plan_info(xr_pair_plan *plan, int64_t *launches, int64_t *workgroups, double *flops,
+                      int64_t *arena_bytes);
+int xr_pair_plan_destroy(xr_pair_plan *plan);
+
+/* HIP graph capture of the context's stream without a framework: calls made
+ * between xr_graph_begin and xr_graph_end are recorded (not run) and
+ * xr_graph_launch replays them on the context's stream.  The context must
+ * have a stream of its own (xr_create / xr_set_stream with a non-null
+ * stream) and the profiler off.  The workspaces are pinned (xr_pin) from
+ * begin until the graph is destroyed, so run the captured sequence once
+ * before capturing it (a workspace that would have to grow fails the
+ * recorded call with XR_EBUSY).  No reference counterpart: the reference
+ * evaluates each element through a Python call. */
+typedef struct xr_graph xr_graph;
+int xr_graph_begin(xr_ctx *ctx);
+int xr_graph_end(xr_ctx *ctx, xr_graph **graph);
+int xr_graph_launch(xr_graph *graph);
+int xr_graph_destroy(xr_graph *graph);
 
 #ifdef __cplusplus
 }

@@ -72,7 +72,29 @@ SIGNATURES = {
     "xr_block_scatter": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P, _L]),
     "xr_braket_assemble": (_I, [_P, _I, _L, _P, _P, _P, _P, _P]),
     "xr_dimer_ExEx": (_I, [_P, _L, _L, _L, _P, _L, _P, _P, _P]),
+    "xr_pair_blocks": (_I, [_P, _P, _I, _P, _P, _P, _I, _I]),
+    "xr_pair_plan_create": (_I, [_P, _P, _I, _P, _P, _P, _I, _I, ctypes.POINTER(_P)]),
+    "xr_pair_plan_run": (_I, [_P]),
+    "xr_pair_plan_info": (_I, [_P, ctypes.POINTER(_L), ctypes.POINTER(_L), ctypes.POINTER(_D),
+                               ctypes.POINTER(_L)]),
+    "xr_pair_plan_destroy": (_I, [_P]),
+    "xr_graph_begin": (_I, [_P]),
+    "xr_graph_end": (_I, [_P, ctypes.POINTER(_P)]),
+    "xr_graph_launch": (_I, [_P]),
+    "xr_graph_destroy": (_I, [_P]),
 }
+
+PAIR_2MIN2PLS, PAIR_1MIN1PLS, PAIR_EXEX = 0, 1, 2
+
+
+class PairJob(ctypes.Structure):
+    """struct xr_pair_job of include/xr_hip.h (one charge combination of one
+    fragment pair for xr_pair_blocks / xr_pair_plan_create)."""
+    _fields_ = [("kind", ctypes.c_int32), ("transposed", ctypes.c_int32),
+                ("n1", _L), ("n2", _L), ("nI1", _L), ("nJ1", _L), ("nI2", _L), ("nJ2", _L),
+                ("R1", _P * 2), ("R2", _P * 2), ("V", _P * 2), ("h", _P * 2),
+                ("scale", _D), ("nuc", _D), ("out", _P), ("ldo", _L), ("mul", _L),
+                ("rows1", _P), ("cols1", _P), ("rows2", _P), ("cols2", _P)]
 
 
 class XRError(RuntimeError):

@@ -133,6 +133,11 @@ struct xr_ctx {
     size_t scratch_bytes = 0;
     bool noalloc = false;
     bool noalloc_blocked = false;
+    // pinned staging of xr_pair_blocks' descriptor tables and the event that
+    // marks the last copy out of it
+    void* pair_host = nullptr;
+    size_t pair_host_bytes = 0;
+    hipEvent_t pair_copied = nullptr;
     std::mutex mu;
 };
 

@@ -328,6 +328,8 @@ extern "C" int xr_destroy(xr_ctx* c) {
         (void)hipEventDestroy(r.b);
     }
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->pair_copied) (void)hipEventDestroy(c->pair_copied);
+    if (c->pair_host) (void)hipHostFree(c->pair_host);
     delete c;
     return XR_OK;
 }

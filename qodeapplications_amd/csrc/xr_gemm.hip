@@ -1126,9 +1126,24 @@ extern "C" int xr_gemm_tune(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int
     XR_HIP(c, hipEventCreate(&e1));
     const bool prof = c->prof;
     c->prof = false;
+    const std::array<int64_t, 6> key{ta, tb, M, N, K, batch};
+    c->tune.erase(key);
+    // the untuned dispatch is the first candidate: a table entry is kept
+    // only for a configuration that beats it
     double best = 1e300;
-    int best_tile = 0, best_split = 0;
-    for (int tile = 0; tile < 4; tile++) {
+    int best_tile = -1, best_split = 0;
+    rc = xr_dgemm_launch(c, ta, tb, M, N, K, 1.0, A, lda, sA, B, ldb, sB, 0.0, C, N, sC, batch);
+    if (!rc) {
+        XR_HIP(c, hipEventRecord(e0, c->stream));
+        for (int r = 0; r < reps; r++)
+            rc |= xr_dgemm_launch(c, ta, tb, M, N, K, 1.0, A, lda, sA, B, ldb, sB, 0.0, C, N, sC, batch);
+        XR_HIP(c, hipEventRecord(e1, c->stream));
+        XR_HIP(c, hipEventSynchronize(e1));
+        float ms = 0.f;
+        XR_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+        best = ms / reps;
+    }
+    for (int tile = 0; tile < 4 && !rc; tile++) {
         for (int sp = 1; sp <= 64; sp *= 2) {
             if (sp > 1 && K / sp < 256) break;
             rc = launch_cfg(c, tile, sp, ta, tb, M, N, K, 1.0, A, lda, sA, B, ldb, sB, 0.0, C, N, sC, batch);
@@ -1149,7 +1164,7 @@ extern "C" int xr_gemm_tune(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    c->tune[std::array<int64_t, 6>{ta, tb, M, N, K, batch}] = std::make_pair(best_tile, best_split);
+    if (best_tile >= 0) c->tune[key] = std::make_pair(best_tile, best_split);
     if (tile_out) *tile_out = best_tile;
     if (splits_out) *splits_out = best_split;
     if (ms_out) *ms_out = best;

@@ -5,3 +5,4 @@
 #include "xr_gemm.hip"
 #include "xr_density.hip"
 #include "xr_contract.hip"
+#include "xr_pairs.hip"

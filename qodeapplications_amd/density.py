@@ -167,7 +167,7 @@ def conjugate(rho, out=None, accumulate=False):
 
 
 def build_density_arrays(blocks, ops_table=OPS_BY_CHG_DIFF_FULL, skip=("ccaa",), derived=None,
-                         conjugates=True):
+                         conjugates=True, out=None):
     """All densities of one fragment on the device:
     rho[op][(bra_chg, ket_chg)] = tensor (nI, nJ, dim, ...).  Operator strings
     in `skip` are not built (the monomer 4-index term goes through
@@ -176,7 +176,8 @@ def build_density_arrays(blocks, ops_table=OPS_BY_CHG_DIFF_FULL, skip=("ccaa",),
     conjugate of that one (<I|op|J> = <J|op^+|I> for real states: half the
     GEMM work of the charge-changing strings); derived, if given, receives
     {(op, (bra, ket)): (op^+, (ket, bra))} for them.  conjugates=False builds
-    every string directly."""
+    every string directly.  out: a previous result whose tensors are
+    overwritten where they have the same shape (fixed device addresses)."""
     rho = {}
     pairs = [(b, k) for b in blocks for k in blocks]
     pairs.sort(key=lambda bk: bk[1] - bk[0])        # bra - ket >= 0 first: the conjugate sources
@@ -186,10 +187,16 @@ def build_density_arrays(blocks, ops_table=OPS_BY_CHG_DIFF_FULL, skip=("ccaa",),
                 continue
             cop = conjugate_string(op)
             src = rho.get(cop, {}).get((ket_chg, bra_chg)) if cop != op and conjugates else None
+            old = (out or {}).get(op, {}).get((bra_chg, ket_chg))
             if src is not None:
-                rho.setdefault(op, {})[(bra_chg, ket_chg)] = conjugate(src)
+                if old is not None and old.shape != (src.shape[1], src.shape[0]) + tuple(src.shape[2:]):
+                    old = None
+                rho.setdefault(op, {})[(bra_chg, ket_chg)] = conjugate(src, out=old)
                 if derived is not None:
                     derived[(op, (bra_chg, ket_chg))] = (cop, (ket_chg, bra_chg))
             else:
-                rho.setdefault(op, {})[(bra_chg, ket_chg)] = density(op, blocks[bra_chg], blocks[ket_chg])
+                bb, kb = blocks[bra_chg], blocks[ket_chg]
+                if old is not None and tuple(old.shape) != (bb.n_states, kb.n_states) + (kb.dim,) * len(op):
+                    old = None
+                rho.setdefault(op, {})[(bra_chg, ket_chg)] = density(op, bb, kb, out=old)
     return rho

@@ -11,10 +11,12 @@ is captured once into a HIP graph and replayed as a single launch:
   its FP64 GEMMs and times the tile / K-split candidates of each shape
   (xr_gemm_tune); later launches of those shapes use the winner;
 * CapturedBuild(system): warm-up (workspaces and caches reach their final
-  sizes, so the captured sequence allocates and synchronises nothing), then
-  capture of build_densities + blocks; replay() recomputes every block for
-  the coefficients currently held in the system's coefficient tensors
-  (update them in place with set_coeffs).
+  sizes, so the captured sequence allocates and synchronises nothing), a
+  pair plan (every pair matrix as one batched program on fixed addresses:
+  the densities are rebuilt in place), then capture of build_densities +
+  blocks; replay() recomputes every block for the coefficients currently
+  held in the system's coefficient tensors (update them in place with
+  set_coeffs).
 """
 import csv
 import ctypes
@@ -82,12 +84,21 @@ class CapturedBuild:
         dev = system.device
         self.ctx = context(dev.index)
 
-        def build():
-            system.build_densities()
+        def eager():
+            system.build_densities(reuse=True)
             with precision(system.precision, dev.index):
                 return system._blocks(with_trimers)
 
-        self.tuned = autotune(build, dev.index) if tune else {}
+        self.tuned = autotune(eager, dev.index) if tune else {}
+        eager()                                   # densities at their final addresses
+        nf = len(system.frags)
+        self.plan, _ = system.pair_plan([(m, n) for m in range(nf) for n in range(m + 1, nf)])
+
+        def build():
+            system.build_densities(reuse=True)
+            with precision(system.precision, dev.index):
+                return system._blocks(with_trimers, pair_plan=self.plan)
+
         build()                                   # final workspace sizes, warm caches
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
@@ -110,6 +121,7 @@ class CapturedBuild:
             self._pinned = False
             self.graph = None
             self.ctx.call("xr_pin", -1)
+            self.plan.close()
 
     def __del__(self):
         try:

@@ -13,7 +13,7 @@ import numpy as np
 import torch
 
 from . import blocks as B
-from ._lib import context, ptr
+from ._lib import PAIR_1MIN1PLS, PAIR_2MIN2PLS, PAIR_EXEX, PairJob, context, ptr
 from .density import OPS_BY_CHG_DIFF, ChargeBlock, build_density_arrays, monomer_2e
 
 
@@ -59,14 +59,16 @@ class XRFragment:
         self.E2 = None
         self.E2W = {}
 
-    def build(self, V_mm, keep_sigma=False, conjugates=True):
+    def build(self, V_mm, keep_sigma=False, conjugates=True, reuse=False):
         """Densities for all charge pairs (not the 4-index one) and the
         monomer two-electron matrices; keep_sigma keeps their intermediate
         W = Veff Y per charge (E2W) for the gradient step; conjugates=False
-        builds the lowering strings directly instead of as conjugates."""
+        builds the lowering strings directly instead of as conjugates;
+        reuse writes the densities into the tensors of the previous build
+        (fixed addresses for a captured build)."""
         self.derived = {}
         self.rho = build_density_arrays(self.blocks, OPS_BY_CHG_DIFF, skip=("ccaa",), derived=self.derived,
-                                        conjugates=conjugates)
+                                        conjugates=conjugates, out=self.rho if reuse else None)
         self.E2, self.E2W = {}, {}
         for chg, b in self.blocks.items():
             if keep_sigma:
@@ -124,10 +126,11 @@ class XRSystem:
         self.build_count = 0     # densities builds so far (a backward checks it matches its forward)
         self._sector_idx = {}    # per pair: fixed-charge block layout and scatter indices
 
-    def build_densities(self, keep_sigma=False):
+    def build_densities(self, keep_sigma=False, reuse=False):
         with precision(self.precision, self.device.index):
             for m, F in enumerate(self.frags):
-                F.build(self.ints.V(m, m, m, m), keep_sigma and not self.plain, conjugates=not self.plain)
+                F.build(self.ints.V(m, m, m, m), keep_sigma and not self.plain, conjugates=not self.plain,
+                        reuse=reuse)
         self.built = True
         self.build_count += 1
 
@@ -206,9 +209,109 @@ class XRSystem:
             if k1[0] == k1[1] and k2[0] == k2[1] and self.nuc[m1, m2] != 0.0:
                 B.scatter(out, ldo, fr, rows, cols, dims, None, [0, 0], diag=[1, 1], scale=half * self.nuc[m1, m2])
 
+    def _pair_job(self, m1, m2, k1, k2, half, out, ldo, rows, cols, mul, keep):
+        """The xr_pair_job of one charge combination: the family and operands
+        build_matrix_elements.dimer (build_H.py:56-102) uses for it, as
+        blocks.dimer_block selects them."""
+        F1, F2 = self.frags[m1], self.frags[m2]
+        I = self.ints
+        d = k1[0] - k1[1]
+        sign2 = float((-1) ** ((F2.n_elec_ref - k2[0]) % 2))
+
+        def P(t):
+            if not t.is_contiguous():
+                raise ValueError("pair job operands must be contiguous")
+            keep.append(t)
+            return t.data_ptr()
+
+        r1 = lambda op: P(F1.rho[op][k1])
+        r2 = lambda op: P(F2.rho[op][k2])
+        j = PairJob()
+        j.nI1, j.nJ1 = F1.n_states[k1[0]], F1.n_states[k1[1]]
+        j.nI2, j.nJ2 = F2.n_states[k2[0]], F2.n_states[k2[1]]
+        j.nuc = 0.0
+        if d in (-2, 2):
+            j.kind = PAIR_2MIN2PLS
+            if d == -2:
+                j.transposed, j.n1, j.n2, j.scale = 0, F1.dim, F2.dim, half
+                j.R1[0], j.R2[0], j.V[0] = r1("cc"), r2("aa"), P(I.V(m1, m1, m2, m2))
+            else:
+                j.transposed, j.n1, j.n2, j.scale = 1, F2.dim, F1.dim, half
+                j.R1[0], j.R2[0], j.V[0] = r2("cc"), r1("aa"), P(I.V(m2, m2, m1, m1))
+        elif d in (-1, 1):
+            j.kind = PAIR_1MIN1PLS
+            if d == -1:
+                j.transposed, j.n1, j.n2, j.scale = 0, F1.dim, F2.dim, half * sign2
+                j.R1[0], j.R1[1] = r1("c"), r1("cca")
+                j.R2[0], j.R2[1] = r2("a"), r2("caa")
+                j.V[0], j.V[1] = P(I.V(m1, m1, m1, m2)), P(I.V(m1, m2, m2, m2))
+                j.h[0] = P(I.h_sum((m1, m2, "1min"), [("T", (m1, m2)), ("U", (m1, m1, m2)), ("U", (m2, m1, m2))]))
+            else:
+                j.transposed, j.n1, j.n2, j.scale = 1, F2.dim, F1.dim, -half * sign2
+                j.R1[0], j.R1[1] = r2("c"), r2("cca")
+                j.R2[0], j.R2[1] = r1("a"), r1("caa")
+                j.V[0], j.V[1] = P(I.V(m2, m2, m2, m1)), P(I.V(m2, m1, m1, m1))
+                j.h[0] = P(I.h_sum((m2, m1, "1pls"), [("T", (m2, m1)), ("U", (m1, m2, m1)), ("U", (m2, m2, m1))]))
+        else:
+            j.kind, j.transposed, j.n1, j.n2, j.scale = PAIR_EXEX, 0, F1.dim, F2.dim, half
+            j.R1[0], j.R2[0], j.V[0] = r1("ca"), r2("ca"), P(I.V(m1, m2, m1, m2))
+            j.h[0], j.h[1] = P(I.U(m1, m2, m2)), P(I.U(m2, m1, m1))
+            j.nuc = float(self.nuc[m1, m2])
+        j.out, j.ldo, j.mul = P(out), int(ldo), int(mul)
+        j.rows1, j.rows2 = P(rows[0]), P(rows[1])
+        j.cols1, j.cols2 = P(cols[0]), P(cols[1])
+        return j
+
+    def pair_jobs(self, pairs, place):
+        """(jobs, keep) for the 'self' (half weight) and 'source' charge
+        combinations of the pairs (the outputs are completed as A + A^T);
+        place(m1, m2, k1, k2) -> (out, ldo, rows, cols, mul)."""
+        self._need()
+        jobs, keep = [], []
+        for (m1, m2) in pairs:
+            for k1, k2 in self.pair_charge_combos(m1, m2):
+                role = self.combo_role(k1, k2)
+                if role == "derived":
+                    continue
+                out, ldo, rows, cols, mul = place(m1, m2, k1, k2)
+                jobs.append(self._pair_job(m1, m2, k1, k2, 0.5 if role == "self" else 1.0,
+                                           out, ldo, rows, cols, mul, keep))
+        return jobs, keep
+
+    def _dense_place(self, H):
+        def place(m1, m2, k1, k2):
+            F1, F2 = self.frags[m1], self.frags[m2]
+            return (H[(m1, m2)], F1.d * F2.d, [F1.pos[k1[0]], F2.pos[k2[0]]],
+                    [F1.pos[k1[1]], F2.pos[k2[1]]], F2.d)
+        return place
+
+    def pair_matrices(self, pairs):
+        """{(m1, m2): H2[m1][m2]} for the pairs, all in one batched program
+        (xr_pair_blocks); plain systems build each combination on its own."""
+        if self.plain:
+            return {(m1, m2): self.pair_matrix(m1, m2) for (m1, m2) in pairs}
+        from .pairs import pair_blocks
+        H = {(m1, m2): torch.empty(self.frags[m1].d * self.frags[m2].d, self.frags[m1].d * self.frags[m2].d,
+                                   dtype=torch.float64, device=self.device) for (m1, m2) in pairs}
+        jobs, keep = self.pair_jobs(pairs, self._dense_place(H))
+        pair_blocks(jobs, list(H.values()), PRECISIONS[self.precision], self.device.index)
+        return H
+
+    def pair_plan(self, pairs, H=None):
+        """A PairPlan writing the dense pair matrices of the pairs into H
+        (allocated when not given): replays read the densities at their
+        current addresses (rebuild them with reuse=True)."""
+        from .pairs import PairPlan
+        if H is None:
+            H = {(m1, m2): torch.empty(self.frags[m1].d * self.frags[m2].d, self.frags[m1].d * self.frags[m2].d,
+                                       dtype=torch.float64, device=self.device) for (m1, m2) in pairs}
+        jobs, keep = self.pair_jobs(pairs, self._dense_place(H))
+        return PairPlan(jobs, [H[p] for p in pairs], PRECISIONS[self.precision], self.device.index, keep), H
+
     def pair_matrix(self, m1, m2):
         """H2[m1][m2] over the product basis [(a, b) for a in states(m1) for b
-        in states(m2)] = build_matrix_elements.dimer (build_H.py:56-102)."""
+        in states(m2)] = build_matrix_elements.dimer (build_H.py:56-102), one
+        charge combination at a time."""
         F1, F2 = self.frags[m1], self.frags[m2]
         D = F1.d * F2.d
         out = torch.zeros(D, D, dtype=torch.float64, device=self.device)
@@ -219,14 +322,12 @@ class XRSystem:
         context(self.device.index).call("xr_symmetrize", ptr(out), D)
         return out
 
-    def pair_sector_blocks(self, m1, m2, segs, segq):
-        """H2[m1][m2] as its square blocks of fixed pair charge, without the
-        dense pair matrix: block i spans the pairs (a, b) with charge(a) +
-        charge(b) = segq[i] in increasing a * d2 + b (segs[i] = (start, count)
-        in that ordering of all pairs).  Pair (a, b) is row off_q[a] + r(b) of
-        its block, r(b) being b's rank among the states of its charge and
-        off_q[a] the number of pairs of charge q with a smaller first state,
-        so the scatter indices stay one per fragment (added, not multiplied)."""
+    def _sector_place(self, m1, m2, segs, segq, blocks):
+        """Placement into the fixed-charge blocks of pair (m1, m2): pair (a, b)
+        is row off_q[a] + r(b) of block q = charge(a) + charge(b), r(b) being
+        b's rank among the states of its charge and off_q[a] the number of
+        charge-q pairs whose first state comes before a (blocks ordered by
+        increasing a * d2 + b; segs[i] = (start, count) in that ordering)."""
         F1, F2 = self.frags[m1], self.frags[m2]
         cache = self._sector_idx.setdefault((m1, m2), {})
         if "layout" not in cache:
@@ -238,7 +339,6 @@ class XRSystem:
             cache["layout"] = (off, rank2, pos1, pos2)
         off, rank2, pos1, pos2 = cache["layout"]
         seg_of = {q: i for i, q in enumerate(segq)}
-        blocks = [torch.zeros(cnt, cnt, dtype=torch.float64, device=self.device) for _, cnt in segs]
         dev = self.device
 
         def place(k1, k2):
@@ -250,13 +350,43 @@ class XRSystem:
                 cache[key] = ([t(off[q][pos1[k1[0]]]), t(rank2[pos2[k2[0]]])],
                               [t(off[q][pos1[k1[1]]]), t(rank2[pos2[k2[1]]])])
             rows, cols = cache[key]
-            return blocks[i], segs[i][1], rows, cols, [1, 1]
+            return blocks[i], segs[i][1], rows, cols
+        return place
+
+    def pair_sector_blocks(self, m1, m2, segs, segq):
+        """H2[m1][m2] as its square blocks of fixed pair charge, without the
+        dense pair matrix (layout of _sector_place), one charge combination
+        at a time."""
+        blocks = [torch.zeros(cnt, cnt, dtype=torch.float64, device=self.device) for _, cnt in segs]
+        pl = self._sector_place(m1, m2, segs, segq, blocks)
+
+        def place(k1, k2):
+            out, ldo, rows, cols = pl(k1, k2)
+            return out, ldo, rows, cols, [1, 1]
 
         with precision(self.precision, self.device.index):
             self._pair_terms(m1, m2, place)
         ctx = context(self.device.index)
         for blk, (_, cnt) in zip(blocks, segs):
             ctx.call("xr_symmetrize", ptr(blk), cnt)
+        return blocks
+
+    def pair_sector_blocks_all(self, layouts):
+        """{(m1, m2): fixed-charge blocks} for layouts {(m1, m2): (segs,
+        segq)}, every pair in one batched program (xr_pair_blocks)."""
+        if self.plain:
+            return {p: self.pair_sector_blocks(p[0], p[1], *lay) for p, lay in layouts.items()}
+        from .pairs import pair_blocks
+        blocks = {p: [torch.empty(cnt, cnt, dtype=torch.float64, device=self.device) for _, cnt in lay[0]]
+                  for p, lay in layouts.items()}
+        places = {p: self._sector_place(p[0], p[1], lay[0], lay[1], blocks[p]) for p, lay in layouts.items()}
+
+        def place(m1, m2, k1, k2):
+            out, ldo, rows, cols = places[(m1, m2)](k1, k2)
+            return out, ldo, rows, cols, 1
+
+        jobs, keep = self.pair_jobs(list(layouts), place)
+        pair_blocks(jobs, [b for p in layouts for b in blocks[p]], PRECISIONS[self.precision], self.device.index)
         return blocks
 
     def trimer_matrix(self, m1, m2, m3):
@@ -302,13 +432,19 @@ class XRSystem:
         with precision(self.precision, self.device.index):
             return self._blocks(with_trimers)
 
-    def _blocks(self, with_trimers):
+    def _blocks(self, with_trimers, pair_plan=None):
+        """(H1, H2, H3); pair_plan: a PairPlan (pair_plan()) whose run
+        gives the pair matrices instead of a fresh batched build."""
         nf = len(self.frags)
         H1 = [self.monomer_matrix(m) for m in range(nf)]
         H2 = [[None] * nf for _ in range(nf)]
-        for m in range(nf):
-            for n in range(m + 1, nf):
-                H2[m][n] = self.pair_matrix(m, n)
+        pairs = [(m, n) for m in range(nf) for n in range(m + 1, nf)]
+        if pair_plan is not None:
+            pm = dict(zip(pairs, pair_plan.run()))
+        else:
+            pm = self.pair_matrices(pairs)
+        for (m, n), h in pm.items():
+            H2[m][n] = h
         H3 = None
         if with_trimers:
             H3 = [[[None] * nf for _ in range(nf)] for _ in range(nf)]
@@ -336,7 +472,7 @@ class XRSystem:
             with precision(self.precision, self.device.index):
                 H1 = [self.monomer_matrix(m) for m in range(nf)]
                 return XROperator(H1, {(m, n): None for m in range(nf) for n in range(m + 1, nf)},
-                                  self.state_charges(), pair_builder=self.pair_sector_blocks)
+                                  self.state_charges(), pairs_builder=self.pair_sector_blocks_all)
         H1, H2, H3 = H
         if H3 is not None:
             raise ValueError("the matrix-free operator carries monomer and pair terms only")

@@ -84,7 +84,7 @@ class XROperator:
     matrix (d_m d_n x d_m d_n)} for m < n; charges: per fragment, the charge
     of each state in state order."""
 
-    def __init__(self, H1, H2, charges, symmetrize=False, pair_builder=None):
+    def __init__(self, H1, H2, charges, symmetrize=False, pair_builder=None, pairs_builder=None):
         self.H1 = [h.contiguous() for h in H1]
         self.dims = [int(h.shape[0]) for h in H1]
         self.nf = len(self.dims)
@@ -97,14 +97,23 @@ class XROperator:
         self.found = None
         self.log = None          # set to a list to record the per-sector solves
         self.grouped = True      # pair products of a sector in one grouped GEMM (else one launch each)
+        pending = {}
         for (m, n), H in H2.items():
             if H is not None:
                 self.pairs[(m, n)] = self._pair_blocks(m, n, H, symmetrize)
+            elif pairs_builder is not None:
+                pending[(m, n)] = self._order(m, n)
             elif pair_builder is not None:
                 # pair_builder(m, n, segs, segq) returns the fixed-charge
                 # blocks in the layout _pair_blocks cuts out of a dense matrix
                 order, inv, segs, segq = self._order(m, n)
                 self.pairs[(m, n)] = self._pair_meta(order, inv, segs, segq, pair_builder(m, n, segs, segq))
+        if pending:
+            # pairs_builder({(m, n): (segs, segq)}) returns those blocks of
+            # every pair at once
+            built = pairs_builder({p: (o[2], o[3]) for p, o in pending.items()})
+            for p, (order, inv, segs, segq) in pending.items():
+                self.pairs[p] = self._pair_meta(order, inv, segs, segq, built[p])
 
     # -- setup ----------------------------------------------------------------
     def _order(self, m, n):

@@ -50,3 +50,20 @@ def test_conjugate_strings():
     for op, adj in pairs.items():
         assert conjugate_string(op) == adj
         assert conjugate_string(adj) == op
+
+
+def test_pair_job_struct_matches_the_header(tmp_path):
+    """ctypes' PairJob has the size and field offsets the C compiler gives
+    struct xr_pair_job (gcc on include/xr_hip.h)."""
+    import subprocess
+    fields = [f for f, _ in _lib.PairJob._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"xr_hip.h\"\nint main(void){\n"
+                   "printf(\"%zu\\n\", sizeof(xr_pair_job));\n"
+                   + "".join(f"printf(\"%zu\\n\", offsetof(xr_pair_job, {f}));\n" for f in fields)
+                   + "return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    vals = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert vals[0] == ctypes.sizeof(_lib.PairJob)
+    assert vals[1:] == [getattr(_lib.PairJob, f).offset for f in fields]
