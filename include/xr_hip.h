@@ -127,8 +127,9 @@ int xr_excitation_fill(xr_ctx *ctx, const char *ops, int n_orbs, int n_core, int
 
 /* Dense transition density <I| ops |J> for all state pairs (replaces
  * a/c/ca/aa/cc/caa/cca/ccaa_tensor, general-XRCC/density_tensors.c:162-556,
- * and caaa/ccca_tensor, Be-states/_attic_/deprecated/density_tensors.c:560-688,
- * including its separate bra coefficient list).  The ket space has
+ * caaa/ccca_tensor, Be-states/_attic_/deprecated/density_tensors.c:560-688,
+ * including its separate bra coefficient list, and the XR order-1 strings
+ * ccaaa/cccaa of the same file, :690-836).  The ket space has
  * n_elec_ket electrons and nQ = C(dim-2n_core, n_elec_ket-2n_core)
  * determinants; the bra space is implied by ops.  d_out: nI*nJ*dim^k doubles. */
 int xr_density(xr_ctx *ctx, const char *ops, int n_orbs, int n_core, int n_elec_ket,
@@ -385,7 +386,7 @@ int xr_axis_apply_multi(xr_ctx *ctx, int n_items, const double *const *d_U, cons
 int xr_symmetrize(xr_ctx *ctx, double *d_A, int64_t n);
 
 /* dst[J, I, x_k .. x_1] = src[I, J, x_1 .. x_k] (accumulate: +=) for a
- * density of k <= 4 spin-orbital indices over d <= 64 spin orbitals: the
+ * density of k <= 5 spin-orbital indices over d <= 64 spin orbitals: the
  * density of the conjugate string between the swapped charges for real
  * states, <I|op|J> = <J|op^+|I> (as Be-states/densities.py:192-206 takes
  * the conjugates by transposition). */

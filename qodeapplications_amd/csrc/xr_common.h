@@ -12,7 +12,7 @@
 #include "../../include/xr_hip.h"
 
 #define XR_MAX_SO 64      // spin orbitals per fragment: one 64-bit occupation word
-#define XR_MAX_OPS 4
+#define XR_MAX_OPS 5
 
 // ---------------------------------------------------------------------------
 // binomial table in constant memory: C(n, k) for 0 <= n, k <= 64

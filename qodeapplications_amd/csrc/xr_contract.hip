@@ -754,7 +754,7 @@ __global__ __launch_bounds__(256) void k_conjugate(const double* __restrict__ sr
 extern "C" int xr_conjugate(xr_ctx* c, const double* src, int64_t nI, int64_t nJ, int d, int k, double* dst,
                             int accumulate) {
     XR_ENTRY(c);
-    if (nI < 0 || nJ < 0 || d < 1 || k < 0 || k > 4) return xr_fail(c, XR_EINVAL, "bad density shape");
+    if (nI < 0 || nJ < 0 || d < 1 || k < 0 || k > 5) return xr_fail(c, XR_EINVAL, "bad density shape");
     if (d > 64) return xr_fail(c, XR_EUNSUPPORTED, "more than 64 spin orbitals");
     if (src == dst && nI * nJ > 0) return xr_fail(c, XR_EINVAL, "xr_conjugate cannot work in place");
     int64_t P = 1;

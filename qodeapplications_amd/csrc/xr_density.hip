@@ -40,6 +40,10 @@ int xr_plan_op(const char* ops, OpPlan* p) {
         {"ccaa", {4, 1, {2, 0, {1, 0}}, {2, 0, {0, 1}}}},
         {"caaa", {4, 1, {1, 0, {0}}, {3, 0, {0, 1, 2}}}},
         {"ccca", {4, 1, {3, 0, {2, 1, 0}}, {1, 0, {0}}}},
+        // the 5-operator strings of XR order 1 (Be-states/densities.py:83-88;
+        // Be-states/_attic_/deprecated/density_tensors.c:690-836)
+        {"ccaaa", {5, 1, {2, 0, {1, 0}}, {3, 0, {0, 1, 2}}}},
+        {"cccaa", {5, 1, {3, 0, {2, 1, 0}}, {2, 0, {0, 1}}}},
     };
     if (!ops) return 0;
     for (const Row& r : rows) {

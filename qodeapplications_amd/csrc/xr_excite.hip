@@ -86,7 +86,7 @@ __global__ void k_excite(OpString os, int dim, int n_orbs, int n_core, const uin
 
 static int check_common(xr_ctx* c, const char* ops, int n_orbs, int n_core, int n_elec_ket,
                         OpString* os) {
-    if (!parse_ops(ops, os)) return xr_fail(c, XR_EUNSUPPORTED, "operator string must be 1-4 of 'c'/'a'");
+    if (!parse_ops(ops, os)) return xr_fail(c, XR_EUNSUPPORTED, "operator string must be 1-5 of 'c'/'a'");
     if (n_orbs <= 0 || 2 * n_orbs > XR_MAX_SO) return xr_fail(c, XR_EUNSUPPORTED, "need 0 < 2*n_orbs <= 64");
     if (n_core < 0 || n_core > n_orbs) return xr_fail(c, XR_EINVAL, "bad n_core");
     int nb = n_elec_ket + os->delta;

@@ -18,7 +18,7 @@ OPS_BY_CHG_DIFF = {+2: ("aa",), +1: ("a", "caa"), 0: ("ca", "ccaa"), -1: ("c", "
 # with the 4-operator strings of Be-states/_attic_/deprecated/density_tensors.c:560-688
 OPS_BY_CHG_DIFF_FULL = {+2: ("aa", "caaa"), +1: ("a", "caa"), 0: ("ca", "ccaa"),
                         -1: ("c", "cca"), -2: ("cc", "ccca")}
-SUPPORTED_OPS = ("a", "c", "ca", "aa", "cc", "caa", "cca", "ccaa", "caaa", "ccca")
+SUPPORTED_OPS = ("a", "c", "ca", "aa", "cc", "caa", "cca", "ccaa", "caaa", "ccca", "ccaaa", "cccaa")
 
 
 def n_elec_change(op):

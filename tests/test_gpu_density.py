@@ -64,6 +64,8 @@ def test_excitation_lists_bit_exact(op, n_orbs, n_core, n_ket):
 @pytest.mark.parametrize("op", ALL_OPS)
 @pytest.mark.parametrize("n_orbs,n_core,n_ket,ns", [(4, 1, 4, 3), (4, 0, 2, 5), (9, 0, 4, 8), (6, 0, 3, 17)])
 def test_density_matches_oracle(op, n_orbs, n_core, n_ket, ns):
+    if len(op) == 5 and n_orbs > 6:
+        pytest.skip("5-operator strings: the oracle's dense dim^5 output is checked at smaller sizes")
     n_bra = n_ket + D.n_elec_change(op)
     if n_bra < 2 * n_core or n_bra > 2 * n_orbs:
         pytest.skip("empty bra space")

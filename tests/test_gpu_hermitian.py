@@ -63,7 +63,7 @@ def test_det_options_and_signature():
     with pytest.raises(NotImplementedError):
         build_tensors(frag.states, 3, 2, options=["compress=SVD,1e-4"])
     with pytest.raises(NotImplementedError):
-        build_tensors(frag.states, 3, 2, xr_order=1)
+        build_tensors(frag.states, 3, 2, xr_order=2)
 
 
 def _dimer_system(seed):
