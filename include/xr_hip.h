@@ -67,6 +67,12 @@ int xr_synchronize(xr_ctx *ctx);
  * counterpart (the reference has no captured launches). */
 int xr_pin(xr_ctx *ctx, int delta);
 
+/* Caps the total bytes of the context's workspaces (0 = no cap): a call
+ * whose scratch would exceed it fails with XR_ENOMEM instead of allocating
+ * (for callers that share HBM with other work; also how the out-of-memory
+ * paths are tested).  No reference counterpart. */
+int xr_set_memory_limit(xr_ctx *ctx, int64_t bytes);
+
 /* Frees the context's workspaces (they grow again on demand); XR_EBUSY while
  * pinned.  The cached excitation and side tables stay.  No reference
  * counterpart. */

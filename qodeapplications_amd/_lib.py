@@ -33,6 +33,7 @@ SIGNATURES = {
     "xr_synchronize": (_I, [_P]),
     "xr_pin": (_I, [_P, _I]),
     "xr_trim": (_I, [_P]),
+    "xr_set_memory_limit": (_I, [_P, _L]),
     "xr_assume_hermitian_v": (_I, [_P, _I]),
     "xr_profile": (_I, [_P, _I]),
     "xr_profile_stats": (_I, [_P, _I, ctypes.POINTER(_L), ctypes.POINTER(_D), ctypes.POINTER(_D), ctypes.POINTER(_D)]),

@@ -111,6 +111,7 @@ struct xr_ctx {
     // growable scratch buffers (device)
     std::vector<void*> ws;
     std::vector<size_t> ws_size;
+    size_t ws_limit = 0;        // cap on the workspaces' total bytes (0: none; xr_set_memory_limit)
     // cache of K-space occupation masks keyed by (dim, n_elec)
     std::map<std::pair<int, int>, std::pair<uint64_t*, int64_t>> kspace;
     // side-adjoint index tables (one int32 code per source determinant and

@@ -40,6 +40,12 @@ void* xr_workspace(xr_ctx* c, int slot, size_t bytes) {
         c->ws_size.resize(slot + 1, 0);
     }
     if (c->ws_size[slot] >= bytes && c->ws[slot]) return c->ws[slot];
+    if (c->ws_limit) {          // the caller's cap on the workspaces' total
+        size_t total = 0;
+        for (size_t i = 0; i < c->ws_size.size(); i++)
+            if ((int)i != slot) total += c->ws_size[i];
+        if (total + bytes > c->ws_limit) return nullptr;
+    }
     if (c->pins > 0) {          // a captured graph reads the current address
         c->pin_blocked = true;
         return nullptr;
@@ -59,6 +65,14 @@ void* xr_workspace(xr_ctx* c, int slot, size_t bytes) {
     c->ws[slot] = p;
     c->ws_size[slot] = sz;
     return p;
+}
+
+extern "C" int xr_set_memory_limit(xr_ctx* c, int64_t bytes) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (bytes < 0) return xr_fail(c, XR_EINVAL, "negative memory limit");
+    c->ws_limit = (size_t)bytes;
+    return XR_OK;
 }
 
 extern "C" int xr_assume_hermitian_v(xr_ctx* c, int on) {

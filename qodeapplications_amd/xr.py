@@ -7,6 +7,7 @@ the layouts of general-XRCC/test_H.py:63-108, and assembles the tensor-
 product XR matrix (general-XRCC/hamiltonian.py:21-83) on the device.
 """
 import ctypes
+import math
 from contextlib import contextmanager
 
 import numpy as np
@@ -65,16 +66,28 @@ class XRFragment:
         W = Veff Y per charge (E2W) for the gradient step; conjugates=False
         builds the lowering strings directly instead of as conjugates;
         reuse writes the densities into the tensors of the previous build
-        (fixed addresses for a captured build)."""
+        (fixed addresses for a captured build).  keep_sigma may be a byte
+        budget per charge block: an intermediate larger than that is not
+        stored (the adjoint recomputes it)."""
         self.derived = {}
         self.rho = build_density_arrays(self.blocks, OPS_BY_CHG_DIFF, skip=("ccaa",), derived=self.derived,
                                         conjugates=conjugates, out=self.rho if reuse else None)
         self.E2, self.E2W = {}, {}
         for chg, b in self.blocks.items():
-            if keep_sigma:
+            if keep_sigma and keep_sigma is not True and self.sigma_bytes(chg) > keep_sigma:
+                self.E2[chg] = monomer_2e(b, b, V_mm)        # recomputed by the adjoint instead
+            elif keep_sigma:
                 self.E2[chg], self.E2W[chg] = monomer_2e(b, b, V_mm, keep=True)
             else:
                 self.E2[chg] = monomer_2e(b, b, V_mm)
+
+    def sigma_bytes(self, chg):
+        """Bytes of the monomer sigma intermediate W = Veff Y of one charge
+        block (kept for the gradient step when the budget allows)."""
+        b = self.blocks[chg]
+        if b.n_elec < 2:
+            return 0
+        return 8 * math.comb(b.dim, 2) * math.comb(b.dim, b.n_elec - 2) * b.n_states
 
     def parity(self):
         return torch.tensor([(self.n_elec_ref - chg) % 2 for chg, _ in self.state_indices],
@@ -105,7 +118,7 @@ class XRSystem:
     .n_core, .n_elec_ref, .states, .state_indices; fragment-blocked
     integrals with T, U, V; nuclear repulsion matrix)."""
 
-    def __init__(self, fragments, integrals, nuc, device=0, precision="f64", plain=False):
+    def __init__(self, fragments, integrals, nuc, device=0, precision="f64", plain=False, sigma_budget=None):
         """precision: 'f64', or 'split_f32' for the two-electron contraction
         GEMMs (hi/lo FP32 matrix-core products accumulated in FP64).
 
@@ -113,7 +126,11 @@ class XRSystem:
         every density string built directly (no conjugates), the monomer
         sigma recomputed in the adjoint, dense pair matrices cut into the
         operator's blocks, one GEMM launch per pair product of the operator
-        (no grouped GEMM) and every charge sector converged (no screening)."""
+        (no grouped GEMM) and every charge sector converged (no screening).
+
+        sigma_budget: bytes up to which a charge block's monomer sigma
+        intermediate is stored for the gradient step (default a quarter of
+        the device's memory); a larger one is recomputed in the adjoint."""
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         self.precision = precision
@@ -125,12 +142,15 @@ class XRSystem:
         self.built = False
         self.build_count = 0     # densities builds so far (a backward checks it matches its forward)
         self._sector_idx = {}    # per pair: fixed-charge block layout and scatter indices
+        if sigma_budget is None:
+            sigma_budget = torch.cuda.get_device_properties(self.device).total_memory // 4
+        self.sigma_budget = int(sigma_budget)
 
     def build_densities(self, keep_sigma=False, reuse=False):
         with precision(self.precision, self.device.index):
             for m, F in enumerate(self.frags):
-                F.build(self.ints.V(m, m, m, m), keep_sigma and not self.plain, conjugates=not self.plain,
-                        reuse=reuse)
+                keep = self.sigma_budget if (keep_sigma and not self.plain) else 0
+                F.build(self.ints.V(m, m, m, m), keep, conjugates=not self.plain, reuse=reuse)
         self.built = True
         self.build_count += 1
 
