@@ -197,7 +197,8 @@ def main():
         ms = (t1 - t0) * 1e3 / steps
         stats = {}
         for name, k in (("gemm_f64", _lib.K_GEMM_F64), ("gemm_split_f32", _lib.K_GEMM_SPLIT),
-                        ("gemm_grouped", _lib.K_GEMM_GROUPED), ("gemm_bandwidth", _lib.K_GEMM_BW),
+                        ("gemm_grouped", _lib.K_GEMM_GROUPED), ("gemm_worklist", _lib.K_GEMM_WORKLIST),
+                        ("gemm_bandwidth", _lib.K_GEMM_BW),
                         ("other", _lib.K_OTHER),
                         ("gather", _lib.K_GATHER), ("expand", _lib.K_EXPAND),
                         ("excitation_table", _lib.K_TABLE), ("block_scatter", _lib.K_SCATTER),
@@ -376,7 +377,7 @@ def bench_c2(args):
     torch.cuda.synchronize()
     t_eager = (time.perf_counter() - t0) / args.steps
     ctx.profile(False)
-    kinds = {"gemm_f64 (densities)": _lib.K_GEMM_F64, "gemm_worklist (pair program)": _lib.K_GEMM_GROUPED,
+    kinds = {"gemm_f64 (densities)": _lib.K_GEMM_F64, "gemm_worklist (pair program)": _lib.K_GEMM_WORKLIST,
              "gemm_bandwidth": _lib.K_GEMM_BW, "scatter": _lib.K_SCATTER, "other": _lib.K_OTHER}
     stats = {}
     for name, k in kinds.items():

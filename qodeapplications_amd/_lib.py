@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libxrhip.so")
 
 PREC_F64, PREC_SPLIT_F32 = 0, 1
 (K_GEMM_F64, K_GATHER, K_EXPAND, K_TABLE, K_SCATTER, K_BRAKET, K_OTHER, K_GEMM_SPLIT, K_GEMM_BW,
- K_GEMM_GROUPED) = range(10)
+ K_GEMM_GROUPED, K_GEMM_WORKLIST) = range(11)
 XR_OK, XR_EINVAL, XR_ESHAPE, XR_EINVAL_CONFIG_ORDER, XR_EPAULI, XR_ENOMEM, XR_EHIP, XR_EUNSUPPORTED, \
     XR_EBUSY = range(9)
 _CODE_NAMES = {1: "XR_EINVAL", 2: "XR_ESHAPE", 3: "XR_EINVAL_CONFIG_ORDER", 4: "XR_EPAULI",

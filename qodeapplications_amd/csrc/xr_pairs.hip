@@ -668,7 +668,7 @@ int run_program(xr_ctx* c, const Program& P, const char* dblob) {
     for (int st = 0; st < 2; st++) {
         for (const GemmLaunch& L : P.launches[st]) {
             if (L.ntiles == 0) continue;
-            ProfScope prof(c, XR_K_GEMM_GROUPED, L.flops, L.bytes);
+            ProfScope prof(c, XR_K_GEMM_WORKLIST, L.flops, L.bytes);
             const dim3 grid((unsigned)L.ntiles);
             const WlTile* t = tiles + L.tile_off;
 #define XR_WL(BM_, BN_)                                                                                   \
