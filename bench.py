@@ -54,10 +54,10 @@ def parse():
                          "4 x 13-orbital chain (the headline line); c3 / c0: the same on the 2 x 18 / 2 x 9 "
                          "orbital dimers; c1: density build of one 13-orbital fragment, 2-6 electrons, 32 "
                          "states; c2: all blocks of six uneven fragments, eager vs graph replay")
-    ap.add_argument("--legs", default="f64,split_f32,plain_f64",
-                    help="comma list of f64 (the value), split_f32, plain_f64")
+    ap.add_argument("--legs", default="f64,split_f32,split_i8,plain_f64",
+                    help="comma list of f64 (the value), split_f32, split_i8, plain_f64")
     ap.add_argument("--side-steps", type=int, default=3,
-                    help="timed steps of the split_f32 and plain_f64 legs (the f64 leg times --steps)")
+                    help="timed steps of the split and plain_f64 legs (the f64 leg times --steps)")
     ap.add_argument("--no-grad", action="store_true", help="skip the adjoint (diagnostics only)")
     ap.add_argument("--tune", action="store_true",
                     help="autotune the GEMM tiles of one iteration before the warm-up (the tuning "
@@ -161,7 +161,7 @@ def main():
     def measure(leg, steps):
         """Warm-up, then the timed steps of one leg; returns the per-step
         time, energy, gradient and the library's kernel statistics."""
-        prec = "split_f32" if leg == "split_f32" else "f64"
+        prec = leg if leg in ("split_f32", "split_i8") else "f64"
         X = XRSystem(w.fragments, w.integrals, w.nuc, precision=prec, plain=(leg == "plain_f64"))
 
         def step():
@@ -197,6 +197,7 @@ def main():
         ms = (t1 - t0) * 1e3 / steps
         stats = {}
         for name, k in (("gemm_f64", _lib.K_GEMM_F64), ("gemm_split_f32", _lib.K_GEMM_SPLIT),
+                        ("gemm_split_i8", _lib.K_GEMM_SPLIT_I8), ("i8_digit_planes", _lib.K_I8_SLICE),
                         ("gemm_grouped", _lib.K_GEMM_GROUPED), ("gemm_worklist", _lib.K_GEMM_WORKLIST),
                         ("gemm_bandwidth", _lib.K_GEMM_BW),
                         ("other", _lib.K_OTHER),
@@ -237,9 +238,10 @@ def main():
             out["grad_rel_err_vs_f64"] = float(torch.linalg.norm(rs["g"] - r0["g"]) / torch.linalg.norm(r0["g"]))
         return out
 
-    split, plain = side("split_f32"), side("plain_f64")
-    if split is not None and plain is not None:
-        split["speedup_vs_plain_f64"] = plain["s_per_iter"] / split["s_per_iter"]
+    split, split_i8, plain = side("split_f32"), side("split_i8"), side("plain_f64")
+    for sp in (split, split_i8):
+        if sp is not None and plain is not None:
+            sp["speedup_vs_plain_f64"] = plain["s_per_iter"] / sp["s_per_iter"]
 
     traffic, tsrc = None, None
     tfile = os.path.join(ROOT, "profiles", f"{WORKLOADS[args.config][1]}_traffic.json")
@@ -275,6 +277,7 @@ def main():
                      "algorithmic_bytes_per_launch": gby / max(n, 1)},
         "kernels": r0["stats"],
         "split_fp32": split,
+        "split_int8": split_i8,
         "plain_fp64": plain,
         "cpu_baseline": cpu,
     }

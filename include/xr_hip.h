@@ -48,7 +48,7 @@ enum {
 typedef struct xr_ctx xr_ctx;
 
 /* Precision of the two-electron contraction GEMMs (xr_set_precision). */
-enum { XR_PREC_F64 = 0, XR_PREC_SPLIT_F32 = 1 };
+enum { XR_PREC_F64 = 0, XR_PREC_SPLIT_F32 = 1, XR_PREC_SPLIT_I8 = 2 };
 
 /* Context on `device`, queuing on `stream` (a hipStream_t; NULL = default).
  * No reference counterpart: the reference is a set of stateless C functions
@@ -233,17 +233,32 @@ int xr_gemm_grouped(xr_ctx *ctx, int n, int transA, int transB, const int64_t *M
                     const double *const *d_B, const int64_t *ldb, double *const *d_C,
                     const int64_t *ldc);
 
-/* Split-precision GEMM: each FP64 operand is split into hi = fp32(x) and
- * lo = fp32(x - hi); the FP32 matrix cores form hi*hi and hi*lo + lo*hi per
- * 4-deep step and every partial is added into FP64 accumulators at once.
- * Same arguments as xr_gemm_f64.  No reference counterpart. */
+/* Split-FP32 GEMM: while a tile is staged, each group of 8 consecutive k
+ * values of a row (column) is put on a power-of-two grid below its largest
+ * value, hi = the grid multiple (11 bits), lo = fp32(x - hi); the FP32
+ * matrix cores form the group's hi*hi sum exactly, which goes into an FP64
+ * accumulator, and hi*lo + lo*(hi + lo) with FP32 accumulation.  Error a
+ * few 2^-35 of max|a| max|b| per group.  Same arguments as xr_gemm_f64.
+ * No reference counterpart. */
 int xr_gemm_split_f32(xr_ctx *ctx, int transA, int transB, int64_t M, int64_t N, int64_t K,
                       double alpha, const double *d_A, int64_t lda, int64_t strideA,
                       const double *d_B, int64_t ldb, int64_t strideB,
                       double beta, double *d_C, int64_t ldc, int64_t strideC, int64_t batch);
 
-/* Selects FP64 (XR_PREC_F64, default) or split-FP32 (XR_PREC_SPLIT_F32)
- * GEMMs for the two-electron contractions (xr_monomer_2e, xr_dimer_2min2pls,
+/* Split-int8 GEMM (the Ozaki scheme on the int8 matrix cores): every row of
+ * op(A) and column of op(B) is scaled by a power of two and cut into 7
+ * signed 7-bit digit planes; the 28 digit-plane products of weight
+ * 2^(-7 t), t <= 6, are summed exactly in int32 (K chunks of at most 65536)
+ * and combined in FP64 in the epilogue.  Elementwise error at most
+ * 36 K 2^-49 max|a_row| max|b_col|.  Same arguments as xr_gemm_f64.
+ * No reference counterpart. */
+int xr_gemm_split_i8(xr_ctx *ctx, int transA, int transB, int64_t M, int64_t N, int64_t K,
+                     double alpha, const double *d_A, int64_t lda, int64_t strideA,
+                     const double *d_B, int64_t ldb, int64_t strideB,
+                     double beta, double *d_C, int64_t ldc, int64_t strideC, int64_t batch);
+
+/* Selects FP64 (XR_PREC_F64, default), split-FP32 (XR_PREC_SPLIT_F32) or
+ * split-int8 (XR_PREC_SPLIT_I8) GEMMs for the two-electron contractions (xr_monomer_2e, xr_dimer_2min2pls,
  * xr_dimer_1min1pls_2e, xr_dimer_ExEx, xr_trimer_*).  Densities, one-electron
  * terms and the eigenproblem stay FP64. */
 int xr_set_precision(xr_ctx *ctx, int mode);
@@ -460,8 +475,8 @@ typedef struct xr_pair_plan xr_pair_plan;
 
 /* Runs the jobs once on the context's stream (intermediates in the context's
  * workspaces).  outs[i] is an out_n[i] x out_n[i] matrix with leading
- * dimension out_ld[i]; mode XR_PREC_F64 or XR_PREC_SPLIT_F32 (the products
- * with two-electron integrals as split-FP32 GEMMs, one launch each). */
+ * dimension out_ld[i]; mode XR_PREC_F64, XR_PREC_SPLIT_F32 or XR_PREC_SPLIT_I8
+ * (the products with two-electron integrals as split GEMMs, one launch each). */
 int xr_pair_blocks(xr_ctx *ctx, const xr_pair_job *jobs, int n_jobs, double *const *outs,
                    const int64_t *out_n, const int64_t *out_ld, int n_outs, int mode);
 

@@ -11,9 +11,9 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libxrhip.so")
 
-PREC_F64, PREC_SPLIT_F32 = 0, 1
+PREC_F64, PREC_SPLIT_F32, PREC_SPLIT_I8 = 0, 1, 2
 (K_GEMM_F64, K_GATHER, K_EXPAND, K_TABLE, K_SCATTER, K_BRAKET, K_OTHER, K_GEMM_SPLIT, K_GEMM_BW,
- K_GEMM_GROUPED, K_GEMM_WORKLIST) = range(11)
+ K_GEMM_GROUPED, K_GEMM_WORKLIST, K_GEMM_SPLIT_I8, K_I8_SLICE) = range(13)
 XR_OK, XR_EINVAL, XR_ESHAPE, XR_EINVAL_CONFIG_ORDER, XR_EPAULI, XR_ENOMEM, XR_EHIP, XR_EUNSUPPORTED, \
     XR_EBUSY = range(9)
 _CODE_NAMES = {1: "XR_EINVAL", 2: "XR_ESHAPE", 3: "XR_EINVAL_CONFIG_ORDER", 4: "XR_EPAULI",
@@ -60,6 +60,7 @@ SIGNATURES = {
     "xr_pair_gather": (_I, [_P, _P, _L, _L, _L, _P, _P, _P, _P, _P, _L, _L, _L, _L, _I, _P]),
     "xr_gemm_f64": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
     "xr_gemm_split_f32": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
+    "xr_gemm_split_i8": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
     "xr_gemm_f64_noalloc": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
     "xr_set_precision": (_I, [_P, _I]),
     "xr_monomer_1e": (_I, [_P, _L, _L, _P, _P, _P]),
@@ -166,7 +167,7 @@ class Context:
         self.check(self.L.xr_set_stream(self.h, _P(stream)))
 
     def set_precision(self, mode):
-        """PREC_F64 or PREC_SPLIT_F32 for the two-electron contraction GEMMs."""
+        """PREC_F64, PREC_SPLIT_F32 or PREC_SPLIT_I8 for the two-electron contraction GEMMs."""
         self.check(self.L.xr_set_precision(self.h, int(mode)))
         self.precision = int(mode)
 

@@ -87,11 +87,13 @@ __device__ __forceinline__ uint64_t d_core_mask(int n_orbs, int n_core) {
 // XR_K_GEMM_F64: the MFMA tile GEMM kernel alone; XR_K_GEMM_BW: the
 // bandwidth-bound tall-skinny product kernels; XR_K_GEMM_GROUPED: the
 // grouped tile GEMM (many products in one grid); XR_K_GEMM_WORKLIST: the
-// pair program's work-list GEMM; reductions of staged GEMMs count as
-// XR_K_OTHER.
+// pair program's work-list GEMM; XR_K_GEMM_SPLIT_I8: the split-int8 GEMM
+// kernel; XR_K_I8_SLICE: its exponent and digit-plane passes; reductions of
+// staged GEMMs count as XR_K_OTHER.
 enum { XR_K_GEMM_F64 = 0, XR_K_GATHER = 1, XR_K_EXPAND = 2, XR_K_TABLE = 3, XR_K_SCATTER = 4,
        XR_K_BRAKET = 5, XR_K_OTHER = 6, XR_K_GEMM_SPLIT = 7, XR_K_GEMM_BW = 8,
-       XR_K_GEMM_GROUPED = 9, XR_K_GEMM_WORKLIST = 10, XR_K_COUNT = 11 };
+       XR_K_GEMM_GROUPED = 9, XR_K_GEMM_WORKLIST = 10, XR_K_GEMM_SPLIT_I8 = 11, XR_K_I8_SLICE = 12,
+       XR_K_COUNT = 13 };
 
 struct ProfRec {
     int kind;

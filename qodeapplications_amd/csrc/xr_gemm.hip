@@ -1406,21 +1406,25 @@ int xr_dgemm_split_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64
     return splitk_reduce(c, Cw, splits, batch, M, N, alpha, beta, C, ldc, sC);
 }
 
-// The contraction GEMMs of the two-electron terms go through here: FP64 or
-// split-FP32 depending on the context's precision setting.
+// The contraction GEMMs of the two-electron terms go through here: FP64,
+// split-FP32 or split-int8 depending on the context's precision setting
+// (split-int8 for the products where it pays, xr_i8_pays; FP64 for the rest).
 int xr_gemm_2e(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, double alpha,
                const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb, int64_t sB,
                double beta, double* C, int64_t ldc, int64_t sC, int64_t batch) {
     if (c->precision == XR_PREC_SPLIT_F32)
         return xr_dgemm_split_launch(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc,
                                      sC, batch);
+    if (c->precision == XR_PREC_SPLIT_I8 && xr_i8_pays(M, N, K, batch))
+        return xr_dgemm_i8_launch(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
     return xr_dgemm_launch(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
 }
 
 extern "C" int xr_set_precision(xr_ctx* c, int mode) {
     if (!c) return XR_EINVAL;
     std::lock_guard<std::mutex> g(c->mu);
-    if (mode != XR_PREC_F64 && mode != XR_PREC_SPLIT_F32) return xr_fail(c, XR_EINVAL, "unknown precision mode");
+    if (mode != XR_PREC_F64 && mode != XR_PREC_SPLIT_F32 && mode != XR_PREC_SPLIT_I8)
+        return xr_fail(c, XR_EINVAL, "unknown precision mode");
     c->precision = mode;
     return XR_OK;
 }

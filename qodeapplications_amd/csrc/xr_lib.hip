@@ -3,6 +3,7 @@
 #include "xr_ctx.hip"
 #include "xr_excite.hip"
 #include "xr_gemm.hip"
+#include "xr_ozaki.hip"
 #include "xr_density.hip"
 #include "xr_contract.hip"
 #include "xr_pairs.hip"

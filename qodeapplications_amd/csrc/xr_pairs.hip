@@ -248,7 +248,7 @@ struct Proto {                      // one product before the K-split decision
     const double* B;
     int64_t ldb;
     int group;                      // output group (several terms may add into one output)
-    bool two_e;                     // a two-electron product (split-FP32 in that mode)
+    bool two_e;                     // a two-electron product (a split GEMM in the split modes)
 };
 struct Group {
     double* dst;
@@ -261,9 +261,9 @@ struct GemmLaunch {
     int64_t tile_off, ntiles;
     double flops, bytes;            // of the launch's products (profiler records)
 };
-struct SplitCall {                  // a product run by its own split-FP32 launch
+struct SplitCall {                  // a product run by its own split launch (FP32 or int8)
     WlGemm g;
-    int ta, tb;
+    int ta, tb, mode;
 };
 
 struct Program {
@@ -462,11 +462,16 @@ struct Builder {
     void finish_stage(int st) {
         std::vector<Proto>& pr = protos[st];
         std::vector<int64_t> splits(pr.size(), 1);
-        const bool sep = (mode == XR_PREC_SPLIT_F32);
+        // two-electron products that run as their own split launches: all of
+        // them in split-FP32, those where int8 pays in split-int8
+        std::vector<char> sep(pr.size(), 0);
+        for (size_t q = 0; q < pr.size(); q++)
+            sep[q] = pr[q].two_e && (mode == XR_PREC_SPLIT_F32 ||
+                                     (mode == XR_PREC_SPLIT_I8 && xr_i8_pays(pr[q].M, pr[q].N, pr[q].K, 1)));
         // per (ta, tb, shape) class: balance the work over the resident workgroups
         std::map<std::array<int, 3>, std::vector<int>> classes;
         for (size_t q = 0; q < pr.size(); q++) {
-            if (sep && pr[q].two_e) continue;
+            if (sep[q]) continue;
             classes[{pr[q].ta, pr[q].tb, shape_of(pr[q].M, pr[q].N)}].push_back((int)q);
         }
         for (auto& kv : classes) {
@@ -560,13 +565,12 @@ struct Builder {
             P.launches[st].push_back(L);
             P.workgroups += L.ntiles;
         }
-        if (sep)
-            for (size_t q = 0; q < pr.size(); q++)
-                if (pr[q].two_e) {
-                    const Proto& p = pr[q];
-                    WlGemm g{p.A, p.B, dst[q], p.M, p.N, p.K, p.lda, p.ldb, ldc[q], p.alpha, p.K, 0};
-                    P.split_calls[st].push_back(SplitCall{g, p.ta, p.tb});
-                }
+        for (size_t q = 0; q < pr.size(); q++)
+            if (sep[q]) {
+                const Proto& p = pr[q];
+                WlGemm g{p.A, p.B, dst[q], p.M, p.N, p.K, p.lda, p.ldb, ldc[q], p.alpha, p.K, 0};
+                P.split_calls[st].push_back(SplitCall{g, p.ta, p.tb, mode});
+            }
     }
 };
 
@@ -681,8 +685,11 @@ int run_program(xr_ctx* c, const Program& P, const char* dblob) {
             XR_LAUNCH_CHECK(c);
         }
         for (const SplitCall& s : P.split_calls[st]) {
-            const int rc = xr_dgemm_split_launch(c, s.ta, s.tb, s.g.M, s.g.N, s.g.K, s.g.alpha, s.g.A, s.g.lda, 0,
-                                                 s.g.B, s.g.ldb, 0, 0.0, s.g.C, s.g.ldc, 0, 1);
+            const int rc = (s.mode == XR_PREC_SPLIT_I8)
+                ? xr_dgemm_i8_launch(c, s.ta, s.tb, s.g.M, s.g.N, s.g.K, s.g.alpha, s.g.A, s.g.lda, 0, s.g.B,
+                                     s.g.ldb, 0, 0.0, s.g.C, s.g.ldc, 0, 1)
+                : xr_dgemm_split_launch(c, s.ta, s.tb, s.g.M, s.g.N, s.g.K, s.g.alpha, s.g.A, s.g.lda, 0,
+                                        s.g.B, s.g.ldb, 0, 0.0, s.g.C, s.g.ldc, 0, 1);
             if (rc) return rc;
         }
         if (!P.reduces[st].empty()) {
@@ -726,7 +733,8 @@ extern "C" int xr_pair_blocks(xr_ctx* c, const xr_pair_job* jobs, int n_jobs, do
     std::lock_guard<std::mutex> lock(c->mu);
     if (n_jobs < 0 || n_outs < 0 || (n_jobs > 0 && !jobs) || (n_outs > 0 && (!outs || !out_n || !out_ld)))
         return xr_fail(c, XR_EINVAL, "bad pair job list");
-    if (mode != XR_PREC_F64 && mode != XR_PREC_SPLIT_F32) return xr_fail(c, XR_EINVAL, "unknown precision mode");
+    if (mode != XR_PREC_F64 && mode != XR_PREC_SPLIT_F32 && mode != XR_PREC_SPLIT_I8)
+        return xr_fail(c, XR_EINVAL, "unknown precision mode");
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(c->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
         return xr_fail(c, XR_EINVAL, "xr_pair_blocks uploads its descriptors from host memory and cannot be "
@@ -768,7 +776,8 @@ extern "C" int xr_pair_plan_create(xr_ctx* c, const xr_pair_job* jobs, int n_job
     *plan = nullptr;
     if (n_jobs < 0 || n_outs < 0 || (n_jobs > 0 && !jobs) || (n_outs > 0 && (!outs || !out_n || !out_ld)))
         return xr_fail(c, XR_EINVAL, "bad pair job list");
-    if (mode != XR_PREC_F64 && mode != XR_PREC_SPLIT_F32) return xr_fail(c, XR_EINVAL, "unknown precision mode");
+    if (mode != XR_PREC_F64 && mode != XR_PREC_SPLIT_F32 && mode != XR_PREC_SPLIT_I8)
+        return xr_fail(c, XR_EINVAL, "unknown precision mode");
     xr_pair_plan* p = new xr_pair_plan();
     p->c = c;
     std::string err;

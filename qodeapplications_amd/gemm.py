@@ -33,7 +33,11 @@ def mm(A, B, transA=0, transB=0, alpha=1.0, out=None, beta=0.0, split=False):
     """out = alpha op(A) op(B) + beta out for 2-D float64 CUDA tensors that
     are row-major with any leading dimension (views of wider buffers work
     without copies).  transA = 1 uses A^T, transB = 1 uses B^T; split=True
-    runs the split-FP32 product."""
+    (or "f32") runs the split-FP32 product, split="i8" the split-int8 one."""
+    entry = {False: "xr_gemm_f64", True: "xr_gemm_split_f32", "f32": "xr_gemm_split_f32",
+             "i8": "xr_gemm_split_i8"}.get(split)
+    if entry is None:
+        raise ValueError("split must be False, True, 'f32' or 'i8'")
     for t in (A, B):
         if t.dim() != 2 or t.dtype != torch.float64 or not t.is_cuda:
             raise ValueError("mm needs 2-D float64 CUDA tensors")
@@ -52,7 +56,7 @@ def mm(A, B, transA=0, transB=0, alpha=1.0, out=None, beta=0.0, split=False):
     if out.stride(1) != 1 or tuple(out.shape) != (M, N):
         raise ValueError("out must be row-major (M, N)")
     ctx = context(A.device.index)
-    ctx.call("xr_gemm_split_f32" if split else "xr_gemm_f64", int(transA), int(transB), M, N, K,
+    ctx.call(entry, int(transA), int(transB), M, N, K,
              float(alpha), ptr(A), A.stride(0), 0, ptr(B), B.stride(0), 0, float(beta), ptr(out),
              out.stride(0), 0, 1)
     return out

@@ -45,8 +45,9 @@ def mm(A, Bm, transA=0, transB=0, alpha=1.0, out=None, beta=0.0, split=False):
 def bilinear_adjoint(G, A, W, Bm, s, Abar=None, Bbar=None, Wbar=None, split=False):
     """For H = s A W B^T (A: nA x p, W: p x q, B: nB x q) and G = dE/dH
     (nA x nB): Abar += s G B W^T, Bbar += s G^T A W, Wbar += s A^T G B.
-    Each triple product is evaluated in the cheaper association; split runs
-    the products on the split-FP32 path (two-electron terms in that mode)."""
+    Each triple product is evaluated in the cheaper association; split
+    ('f32' or 'i8') runs the products on that split path (two-electron terms
+    in the split modes)."""
     nA, p = A.shape
     nB, q = Bm.shape
     x = split
@@ -215,7 +216,7 @@ class XRAdjoint:
         X = self.X
         F1, F2 = X.frags[m1], X.frags[m2]
         (i1c, j1c), (i2c, j2c) = k1, k2
-        sp = X.precision == "split_f32"
+        sp = {"split_f32": "f32", "split_i8": "i8"}.get(X.precision, False)
         nI1, nJ1, nI2, nJ2 = F1.n_states[i1c], F1.n_states[j1c], F2.n_states[i2c], F2.n_states[j2c]
         nA, nB = nI1 * nJ1, nI2 * nJ2
         d = i1c - j1c

@@ -18,13 +18,14 @@ from ._lib import PAIR_1MIN1PLS, PAIR_2MIN2PLS, PAIR_EXEX, PairJob, context, ptr
 from .density import OPS_BY_CHG_DIFF, ChargeBlock, build_density_arrays, monomer_2e
 
 
-PRECISIONS = {"f64": 0, "split_f32": 1}
+PRECISIONS = {"f64": 0, "split_f32": 1, "split_i8": 2}
 
 
 @contextmanager
 def precision(mode, device=0):
-    """Runs the enclosed two-electron contractions in `mode` ('f64' or
-    'split_f32') on the device's context, restoring the previous mode."""
+    """Runs the enclosed two-electron contractions in `mode` ('f64',
+    'split_f32' or 'split_i8') on the device's context, restoring the
+    previous mode."""
     ctx = context(device)
     old = ctx.precision
     ctx.set_precision(PRECISIONS[mode] if isinstance(mode, str) else int(mode))
@@ -119,8 +120,10 @@ class XRSystem:
     integrals with T, U, V; nuclear repulsion matrix)."""
 
     def __init__(self, fragments, integrals, nuc, device=0, precision="f64", plain=False, sigma_budget=None):
-        """precision: 'f64', or 'split_f32' for the two-electron contraction
-        GEMMs (hi/lo FP32 matrix-core products accumulated in FP64).
+        """precision: 'f64', or 'split_f32' / 'split_i8' for the
+        two-electron contraction GEMMs (hi/lo FP32 matrix-core products
+        accumulated in FP64, or int8 digit-plane products accumulated
+        exactly in int32).
 
         plain=True is the unbatched FP64 route kept as a comparison point:
         every density string built directly (no conjugates), the monomer

@@ -11,13 +11,14 @@ tests/golden/make_sample_fixtures.py):
     determinant count (the lowering strings derived by conjugation included);
   * one pair-matrix element per charge combination against the oracle's
     build_matrix_elements.dimer (general-XRCC/build_H.py:56-102 over the
-    H_contractions.c restatement) on those densities, in FP64 and
-    split-FP32;
+    H_contractions.c restatement) on those densities, in FP64, split-FP32
+    and split-int8;
   * the fixed-charge pair blocks of the iteration path against the blocks
     cut from the dense pair matrix.
-Split-FP32 against FP64: the per-block max |diff| of both monomer blocks and
-of every fixed-charge pair block, and |dE| of the lowest 10 XR eigenvalues
-(<= 1e-9 Eh).
+Split-FP32 and split-int8 against FP64: the per-block max |diff| of both
+monomer blocks and of every fixed-charge pair block, and |dE| of the lowest
+10 XR eigenvalues (<= 1e-9 Eh); the split-int8 system must have run int8
+products (its shape rule keeps the memory-bound ones on FP64).
 """
 import os
 
@@ -39,15 +40,18 @@ DENS_TOL = 1e-12      # densities, absolute
 @pytest.fixture(scope="module")
 def c3():
     w = syn.config3(seed=3)
-    print("c3: building FP64 and split-FP32 systems", flush=True)
+    print("c3: building FP64, split-FP32 and split-int8 systems", flush=True)
     Xf = XRSystem(w.fragments, w.integrals, w.nuc)
     Xs = XRSystem(w.fragments, w.integrals, w.nuc, precision="split_f32")
+    Xi = XRSystem(w.fragments, w.integrals, w.nuc, precision="split_i8")
     Xf.build_densities()
     Xs.build_densities()
+    Xi.build_densities()
     torch.cuda.synchronize()
+    print(f"c3: {torch.cuda.memory_allocated() / 2**30:.1f} GiB allocated", flush=True)
     fx = dict(np.load(os.path.join(os.path.dirname(__file__), "golden", "c3_samples.npz")))
-    yield w, Xf, Xs, fx
-    del Xf, Xs
+    yield w, Xf, {"split_f32": Xs, "split_i8": Xi}, fx
+    del Xf, Xs, Xi
     torch.cuda.empty_cache()
 
 
@@ -74,10 +78,10 @@ def test_c3_densities_match_oracle_samples(c3, m):
     assert n == 13
 
 
-@pytest.mark.parametrize("prec", ["f64", "split_f32"])
+@pytest.mark.parametrize("prec", ["f64", "split_f32", "split_i8"])
 def test_c3_pair_elements_match_oracle(c3, prec):
-    w, Xf, Xs, fx = c3
-    X = Xf if prec == "f64" else Xs
+    w, Xf, Xsplit, fx = c3
+    X = Xf if prec == "f64" else Xsplit[prec]
     H2 = X.pair_matrix(0, 1)
     F1, F2 = X.frags
     worst = 0.0
@@ -104,8 +108,10 @@ def test_c3_fixed_charge_blocks_equal_dense_cut(c3):
         assert float((a - b).abs().max()) <= 1e-12 * max(1.0, float(a.abs().max()))
 
 
-def test_c3_split_per_block_errors(c3):
-    w, Xf, Xs, _ = c3
+@pytest.mark.parametrize("prec", ["split_f32", "split_i8"])
+def test_c3_split_per_block_errors(c3, prec):
+    w, Xf, Xsplit, _ = c3
+    Xs = Xsplit[prec]
     rows = []
     for m in range(2):
         d = float((Xf.monomer_matrix(m) - Xs.monomer_matrix(m)).abs().max())
@@ -115,19 +121,21 @@ def test_c3_split_per_block_errors(c3):
     for q, bf, bs in zip(Pf["segq"], Pf["blocks"], Ps["blocks"]):
         rows.append((f"pair block Q={q} ({bf.shape[0]}x{bf.shape[0]})", float((bf - bs).abs().max())))
     for name, d in rows:
-        print(f"c3 split-FP32 vs FP64: {name}: max |diff| = {d:.3e}", flush=True)
+        print(f"c3 {prec} vs FP64: {name}: max |diff| = {d:.3e}", flush=True)
     assert len(rows) == 2 + 5
     assert max(d for _, d in rows) <= 1e-9
 
 
-def test_c3_lowest_10_eigenvalues_split_vs_f64(c3):
-    w, Xf, Xs, _ = c3
+@pytest.mark.parametrize("prec", ["split_f32", "split_i8"])
+def test_c3_lowest_10_eigenvalues_split_vs_f64(c3, prec):
+    w, Xf, Xsplit, _ = c3
+    Xs = Xsplit[prec]
     of, os_ = Xf.operator(), Xs.operator()
     wf, vf = of.lowest(k=10, tol=1e-10)
     ws, _ = os_.lowest(k=10, tol=1e-10)
     dE = (wf - ws).abs()
     print("c3 lowest 10 (FP64):", [f"{float(x):.12f}" for x in wf], flush=True)
-    print("c3 |dE| split vs FP64:", [f"{float(x):.2e}" for x in dE], flush=True)
+    print(f"c3 |dE| {prec} vs FP64:", [f"{float(x):.2e}" for x in dE], flush=True)
     assert wf.numel() == 10 and ws.numel() == 10
     assert float(dE.max()) <= E_TOL
     # each FP64 root is an eigenpair of the operator
@@ -135,3 +143,25 @@ def test_c3_lowest_10_eigenvalues_split_vs_f64(c3):
     of.matvec(vf.contiguous(), R)
     res = (R - vf * wf[None, :]).norm(dim=0)
     assert float(res.max()) <= 1e-8 * max(1.0, float(wf.abs().max()))
+
+
+def test_c3_split_i8_runs_int8_products(c3):
+    """The split-int8 system's pair blocks launch the int8 GEMM (the c3
+    two-electron products are compute bound) and give the same blocks as a
+    rebuild."""
+    from qodeapplications_amd import _lib
+    w, Xf, Xsplit, _ = c3
+    Xi = Xsplit["split_i8"]
+    ctx = _lib.context(0)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    H2 = Xi.pair_matrix(0, 1)
+    torch.cuda.synchronize()
+    n, ms, fl, _ = ctx.profile_stats(_lib.K_GEMM_SPLIT_I8)
+    ctx.profile(False)
+    print(f"c3 split_i8 pair matrix: {n} int8 GEMM launches, {fl / 1e12:.2f} TFLOP at "
+          f"{fl / max(ms, 1e-9) / 1e9:.1f} TFLOP/s", flush=True)
+    assert n > 0 and fl > 0
+    ref = Xf.pair_matrix(0, 1)
+    assert float((H2 - ref).abs().max()) <= 1e-10 * max(1.0, float(ref.abs().max()))
+    del H2, ref
