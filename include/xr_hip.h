@@ -257,6 +257,16 @@ int xr_gemm_split_i8(xr_ctx *ctx, int transA, int transB, int64_t M, int64_t N, 
                      const double *d_B, int64_t ldb, int64_t strideB,
                      double beta, double *d_C, int64_t ldc, int64_t strideC, int64_t batch);
 
+/* The product as the two-electron contractions run it in precision `mode`
+ * (XR_PREC_*): FP64; split-FP32; or in XR_PREC_SPLIT_I8 the split-int8
+ * product where it pays (at least 2^33 flops, every dimension >= 128 and
+ * M or N >= 1024) and FP64 elsewhere.  Same other arguments as xr_gemm_f64.
+ * Used by the adjoints of those contractions.  No reference counterpart. */
+int xr_gemm_mode(xr_ctx *ctx, int mode, int transA, int transB, int64_t M, int64_t N, int64_t K,
+                 double alpha, const double *d_A, int64_t lda, int64_t strideA,
+                 const double *d_B, int64_t ldb, int64_t strideB,
+                 double beta, double *d_C, int64_t ldc, int64_t strideC, int64_t batch);
+
 /* Selects FP64 (XR_PREC_F64, default), split-FP32 (XR_PREC_SPLIT_F32) or
  * split-int8 (XR_PREC_SPLIT_I8) GEMMs for the two-electron contractions (xr_monomer_2e, xr_dimer_2min2pls,
  * xr_dimer_1min1pls_2e, xr_dimer_ExEx, xr_trimer_*).  Densities, one-electron

@@ -90,9 +90,22 @@ __global__ __launch_bounds__(256) void k_i8_rowmax(I8Operand a, I8Operand b, int
     }
 }
 
+// Row exponents E_r = max of the row's partials, for both operands.
+__global__ __launch_bounds__(256) void k_i8_rowexp(I8Operand a, I8Operand b, int64_t batch) {
+    const int64_t nA = batch * a.Rp;
+    XR_GRID_STRIDE(i, nA + batch * b.Rp) {
+        const bool isA = i < nA;
+        const I8Operand& o = isA ? a : b;
+        const int64_t j = isA ? i : i - nA, bt = j / o.Rp, r = j - bt * o.Rp;
+        const int* P = o.P + bt * o.nke * o.Rp + r;
+        int e = 0;
+        for (int64_t q = 0; q < o.nke; q++) e = max(e, P[q * o.Rp]);
+        o.E[j] = e;
+    }
+}
+
 // Digit planes of 64 x 64 tiles of both operands' padded Rp x Kp layouts
-// (zeros outside R x K).  Row exponent E_r = max of the row's partials; the
-// tiles at k = 0 also store it for the product's epilogue.
+// (zeros outside R x K), on the rows' exponents.
 __global__ __launch_bounds__(256) void k_i8_slice(I8Operand a, I8Operand b, int64_t batch) {
     __shared__ double T[64][65];
     __shared__ int Es[64];
@@ -113,13 +126,7 @@ __global__ __launch_bounds__(256) void k_i8_slice(I8Operand a, I8Operand b, int6
             const int64_t gr = r0 + r, gk = k0 + k;
             T[r][k] = (gr < o.R && gk < o.K) ? X[o.kc ? gr * o.ld + gk : gk * o.ld + gr] : 0.0;
         }
-        if (tid < 64) {
-            const int* P = o.P + bt * o.nke * o.Rp + r0 + tid;
-            int e = 0;
-            for (int64_t q = 0; q < o.nke; q++) e = max(e, P[q * o.Rp]);
-            Es[tid] = e;
-            if (kb == 0) o.E[bt * o.Rp + r0 + tid] = e;
-        }
+        if (tid < 64) Es[tid] = o.E[bt * o.Rp + r0 + tid];
         __syncthreads();
         const int r = tid >> 2, g = tid & 3;
         const int sh = 1028 - Es[r];
@@ -248,7 +255,7 @@ __global__ __launch_bounds__(256, 2) void k_i8gemm(
 
 enum { WS_I8_DA = 80, WS_I8_DB, WS_I8_E, WS_I8_C };
 
-// Split-int8 product: digit planes of both operands (two launches), the
+// Split-int8 product: digit planes of both operands (three launches), the
 // int8 GEMM (K split toward a full chip and into chunks of at most
 // XR_I8_KMAX), and the fixed-order reduction when the product is split or
 // accumulates (beta).
@@ -284,8 +291,11 @@ int xr_dgemm_i8_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t 
     const I8Operand ob{B, ldb, sB, N, K, RpB, Kp, nke, tb != 0, PB, EB, DB};
     {
         ProfScope prof(c, XR_K_I8_SLICE, 0.0, 8.0 * (M * K + K * N) * batch * 2 + (double)XR_I8_S * (RpA + RpB) * Kp * batch);
+        prof.shape(M, N, K, batch, ta, tb);
         const int64_t ne = batch * (RpA / 64 + RpB / 64) * nke, ns = batch * (RpA / 64 + RpB / 64) * (Kp / 64);
         k_i8_rowmax<<<(unsigned)std::min<int64_t>(ne, 1 << 20), 256, 0, c->stream>>>(oa, ob, batch);
+        XR_LAUNCH_CHECK(c);
+        k_i8_rowexp<<<xr_grid(batch * (RpA + RpB)), 256, 0, c->stream>>>(oa, ob, batch);
         XR_LAUNCH_CHECK(c);
         k_i8_slice<<<(unsigned)std::min<int64_t>(ns, 1 << 20), 256, 0, c->stream>>>(oa, ob, batch);
         XR_LAUNCH_CHECK(c);
@@ -307,15 +317,16 @@ int xr_dgemm_i8_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t 
     return splitk_reduce(c, Cw, splits, batch, M, N, alpha, beta, C, ldc, sC);
 }
 
-// Whether a product of this shape gains from the split-int8 path: it must
-// be compute bound in FP64 with room for the digit-plane passes (harmonic
-// mean of M, N, K of at least 96) and large enough that their two extra
-// launches do not dominate (at least 2^31 flops); other products of the
-// split-int8 mode run on the FP64 matrix cores (exact, and no slower).
+// Whether a product of this shape gains from the split-int8 path (measured
+// on the XR shapes): at least 2^33 flops, so the three digit-plane launches
+// and passes stay small beside it; every dimension at least 128 (narrower
+// ones leave the 64-wide tiles and the digit passes mostly idle); and M or
+// N at least 1024 (a small output over a long K is bound by re-reading the
+// 7 digit planes through L2, where the FP64 kernel's larger tiles win).
+// Other products of the split-int8 mode run on the FP64 matrix cores.
 bool xr_i8_pays(int64_t M, int64_t N, int64_t K, int64_t batch) {
-    if (M <= 0 || N <= 0 || K <= 0) return false;
-    const double h = 1.0 / (1.0 / M + 1.0 / N + 1.0 / K);
-    return h >= 96.0 && 2.0 * M * N * K * batch >= 2147483648.0;
+    if (M < 128 || N < 128 || K < 128) return false;
+    return (M >= 1024 || N >= 1024) && 2.0 * M * N * K * batch >= 8589934592.0;
 }
 
 extern "C" int xr_gemm_split_i8(xr_ctx* c, int transA, int transB, int64_t M, int64_t N, int64_t K,
@@ -326,5 +337,25 @@ extern "C" int xr_gemm_split_i8(xr_ctx* c, int transA, int transB, int64_t M, in
     std::lock_guard<std::mutex> g(c->mu);
     if (M < 0 || N < 0 || K < 0 || batch < 0) return xr_fail(c, XR_EINVAL, "negative GEMM size");
     if (ldc < N) return xr_fail(c, XR_ESHAPE, "ldc < N");
+    if ((transA ? lda < M : lda < K) && M > 0 && K > 0) return xr_fail(c, XR_ESHAPE, "lda too small");
+    if ((transB ? ldb < K : ldb < N) && N > 0 && K > 0) return xr_fail(c, XR_ESHAPE, "ldb too small");
     return xr_dgemm_i8_launch(c, transA, transB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+}
+
+extern "C" int xr_gemm_mode(xr_ctx* c, int mode, int transA, int transB, int64_t M, int64_t N, int64_t K,
+                            double alpha, const double* A, int64_t lda, int64_t sA, const double* B, int64_t ldb,
+                            int64_t sB, double beta, double* C, int64_t ldc, int64_t sC, int64_t batch) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (M < 0 || N < 0 || K < 0 || batch < 0) return xr_fail(c, XR_EINVAL, "negative GEMM size");
+    if (ldc < N) return xr_fail(c, XR_ESHAPE, "ldc < N");
+    if ((transA ? lda < M : lda < K) && M > 0 && K > 0) return xr_fail(c, XR_ESHAPE, "lda too small");
+    if ((transB ? ldb < K : ldb < N) && N > 0 && K > 0) return xr_fail(c, XR_ESHAPE, "ldb too small");
+    if (mode != XR_PREC_F64 && mode != XR_PREC_SPLIT_F32 && mode != XR_PREC_SPLIT_I8)
+        return xr_fail(c, XR_EINVAL, "unknown precision mode");
+    const int saved = c->precision;
+    c->precision = mode;
+    const int rc = xr_gemm_2e(c, transA, transB, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+    c->precision = saved;
+    return rc;
 }

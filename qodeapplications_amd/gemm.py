@@ -29,15 +29,23 @@ def gemm_f64(A, B, C, transA=0, transB=0, alpha=1.0, beta=0.0):
     return C
 
 
-def mm(A, B, transA=0, transB=0, alpha=1.0, out=None, beta=0.0, split=False):
+MODES = {"f64": 0, "split_f32": 1, "split_i8": 2}
+
+
+def mm(A, B, transA=0, transB=0, alpha=1.0, out=None, beta=0.0, split=False, mode=None):
     """out = alpha op(A) op(B) + beta out for 2-D float64 CUDA tensors that
     are row-major with any leading dimension (views of wider buffers work
     without copies).  transA = 1 uses A^T, transB = 1 uses B^T; split=True
-    (or "f32") runs the split-FP32 product, split="i8" the split-int8 one."""
+    (or "f32") runs the split-FP32 product, split="i8" the split-int8 one.
+    mode ('f64', 'split_f32', 'split_i8'), when given, runs the product as
+    the two-electron contractions do in that precision mode (xr_gemm_mode:
+    split-int8 only where it pays)."""
     entry = {False: "xr_gemm_f64", True: "xr_gemm_split_f32", "f32": "xr_gemm_split_f32",
              "i8": "xr_gemm_split_i8"}.get(split)
     if entry is None:
         raise ValueError("split must be False, True, 'f32' or 'i8'")
+    if mode is not None and mode not in MODES:
+        raise ValueError(f"mode must be one of {sorted(MODES)}")
     for t in (A, B):
         if t.dim() != 2 or t.dtype != torch.float64 or not t.is_cuda:
             raise ValueError("mm needs 2-D float64 CUDA tensors")
@@ -56,6 +64,10 @@ def mm(A, B, transA=0, transB=0, alpha=1.0, out=None, beta=0.0, split=False):
     if out.stride(1) != 1 or tuple(out.shape) != (M, N):
         raise ValueError("out must be row-major (M, N)")
     ctx = context(A.device.index)
+    if mode is not None:
+        ctx.call("xr_gemm_mode", MODES[mode], int(transA), int(transB), M, N, K, float(alpha), ptr(A), A.stride(0),
+                 0, ptr(B), B.stride(0), 0, float(beta), ptr(out), out.stride(0), 0, 1)
+        return out
     ctx.call(entry, int(transA), int(transB), M, N, K,
              float(alpha), ptr(A), A.stride(0), 0, ptr(B), B.stride(0), 0, float(beta), ptr(out),
              out.stride(0), 0, 1)

@@ -36,36 +36,37 @@ from .gemm import mm as _mm
 from .xr import precision
 
 
-def mm(A, Bm, transA=0, transB=0, alpha=1.0, out=None, beta=0.0, split=False):
+def mm(A, Bm, transA=0, transB=0, alpha=1.0, out=None, beta=0.0, split=False, mode=None):
     """2-D GEMM on libxrhip: out = alpha op(A) op(B) + beta out (split-FP32
-    when split is set)."""
-    return _mm(A.contiguous(), Bm.contiguous(), transA, transB, alpha, out, beta, split)
+    when split is set; as the two-electron contractions run in precision
+    `mode` when that is given)."""
+    return _mm(A.contiguous(), Bm.contiguous(), transA, transB, alpha, out, beta, split, mode)
 
 
-def bilinear_adjoint(G, A, W, Bm, s, Abar=None, Bbar=None, Wbar=None, split=False):
+def bilinear_adjoint(G, A, W, Bm, s, Abar=None, Bbar=None, Wbar=None, split="f64"):
     """For H = s A W B^T (A: nA x p, W: p x q, B: nB x q) and G = dE/dH
     (nA x nB): Abar += s G B W^T, Bbar += s G^T A W, Wbar += s A^T G B.
-    Each triple product is evaluated in the cheaper association; split
-    ('f32' or 'i8') runs the products on that split path (two-electron terms
-    in the split modes)."""
+    Each triple product is evaluated in the cheaper association; split is
+    the precision mode the products run in ('f64', 'split_f32', 'split_i8':
+    two-electron terms follow the system's mode)."""
     nA, p = A.shape
     nB, q = Bm.shape
     x = split
     if Abar is not None:
         if nA * nB * q + nA * q * p <= nB * q * p + nA * nB * p:
-            mm(mm(G, Bm, split=x), W, transB=1, alpha=s, out=Abar, beta=1.0, split=x)   # (G B) W^T
+            mm(mm(G, Bm, mode=x), W, transB=1, alpha=s, out=Abar, beta=1.0, mode=x)   # (G B) W^T
         else:
-            mm(G, mm(Bm, W, transB=1, split=x), alpha=s, out=Abar, beta=1.0, split=x)   # G (B W^T)
+            mm(G, mm(Bm, W, transB=1, mode=x), alpha=s, out=Abar, beta=1.0, mode=x)   # G (B W^T)
     if Bbar is not None:
         if nB * nA * p + nB * p * q <= nA * p * q + nB * nA * q:
-            mm(mm(G, A, transA=1, split=x), W, alpha=s, out=Bbar, beta=1.0, split=x)    # (G^T A) W
+            mm(mm(G, A, transA=1, mode=x), W, alpha=s, out=Bbar, beta=1.0, mode=x)    # (G^T A) W
         else:
-            mm(G, mm(A, W, split=x), transA=1, alpha=s, out=Bbar, beta=1.0, split=x)    # G^T (A W)
+            mm(G, mm(A, W, mode=x), transA=1, alpha=s, out=Bbar, beta=1.0, mode=x)    # G^T (A W)
     if Wbar is not None:
         if p * nA * nB + p * nB * q <= nA * nB * q + p * nA * q:
-            mm(mm(A, G, transA=1, split=x), Bm, alpha=s, out=Wbar, beta=1.0, split=x)   # (A^T G) B
+            mm(mm(A, G, transA=1, mode=x), Bm, alpha=s, out=Wbar, beta=1.0, mode=x)   # (A^T G) B
         else:
-            mm(A, mm(G, Bm, split=x), transA=1, alpha=s, out=Wbar, beta=1.0, split=x)   # A^T (G B)
+            mm(A, mm(G, Bm, mode=x), transA=1, alpha=s, out=Wbar, beta=1.0, mode=x)   # A^T (G B)
 
 
 class _Acc:
@@ -216,7 +217,7 @@ class XRAdjoint:
         X = self.X
         F1, F2 = X.frags[m1], X.frags[m2]
         (i1c, j1c), (i2c, j2c) = k1, k2
-        sp = {"split_f32": "f32", "split_i8": "i8"}.get(X.precision, False)
+        sp = X.precision
         nI1, nJ1, nI2, nJ2 = F1.n_states[i1c], F1.n_states[j1c], F2.n_states[i2c], F2.n_states[j2c]
         nA, nB = nI1 * nJ1, nI2 * nJ2
         d = i1c - j1c

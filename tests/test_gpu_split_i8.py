@@ -131,6 +131,27 @@ def test_split_i8_batched_strides():
         assert bool((err <= _bound(A[b], B[b].T, 2.0) + 2.0 ** -51 * ref[b].abs()).all()), b
 
 
+def test_mode_routing():
+    """xr_gemm_mode in split-int8 mode: a small product (below 2^33 flops)
+    runs the FP64 kernel (bit-identical to xr_gemm_f64), a large one the
+    int8 kernel (bit-identical to xr_gemm_split_i8); split-FP32 mode is the
+    split-FP32 product."""
+    g = torch.Generator().manual_seed(17)
+    for (M, N, K) in [(300, 260, 700), (2048, 2048, 1100)]:
+        A = torch.randn(M, K, dtype=torch.float64, generator=g).cuda()
+        B = torch.randn(K, N, dtype=torch.float64, generator=g).cuda()
+        routed = mm(A, B, mode="split_i8")
+        f64 = mm(A, B)
+        i8 = mm(A, B, split="i8")
+        f32 = mm(A, B, split="f32")
+        torch.cuda.synchronize()
+        big = 2.0 * M * N * K >= 2.0 ** 33
+        assert torch.equal(routed, i8 if big else f64), (M, N, K)
+        assert not torch.equal(i8, f64)
+        assert torch.equal(mm(A, B, mode="split_f32"), f32)
+        assert torch.equal(mm(A, B, mode="f64"), f64)
+
+
 def test_split_i8_rejects_bad_sizes():
     ctx = context(0)
     L = ctx.L
