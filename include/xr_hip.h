@@ -257,6 +257,40 @@ int xr_gemm_split_i8(xr_ctx *ctx, int transA, int transB, int64_t M, int64_t N, 
                      const double *d_B, int64_t ldb, int64_t strideB,
                      double beta, double *d_C, int64_t ldc, int64_t strideC, int64_t batch);
 
+/* Vector steps of the block-Davidson solver behind xr_operator's lowest
+ * roots (the matrix-free counterpart of general-XRCC/excitonic.py:56-82,
+ * numpy.linalg.eigh of the assembled XR matrix), each one pass over an
+ * n x b row-major block (b <= 32 for the column reductions):
+ *   xr_precond_divide: Y[i,j] = X[i,j] / d, d = sign (lam[i] - theta[j]),
+ *       d := +floor where |d| < floor (X == Y allowed);
+ *   xr_residual: R = AX - X diag(theta) and norms[j] = ||R[:, j]|| for j < k;
+ *   xr_column_norms: out[j] = sum_i A[i,j]^2 (squared != 0) or its root;
+ *   xr_transpose: B[j*ldb + i] = A[i*lda + j] (rows x cols).
+ * The column sums are fixed-order (bitwise reproducible).  No reference
+ * counterpart (numpy does these steps inside eigh). */
+int xr_precond_divide(xr_ctx *ctx, int64_t n, int64_t b, const double *d_X, int64_t ldx, double *d_Y,
+                      int64_t ldy, const double *d_lam, const double *d_theta, double sign, double floor_);
+int xr_residual(xr_ctx *ctx, int64_t n, int b, int k, const double *d_X, int64_t ldx, const double *d_AX,
+                int64_t ldax, const double *d_theta, double *d_R, int64_t ldr, double *d_norms);
+int xr_column_norms(xr_ctx *ctx, int64_t n, int b, const double *d_A, int64_t lda, double *d_out, int squared);
+int xr_transpose(xr_ctx *ctx, int64_t rows, int64_t cols, const double *d_A, int64_t lda, double *d_B,
+                 int64_t ldb);
+
+/* Spin-orbital integral blocks cut from spatial integrals (alpha spin
+ * orbitals of a fragment first), one block per list entry: kinds[i] = 1, a
+ * one-body block out[(s,p),(s',q)] = d(s,s') A[o0+p, o1+q] of the n x n
+ * matrix src[i] (dims[4i..4i+1] = n0, n1, offs likewise); kinds[i] = 2, a
+ * two-body block out[(s0,p),(s1,q),(s2,r),(s3,s)] = 1/4 (d(s0,s2) d(s1,s3)
+ * V[o0+p,o2+r,o1+q,o3+s] - d(s0,s3) d(s1,s2) V[o0+p,o3+s,o1+q,o2+r]) of the
+ * n^4 chemist-notation tensor src[i].  Replaces the spin-orbital integral
+ * assembly of hermitian-XRCC/get_ints.py:62-155 (orbital_solver.py:323-437)
+ * for the fragment blocks the contractions read.  The adjoint adds every
+ * block gradient grad[i] (null: none) into dsrc[i] in list order. */
+int xr_spin_blocks(xr_ctx *ctx, int n_keys, const int *kinds, const int *dims, const int *offs,
+                   const double *const *d_src, double *const *d_dst, int64_t n);
+int xr_spin_blocks_adjoint(xr_ctx *ctx, int n_keys, const int *kinds, const int *dims, const int *offs,
+                           const double *const *d_grad, double *const *d_dsrc, int64_t n);
+
 /* The product as the two-electron contractions run it in precision `mode`
  * (XR_PREC_*): FP64; split-FP32; or in XR_PREC_SPLIT_I8 the split-int8
  * product where it pays (at least 2^33 flops, every dimension >= 128 and

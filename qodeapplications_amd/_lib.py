@@ -62,6 +62,12 @@ SIGNATURES = {
     "xr_gemm_split_f32": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
     "xr_gemm_split_i8": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
     "xr_gemm_mode": (_I, [_P, _I, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
+    "xr_precond_divide": (_I, [_P, _L, _L, _P, _L, _P, _L, _P, _P, _D, _D]),
+    "xr_residual": (_I, [_P, _L, _I, _I, _P, _L, _P, _L, _P, _P, _L, _P]),
+    "xr_column_norms": (_I, [_P, _L, _I, _P, _L, _P, _I]),
+    "xr_transpose": (_I, [_P, _L, _L, _P, _L, _P, _L]),
+    "xr_spin_blocks": (_I, [_P, _I, _P, _P, _P, _P, _P, _L]),
+    "xr_spin_blocks_adjoint": (_I, [_P, _I, _P, _P, _P, _P, _P, _L]),
     "xr_gemm_f64_noalloc": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
     "xr_set_precision": (_I, [_P, _I]),
     "xr_monomer_1e": (_I, [_P, _L, _L, _P, _P, _P]),

@@ -7,3 +7,5 @@
 #include "xr_density.hip"
 #include "xr_contract.hip"
 #include "xr_pairs.hip"
+#include "xr_solver.hip"
+#include "xr_spin.hip"

@@ -10,9 +10,11 @@ product states of H0.
 import numpy as np
 import torch
 
+from ._lib import context, ptr
 from .gemm import gemm_f64, mm
 
 DENSE_MAX = 6144
+PRECOND_FLOOR = 1e-8     # |denominator| floor of the diagonal preconditioners
 
 
 def upload(a, device):
@@ -23,6 +25,49 @@ def upload(a, device):
     if t.dtype == torch.float32:
         t = t.double()
     return t.pin_memory().to(device, non_blocking=True)
+
+
+def precond_divide(X, lam, theta, sign=1.0, out=None):
+    """out[i, j] = X[i, j] / d, d = sign (lam[i] - theta[j]) with |d| floored
+    at PRECOND_FLOOR (to +PRECOND_FLOOR), one library pass (xr_precond_divide);
+    out may be X."""
+    if out is None:
+        out = torch.empty_like(X)
+    n, b = X.shape
+    context(X.device.index).call("xr_precond_divide", n, b, ptr(X), X.stride(0), ptr(out), out.stride(0),
+                                 ptr(lam), ptr(theta.contiguous()), float(sign), PRECOND_FLOOR)
+    return out
+
+
+def residual(X, AX, theta, k):
+    """(R, norms): R = AX - X diag(theta) and the norms of R's first k
+    columns, one library pass (xr_residual; fixed-order column sums)."""
+    n, b = X.shape
+    R = torch.empty(n, b, dtype=X.dtype, device=X.device)
+    norms = torch.empty(k, dtype=X.dtype, device=X.device)
+    context(X.device.index).call("xr_residual", n, b, k, ptr(X), X.stride(0), ptr(AX), AX.stride(0),
+                                 ptr(theta.contiguous()), ptr(R), b, ptr(norms))
+    return R, norms
+
+
+def column_sumsq(A):
+    """sum_i A[i, j]^2 per column, one library pass (xr_column_norms)."""
+    n, b = A.shape
+    out = torch.empty(b, dtype=A.dtype, device=A.device)
+    context(A.device.index).call("xr_column_norms", n, b, ptr(A), A.stride(0), ptr(out), 1)
+    return out
+
+
+def store_rows(dst, A):
+    """dst (b x n, row stride dst.stride(0)) = A^T for A (n x b): a block of
+    vectors written into a basis kept one vector per row (xr_transpose)."""
+    n, b = A.shape
+    context(A.device.index).call("xr_transpose", n, b, ptr(A), A.stride(0), ptr(dst), dst.stride(0))
+
+
+def _narrow(b):
+    """The library's column-reduction kernels take blocks of at most 32."""
+    return b <= 32
 
 
 class KroneckerSumPreconditioner:
@@ -72,9 +117,7 @@ class KroneckerSumPreconditioner:
 
     def __call__(self, R, theta):
         Y = self._to_eig(R, True)
-        den = self.lam[:, None] - theta[None, :]
-        den = torch.where(den.abs() < 1e-8, torch.full_like(den, 1e-8), den)
-        return self._to_eig(Y / den, False)
+        return self._to_eig(precond_divide(Y, self.lam, theta, out=Y), False)
 
     def guess(self, b):
         idx = torch.argsort(self.lam)[:b]
@@ -115,7 +158,7 @@ def _orthonormalize(T, V=None, passes=2, Vt=None, extra=None):
         n0_t = None
         if (V is not None and V.shape[1]) or (Vt is not None and Vt.shape[0]):
             if p == 0:
-                n0_t = (T * T).sum(0)
+                n0_t = column_sumsq(T) if _narrow(T.shape[1]) and T.stride(1) == 1 else (T * T).sum(0)
             if Vt is not None:          # basis stored one vector per row
                 mm(Vt, mm(Vt, T), transA=1, alpha=-1.0, out=T, beta=1.0)
             else:
@@ -204,10 +247,10 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         AVt = torch.empty(max_space + b, n, dtype=dt, device=device)
         V0 = _orthonormalize(V0.contiguous())
         m = V0.shape[1]
-        Vt[:m] = V0.T
+        store_rows(Vt[:m], V0)
         AX0 = torch.empty(n, m, dtype=dt, device=device)
         matmul(V0, AX0)
-        AVt[:m] = AX0.T
+        store_rows(AVt[:m], AX0)
         S = mm(V0, AX0, transA=1).cpu().numpy()
 
     def keep_state():
@@ -229,11 +272,14 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         theta = upload(w[:nb], device)
         X = mm(Vt[:m], yb, transA=1)
         AX = mm(AVt[:m], yb, transA=1)
-        R = AX - X * theta
         # the residual norms reach the host with the orthonormalisation's
         # first transfer (one synchronisation fewer per iteration); the
         # correction computed meanwhile is dropped when they show convergence
-        rn_dev = torch.linalg.norm(R[:, :k], dim=0)
+        if _narrow(nb):
+            R, rn_dev = residual(X, AX, theta, min(k, nb))
+        else:
+            R = AX - X * theta
+            rn_dev = torch.linalg.norm(R[:, :k], dim=0)
         target = tol * np.maximum(np.abs(w[:k]), 1.0)
         if precond is not None:
             T = precond(R, theta)
@@ -244,15 +290,14 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
                 eps = (X * T).sum(0) / (X * PX).sum(0)
                 T = T - PX * eps[None, :]
         else:
-            T = R / torch.where((theta[None, :] - diag[:, None]).abs() < 1e-8, torch.full_like(R, 1e-8),
-                                theta[None, :] - diag[:, None])
+            T = precond_divide(R, diag, theta, sign=-1.0)     # R / (theta - diag), floored
         restart = m + nb > max_space
         if restart:                         # thick restart on the lowest Ritz vectors
             nk = min(m, max(2 * nb, keep))
             yk = upload(y[:, :nk], device)
             Xk, AXk = mm(Vt[:m], yk, transA=1), mm(AVt[:m], yk, transA=1)
-            Vt[:nk] = Xk.T
-            AVt[:nk] = AXk.T
+            store_rows(Vt[:nk], Xk)
+            store_rows(AVt[:nk], AXk)
             S = np.diag(w[:nk])
             m = nk
         Q, rn = _orthonormalize(T.contiguous(), Vt=Vt[:m], extra=rn_dev)
@@ -277,8 +322,8 @@ def davidson(matmul, diag, n, k=1, tol=1e-9, max_iter=400, block=None, device=No
         matmul(Q, AQ)
         # grow the projected matrix by the new rows / columns: one product
         # over the extended basis gives V^T A Q and Q^T A Q in one transfer
-        Vt[m:m + q] = Q.T
-        AVt[m:m + q] = AQ.T
+        store_rows(Vt[m:m + q], Q)
+        store_rows(AVt[m:m + q], AQ)
         VA = mm(Vt[:m + q], AQ).cpu().numpy()
         S = np.block([[S, VA[:m]], [VA[:m].T, VA[m:]]])
         m += q

@@ -43,12 +43,13 @@ def mm(A, Bm, transA=0, transB=0, alpha=1.0, out=None, beta=0.0, split=False, mo
     return _mm(A.contiguous(), Bm.contiguous(), transA, transB, alpha, out, beta, split, mode)
 
 
-def bilinear_adjoint(G, A, W, Bm, s, Abar=None, Bbar=None, Wbar=None, split="f64"):
+def bilinear_adjoint(G, A, W, Bm, s, Abar=None, Bbar=None, Wbar=None, split="f64", wbar_beta=1.0):
     """For H = s A W B^T (A: nA x p, W: p x q, B: nB x q) and G = dE/dH
     (nA x nB): Abar += s G B W^T, Bbar += s G^T A W, Wbar += s A^T G B.
     Each triple product is evaluated in the cheaper association; split is
     the precision mode the products run in ('f64', 'split_f32', 'split_i8':
-    two-electron terms follow the system's mode)."""
+    two-electron terms follow the system's mode).  wbar_beta = 0 overwrites
+    Wbar instead of adding to it (a fresh accumulator needs no zero fill)."""
     nA, p = A.shape
     nB, q = Bm.shape
     x = split
@@ -64,9 +65,9 @@ def bilinear_adjoint(G, A, W, Bm, s, Abar=None, Bbar=None, Wbar=None, split="f64
             mm(G, mm(A, W, mode=x), transA=1, alpha=s, out=Bbar, beta=1.0, mode=x)    # G^T (A W)
     if Wbar is not None:
         if p * nA * nB + p * nB * q <= nA * nB * q + p * nA * q:
-            mm(mm(A, G, transA=1, mode=x), Bm, alpha=s, out=Wbar, beta=1.0, mode=x)   # (A^T G) B
+            mm(mm(A, G, transA=1, mode=x), Bm, alpha=s, out=Wbar, beta=wbar_beta, mode=x)   # (A^T G) B
         else:
-            mm(A, mm(G, Bm, mode=x), transA=1, alpha=s, out=Wbar, beta=1.0, mode=x)   # A^T (G B)
+            mm(A, mm(G, Bm, mode=x), transA=1, alpha=s, out=Wbar, beta=wbar_beta, mode=x)   # A^T (G B)
 
 
 class _Acc:
@@ -238,18 +239,18 @@ class XRAdjoint:
             mapped back through the same permutation."""
             Vt = X.ints.V(*key)
             W = Vt.permute(*perm).contiguous().view(*shape2)
-            Wb = torch.zeros_like(W)
+            Wb = torch.empty_like(W)          # written whole by its one product (wbar_beta = 0)
             return W, Wb, lambda: ib("V", *key).add_(Wb.view([Vt.shape[i] for i in perm]).permute(*_inv(perm)))
 
         if d in (-2, +2):
             if d == -2:
                 W, Wb, fin = vperm((m1, m1, m2, m2), (0, 1, 3, 2), (n1 * n1, n2 * n2))
                 bilinear_adjoint(G(False), _flat(R1("cc")), W, _flat(R2("aa")), 1.0,
-                                 R1b("cc"), R2b("aa"), Wb, split=sp)
+                                 R1b("cc"), R2b("aa"), Wb, split=sp, wbar_beta=0.0)
             else:
                 W, Wb, fin = vperm((m2, m2, m1, m1), (0, 1, 3, 2), (n2 * n2, n1 * n1))
                 bilinear_adjoint(G(True), _flat(R2("cc")), W, _flat(R1("aa")), 1.0,
-                                 R2b("cc"), R1b("aa"), Wb, split=sp)
+                                 R2b("cc"), R1b("aa"), Wb, split=sp, wbar_beta=0.0)
             fin()
             return
         if d in (-1, +1):
@@ -269,22 +270,23 @@ class XRAdjoint:
             Rc, Rcb = cre[0], cre[1]
             Ra, Rab = ann[0], ann[1]
             h = X.ints.h_sum(hkey, parts)
-            hb = torch.zeros_like(h)
-            bilinear_adjoint(Gm, _flat(Rc("c")), h, _flat(Ra("a")), s, Rcb("c"), Rab("a"), hb)
+            hb = torch.empty_like(h)
+            bilinear_adjoint(Gm, _flat(Rc("c")), h, _flat(Ra("a")), s, Rcb("c"), Rab("a"), hb, wbar_beta=0.0)
             h_bar(parts, hb)
             W1, W1b, fin1 = vperm(k11, (1, 0, 2, 3), (nc ** 3, na))
             bilinear_adjoint(Gm, _flat(Rc("cca")), W1, _flat(Ra("a")), 2.0 * s, Rcb("cca"), Rab("a"), W1b,
-                             split=sp)
+                             split=sp, wbar_beta=0.0)
             fin1()
             W2, W2b, fin2 = vperm(k22, (0, 1, 3, 2), (nc, na ** 3))
             bilinear_adjoint(Gm, _flat(Rc("c")), W2, _flat(Ra("caa")), 2.0 * s, Rcb("c"), Rab("caa"), W2b,
-                             split=sp)
+                             split=sp, wbar_beta=0.0)
             fin2()
             return
         # d == 0
         Gm = G(False)
         W, Wb, fin = vperm((m1, m2, m1, m2), (0, 2, 1, 3), (n1 * n1, n2 * n2))
-        bilinear_adjoint(Gm, _flat(R1("ca")), W, _flat(R2("ca")), 4.0, R1b("ca"), R2b("ca"), Wb, split=sp)
+        bilinear_adjoint(Gm, _flat(R1("ca")), W, _flat(R2("ca")), 4.0, R1b("ca"), R2b("ca"), Wb, split=sp,
+                         wbar_beta=0.0)
         fin()
         # extPot terms: e2[b] on I1 == J1 rows, e1[a] on I2 == J2 columns
         dia1 = torch.zeros(nA, 1, dtype=torch.float64, device=X.device)

@@ -11,8 +11,6 @@ update of StateSpaceOptimizer/orbital_solver.py:420-437 to first order
 antisymmetric generator.  Everything is torch on the device, so autograd
 carries the integral-block adjoints of the XR energy back to kappa.
 """
-import itertools
-
 import numpy as np
 import torch
 
@@ -85,54 +83,88 @@ class OrbitalRotation:
         U = None if self.U is None else torch.stack([transform(u, R) for u in self.U])
         return h, V, U
 
-    def _sl(self, m):
-        return slice(int(self.off[m]), int(self.off[m + 1]))
-
-    def one_body(self, A, m1, m2):
-        n1, n2 = self.frag_orbs[m1], self.frag_orbs[m2]
-        blk = A[self._sl(m1), self._sl(m2)]
-        z = torch.zeros(n1, n2, dtype=torch.float64, device=self.device)
-        return torch.cat([torch.cat([blk, z], 1), torch.cat([z, blk], 1)], 0)
-
-    def two_body(self, V, ms):
-        n = [self.frag_orbs[m] for m in ms]
-        s = [self._sl(m) for m in ms]
-        C = V[s[0], s[2], s[1], s[3]].permute(0, 2, 1, 3)
-        X = V[s[0], s[3], s[1], s[2]].permute(0, 2, 3, 1)
-        zero = torch.zeros(n[0], n[1], n[2], n[3], dtype=torch.float64, device=self.device)
-        cube = {}
-        for sp in itertools.product((0, 1), repeat=4):
-            blk = zero
-            if sp[0] == sp[2] and sp[1] == sp[3]:
-                blk = blk + C
-            if sp[0] == sp[3] and sp[1] == sp[2]:
-                blk = blk - X
-            cube[sp] = 0.25 * blk
-        # assemble the (2n0, 2n1, 2n2, 2n3) block from the 16 spin cases
-        def cat(level, prefix):
-            if level == 4:
-                return cube[prefix]
-            return torch.cat([cat(level + 1, prefix + (b,)) for b in (0, 1)], dim=level)
-        return cat(0, ())
-
     def blocks(self, kappas, keys, at_zero=False):
         """Differentiable blocks for IntegralCache keys ('T', m, n), ('U', a, m, n),
-        ('V', m, n, o, p)."""
+        ('V', m, n, o, p), cut from the rotated spatial integrals by the
+        library (xr_spin_blocks; the adjoint xr_spin_blocks_adjoint carries
+        the block gradients back to h, V and U)."""
         h, V, U = self.rotated(kappas, at_zero)
         out = {}
+        lib = []
         for key in keys:
-            if key[0] == "T":
-                out[key] = self.one_body(h, *key[1:])
-            elif key[0] == "U":
-                a, m1, m2 = key[1:]
-                if U is None:
-                    n1, n2 = self.frag_orbs[m1], self.frag_orbs[m2]
-                    out[key] = torch.zeros(2 * n1, 2 * n2, dtype=torch.float64, device=self.device)
-                else:
-                    out[key] = self.one_body(U[a], m1, m2)
+            if key[0] == "U" and U is None:
+                n1, n2 = self.frag_orbs[key[2]], self.frag_orbs[key[3]]
+                out[key] = torch.zeros(2 * n1, 2 * n2, dtype=torch.float64, device=self.device)
             else:
-                out[key] = self.two_body(V, key[1:])
+                lib.append(key)
+        if lib:
+            desc = _spin_desc(lib, self.frag_orbs, self.off)
+            res = _SpinBlocks.apply(desc, h, V, U if U is not None else h.new_zeros(1, 1, 1))
+            out.update(zip(lib, res))
         return out
+
+
+def _spin_desc(keys, frag_orbs, off):
+    """Launch arguments of xr_spin_blocks for the keys: kinds, dims, offsets,
+    the source of each block (-1 = h, -2 = V, a >= 0 = U[a]) and its shape."""
+    k = len(keys)
+    kinds = np.zeros(k, dtype=np.int32)
+    dims = np.zeros((k, 4), dtype=np.int32)
+    offs = np.zeros((k, 4), dtype=np.int32)
+    srcs, shapes = [], []
+    for i, key in enumerate(keys):
+        ms = key[1:] if key[0] in ("T", "V") else key[2:]
+        kinds[i] = 2 if key[0] == "V" else 1
+        for a, m in enumerate(ms):
+            dims[i, a] = frag_orbs[m]
+            offs[i, a] = off[m]
+        srcs.append(-2 if key[0] == "V" else (-1 if key[0] == "T" else key[1]))
+        shapes.append(tuple(2 * frag_orbs[m] for m in ms))
+    return {"k": k, "kinds": kinds, "dims": dims.reshape(-1), "offs": offs.reshape(-1), "srcs": srcs,
+            "shapes": shapes}
+
+
+class _SpinBlocks(torch.autograd.Function):
+    """Spin-orbital blocks of (h, V, U) for a key list (forward one library
+    call, backward one library call accumulating into dh, dV, dU)."""
+
+    @staticmethod
+    def forward(ctx, desc, h, V, U):
+        from ._lib import context
+        dev = h.device
+        outs = [torch.empty(sh, dtype=torch.float64, device=dev) for sh in desc["shapes"]]
+        # the pointer arrays stay referenced through the call
+        sp = np.asarray([(V if s == -2 else h if s == -1 else U[s]).data_ptr() for s in desc["srcs"]],
+                        dtype=np.uint64)
+        op = np.asarray([o.data_ptr() for o in outs], dtype=np.uint64)
+        c = context(dev.index)
+        c.call("xr_spin_blocks", desc["k"], _np_ptr(desc["kinds"]), _np_ptr(desc["dims"]), _np_ptr(desc["offs"]),
+               _np_ptr(sp), _np_ptr(op), h.shape[0])
+        ctx.desc = desc
+        ctx.shapes = (h.shape, V.shape, U.shape)
+        ctx.dev = dev
+        return tuple(outs)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        from ._lib import context
+        desc = ctx.desc
+        dh = torch.zeros(ctx.shapes[0], dtype=torch.float64, device=ctx.dev)
+        dV = torch.zeros(ctx.shapes[1], dtype=torch.float64, device=ctx.dev)
+        dU = torch.zeros(ctx.shapes[2], dtype=torch.float64, device=ctx.dev)
+        gs = [None if g is None else g.contiguous() for g in grads]
+        gp = np.asarray([0 if g is None else g.data_ptr() for g in gs], dtype=np.uint64)
+        dp = np.asarray([(dV if s == -2 else dh if s == -1 else dU[s]).data_ptr() for s in desc["srcs"]],
+                        dtype=np.uint64)
+        c = context(ctx.dev.index)
+        c.call("xr_spin_blocks_adjoint", desc["k"], _np_ptr(desc["kinds"]), _np_ptr(desc["dims"]),
+               _np_ptr(desc["offs"]), _np_ptr(gp), _np_ptr(dp), ctx.shapes[0][0])
+        return None, dh, dV, dU
+
+
+def _np_ptr(a):
+    import ctypes
+    return ctypes.c_void_p(a.ctypes.data)
 
 
 def integral_keys(nf):

@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from ._lib import context, ptr
-from .eigen import davidson, upload
+from .eigen import davidson, precond_divide, upload
 from .gemm import gemm_f64, mm
 
 
@@ -760,9 +760,7 @@ class SectorSpace:
         """(Hd - theta)^-1 R, Hd the diagonal of H in the product eigenbasis of
         the monomer blocks (monomer part exact, pair terms by their diagonal)."""
         Y = self._axes(R.contiguous(), False)
-        den = self.lam[:, None] - theta[None, :]
-        den = torch.where(den.abs() < 1e-8, torch.full_like(den, 1e-8), den)
-        return self._axes(Y.div_(den), True)
+        return self._axes(precond_divide(Y, self.lam, theta, out=Y), True)
 
     def guess(self, b):
         """The b product eigenstates with the lowest diagonal energy."""

@@ -1,0 +1,78 @@
+"""Vector kernels of the block-Davidson solver (xr_precond_divide,
+xr_residual, xr_column_norms, xr_transpose) against the torch expressions
+they replace, and the fixed-order column sums reproducing bit for bit."""
+import pytest
+import torch
+
+from qodeapplications_amd._lib import XR_EUNSUPPORTED, context, ptr
+from qodeapplications_amd.eigen import column_sumsq, precond_divide, residual, store_rows
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n,b", [(1, 1), (1000, 3), (100_003, 8), (70_000, 32)])
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_precond_divide_matches_torch(n, b, sign):
+    g = torch.Generator().manual_seed(n + b)
+    X = torch.randn(n, b, dtype=torch.float64, generator=g).cuda()
+    lam = torch.randn(n, dtype=torch.float64, generator=g).cuda()
+    theta = torch.randn(b, dtype=torch.float64, generator=g).cuda()
+    lam[::7] = theta[0]                  # exact zeros of the denominator hit the floor
+    if n > 1:
+        lam[1] = theta[0] + 3e-9         # below the floor, positive
+    d = sign * (lam[:, None] - theta[None, :])
+    ref = X / torch.where(d.abs() < 1e-8, torch.full_like(d, 1e-8), d)
+    out = precond_divide(X, lam, theta, sign=sign)
+    Y = X.clone()
+    precond_divide(Y, lam, theta, sign=sign, out=Y)          # in place
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    assert torch.equal(Y, ref)
+
+
+@pytest.mark.parametrize("n,b,k", [(1, 1, 1), (5000, 4, 2), (300_001, 8, 8), (20_000, 32, 3), (1000, 3, 0)])
+def test_residual_and_norms(n, b, k):
+    g = torch.Generator().manual_seed(7 * n + b)
+    X = torch.randn(n, b, dtype=torch.float64, generator=g).cuda()
+    AX = torch.randn(n, b, dtype=torch.float64, generator=g).cuda()
+    theta = torch.randn(b, dtype=torch.float64, generator=g).cuda()
+    R, nr = residual(X, AX, theta, k)
+    torch.cuda.synchronize()
+    Rref = AX - X * theta
+    # one rounding of X theta may be fused away (at most an ulp of either term)
+    assert float((R - Rref).abs().max()) <= 2.3e-16 * float((X * theta).abs().max() + Rref.abs().max())
+    if k:
+        ref = torch.linalg.norm(Rref[:, :k], dim=0)
+        assert float(((nr - ref).abs() / ref).max()) <= 1e-13
+        R2, nr2 = residual(X, AX, theta, k)
+        torch.cuda.synchronize()
+        assert torch.equal(nr, nr2) and torch.equal(R, R2)      # fixed-order sums
+
+
+def test_column_sumsq_and_strided_rows():
+    g = torch.Generator().manual_seed(3)
+    buf = torch.randn(50_000, 12, dtype=torch.float64, generator=g).cuda()
+    A = buf[:, :7]                           # row stride 12
+    s = column_sumsq(A)
+    torch.cuda.synchronize()
+    ref = (A * A).sum(0)
+    assert float(((s - ref).abs() / ref).max()) <= 1e-13
+
+
+@pytest.mark.parametrize("n,b", [(1, 1), (33, 5), (100_000, 8), (2_200_000, 2)])
+def test_store_rows_is_the_transpose(n, b):
+    g = torch.Generator().manual_seed(n)
+    A = torch.randn(n, b, dtype=torch.float64, generator=g).cuda()
+    Vt = torch.full((b + 3, n), 5.0, dtype=torch.float64, device="cuda")
+    store_rows(Vt[1:1 + b], A)
+    torch.cuda.synchronize()
+    assert torch.equal(Vt[1:1 + b], A.T)
+    assert bool((Vt[0] == 5.0).all()) and bool((Vt[1 + b:] == 5.0).all())
+
+
+def test_solver_kernels_reject_wide_blocks():
+    ctx = context(0)
+    A = torch.zeros(10, 40, dtype=torch.float64, device="cuda")
+    out = torch.zeros(40, dtype=torch.float64, device="cuda")
+    rc = ctx.L.xr_column_norms(ctx.h, 10, 40, ptr(A), 40, ptr(out), 1)
+    assert rc == XR_EUNSUPPORTED
