@@ -309,29 +309,50 @@ extern "C" int xr_block_scatter(xr_ctx* c, int nfrag, const int64_t* nI, const i
 // mode 0 copies src -> packed, mode 1 adds alpha * packed into src.  Every
 // element of src maps to exactly one packed element (no write conflicts).
 // ---------------------------------------------------------------------------
+// Division by a divisor fixed for a launch (Granlund & Montgomery,
+// "Division by invariant integers using multiplication", PLDI 1994,
+// Thm. 4.2): with l = ceil(log2 d), s = 31 + l and m = ceil(2^s / d),
+// floor(x / d) = (x m) >> s exactly for 0 <= x < 2^31 (x m < 2^64).
+// The multiplier (< 2^33) and the shift share one word (shift in bits
+// 40..47) to keep the batched launch arguments small.
+typedef uint64_t FastDiv;
+static inline FastDiv fast_div(int64_t d) {
+    int l = 0;
+    while (((int64_t)1 << l) < d) l++;
+    const int s = 31 + l;
+    const unsigned __int128 num = ((unsigned __int128)1 << s) + (unsigned __int128)(d - 1);
+    return (uint64_t)(num / (unsigned __int128)d) | ((uint64_t)s << 40);
+}
+__device__ __forceinline__ int64_t fdiv(int64_t x, FastDiv v) {
+    return (int64_t)(((uint64_t)x * (v & 0xFFFFFFFFFFull)) >> (int)(v >> 40));
+}
+
 struct AxesDesc {
     int nrest;
+    int small;                 // every element index < 2^31: the FastDiv forms apply
     int64_t s0, s1;            // strides of the front axes (s1 unused when one)
     int64_t d1;                // dims of the second front axis (1 when one)
     int64_t nfront;            // size of the front index space
     int64_t rest_dim[XR_MAX_FRAG_AX], rest_stride[XR_MAX_FRAG_AX];
     int64_t ncol, xsize, nrows;
+    FastDiv fx, fc, fd1, frest[XR_MAX_FRAG_AX];   // by xsize, ncol, d1, rest_dim[k]
 };
 
 __device__ __forceinline__ void axes_pack_elem(const AxesDesc& ad, const int32_t* __restrict__ rowmap,
                                                double* __restrict__ full, double* __restrict__ packed,
                                                int64_t ldp, int mode, double alpha, int64_t i) {
     {
-        const int64_t r = i / ad.xsize, x = i - r * ad.xsize;
+        const bool sm = ad.small;
+        const int64_t r = sm ? fdiv(i, ad.fx) : i / ad.xsize, x = i - r * ad.xsize;
         const int64_t pi = r * ldp + x;
         const int64_t f = rowmap ? (int64_t)rowmap[r] : r;
         if (f < 0 || f >= ad.nfront) return;
-        const int64_t a = f / ad.d1, b = f - a * ad.d1;
+        const int64_t a = sm ? fdiv(f, ad.fd1) : f / ad.d1, b = f - a * ad.d1;
         int64_t off = a * ad.s0 + b * ad.s1;
-        int64_t y = x / ad.ncol;
+        int64_t y = sm ? fdiv(x, ad.fc) : x / ad.ncol;
         off += x - y * ad.ncol;
         for (int k = ad.nrest - 1; k >= 0; k--) {
-            const int64_t q = y / ad.rest_dim[k];
+            const int64_t q = sm ? fdiv(y, ad.frest[k]) : y / ad.rest_dim[k];
             off += (y - q * ad.rest_dim[k]) * ad.rest_stride[k];
             y = q;
         }
@@ -375,6 +396,13 @@ static int axes_desc(xr_ctx* c, int nf, const int64_t* dims, int64_t ncol, int f
     ad->ncol = ncol;
     ad->nrows = nrows;
     if (!d_rowmap && nrows != ad->nfront) return xr_fail(c, XR_ESHAPE, "without a row map nrows must be the front size");
+    // element indices (and the front indices, below nfront) under 2^31
+    const int64_t lim = (int64_t)1 << 31;
+    ad->small = (nrows * ad->xsize < lim && ad->nfront < lim) ? 1 : 0;
+    ad->fx = fast_div(ad->xsize);
+    ad->fc = fast_div(ncol);
+    ad->fd1 = fast_div(ad->d1);
+    for (int k = 0; k < ad->nrest; k++) ad->frest[k] = fast_div(ad->rest_dim[k]);
     return XR_OK;
 }
 
@@ -427,10 +455,11 @@ struct AxesBatch {
 
 // offset in the full tensor of packed column x (rest axes and column)
 __device__ __forceinline__ int64_t axes_rest_offset(const AxesDesc& ad, int64_t x) {
-    int64_t y = x / ad.ncol;
+    const bool sm = ad.small;
+    int64_t y = sm ? fdiv(x, ad.fc) : x / ad.ncol;
     int64_t off = x - y * ad.ncol;
     for (int k = ad.nrest - 1; k >= 0; k--) {
-        const int64_t q = y / ad.rest_dim[k];
+        const int64_t q = sm ? fdiv(y, ad.frest[k]) : y / ad.rest_dim[k];
         off += (y - q * ad.rest_dim[k]) * ad.rest_stride[k];
         y = q;
     }
@@ -588,25 +617,60 @@ __global__ __launch_bounds__(256) void k_axis_apply_multi(AxisBatch bt) {
         if (i < n && j < n) v = I.trans ? I.U[j * n + i] : I.U[i * n + j];
         Ut[j][i] = v;
     }
-    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    // operand tile t in registers (at most RB values a thread): the next
+    // tile's loads are in flight while the current one is computed
+    double pf[RB];
+    auto fetch = [&](int64_t t) {
         const int64_t c0 = t << 6;
         const int width = (cols - c0 < 64) ? (int)(cols - c0) : 64;
-        __syncthreads();                      // Ut written / previous tile consumed
-        int64_t base = 0;                     // this lane's column: p * n * post + q
         if (post == 1) {
-            const double* s = I.src + c0 * n;
-            for (int e = tid; e < width * n; e += 256) {
-                const int c = e / n, j = e - c * n;
-                Xs[j][c] = s[e];
+            const double* src = I.src + c0 * n;
+#pragma unroll
+            for (int u = 0; u < RB; u++) {
+                const int e = tid + 256 * u;
+                pf[u] = (e < width * n) ? src[e] : 0.0;
             }
         } else {
             const int64_t c = c0 + lane;
             const int64_t p = c / post, q = c - p * post;
-            base = p * n * post + q;
-            if (lane < width)
-                for (int j = w; j < n; j += 4) Xs[j][lane] = I.src[base + j * post];
+            const int64_t base = p * n * post + q;
+#pragma unroll
+            for (int u = 0; u < RB; u++) {
+                const int j = w + 4 * u;
+                pf[u] = (lane < width && j < n) ? I.src[base + j * post] : 0.0;
+            }
+        }
+    };
+    int64_t t = blockIdx.x;
+    if (t < tiles) fetch(t);
+    for (; t < tiles; t += gridDim.x) {
+        const int64_t c0 = t << 6;
+        const int width = (cols - c0 < 64) ? (int)(cols - c0) : 64;
+        __syncthreads();                      // Ut written / previous tile consumed
+        if (post == 1) {
+#pragma unroll
+            for (int u = 0; u < RB; u++) {
+                const int e = tid + 256 * u;
+                if (e < width * n) {
+                    const int c = e / n, j = e - c * n;
+                    Xs[j][c] = pf[u];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < RB; u++) {
+                const int j = w + 4 * u;
+                if (j < n) Xs[j][lane] = pf[u];
+            }
         }
         __syncthreads();
+        if (t + gridDim.x < tiles) fetch(t + gridDim.x);
+        int64_t base = 0;                     // this lane's column: p * n * post + q
+        if (post != 1) {
+            const int64_t c = c0 + lane;
+            const int64_t p = c / post, q = c - p * post;
+            base = p * n * post + q;
+        }
         double acc[RB];
 #pragma unroll
         for (int k = 0; k < RB; k++) acc[k] = 0.0;
@@ -667,8 +731,9 @@ extern "C" int xr_axis_apply_multi(xr_ctx* c, int n_items, const double* const* 
         }
         if (most <= 0) continue;
         ProfScope prof(c, XR_K_GEMM_BW, 2.0 * flops, (beta != 0.0 ? 24.0 : 16.0) * elems);
-        // enough workgroups to fill the chip; each loops over its item's tiles
-        int64_t gx = tiles, cap = 4096 / bt.nitems;
+        // enough workgroups to fill the chip, each looping over at least two of
+        // its item's tiles (the second one's loads overlap the first's work)
+        int64_t gx = ceil_div(tiles, 2), cap = 4096 / bt.nitems;
         gx = gx < cap ? gx : cap;
         dim3 grid((unsigned)(gx > 0 ? gx : 1), (unsigned)bt.nitems);
         if (nmax <= 4) k_axis_apply_multi<1><<<grid, 256, 0, c->stream>>>(bt);
