@@ -291,6 +291,13 @@ int xr_spin_blocks(xr_ctx *ctx, int n_keys, const int *kinds, const int *dims, c
 int xr_spin_blocks_adjoint(xr_ctx *ctx, int n_keys, const int *kinds, const int *dims, const int *offs,
                            const double *const *d_grad, double *const *d_dsrc, int64_t n);
 
+/* dst = src.permute(perm) (or dst += it, accumulate != 0) for a row-major
+ * 4-index src of dims[4]: dst axis a is src axis perm[a], dst row-major.
+ * The permuted integral blocks of the pair adjoint and their gradients.
+ * No reference counterpart (numpy transposes in the reference). */
+int xr_permute4(xr_ctx *ctx, const double *d_src, const int64_t *dims, const int *perm, double *d_dst,
+                int accumulate);
+
 /* The product as the two-electron contractions run it in precision `mode`
  * (XR_PREC_*): FP64; split-FP32; or in XR_PREC_SPLIT_I8 the split-int8
  * product where it pays (at least 2^33 flops, every dimension >= 128 and

@@ -153,3 +153,48 @@ extern "C" int xr_spin_blocks_adjoint(xr_ctx* c, int n_keys, const int* kinds, c
     }
     return XR_OK;
 }
+
+// dst = src.permute(perm) for a row-major 4-index src of dims d (dst axis a
+// is src axis perm[a]; dst row-major), written or added (accumulate): one
+// thread per dst element (coalesced stores, gathered loads).
+__global__ void k_permute4(const double* __restrict__ src, int4 d, int4 p, double* __restrict__ dst, int accumulate) {
+    const int dd[4] = {d.x, d.y, d.z, d.w};
+    const int pp[4] = {p.x, p.y, p.z, p.w};
+    int64_t sst[4];                 // src strides
+    sst[3] = 1;
+    sst[2] = dd[3];
+    sst[1] = (int64_t)dd[2] * dd[3];
+    sst[0] = (int64_t)dd[1] * dd[2] * dd[3];
+    const int e0 = dd[pp[0]], e1 = dd[pp[1]], e2 = dd[pp[2]], e3 = dd[pp[3]];
+    const int64_t t0 = sst[pp[0]], t1 = sst[pp[1]], t2 = sst[pp[2]], t3 = sst[pp[3]];
+    XR_GRID_STRIDE(e, (int64_t)e0 * e1 * e2 * e3) {
+        int64_t t = e;
+        const int64_t j3 = t % e3; t /= e3;
+        const int64_t j2 = t % e2; t /= e2;
+        const int64_t j1 = t % e1;
+        const int64_t j0 = t / e1;
+        const double v = src[j0 * t0 + j1 * t1 + j2 * t2 + j3 * t3];
+        dst[e] = accumulate ? dst[e] + v : v;
+    }
+}
+
+extern "C" int xr_permute4(xr_ctx* c, const double* src, const int64_t* dims, const int* perm, double* dst,
+                           int accumulate) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (!dims || !perm) return xr_fail(c, XR_EINVAL, "null dims / perm");
+    int seen = 0;
+    for (int a = 0; a < 4; a++) {
+        if (dims[a] < 0 || dims[a] > 0x7fffffff) return xr_fail(c, XR_EINVAL, "bad dimension");
+        if (perm[a] < 0 || perm[a] > 3 || (seen >> perm[a]) & 1) return xr_fail(c, XR_EINVAL, "perm is not a permutation of 0..3");
+        seen |= 1 << perm[a];
+    }
+    const int64_t n = dims[0] * dims[1] * dims[2] * dims[3];
+    if (n == 0) return XR_OK;
+    if (src == dst) return xr_fail(c, XR_EINVAL, "xr_permute4 cannot work in place");
+    ProfScope prof(c, XR_K_OTHER, 0.0, (accumulate ? 24.0 : 16.0) * n);
+    k_permute4<<<xr_grid(n), 256, 0, c->stream>>>(src, make_int4((int)dims[0], (int)dims[1], (int)dims[2], (int)dims[3]),
+                                                  make_int4(perm[0], perm[1], perm[2], perm[3]), dst, accumulate);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}

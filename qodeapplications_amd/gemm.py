@@ -72,3 +72,24 @@ def mm(A, B, transA=0, transB=0, alpha=1.0, out=None, beta=0.0, split=False, mod
              float(alpha), ptr(A), A.stride(0), 0, ptr(B), B.stride(0), 0, float(beta), ptr(out),
              out.stride(0), 0, 1)
     return out
+
+
+def permute4(src, perm, out=None, accumulate=False):
+    """out = src.permute(perm) as a contiguous tensor (or out += it) for a
+    contiguous 4-index float64 CUDA tensor, one library pass (xr_permute4)."""
+    import numpy as np
+    import ctypes
+    if src.dim() != 4 or not src.is_contiguous() or src.dtype != torch.float64:
+        raise ValueError("permute4 needs a contiguous 4-index float64 tensor")
+    shape = tuple(src.shape[p] for p in perm)
+    if out is None:
+        if accumulate:
+            raise ValueError("accumulate needs out")
+        out = torch.empty(shape, dtype=torch.float64, device=src.device)
+    elif out.numel() != src.numel() or not out.is_contiguous():
+        raise ValueError("out must be contiguous with the permuted shape's size")
+    dims = np.asarray(src.shape, dtype=np.int64)
+    pp = np.asarray(perm, dtype=np.int32)
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    context(src.device.index).call("xr_permute4", ptr(src), P(dims), P(pp), ptr(out), 1 if accumulate else 0)
+    return out

@@ -32,7 +32,7 @@ import torch
 from ._lib import context, ptr
 from .density import conjugate
 from .eigen import DENSE_MAX
-from .gemm import mm as _mm
+from .gemm import mm as _mm, permute4
 from .xr import precision
 
 
@@ -236,11 +236,12 @@ class XRAdjoint:
 
         def vperm(key, perm, shape2):
             """W = V[key] with axes permuted and flattened, plus its accumulator
-            mapped back through the same permutation."""
-            Vt = X.ints.V(*key)
-            W = Vt.permute(*perm).contiguous().view(*shape2)
+            mapped back through the same permutation (library passes)."""
+            Vt = X.ints.V(*key).contiguous()
+            W = permute4(Vt, perm).view(*shape2)
             Wb = torch.empty_like(W)          # written whole by its one product (wbar_beta = 0)
-            return W, Wb, lambda: ib("V", *key).add_(Wb.view([Vt.shape[i] for i in perm]).permute(*_inv(perm)))
+            return W, Wb, lambda: permute4(Wb.view([Vt.shape[i] for i in perm]), _inv(perm),
+                                           out=ib("V", *key), accumulate=True)
 
         if d in (-2, +2):
             if d == -2:
@@ -335,10 +336,11 @@ class XREnergy(torch.autograd.Function):
     def backward(ctx, gE):
         zbar, ibar = ctx.adj.backward(lagrangian=ctx.lag)
         g = float(gE)
-        out = [zbar[m][c] * g for (m, c) in ctx.z_keys]
+        scale = (lambda t: t) if g == 1.0 else (lambda t: t * g)    # dE/dE = 1 on the gradient path
+        out = [scale(zbar[m][c]) for (m, c) in ctx.z_keys]
         for key in ctx.int_keys:
             t = ibar.get(key)
-            out.append(None if t is None else t * g)
+            out.append(None if t is None else scale(t))
         return (None, None, None, None) + tuple(out)
 
 

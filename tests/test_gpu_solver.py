@@ -76,3 +76,18 @@ def test_solver_kernels_reject_wide_blocks():
     out = torch.zeros(40, dtype=torch.float64, device="cuda")
     rc = ctx.L.xr_column_norms(ctx.h, 10, 40, ptr(A), 40, ptr(out), 1)
     assert rc == XR_EUNSUPPORTED
+
+
+@pytest.mark.parametrize("perm", [(0, 1, 3, 2), (1, 0, 2, 3), (0, 2, 1, 3), (3, 2, 1, 0), (0, 1, 2, 3)])
+def test_permute4_matches_torch(perm):
+    from qodeapplications_amd.gemm import permute4
+    g = torch.Generator().manual_seed(sum(perm) + perm[0])
+    src = torch.randn(5, 7, 3, 11, dtype=torch.float64, generator=g).cuda()
+    out = permute4(src, perm)
+    acc = torch.randn(out.shape, dtype=torch.float64, generator=g).cuda()
+    acc0 = acc.clone()
+    permute4(src, perm, out=acc, accumulate=True)
+    torch.cuda.synchronize()
+    ref = src.permute(*perm).contiguous()
+    assert torch.equal(out, ref)
+    assert torch.equal(acc, acc0 + ref)
