@@ -111,3 +111,42 @@ def hamiltonian(h, V, cache=None):
 def number_operator(dim):
     n = 1 << dim
     return np.array([popcount(s) for s in range(n)], dtype=np.int64)
+
+
+def density_elements(ops, tuples, zbra, zket, bra_configs, ket_configs):
+    """<I| o_1(p_1) ... o_k(p_k) |J> for the index tuples (p_1..p_k), the
+    operators applied right to left to every ket determinant as a bit mask
+    (a_p|n> = (-1)^popcount(n >> (p+1)) |n - p>, c_p likewise on the mask
+    before the bit is set), the results looked up among the bra
+    determinants.  zbra, zket: one state's coefficients over bra / ket
+    configs (tuples of occupied orbitals).  O(n_ket) per tuple, no tensor
+    built: usable at full determinant counts."""
+    def masks(configs):
+        out = np.zeros(len(configs), dtype=np.int64)
+        for k, cfg in enumerate(configs):
+            for p in cfg:
+                out[k] |= np.int64(1) << np.int64(int(p))
+        return out
+    km = masks(ket_configs)
+    bm = masks(bra_configs)
+    order = np.argsort(bm)
+    bs = bm[order]
+    zk = np.asarray(zket, dtype=np.float64)
+    zb = np.asarray(zbra, dtype=np.float64)
+    vals = np.empty(len(tuples))
+    for t, idx in enumerate(tuples):
+        m = km.copy()
+        sign = np.ones(len(km))
+        ok = np.ones(len(km), dtype=bool)
+        for o, p in zip(reversed(ops), reversed(idx)):
+            bit = np.int64(1) << np.int64(int(p))
+            occ = (m & bit) != 0
+            ok &= occ if o == "a" else ~occ
+            above = np.bitwise_count((m >> np.int64(int(p) + 1)).astype(np.uint64))
+            sign *= np.where(above & 1, -1.0, 1.0)
+            m = m ^ bit
+        pos = np.searchsorted(bs, m)
+        pos_c = np.minimum(pos, len(bs) - 1)
+        found = ok & (pos < len(bs)) & (bs[pos_c] == m)
+        vals[t] = float(np.sum(zk[found] * sign[found] * zb[order[pos_c[found]]]))
+    return vals

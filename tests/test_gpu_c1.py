@@ -91,6 +91,31 @@ def test_c1_one_state_pair_matches_oracle(c1, op):
         assert np.abs(got - ref).max() < 1e-12, (op, bc, kc, I, J)
 
 
+@pytest.mark.parametrize("combo", sorted(SLOW_ORACLE))
+def test_c1_slow_combinations_sampled_elements(c1, combo):
+    """The combinations whose whole-pair oracle walk takes minutes, element
+    by element: one (I, J) pair, 192 seeded index tuples plus the pair's 64
+    largest entries, each against oracle.brute.density_elements (operators
+    applied to every ket determinant as bit masks, at the full determinant
+    count), so errors the trace identities cannot see are caught too."""
+    from oracle import brute
+    op, bc, kc = combo
+    frag, blocks, rho = c1
+    R = rho[op][(bc, kc)]
+    rng = np.random.default_rng(sum(ord(ch) for ch in op) * 100 + (bc + 5) * 10 + kc + 5)
+    I, J = (int(x) for x in rng.integers(0, 32, 2))
+    got = R[I, J].cpu().numpy()
+    dim = 2 * N_ORBS
+    tuples = [tuple(int(x) for x in rng.integers(0, dim, len(op))) for _ in range(192)]
+    big = np.argsort(-np.abs(got).ravel())[:64]
+    tuples += [tuple(int(x) for x in np.unravel_index(k, got.shape)) for k in big]
+    ref = brute.density_elements(op, tuples, frag.states[bc].coeffs[I], frag.states[kc].coeffs[J],
+                                 frag.states[bc].configs, frag.states[kc].configs)
+    vals = np.array([got[t] for t in tuples])
+    assert np.abs(vals - ref).max() < 1e-12, (combo, I, J, float(np.abs(vals - ref).max()))
+    assert np.abs(ref[192:]).min() > 0.0
+
+
 def test_c1_traces(c1):
     """sum_q q+ q = N on the ket: contracting one creator with one annihilator
     gives the lower-rank density times an electron count."""

@@ -217,3 +217,26 @@ def test_dimer_xr_equals_fci():
 
 def test_trimer_xr_equals_fci():
     _xr_vs_fci([1, 1, 1], 1, True, 12)
+
+
+@pytest.mark.parametrize("ops", ["ca", "ccaa", "caaa", "ccca", "caa", "cca", "aa", "cc", "a", "c", "ccaaa"])
+def test_density_elements_match_brute_force(ops):
+    """oracle.brute.density_elements (bit-mask walk over the ket
+    determinants, used for sampled elements at full size) against the
+    explicit-operator densities on a 7-orbital space."""
+    import itertools
+    from oracle import brute
+    dim = 7
+    rng = np.random.default_rng(len(ops) * 31 + ops.count("c"))
+    nk = 3
+    nb = nk + ops.count("c") - ops.count("a")
+    kc = list(itertools.combinations(range(dim), nk))
+    bc = list(itertools.combinations(range(dim), nb))
+    zk = rng.standard_normal(len(kc))
+    zb = rng.standard_normal(len(bc))
+    full = brute.density_fast(ops, brute.embed(zb[None], bc, dim), brute.embed(zk[None], kc, dim), dim)[0, 0]
+    tuples = list(itertools.product(range(dim), repeat=len(ops)))
+    got = brute.density_elements(ops, tuples, zb, zk, bc, kc)
+    ref = np.array([full[t] for t in tuples])
+    assert np.abs(got - ref).max() < 1e-13
+    assert np.abs(ref).max() > 0.1
