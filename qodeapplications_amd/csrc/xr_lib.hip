@@ -9,3 +9,4 @@
 #include "xr_pairs.hip"
 #include "xr_solver.hip"
 #include "xr_spin.hip"
+#include "xr_bound.hip"

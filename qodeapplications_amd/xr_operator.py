@@ -79,6 +79,41 @@ def _i32(a, device):
     return t
 
 
+BOUND_MAX_BLOCK = 4096     # pair blocks with more states get no certified bound
+BOUND_STEPS = 30           # Lanczos steps per pair block
+_U64 = 2.0 ** -53          # unit roundoff of FP64
+
+
+def charge_sector_bound(lines, Q):
+    """max over mu of f(mu) = sum_p min over (lam, c) in lines[p] of
+    (lam + mu c), minus mu Q.
+
+    lines[p] lists, for term p of a decomposition of H, the lower bound lam
+    of its lowest eigenvalue on each block the sector of total charge Q
+    reaches and that block's coefficient c of the charge multiplier mu.
+    f(mu) bounds the sector's lowest eigenvalue from below for every mu; it
+    is concave and piecewise linear, so its maximum lies at mu = 0 or where
+    two lines of one term cross.  A term with no finite line makes the bound
+    -inf."""
+    if not lines:
+        return float("-inf")
+    for L in lines:
+        if not L or not any(np.isfinite(lam) for lam, _ in L):
+            return float("-inf")
+
+    def f(mu):
+        return sum(min(lam + mu * c for lam, c in L) for L in lines) - mu * Q
+
+    cands = {0.0}
+    for L in lines:
+        for i in range(len(L)):
+            for j in range(i + 1, len(L)):
+                (l1, c1), (l2, c2) = L[i], L[j]
+                if c1 != c2 and np.isfinite(l1) and np.isfinite(l2):
+                    cands.add((l1 - l2) / (c2 - c1))
+    return max(f(mu) for mu in cands)
+
+
 class XROperator:
     """H1: list of monomer matrices (d_m x d_m); H2: {(m, n): dense pair
     matrix (d_m d_n x d_m d_n)} for m < n; charges: per fragment, the charge
@@ -296,8 +331,153 @@ class XROperator:
             tot = {t + x for t in tot for x in q}
         return sorted(tot)
 
+    def sector_bounds(self, steps=BOUND_STEPS, max_block=BOUND_MAX_BLOCK):
+        """{Q: a certified lower bound of the lowest eigenvalue in the sector
+        of total charge Q}, from the pair decomposition of H ({} unless
+        there are at least 3 fragments and every pair has its term).
+
+        Every fragment lies in the nf - 1 pairs (m, n), so
+        H = sum_(m<n) h_mn with h_mn = H2_mn + (H1_m + H1_n) / (nf - 1), and
+        the total charge N = sum_m n_m is Q on the sector.  For a unit
+        vector v of the sector and any mu,
+        <v|H|v> = -mu Q + sum_(m<n) <v| h_mn + mu (n_m + n_n) / (nf - 1) |v>.
+        Term (m, n) conserves the pair charge s = n_m + n_n, and on the
+        sector s takes only the values the other fragments' charges complete
+        to Q, so the term is at least the lowest of
+        lambda_mn(s) + mu s / (nf - 1) over those s, lambda_mn(s) the lowest
+        eigenvalue of h_mn on its block of charge sum s.  The bound is the
+        maximum over mu (charge_sector_bound).
+
+        lambda_mn(s) is certified from below: xr_pair_dress forms the block
+        (symmetric parts), xr_lanczos gives an estimate theta with residual
+        r, and t = theta - max(2 r, 1e-3 max(1, |theta|)) stands when the
+        Cholesky factorisation of the block minus t I succeeds; lambda >= t
+        minus the factorisation's backward-error allowance
+        4 (n + 1) n u (|A|_inf + |t|).  A failed factorisation is retried
+        once at theta - max(8 r, 0.05 max(1, |theta|)); then, and for blocks
+        of more than max_block states, the block's bound is -inf."""
+        nf = self.nf
+        self.block_bounds = {}     # (m, n, s) -> the certified bound of lambda_mn(s)
+        if nf < 3 or len(self.pairs) != nf * (nf - 1) // 2:
+            return {}
+        wgt = 1.0 / (nf - 1)
+        dev = self.device
+        ctx = self.ctx
+        pairs = list(self.pairs.items())
+        # pair-state digits (a, b) of every pair, one cached device array
+        digs, doff, o = [], {}, 0
+        for (m, n), P in pairs:
+            order = P["order_np"]
+            digs.append(np.stack([order // self.dims[n], order % self.dims[n]]).reshape(-1))
+            doff[(m, n)] = o
+            o += 2 * order.size
+        dig = _i32(np.concatenate(digs), dev)
+        items = sorted((cnt, m, n, i) for (m, n), P in pairs for i, (_, cnt) in enumerate(P["segs"])
+                       if cnt <= max_block)
+        lam = {}
+        if items:
+            sizes = np.array([cnt * cnt for cnt, _, _, _ in items], dtype=np.int64)
+            offs = np.concatenate([[0], np.cumsum(sizes)])
+            buf = torch.empty(int(offs[-1]), dtype=torch.float64, device=dev)
+            base = buf.data_ptr()
+            for j, (cnt, m, n, i) in enumerate(items):
+                P = self.pairs[(m, n)]
+                start = P["segs"][i][0]
+                blk = P["blocks"][i]
+                if blk.stride(1) != 1:
+                    raise ValueError("pair block rows must be contiguous")
+                pa = dig.data_ptr() + 4 * (doff[(m, n)] + start)
+                pb = pa + 4 * P["order_np"].size
+                ctx.call("xr_pair_dress", cnt, blk.data_ptr(), blk.stride(0), pa, pb, self.H1[m].data_ptr(),
+                         self.dims[m], self.H1[n].data_ptr(), self.dims[n], wgt, base + 8 * int(offs[j]), cnt)
+            nb = len(items)
+            ns = _arr([cnt for cnt, _, _, _ in items], np.int64)
+            ptrs = _arr([base + 8 * int(offs[j]) for j in range(nb)], np.uint64)
+            stats = torch.empty(2 * nb * steps + nb, dtype=torch.float64, device=dev)
+            sp_ = stats.data_ptr()
+            for c0 in range(0, nb, 64):
+                c1 = min(nb, c0 + 64)
+                ctx.call("xr_lanczos", c1 - c0, _ptr(ns[c0:c1]), _ptr(ptrs[c0:c1]), _ptr(ns[c0:c1]), steps,
+                         sp_ + 8 * c0 * steps, sp_ + 8 * (nb + c0) * steps, sp_ + 8 * (2 * nb * steps + c0))
+            h = stats.cpu().numpy()
+            A_, B_, anorm = h[:nb * steps].reshape(nb, steps), h[nb * steps:2 * nb * steps].reshape(nb, steps), \
+                h[2 * nb * steps:]
+            theta, resid = np.empty(nb), np.empty(nb)
+            for j in range(nb):
+                a_, b_ = A_[j], B_[j]
+                tiny = 1e-14 * max(anorm[j], 1e-300)
+                kk = steps
+                for q in range(steps):
+                    if not b_[q] > tiny:        # breakdown (or NaN): T of the first q + 1 steps
+                        kk = q + 1
+                        break
+                T = np.diag(a_[:kk]) + np.diag(b_[:kk - 1], 1) + np.diag(b_[:kk - 1], -1)
+                if not np.all(np.isfinite(T)):
+                    theta[j], resid[j] = np.nan, np.nan
+                    continue
+                w, y = np.linalg.eigh(T)
+                theta[j], resid[j] = w[0], abs(b_[kk - 1] * y[kk - 1, 0])
+            ok = np.isfinite(theta) & np.isfinite(resid) & np.isfinite(anorm)
+            t = np.where(ok, theta - np.maximum(2 * resid, 1e-3 * np.maximum(1.0, np.abs(theta))), 0.0)
+            info = self._cholesky_test(items, offs, buf, ns, ptrs, -t)
+            retry = [j for j in range(nb) if ok[j] and info[j] != 0]
+            t2 = t.copy()
+            if retry:
+                for j in retry:
+                    t2[j] = theta[j] - max(8 * resid[j], 0.05 * max(1.0, abs(theta[j])))
+                sel = np.array(retry)
+                info2 = self._cholesky_test([items[j] for j in retry], offs[sel], buf, ns[sel], ptrs[sel],
+                                            t[sel] - t2[sel], single=True)
+                for q, j in enumerate(retry):
+                    info[j] = info2[q]
+            for j, (cnt, m, n, i) in enumerate(items):
+                if ok[j] and info[j] == 0:
+                    slack = 4.0 * (cnt + 1) * cnt * _U64 * (anorm[j] + abs(t2[j]))
+                    lam[(m, n, self.pairs[(m, n)]["segq"][i])] = float(t2[j] - slack)
+            del buf
+        self.block_bounds = lam
+        cs = [sorted(set(int(x) for x in c)) for c in self.charges]
+        rest = {}
+        for (m, n), _ in pairs:
+            tot = {0}
+            for f in range(nf):
+                if f not in (m, n):
+                    tot = {x + y for x in tot for y in cs[f]}
+            rest[(m, n)] = tot
+        out = {}
+        for Q in self.total_charges():
+            lines = [[(lam.get((m, n, s), float("-inf")), s * wgt) for s in P["segq"] if Q - s in rest[(m, n)]]
+                     for (m, n), P in pairs]
+            out[Q] = charge_sector_bound(lines, Q)
+        return out
+
+    def _cholesky_test(self, items, offs, buf, ns, ptrs, shift, single=False):
+        """Adds shift[j] to the diagonal of every listed block of buf, then
+        Cholesky-factorises each (batched over runs of equal size unless
+        single); returns the factorisations' info codes (0: positive
+        definite)."""
+        nb = len(items)
+        for c0 in range(0, nb, 64):
+            c1 = min(nb, c0 + 64)
+            self.ctx.call("xr_add_diagonal", c1 - c0, _ptr(_arr(ns[c0:c1], np.int64)),
+                          _ptr(_arr(ptrs[c0:c1], np.uint64)), _ptr(_arr(ns[c0:c1], np.int64)),
+                          _ptr(_arr(shift[c0:c1], np.float64)))
+        infos = []
+        j = 0
+        while j < nb:
+            cnt = items[j][0]
+            k = 1
+            if not single:
+                while j + k < nb and items[j + k][0] == cnt:
+                    k += 1
+            o = int(offs[j])
+            view = buf[o:o + k * cnt * cnt].view(k, cnt, cnt)
+            infos.append(torch.linalg.cholesky_ex(view)[1].reshape(-1))
+            j += k
+        return torch.cat(infos).cpu().numpy()
+
     def lowest(self, k=1, tol=1e-9, max_iter=400, block=None, charges=None, chunk=4, margin=4.0,
-               screen=True, verify=True, verify_tol=1e-4):
+               screen=True, verify=True, verify_tol=1e-4, bound=True):
         """(values[k], vectors[n, k]) of the lowest roots over all total
         charges (or the given ones), in the product layout.
 
@@ -328,9 +508,16 @@ class XROperator:
         the early drop final (faster, heuristic).  screen=False converges
         every sector to `tol` without setting any aside.
 
+        bound (with screen, default): every sector first gets a certified
+        lower bound of its lowest eigenvalue (sector_bounds); a sector whose
+        bound lies above the k-th lowest Ritz value at hand (an upper bound
+        of the k-th lowest eigenvalue) is set aside for good, with no
+        further iteration.  The bounds are kept in self.bounds.
+
         self.log receives (Q, size, Ritz values, residual, iterations, how)
         per sector when it is a list; how is 'converged', 'dense',
-        'verified-above' or 'dropped'."""
+        'verified-above', 'bounded-above' (the values are then [the bound])
+        or 'dropped'."""
         found = []                    # converged: (value, sector space, vector)
         active = []
         iterative = []
@@ -366,8 +553,23 @@ class XROperator:
                 self.log.append((a["sp"].Q, a["sp"].size, a["w"], float(np.max(inf.get("rn", [0.0]))),
                                  a["iters"], how))
 
+        self.bounds = self.sector_bounds() if screen and bound and len(iterative) > 1 else {}
+
+        def bounded(a, ub):
+            """The sector's certified bound lies above ub: it is set aside."""
+            lb = self.bounds.get(a["sp"].Q)
+            if lb is None or ub is None or not lb > ub + 1e-9 * max(1.0, abs(ub)):
+                return False
+            a["st"].clear()
+            a["w"] = [lb]
+            record(a, {}, "bounded-above")
+            return True
+
         while active:
             for a in list(active):
+                if bounded(a, upper()):
+                    active.remove(a)
+                    continue
                 sp, inf = a["sp"], {}
                 w, v = davidson(sp.matvec, None, sp.size, a["kk"], tol, min(chunk, max_iter - a["iters"]), a["b"],
                                 self.device, precond=sp.precondition,
@@ -392,6 +594,8 @@ class XROperator:
                         a["st"].clear()
                         record(a, inf, "dropped")
         for a in set_aside:
+            if bounded(a, upper(pending=False)):
+                continue
             sp, inf = a["sp"], {}
             w, v = davidson(sp.matvec, None, sp.size, 1, verify_tol, max(max_iter - a["iters"], 1), a["b"],
                             self.device, precond=sp.precondition, state=a["st"], info=inf)

@@ -7,9 +7,11 @@ per-sector Davidson solve it (the reference's dense assembly cannot).
 
 At this size the checks are:
   * the root is an eigenpair of the operator: ||H v - E v|| small;
-  * the default solve (sectors set aside early, then each set-aside
-    sector's lowest root converged and compared) returns the same root as
-    converging every sector to full tolerance;
+  * the default solve (sectors whose certified lower bound lies above the
+    root skipped, others set aside early, then each set-aside sector's
+    lowest root converged and compared) returns the same root as converging
+    every sector to full tolerance, and every certified bound lies below its
+    sector's converged lowest root;
   * dE/dz and dE/dkappa of the hand-written adjoint against one central
     finite difference each (h = 1e-5) along a random unit direction;
   * split-FP32 against FP64: |dE| <= 1e-9 Eh and the gradients within 1e-7
@@ -74,9 +76,13 @@ def test_c4_root_is_eigenpair_and_every_sector_agrees(c4):
     assert abs(float(E_all[0]) - float(E[0])) <= 1e-10
     lows = sorted(r[2][0] for r in op.log)
     assert abs(lows[0] - float(E[0])) <= 1e-10
-    # the default path checked each set-aside sector's converged lowest root
-    assert all(r[5] in ("converged", "verified-above", "dense") for r in log_default)
+    # the default path bounded or checked every sector it did not converge
+    assert all(r[5] in ("converged", "verified-above", "dense", "bounded-above") for r in log_default)
     assert len(log_default) == len(op.total_charges())
+    low = {r[0]: r[2][0] for r in op.log}
+    for r in log_default:
+        if r[5] == "bounded-above":
+            assert float(E[0]) < r[2][0] <= low[r[0]] + 1e-9, r
 
 
 def _set_coeffs(X, coeffs):
