@@ -312,7 +312,12 @@ int xr_permute4(xr_ctx *ctx, const double *d_src, const int64_t *dims, const int
  *     breakdown) and anorm[b] = the largest absolute row sum of matrix b;
  *   xr_add_diagonal: M[b][i,i] += shift[b] (host array) for nb <= 64
  *     matrices.
- * A Cholesky factorisation of M - t I then certifies t (to its rounding
+ *   xr_cholesky_test: in-place Cholesky factorisation of nb <= 64
+ *     symmetric matrices (their lower triangles; 64-column panels, the
+ *     trailing updates on the FP64 matrix cores); d_info[b] = 0 when matrix b
+ *     is positive definite to working precision, else the 1-based column
+ *     whose pivot was not positive.  The matrices are overwritten.
+ * Success on M - t I certifies t (to the factorisation's rounding
  * allowance) as a lower bound of the lowest eigenvalue.  No reference
  * counterpart (the reference diagonalises the assembled XR matrix densely,
  * general-XRCC/excitonic.py:56-82, and has no charge-sector solver). */
@@ -323,6 +328,8 @@ int xr_lanczos(xr_ctx *ctx, int nb, const int64_t *n, const double *const *d_M, 
                double *d_alpha, double *d_beta, double *d_anorm);
 int xr_add_diagonal(xr_ctx *ctx, int nb, const int64_t *n, double *const *d_M, const int64_t *ld,
                     const double *shift);
+int xr_cholesky_test(xr_ctx *ctx, int nb, const int64_t *n, double *const *d_M, const int64_t *ld,
+                     int *d_info);
 
 /* The product as the two-electron contractions run it in precision `mode`
  * (XR_PREC_*): FP64; split-FP32; or in XR_PREC_SPLIT_I8 the split-int8

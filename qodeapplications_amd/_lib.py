@@ -72,6 +72,7 @@ SIGNATURES = {
     "xr_pair_dress": (_I, [_P, _L, _P, _L, _P, _P, _P, _L, _P, _L, _D, _P, _L]),
     "xr_lanczos": (_I, [_P, _I, _P, _P, _P, _I, _P, _P, _P]),
     "xr_add_diagonal": (_I, [_P, _I, _P, _P, _P, _P]),
+    "xr_cholesky_test": (_I, [_P, _I, _P, _P, _P, _P]),
     "xr_gemm_f64_noalloc": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
     "xr_set_precision": (_I, [_P, _I]),
     "xr_monomer_1e": (_I, [_P, _L, _L, _P, _P, _P]),

@@ -269,3 +269,204 @@ extern "C" int xr_add_diagonal(xr_ctx* c, int nb, const int64_t* n, double* cons
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Positive-definiteness test by Cholesky factorisation, in place, of up to 64
+// matrices at once: right-looking with 64-column panels.  Step p:
+//   k_chol_diag    one wave per matrix factors the diagonal block
+//                  L11 L11^T = A11 and forms L11^-1;
+//   panel GEMM     W = A21 L11^-T (grouped, into a workspace);
+//   trailing GEMM  A22 -= W W^T on the lower-triangle tiles (grouped,
+//                  accumulating).
+// info[b] = 0 when matrix b factored, else the 1-based column whose pivot was
+// not positive (NaN included).  Only the verdict is kept; the matrices are
+// left overwritten.
+// ---------------------------------------------------------------------------
+enum { WS_CHOL = 92 };
+
+struct CholBatch {
+    int nb;
+    int p;                                  // panel index
+    int64_t n[XR_LZ_MAXB];
+    int64_t ld[XR_LZ_MAXB];
+    double* M[XR_LZ_MAXB];
+    int id[XR_LZ_MAXB];                     // the listed matrix's entry in info and in the L11^-1 slots
+};
+
+__global__ void k_zero_int(int* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// A double from lane `src` of the wave (src uniform): two scalar reads.
+__device__ __forceinline__ double lane_read(double x, int src) {
+    const unsigned long long v = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// One 64-lane wave per listed matrix.  Lane r keeps row r of the diagonal
+// block (lower triangle, identity-padded beyond n) in registers; column j of
+// L is computed in every lane and its entries reach the other lanes by
+// v_readlane, so the 64 column steps need no barrier.  L then goes to LDS and
+// lane j solves L x = e_j for column j of L11^-1, written row-major (zeros
+// above the diagonal) to Linv[id].
+__global__ __launch_bounds__(64) void k_chol_diag(CholBatch cb, int* __restrict__ info, double* __restrict__ Linv) {
+    __shared__ double Ls[64][65];
+    const int b = blockIdx.x;
+    const int id = cb.id[b];
+    if (info[id] != 0) return;              // failed at an earlier step (uniform)
+    const int64_t n = cb.n[b], ld = cb.ld[b];
+    const double* A = cb.M[b];
+    const int64_t c0 = (int64_t)cb.p * 64;
+    const int nk = (int)(n - c0 < 64 ? n - c0 : 64);
+    const int lane = threadIdx.x;
+    double a[64];
+#pragma unroll
+    for (int k = 0; k < 64; k++)
+        a[k] = (lane < nk && k < nk) ? (k <= lane ? A[(c0 + lane) * ld + c0 + k] : 0.0) : (lane == k ? 1.0 : 0.0);
+    int fail = 0;
+#pragma unroll
+    for (int j = 0; j < 64; j++) {
+        if (!fail) {
+            const double d = lane_read(a[j], j);
+            if (!(d > 0.0)) {
+                fail = j + 1;               // d is the same in every lane: uniform
+            } else {
+                const double s = sqrt(d);
+                const double rs = 1.0 / s;
+                const double l = lane > j ? a[j] * rs : (lane == j ? s : 0.0);
+                a[j] = l;
+#pragma unroll
+                for (int c = j + 1; c < 64; c++) {
+                    const double lc = lane_read(l, c);
+                    if (lane >= c) a[c] -= l * lc;
+                }
+            }
+        }
+    }
+    if (fail) {
+        if (lane == 0) info[id] = (int)(c0 + fail);
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 64; k++) Ls[lane][k] = k <= lane ? a[k] : 0.0;
+    __syncthreads();
+    // lane j: x_r = (delta_rj - sum_{k<r} L[r][k] x_k) / L[r][r], x_r = 0 for r < j
+    double* out = Linv + (int64_t)id * 4096;
+    double x[64];
+#pragma unroll
+    for (int r = 0; r < 64; r++) {
+        double acc = lane == r ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < r; k++) acc -= Ls[r][k] * x[k];
+        x[r] = lane <= r ? acc / Ls[r][r] : 0.0;
+        out[r * 64 + lane] = x[r];
+    }
+}
+
+extern "C" int xr_cholesky_test(xr_ctx* c, int nb, const int64_t* n, double* const* M, const int64_t* ld,
+                                int* info) {
+    XR_ENTRY(c);
+    if (nb < 0 || nb > XR_LZ_MAXB) return xr_fail(c, XR_EINVAL, "between 0 and 64 matrices per call");
+    if (nb == 0) return XR_OK;
+    if (!n || !M || !ld || !info) return xr_fail(c, XR_EINVAL, "null argument");
+    int64_t nmax = 0, wrows = 0;
+    std::vector<int64_t> woff(nb);
+    for (int b = 0; b < nb; b++) {
+        if (n[b] < 1) return xr_fail(c, XR_EINVAL, "empty matrix");
+        if (ld[b] < n[b]) return xr_fail(c, XR_ESHAPE, "row stride below the matrix size");
+        if (!M[b]) return xr_fail(c, XR_EINVAL, "null matrix");
+        nmax = std::max(nmax, n[b]);
+        woff[b] = wrows;
+        wrows += n[b];
+    }
+    if (nmax > (int64_t)1 << 30) return xr_fail(c, XR_EUNSUPPORTED, "matrix too large");
+    // workspace: L11^-1 of every matrix (64 x 64), then W (n[b] x 64 rows each)
+    double* ws = (double*)xr_workspace(c, WS_CHOL, sizeof(double) * (size_t)(4096 * nb + 64 * wrows));
+    if (!ws) return xr_fail(c, XR_ENOMEM, "workspace (Cholesky panels)");
+    double* Linv = ws;
+    double* W = ws + 4096 * (int64_t)nb;
+    k_zero_int<<<1, 64, 0, c->stream>>>(info, nb);
+    XR_LAUNCH_CHECK(c);
+    std::vector<int64_t> Ms, Ns, Ks, lda, ldb, ldc;
+    std::vector<const double*> As, Bs;
+    std::vector<double*> Cs;
+    std::vector<int> wide, narrow;
+    for (int64_t p = 0; p * 64 < nmax; p++) {
+        CholBatch cb{};
+        cb.p = (int)p;
+        int k = 0;
+        for (int b = 0; b < nb; b++) {
+            if (n[b] <= p * 64) continue;   // already factored
+            cb.n[k] = n[b];
+            cb.ld[k] = ld[b];
+            cb.M[k] = M[b];
+            cb.id[k] = b;
+            k++;
+        }
+        cb.nb = k;
+        {
+            ProfScope prof(c, XR_K_OTHER, 0.0, 0.0);
+            k_chol_diag<<<(unsigned)k, 64, 0, c->stream>>>(cb, info, Linv);
+            XR_LAUNCH_CHECK(c);
+        }
+        // W = A21 L11^-T for every matrix with rows below the panel
+        Ms.clear(); Ns.clear(); Ks.clear(); lda.clear(); ldb.clear(); ldc.clear();
+        As.clear(); Bs.clear(); Cs.clear(); wide.clear(); narrow.clear();
+        for (int q = 0; q < k; q++) {
+            const int64_t rest = cb.n[q] - (p + 1) * 64;
+            if (rest <= 0) continue;
+            const int b = cb.id[q];
+            Ms.push_back(rest);
+            Ns.push_back(64);
+            Ks.push_back(64);
+            As.push_back(cb.M[q] + ((p + 1) * 64) * cb.ld[q] + p * 64);
+            lda.push_back(cb.ld[q]);
+            Bs.push_back(Linv + 4096 * (int64_t)b);
+            ldb.push_back(64);
+            Cs.push_back(W + 64 * woff[b]);
+            ldc.push_back(64);
+        }
+        for (size_t q = 0; q < Ms.size(); q++) {
+            if (Ms[q] > 64) {
+                narrow.push_back((int)q);
+            } else {
+                const int rc = xr_dgemm_launch(c, 0, 1, Ms[q], 64, 64, 1.0, As[q], lda[q], 0, Bs[q], 64, 0, 0.0,
+                                               Cs[q], 64, 0, 1);
+                if (rc) return rc;
+            }
+        }
+        if (!narrow.empty()) {
+            const int rc = launch_grouped<128, 64>(c, narrow, 0, 1, Ms.data(), Ns.data(), Ks.data(), 1.0, As.data(),
+                                                   lda.data(), Bs.data(), ldb.data(), Cs.data(), ldc.data(), 0.0, 0,
+                                                   XR_K_OTHER);
+            if (rc) return rc;
+        }
+        // A22 -= W W^T (lower-triangle tiles)
+        std::vector<const double*> Ws(Cs.begin(), Cs.end());
+        std::vector<double*> Cu;
+        std::vector<int64_t> ldu;
+        for (size_t q = 0, r = 0; q < (size_t)k; q++) {
+            const int64_t rest = cb.n[q] - (p + 1) * 64;
+            if (rest <= 0) continue;
+            Cu.push_back(cb.M[q] + ((p + 1) * 64) * cb.ld[q] + (p + 1) * 64);
+            ldu.push_back(cb.ld[q]);
+            if (Ms[r] > 64) wide.push_back((int)r);
+            else {
+                const int rc = xr_dgemm_launch(c, 0, 1, Ms[r], Ms[r], 64, -1.0, Ws[r], 64, 0, Ws[r], 64, 0, 1.0,
+                                               Cu.back(), ldu.back(), 0, 1);
+                if (rc) return rc;
+            }
+            r++;
+        }
+        if (!wide.empty()) {
+            const int rc = launch_grouped<128, 128>(c, wide, 0, 1, Ms.data(), Ms.data(), Ks.data(), -1.0, Ws.data(),
+                                                    ldc.data(), Ws.data(), ldc.data(), Cu.data(), ldu.data(), 1.0, 1,
+                                                    XR_K_OTHER);
+            if (rc) return rc;
+        }
+    }
+    return XR_OK;
+}

@@ -171,20 +171,35 @@ __device__ __forceinline__ void dgemm_tile(int64_t M, int64_t N, int64_t K, doub
     }
 
     // D[row][col] of a 16x16 tile (v_mfma_f64_16x16x4f64, checked on gfx950):
-    // col = lane % 16, row = lane / 16 + 4 * r for accumulator element r
+    // col = lane % 16, row = lane / 16 + 4 * r for accumulator element r.
+    // With ACC the C values of one row of MFMA tiles are all loaded before
+    // any is stored: loads and stores through the one C pointer would
+    // otherwise stay in program order, one memory round trip per element.
 #pragma unroll
-    for (int i = 0; i < FM; i++)
+    for (int i = 0; i < FM; i++) {
+        double cv[FN][4];
+        if (ACC) {
+#pragma unroll
+            for (int j = 0; j < FN; j++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int64_t gm = m0 + wm * WM + i * 16 + fk + 4 * r;
+                    const int64_t gn = n0 + wn * WN + j * 16 + fr;
+                    cv[j][r] = (gm < M && gn < N) ? C[gm * ldc + gn] : 0.0;
+                }
+        }
 #pragma unroll
         for (int j = 0; j < FN; j++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                int64_t gm = m0 + wm * WM + i * 16 + fk + 4 * r;
-                int64_t gn = n0 + wn * WN + j * 16 + fr;
+                const int64_t gm = m0 + wm * WM + i * 16 + fk + 4 * r;
+                const int64_t gn = n0 + wn * WN + j * 16 + fr;
                 if (gm < M && gn < N) {
-                    if (ACC) C[gm * ldc + gn] = acc[i][j][r] + beta * C[gm * ldc + gn];
+                    if (ACC) C[gm * ldc + gn] = acc[i][j][r] + beta * cv[j][r];
                     else C[gm * ldc + gn] = acc[i][j][r];
                 }
             }
+    }
 }
 
 
@@ -259,17 +274,18 @@ struct GemmProb {
 };
 struct GemmGroup {
     int n;
-    double alpha;
+    int lower;           // square products whose strictly upper tiles are skipped
+    double alpha, beta;  // beta: the accumulating variant only
     GemmProb p[XR_GROUP_PROBS];
 };
 
-template <int BM, int BN, bool TA, bool TB>
+template <int BM, int BN, bool TA, bool TB, bool ACC = false>
 __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm_grouped(GemmGroup g) {
     const int x = (int)(blockIdx.x & 7);
     int64_t j = blockIdx.x >> 3, local = -1;
     int q = 0;
     for (; q < g.n; q++) {                       // this XCD's share of each problem, in order
-        const int64_t T = g.p[q].tiles_m * g.p[q].tiles_n;
+        const int64_t T = g.lower ? g.p[q].tiles_m * (g.p[q].tiles_m + 1) / 2 : g.p[q].tiles_m * g.p[q].tiles_n;
         const int64_t base = T >> 3, r = T & 7;
         const int64_t share = base + (x < r ? 1 : 0);
         if (j < share) {
@@ -280,14 +296,23 @@ __global__ __launch_bounds__(GEMM_THREADS) void k_dgemm_grouped(GemmGroup g) {
     }
     if (local < 0) return;                       // padding workgroup (uneven XCD shares)
     const GemmProb& P = g.p[q];
-    const int64_t GROUP = 8;
-    const int64_t per_group = GROUP * P.tiles_n;
-    const int64_t gi = local / per_group;
-    const int64_t first_m = gi * GROUP;
-    const int64_t gsize = (P.tiles_m - first_m < GROUP) ? (P.tiles_m - first_m) : GROUP;
-    const int64_t tm = first_m + (local % per_group) % gsize;
-    const int64_t tn = (local % per_group) / gsize;
-    dgemm_tile<BM, BN, TA, TB>(P.M, P.N, P.K, g.alpha, P.A, P.lda, P.B, P.ldb, P.C, P.ldc, tm * BM, tn * BN);
+    int64_t tm, tn;
+    if (g.lower) {                               // the lower-triangle tiles (tm >= tn) row by row
+        tm = (int64_t)((sqrt(8.0 * (double)local + 1.0) - 1.0) * 0.5);
+        while ((tm + 1) * (tm + 2) / 2 <= local) tm++;
+        while (tm * (tm + 1) / 2 > local) tm--;
+        tn = local - tm * (tm + 1) / 2;
+    } else {
+        const int64_t GROUP = 8;
+        const int64_t per_group = GROUP * P.tiles_n;
+        const int64_t gi = local / per_group;
+        const int64_t first_m = gi * GROUP;
+        const int64_t gsize = (P.tiles_m - first_m < GROUP) ? (P.tiles_m - first_m) : GROUP;
+        tm = first_m + (local % per_group) % gsize;
+        tn = (local % per_group) / gsize;
+    }
+    dgemm_tile<BM, BN, TA, TB, ACC>(P.M, P.N, P.K, g.alpha, P.A, P.lda, P.B, P.ldb, P.C, P.ldc, tm * BM, tn * BN,
+                                    g.beta);
 }
 
 // C = alpha * sum_s P[s * step] + beta * C, summed in split order.
@@ -1016,12 +1041,16 @@ extern "C" int xr_gemm_f64(xr_ctx* c, int transA, int transB, int64_t M, int64_t
                            sC, batch);
 }
 
-// Launches the problems of one tile shape, XR_GROUP_PROBS per grid.
+// Launches the problems of one tile shape, XR_GROUP_PROBS per grid (with
+// beta != 0 the accumulating kernel, C_q = alpha op(A_q) op(B_q) + beta C_q;
+// lower: square products with BM == BN whose grids hold only the tiles on
+// and below the diagonal; kind: the profiler class of the launches).
 template <int BM, int BN>
 static int launch_grouped(xr_ctx* c, const std::vector<int>& ids, int ta, int tb, const int64_t* M,
                           const int64_t* N, const int64_t* K, double alpha, const double* const* A,
                           const int64_t* lda, const double* const* B, const int64_t* ldb, double* const* C,
-                          const int64_t* ldc) {
+                          const int64_t* ldc, double beta = 0.0, int lower = 0, int kind = XR_K_GEMM_GROUPED) {
+    if (lower && BM != BN) return xr_fail(c, XR_EUNSUPPORTED, "lower-triangle grids need square tiles");
     // launches of equal problem counts, problems by decreasing K
     std::vector<int> order(ids);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return K[a] > K[b]; });
@@ -1029,7 +1058,9 @@ static int launch_grouped(xr_ctx* c, const std::vector<int>& ids, int ta, int tb
     for (size_t l = 0; l < nl; l++) {
         GemmGroup g;
         g.n = 0;
+        g.lower = lower;
         g.alpha = alpha;
+        g.beta = beta;
         int64_t slots = 0;                         // workgroups per XCD
         double flops = 0.0, bytes = 0.0;
         for (size_t t = l; t < order.size(); t += nl) {
@@ -1037,15 +1068,19 @@ static int launch_grouped(xr_ctx* c, const std::vector<int>& ids, int ta, int tb
             GemmProb& P = g.p[g.n++];
             P = GemmProb{A[q], B[q], C[q], M[q], N[q], K[q], lda[q], ldb[q], ldc[q],
                          ceil_div(M[q], BM), ceil_div(N[q], BN)};
-            slots += ceil_div(P.tiles_m * P.tiles_n, 8);
-            flops += 2.0 * M[q] * N[q] * K[q];
+            if (lower && M[q] != N[q]) return xr_fail(c, XR_EUNSUPPORTED, "lower-triangle grid of a non-square C");
+            slots += ceil_div(lower ? P.tiles_m * (P.tiles_m + 1) / 2 : P.tiles_m * P.tiles_n, 8);
+            flops += (lower ? 1.0 : 2.0) * M[q] * N[q] * K[q];
             bytes += 8.0 * (M[q] * K[q] + K[q] * N[q] + M[q] * N[q]);
         }
         if (8 * slots > 0x7fffffff) return xr_fail(c, XR_EUNSUPPORTED, "GEMM grid too large");
-        ProfScope prof(c, XR_K_GEMM_GROUPED, flops, bytes);
+        ProfScope prof(c, kind, flops, bytes);
         prof.shape(g.p[0].M, g.p[0].N, g.p[0].K, g.n, ta, tb);
         const dim3 grid((unsigned)(8 * slots));
-        if (ta && tb) k_dgemm_grouped<BM, BN, true, true><<<grid, GEMM_THREADS, 0, c->stream>>>(g);
+        if (beta != 0.0) {
+            if (ta || !tb) return xr_fail(c, XR_EUNSUPPORTED, "accumulating grouped GEMM: A B^T only");
+            k_dgemm_grouped<BM, BN, false, true, true><<<grid, GEMM_THREADS, 0, c->stream>>>(g);
+        } else if (ta && tb) k_dgemm_grouped<BM, BN, true, true><<<grid, GEMM_THREADS, 0, c->stream>>>(g);
         else if (ta) k_dgemm_grouped<BM, BN, true, false><<<grid, GEMM_THREADS, 0, c->stream>>>(g);
         else if (tb) k_dgemm_grouped<BM, BN, false, true><<<grid, GEMM_THREADS, 0, c->stream>>>(g);
         else k_dgemm_grouped<BM, BN, false, false><<<grid, GEMM_THREADS, 0, c->stream>>>(g);

@@ -351,11 +351,12 @@ class XROperator:
         lambda_mn(s) is certified from below: xr_pair_dress forms the block
         (symmetric parts), xr_lanczos gives an estimate theta with residual
         r, and t = theta - max(2 r, 1e-3 max(1, |theta|)) stands when the
-        Cholesky factorisation of the block minus t I succeeds; lambda >= t
-        minus the factorisation's backward-error allowance
-        4 (n + 1) n u (|A|_inf + |t|).  A failed factorisation is retried
-        once at theta - max(8 r, 0.05 max(1, |theta|)); then, and for blocks
-        of more than max_block states, the block's bound is -inf."""
+        Cholesky factorisation of the block minus t I succeeds
+        (xr_cholesky_test); lambda >= t minus the factorisation's
+        backward-error allowance 4 (n + 1) n u (|A|_inf + |t|).  A failed
+        factorisation is retried once at theta - max(8 r, 0.05 max(1,
+        |theta|)); then, and for blocks of more than max_block states, the
+        block's bound is -inf."""
         nf = self.nf
         self.block_bounds = {}     # (m, n, s) -> the certified bound of lambda_mn(s)
         if nf < 3 or len(self.pairs) != nf * (nf - 1) // 2:
@@ -381,15 +382,7 @@ class XROperator:
             buf = torch.empty(int(offs[-1]), dtype=torch.float64, device=dev)
             base = buf.data_ptr()
             for j, (cnt, m, n, i) in enumerate(items):
-                P = self.pairs[(m, n)]
-                start = P["segs"][i][0]
-                blk = P["blocks"][i]
-                if blk.stride(1) != 1:
-                    raise ValueError("pair block rows must be contiguous")
-                pa = dig.data_ptr() + 4 * (doff[(m, n)] + start)
-                pb = pa + 4 * P["order_np"].size
-                ctx.call("xr_pair_dress", cnt, blk.data_ptr(), blk.stride(0), pa, pb, self.H1[m].data_ptr(),
-                         self.dims[m], self.H1[n].data_ptr(), self.dims[n], wgt, base + 8 * int(offs[j]), cnt)
+                self._dress(m, n, i, cnt, wgt, dig, doff, base + 8 * int(offs[j]))
             nb = len(items)
             ns = _arr([cnt for cnt, _, _, _ in items], np.int64)
             ptrs = _arr([base + 8 * int(offs[j]) for j in range(nb)], np.uint64)
@@ -419,17 +412,21 @@ class XROperator:
                 theta[j], resid[j] = w[0], abs(b_[kk - 1] * y[kk - 1, 0])
             ok = np.isfinite(theta) & np.isfinite(resid) & np.isfinite(anorm)
             t = np.where(ok, theta - np.maximum(2 * resid, 1e-3 * np.maximum(1.0, np.abs(theta))), 0.0)
-            info = self._cholesky_test(items, offs, buf, ns, ptrs, -t)
-            retry = [j for j in range(nb) if ok[j] and info[j] != 0]
             t2 = t.copy()
+            info = np.ones(nb, dtype=np.int64)           # not certified
+            sel = np.flatnonzero(ok)
+            if sel.size:
+                info[sel] = self._cholesky_test(ns[sel], ptrs[sel], -t[sel])
+            retry = [j for j in sel if info[j] != 0]
             if retry:
+                # the factorisation overwrote the block: form it again, then
+                # test a lower shift
                 for j in retry:
                     t2[j] = theta[j] - max(8 * resid[j], 0.05 * max(1.0, abs(theta[j])))
+                    cnt, m, n, i = items[j]
+                    self._dress(m, n, i, cnt, wgt, dig, doff, int(ptrs[j]))
                 sel = np.array(retry)
-                info2 = self._cholesky_test([items[j] for j in retry], offs[sel], buf, ns[sel], ptrs[sel],
-                                            t[sel] - t2[sel], single=True)
-                for q, j in enumerate(retry):
-                    info[j] = info2[q]
+                info[sel] = self._cholesky_test(ns[sel], ptrs[sel], -t2[sel])
             for j, (cnt, m, n, i) in enumerate(items):
                 if ok[j] and info[j] == 0:
                     slack = 4.0 * (cnt + 1) * cnt * _U64 * (anorm[j] + abs(t2[j]))
@@ -451,30 +448,34 @@ class XROperator:
             out[Q] = charge_sector_bound(lines, Q)
         return out
 
-    def _cholesky_test(self, items, offs, buf, ns, ptrs, shift, single=False):
-        """Adds shift[j] to the diagonal of every listed block of buf, then
-        Cholesky-factorises each (batched over runs of equal size unless
-        single); returns the factorisations' info codes (0: positive
-        definite)."""
-        nb = len(items)
+    def _dress(self, m, n, i, cnt, wgt, dig, doff, out):
+        """Block i of pair (m, n) with its monomer shares into out (cnt x cnt)."""
+        P = self.pairs[(m, n)]
+        start = P["segs"][i][0]
+        blk = P["blocks"][i]
+        if blk.stride(1) != 1:
+            raise ValueError("pair block rows must be contiguous")
+        pa = dig.data_ptr() + 4 * (doff[(m, n)] + start)
+        pb = pa + 4 * P["order_np"].size
+        self.ctx.call("xr_pair_dress", cnt, blk.data_ptr(), blk.stride(0), pa, pb, self.H1[m].data_ptr(),
+                      self.dims[m], self.H1[n].data_ptr(), self.dims[n], wgt, out, cnt)
+
+    def _cholesky_test(self, ns, ptrs, shift):
+        """Adds shift[j] to the diagonal of every listed block, then
+        Cholesky-factorises them in place (xr_cholesky_test, 64 blocks per
+        call); returns the info codes (0: positive definite)."""
+        nb = len(ns)
+        ns = _arr(ns, np.int64)
+        ptrs = _arr(ptrs, np.uint64)
+        shift = _arr(shift, np.float64)
+        info = torch.empty(nb, dtype=torch.int32, device=self.device)
         for c0 in range(0, nb, 64):
             c1 = min(nb, c0 + 64)
-            self.ctx.call("xr_add_diagonal", c1 - c0, _ptr(_arr(ns[c0:c1], np.int64)),
-                          _ptr(_arr(ptrs[c0:c1], np.uint64)), _ptr(_arr(ns[c0:c1], np.int64)),
-                          _ptr(_arr(shift[c0:c1], np.float64)))
-        infos = []
-        j = 0
-        while j < nb:
-            cnt = items[j][0]
-            k = 1
-            if not single:
-                while j + k < nb and items[j + k][0] == cnt:
-                    k += 1
-            o = int(offs[j])
-            view = buf[o:o + k * cnt * cnt].view(k, cnt, cnt)
-            infos.append(torch.linalg.cholesky_ex(view)[1].reshape(-1))
-            j += k
-        return torch.cat(infos).cpu().numpy()
+            self.ctx.call("xr_add_diagonal", c1 - c0, _ptr(ns[c0:c1]), _ptr(ptrs[c0:c1]), _ptr(ns[c0:c1]),
+                          _ptr(shift[c0:c1]))
+            self.ctx.call("xr_cholesky_test", c1 - c0, _ptr(ns[c0:c1]), _ptr(ptrs[c0:c1]), _ptr(ns[c0:c1]),
+                          info.data_ptr() + 4 * c0)
+        return info.cpu().numpy()
 
     def lowest(self, k=1, tol=1e-9, max_iter=400, block=None, charges=None, chunk=4, margin=4.0,
                screen=True, verify=True, verify_tol=1e-4, bound=True):

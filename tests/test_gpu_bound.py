@@ -8,6 +8,11 @@
     40 steps on 700 x 700 and 3000 x 3000 matrices with a gap at the bottom
     of the spectrum; anorm is the largest absolute row sum;
   * xr_add_diagonal shifts exactly the diagonals;
+  * xr_cholesky_test: on a batch of sizes 1..2000 (row strides above n
+    included) the verdict for A - t I at t = lambda_min -/+ 1e-3 is positive
+    definite / not, as LAPACK's dpotrf says, and a diagonally dominant matrix
+    with one negative diagonal entry at column k fails exactly at k + 1
+    (in the first, second and last panel); a NaN fails;
   * sector_bounds on 3- and 4-fragment chains: every certified block bound
     lies below the block's exact lowest eigenvalue (torch eigvalsh) and
     within 0.05 of it, and every sector's bound below the sector's exact
@@ -122,6 +127,52 @@ def test_add_diagonal_shifts_only_the_diagonals():
     refA[:, :5] += 0.25 * torch.eye(5, dtype=torch.float64, device="cuda")
     refB += -2.0 * torch.eye(3, dtype=torch.float64, device="cuda")
     assert torch.equal(A, refA) and torch.equal(B, refB)
+
+
+def _chol(mats, lds=None):
+    ctx = context(0)
+    lds = lds or [A.shape[0] for A in mats]
+    bufs = []
+    for A, ld in zip(mats, lds):
+        n = A.shape[0]
+        B = torch.full((n, ld), 7.0, dtype=torch.float64)
+        B[:, :n] = A
+        bufs.append(B.cuda())
+    ns = np.array([A.shape[0] for A in mats], dtype=np.int64)
+    lda = np.array(lds, dtype=np.int64)
+    ps = np.array([B.data_ptr() for B in bufs], dtype=np.uint64)
+    info = torch.full((len(mats),), -5, dtype=torch.int32, device="cuda")
+    ctx.call("xr_cholesky_test", len(mats), ns.ctypes.data, ps.ctypes.data, lda.ctypes.data, ptr(info))
+    return info.cpu().numpy()
+
+
+def test_cholesky_test_verdicts():
+    from scipy.linalg import lapack
+    gen = torch.Generator().manual_seed(21)
+    sizes = [1, 5, 64, 65, 130, 700, 2000]
+    base = []
+    for n in sizes:
+        G = torch.randn(n, n, dtype=torch.float64, generator=gen)
+        base.append(G @ G.T / n + 0.1 * torch.eye(n, dtype=torch.float64))
+    lows = [float(torch.linalg.eigvalsh(A)[0]) for A in base]
+    for sign in (-1.0, 1.0):
+        mats = [A - (lo + sign * 1e-3) * torch.eye(A.shape[0], dtype=torch.float64) for A, lo in zip(base, lows)]
+        info = _chol(mats, [A.shape[0] + (3 if j % 2 else 0) for j, A in enumerate(mats)])
+        for j, A in enumerate(mats):
+            ref = lapack.dpotrf(A.numpy(), lower=1)[1]
+            assert (info[j] == 0) == (ref == 0) == (sign < 0), (sizes[j], sign, info[j], ref)
+    mats = []
+    for n, k in [(200, 10), (200, 100), (300, 299), (64, 63)]:
+        A = 0.01 * torch.rand(n, n, dtype=torch.float64, generator=gen)
+        A = A + A.T + 2.0 * torch.eye(n, dtype=torch.float64)
+        A[k, k] = -1.0
+        mats.append(A)
+    N = base[4].clone()
+    N[70, 3] = N[3, 70] = float("nan")
+    mats.append(N)
+    info = _chol(mats)
+    assert list(info[:4]) == [11, 101, 300, 64]
+    assert info[4] != 0
 
 
 def _sector_lows(op):
