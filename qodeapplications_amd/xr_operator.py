@@ -20,6 +20,8 @@ sum_rest v v^T on (m, n), block diagonal in the same order.
 The spectator-fragment sign of one-electron transfers across fragments
 between m and n is not applied, as in the reference loops.
 """
+import contextlib
+
 import numpy as np
 import torch
 
@@ -112,6 +114,209 @@ def charge_sector_bound(lines, Q):
                 if c1 != c2 and np.isfinite(l1) and np.isfinite(l2):
                     cands.add((l1 - l2) / (c2 - c1))
     return max(f(mu) for mu in cands)
+
+
+_SIDE = {}                 # device -> (stream, library context) of the sector-bound work
+
+
+def _side(device):
+    """A second stream on the device (default priority) with its own library
+    context, for the sector bounds computed beside the Davidson."""
+    hit = _SIDE.get(device.index)
+    if hit is None:
+        from ._lib import Context
+        st = torch.cuda.Stream(device=device)
+        with torch.cuda.stream(st):
+            hit = _SIDE[device.index] = (st, Context(device.index))
+    return hit
+
+
+class _SectorBoundJob:
+    """The staged work behind XROperator.sector_bounds: the dressed pair
+    blocks and their Lanczos passes; once those reach the host, the shifted
+    Cholesky tests; once their verdicts arrive (and any retry's), the block
+    and sector bounds.  With a side stream every stage is queued there with
+    its own library context, after the caller's earlier work on its stream,
+    and step() only advances past stages that have finished, so the stages
+    run beside the caller's own GPU work."""
+
+    def __init__(self, op, steps, max_block, side=False):
+        self.op, self.steps = op, steps
+        self.done, self.out, self.lam = False, {}, {}
+        op.block_bounds = {}
+        nf = op.nf
+        if nf < 3 or len(op.pairs) != nf * (nf - 1) // 2:
+            self.done = True
+            return
+        self.wgt = 1.0 / (nf - 1)
+        dev = op.device
+        self.pairs = list(op.pairs.items())
+        # pair-state digits (a, b) of every pair, one cached device array
+        digs, self.doff, o = [], {}, 0
+        for (m, n), P in self.pairs:
+            order = P["order_np"]
+            digs.append(np.stack([order // op.dims[n], order % op.dims[n]]).reshape(-1))
+            self.doff[(m, n)] = o
+            o += 2 * order.size
+        self.dig = _i32(np.concatenate(digs), dev)
+        self.items = sorted((cnt, m, n, i) for (m, n), P in self.pairs for i, (_, cnt) in enumerate(P["segs"])
+                            if cnt <= max_block)
+        if not self.items:
+            self._finish(None)
+            return
+        self.stream, self.ctx = (None, op.ctx)
+        if side:
+            self.stream, self.ctx = _side(dev)
+            self.stream.wait_stream(torch.cuda.current_stream(dev))
+        nb = len(self.items)
+        sizes = np.array([cnt * cnt for cnt, _, _, _ in self.items], dtype=np.int64)
+        offs = np.concatenate([[0], np.cumsum(sizes)])
+        with self._on():
+            self.buf = torch.empty(int(offs[-1]), dtype=torch.float64, device=dev)
+            base = self.buf.data_ptr()
+            self.ns = _arr([cnt for cnt, _, _, _ in self.items], np.int64)
+            self.ptrs = _arr([base + 8 * int(offs[j]) for j in range(nb)], np.uint64)
+            for j in range(nb):
+                self._dress(j)
+            stats = torch.empty(2 * nb * steps + nb, dtype=torch.float64, device=dev)
+            sp_ = stats.data_ptr()
+            for c0 in range(0, nb, 64):
+                c1 = min(nb, c0 + 64)
+                self.ctx.call("xr_lanczos", c1 - c0, _ptr(self.ns[c0:c1]), _ptr(self.ptrs[c0:c1]),
+                              _ptr(self.ns[c0:c1]), steps, sp_ + 8 * c0 * steps, sp_ + 8 * (nb + c0) * steps,
+                              sp_ + 8 * (2 * nb * steps + c0))
+            self.host = self._to_host(stats)
+        self.stage = 1
+
+    def _on(self):
+        return torch.cuda.stream(self.stream) if self.stream is not None else contextlib.nullcontext()
+
+    def _to_host(self, t):
+        """Queues the copy of t into pinned host memory and the event that marks it."""
+        h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+        h.copy_(t, non_blocking=True)
+        self.ev = torch.cuda.Event()
+        self.ev.record()
+        self.keep = t
+        return h
+
+    def _dress(self, j):
+        """Item j's pair block with its monomer shares into its slot of buf."""
+        cnt, m, n, i = self.items[j]
+        op = self.op
+        P = op.pairs[(m, n)]
+        start = P["segs"][i][0]
+        blk = P["blocks"][i]
+        if blk.stride(1) != 1:
+            raise ValueError("pair block rows must be contiguous")
+        pa = self.dig.data_ptr() + 4 * (self.doff[(m, n)] + start)
+        pb = pa + 4 * P["order_np"].size
+        self.ctx.call("xr_pair_dress", cnt, blk.data_ptr(), blk.stride(0), pa, pb, op.H1[m].data_ptr(),
+                      op.dims[m], op.H1[n].data_ptr(), op.dims[n], self.wgt, int(self.ptrs[j]), cnt)
+
+    def _cholesky(self, sel, shift):
+        """Queues: shift[q] added to the diagonal of block sel[q], the
+        in-place Cholesky tests of those blocks, and the copy of their info
+        codes to the host."""
+        ns, ptrs = _arr(self.ns[sel], np.int64), _arr(self.ptrs[sel], np.uint64)
+        shift = _arr(shift, np.float64)
+        with self._on():
+            info = torch.empty(len(sel), dtype=torch.int32, device=self.op.device)
+            for c0 in range(0, len(sel), 64):
+                c1 = min(len(sel), c0 + 64)
+                self.ctx.call("xr_add_diagonal", c1 - c0, _ptr(ns[c0:c1]), _ptr(ptrs[c0:c1]), _ptr(ns[c0:c1]),
+                              _ptr(shift[c0:c1]))
+                self.ctx.call("xr_cholesky_test", c1 - c0, _ptr(ns[c0:c1]), _ptr(ptrs[c0:c1]), _ptr(ns[c0:c1]),
+                              info.data_ptr() + 4 * c0)
+            return self._to_host(info)
+
+    def step(self, block=False):
+        """Advances through every stage whose work has finished (waiting for
+        it when block); True once the bounds are final."""
+        while not self.done:
+            if not block and not self.ev.query():
+                return False
+            self.ev.synchronize()
+            if self.stage == 1:
+                self._estimates()
+            elif self.stage == 2:
+                self._verdicts()
+            else:
+                self.info[self.retry] = self.host.numpy()
+                self._finish(self.info)
+        return True
+
+    def _estimates(self):
+        """Lanczos tridiagonals -> theta, residual, the first shift t; queues
+        the Cholesky tests."""
+        nb, steps = len(self.items), self.steps
+        h = self.host.numpy()
+        A_, B_ = h[:nb * steps].reshape(nb, steps), h[nb * steps:2 * nb * steps].reshape(nb, steps)
+        self.anorm = h[2 * nb * steps:].copy()
+        theta, resid = np.full(nb, np.nan), np.full(nb, np.nan)
+        for j in range(nb):
+            a_, b_ = A_[j], B_[j]
+            tiny = 1e-14 * max(self.anorm[j], 1e-300)
+            kk = steps
+            for q in range(steps):
+                if not b_[q] > tiny:            # breakdown (or NaN): T of the first q + 1 steps
+                    kk = q + 1
+                    break
+            T = np.diag(a_[:kk]) + np.diag(b_[:kk - 1], 1) + np.diag(b_[:kk - 1], -1)
+            if np.all(np.isfinite(T)):
+                w, y = np.linalg.eigh(T)
+                theta[j], resid[j] = w[0], abs(b_[kk - 1] * y[kk - 1, 0])
+        self.theta, self.resid = theta, resid
+        self.ok = np.isfinite(theta) & np.isfinite(resid) & np.isfinite(self.anorm)
+        self.t = np.where(self.ok, theta - np.maximum(2 * resid, 1e-3 * np.maximum(1.0, np.abs(theta))), 0.0)
+        self.sel = np.flatnonzero(self.ok)
+        self.info = np.ones(nb, dtype=np.int64)         # not certified
+        if self.sel.size == 0:
+            self._finish(self.info)
+            return
+        self.host = self._cholesky(self.sel, -self.t[self.sel])
+        self.stage = 2
+
+    def _verdicts(self):
+        """First verdicts; a failed block is formed again (the factorisation
+        overwrote it) and tested once more at a lower shift."""
+        self.info[self.sel] = self.host.numpy()
+        self.retry = np.array([j for j in self.sel if self.info[j] != 0], dtype=np.int64)
+        if self.retry.size == 0:
+            self._finish(self.info)
+            return
+        for j in self.retry:
+            self.t[j] = self.theta[j] - max(8 * self.resid[j], 0.05 * max(1.0, abs(self.theta[j])))
+        with self._on():
+            for j in self.retry:
+                self._dress(int(j))
+        self.host = self._cholesky(self.retry, -self.t[self.retry])
+        self.stage = 3
+
+    def _finish(self, info):
+        op = self.op
+        lam = {}
+        if info is not None:
+            for j, (cnt, m, n, i) in enumerate(self.items):
+                if self.ok[j] and info[j] == 0:
+                    slack = 4.0 * (cnt + 1) * cnt * _U64 * (self.anorm[j] + abs(self.t[j]))
+                    lam[(m, n, op.pairs[(m, n)]["segq"][i])] = float(self.t[j] - slack)
+        self.buf = self.keep = None
+        op.block_bounds = lam
+        cs = [sorted(set(int(x) for x in c)) for c in op.charges]
+        rest = {}
+        for (m, n), _ in self.pairs:
+            tot = {0}
+            for f in range(op.nf):
+                if f not in (m, n):
+                    tot = {x + y for x in tot for y in cs[f]}
+            rest[(m, n)] = tot
+        self.out = {}
+        for Q in op.total_charges():
+            lines = [[(lam.get((m, n, s), float("-inf")), s * self.wgt) for s in P["segq"] if Q - s in rest[(m, n)]]
+                     for (m, n), P in self.pairs]
+            self.out[Q] = charge_sector_bound(lines, Q)
+        self.done = True
 
 
 class XROperator:
@@ -356,126 +561,11 @@ class XROperator:
         backward-error allowance 4 (n + 1) n u (|A|_inf + |t|).  A failed
         factorisation is retried once at theta - max(8 r, 0.05 max(1,
         |theta|)); then, and for blocks of more than max_block states, the
-        block's bound is -inf."""
-        nf = self.nf
-        self.block_bounds = {}     # (m, n, s) -> the certified bound of lambda_mn(s)
-        if nf < 3 or len(self.pairs) != nf * (nf - 1) // 2:
-            return {}
-        wgt = 1.0 / (nf - 1)
-        dev = self.device
-        ctx = self.ctx
-        pairs = list(self.pairs.items())
-        # pair-state digits (a, b) of every pair, one cached device array
-        digs, doff, o = [], {}, 0
-        for (m, n), P in pairs:
-            order = P["order_np"]
-            digs.append(np.stack([order // self.dims[n], order % self.dims[n]]).reshape(-1))
-            doff[(m, n)] = o
-            o += 2 * order.size
-        dig = _i32(np.concatenate(digs), dev)
-        items = sorted((cnt, m, n, i) for (m, n), P in pairs for i, (_, cnt) in enumerate(P["segs"])
-                       if cnt <= max_block)
-        lam = {}
-        if items:
-            sizes = np.array([cnt * cnt for cnt, _, _, _ in items], dtype=np.int64)
-            offs = np.concatenate([[0], np.cumsum(sizes)])
-            buf = torch.empty(int(offs[-1]), dtype=torch.float64, device=dev)
-            base = buf.data_ptr()
-            for j, (cnt, m, n, i) in enumerate(items):
-                self._dress(m, n, i, cnt, wgt, dig, doff, base + 8 * int(offs[j]))
-            nb = len(items)
-            ns = _arr([cnt for cnt, _, _, _ in items], np.int64)
-            ptrs = _arr([base + 8 * int(offs[j]) for j in range(nb)], np.uint64)
-            stats = torch.empty(2 * nb * steps + nb, dtype=torch.float64, device=dev)
-            sp_ = stats.data_ptr()
-            for c0 in range(0, nb, 64):
-                c1 = min(nb, c0 + 64)
-                ctx.call("xr_lanczos", c1 - c0, _ptr(ns[c0:c1]), _ptr(ptrs[c0:c1]), _ptr(ns[c0:c1]), steps,
-                         sp_ + 8 * c0 * steps, sp_ + 8 * (nb + c0) * steps, sp_ + 8 * (2 * nb * steps + c0))
-            h = stats.cpu().numpy()
-            A_, B_, anorm = h[:nb * steps].reshape(nb, steps), h[nb * steps:2 * nb * steps].reshape(nb, steps), \
-                h[2 * nb * steps:]
-            theta, resid = np.empty(nb), np.empty(nb)
-            for j in range(nb):
-                a_, b_ = A_[j], B_[j]
-                tiny = 1e-14 * max(anorm[j], 1e-300)
-                kk = steps
-                for q in range(steps):
-                    if not b_[q] > tiny:        # breakdown (or NaN): T of the first q + 1 steps
-                        kk = q + 1
-                        break
-                T = np.diag(a_[:kk]) + np.diag(b_[:kk - 1], 1) + np.diag(b_[:kk - 1], -1)
-                if not np.all(np.isfinite(T)):
-                    theta[j], resid[j] = np.nan, np.nan
-                    continue
-                w, y = np.linalg.eigh(T)
-                theta[j], resid[j] = w[0], abs(b_[kk - 1] * y[kk - 1, 0])
-            ok = np.isfinite(theta) & np.isfinite(resid) & np.isfinite(anorm)
-            t = np.where(ok, theta - np.maximum(2 * resid, 1e-3 * np.maximum(1.0, np.abs(theta))), 0.0)
-            t2 = t.copy()
-            info = np.ones(nb, dtype=np.int64)           # not certified
-            sel = np.flatnonzero(ok)
-            if sel.size:
-                info[sel] = self._cholesky_test(ns[sel], ptrs[sel], -t[sel])
-            retry = [j for j in sel if info[j] != 0]
-            if retry:
-                # the factorisation overwrote the block: form it again, then
-                # test a lower shift
-                for j in retry:
-                    t2[j] = theta[j] - max(8 * resid[j], 0.05 * max(1.0, abs(theta[j])))
-                    cnt, m, n, i = items[j]
-                    self._dress(m, n, i, cnt, wgt, dig, doff, int(ptrs[j]))
-                sel = np.array(retry)
-                info[sel] = self._cholesky_test(ns[sel], ptrs[sel], -t2[sel])
-            for j, (cnt, m, n, i) in enumerate(items):
-                if ok[j] and info[j] == 0:
-                    slack = 4.0 * (cnt + 1) * cnt * _U64 * (anorm[j] + abs(t2[j]))
-                    lam[(m, n, self.pairs[(m, n)]["segq"][i])] = float(t2[j] - slack)
-            del buf
-        self.block_bounds = lam
-        cs = [sorted(set(int(x) for x in c)) for c in self.charges]
-        rest = {}
-        for (m, n), _ in pairs:
-            tot = {0}
-            for f in range(nf):
-                if f not in (m, n):
-                    tot = {x + y for x in tot for y in cs[f]}
-            rest[(m, n)] = tot
-        out = {}
-        for Q in self.total_charges():
-            lines = [[(lam.get((m, n, s), float("-inf")), s * wgt) for s in P["segq"] if Q - s in rest[(m, n)]]
-                     for (m, n), P in pairs]
-            out[Q] = charge_sector_bound(lines, Q)
-        return out
-
-    def _dress(self, m, n, i, cnt, wgt, dig, doff, out):
-        """Block i of pair (m, n) with its monomer shares into out (cnt x cnt)."""
-        P = self.pairs[(m, n)]
-        start = P["segs"][i][0]
-        blk = P["blocks"][i]
-        if blk.stride(1) != 1:
-            raise ValueError("pair block rows must be contiguous")
-        pa = dig.data_ptr() + 4 * (doff[(m, n)] + start)
-        pb = pa + 4 * P["order_np"].size
-        self.ctx.call("xr_pair_dress", cnt, blk.data_ptr(), blk.stride(0), pa, pb, self.H1[m].data_ptr(),
-                      self.dims[m], self.H1[n].data_ptr(), self.dims[n], wgt, out, cnt)
-
-    def _cholesky_test(self, ns, ptrs, shift):
-        """Adds shift[j] to the diagonal of every listed block, then
-        Cholesky-factorises them in place (xr_cholesky_test, 64 blocks per
-        call); returns the info codes (0: positive definite)."""
-        nb = len(ns)
-        ns = _arr(ns, np.int64)
-        ptrs = _arr(ptrs, np.uint64)
-        shift = _arr(shift, np.float64)
-        info = torch.empty(nb, dtype=torch.int32, device=self.device)
-        for c0 in range(0, nb, 64):
-            c1 = min(nb, c0 + 64)
-            self.ctx.call("xr_add_diagonal", c1 - c0, _ptr(ns[c0:c1]), _ptr(ptrs[c0:c1]), _ptr(ns[c0:c1]),
-                          _ptr(shift[c0:c1]))
-            self.ctx.call("xr_cholesky_test", c1 - c0, _ptr(ns[c0:c1]), _ptr(ptrs[c0:c1]), _ptr(ns[c0:c1]),
-                          info.data_ptr() + 4 * c0)
-        return info.cpu().numpy()
+        block's bound is -inf.  The certified block bounds are kept in
+        self.block_bounds ((m, n, s) -> bound)."""
+        job = _SectorBoundJob(self, steps, max_block)
+        job.step(block=True)
+        return job.out
 
     def lowest(self, k=1, tol=1e-9, max_iter=400, block=None, charges=None, chunk=4, margin=4.0,
                screen=True, verify=True, verify_tol=1e-4, bound=True):
@@ -554,7 +644,20 @@ class XROperator:
                 self.log.append((a["sp"].Q, a["sp"].size, a["w"], float(np.max(inf.get("rn", [0.0]))),
                                  a["iters"], how))
 
-        self.bounds = self.sector_bounds() if screen and bound and len(iterative) > 1 else {}
+        # the sector bounds are computed on a side stream while the sector of
+        # lowest diagonal energy (the likeliest home of the root) iterates
+        self.bounds = {}
+        job = None
+        if screen and bound and len(iterative) > 1:
+            job = _SectorBoundJob(self, BOUND_STEPS, BOUND_MAX_BLOCK, side=True)
+
+        def settle(block):
+            """Takes the bounds once the job is done (waiting for it when block)."""
+            nonlocal job
+            if job is not None and job.step(block):
+                self.bounds = job.out
+                job = None
+            return job is None
 
         def bounded(a, ub):
             """The sector's certified bound lies above ub: it is set aside."""
@@ -566,34 +669,44 @@ class XROperator:
             record(a, {}, "bounded-above")
             return True
 
+        def run(a):
+            """One turn of `chunk` Davidson iterations on sector a."""
+            sp, inf = a["sp"], {}
+            w, v = davidson(sp.matvec, None, sp.size, a["kk"], tol, min(chunk, max_iter - a["iters"]), a["b"],
+                            self.device, precond=sp.precondition,
+                            start=sp.guess(a["b"]) if a["w"] is None else None, strict=False,
+                            state=a["st"], stop_above=upper() if screen else None, stop_margin=margin,
+                            info=inf)
+            a["w"] = [float(x) for x in w]
+            a["iters"] += inf.get("iters", 0)
+            done = inf.get("abandoned", False) or inf.get("converged", False)
+            if not done and a["iters"] >= max_iter:
+                raise RuntimeError(f"Davidson did not converge in {max_iter} iterations in the sector of "
+                                   f"total charge {sp.Q} (residual {float(np.max(inf['rn'])):.2e})")
+            if done:
+                active.remove(a)
+                if inf.get("converged", False):
+                    a["st"].clear()             # free the subspace
+                    found.extend((float(w[i]), sp, v[:, i]) for i in range(w.shape[0]))
+                    record(a, inf, "converged")
+                elif verify:
+                    set_aside.append(a)         # keeps its subspace for the verification
+                else:
+                    a["st"].clear()
+                    record(a, inf, "dropped")
+
+        first = active[0] if active else None
         while active:
+            if not settle(False) and first in active:
+                run(first)
+                continue
+            settle(True)
             for a in list(active):
                 if bounded(a, upper()):
                     active.remove(a)
                     continue
-                sp, inf = a["sp"], {}
-                w, v = davidson(sp.matvec, None, sp.size, a["kk"], tol, min(chunk, max_iter - a["iters"]), a["b"],
-                                self.device, precond=sp.precondition,
-                                start=sp.guess(a["b"]) if a["w"] is None else None, strict=False,
-                                state=a["st"], stop_above=upper() if screen else None, stop_margin=margin,
-                                info=inf)
-                a["w"] = [float(x) for x in w]
-                a["iters"] += inf.get("iters", 0)
-                done = inf.get("abandoned", False) or inf.get("converged", False)
-                if not done and a["iters"] >= max_iter:
-                    raise RuntimeError(f"Davidson did not converge in {max_iter} iterations in the sector of "
-                                       f"total charge {sp.Q} (residual {float(np.max(inf['rn'])):.2e})")
-                if done:
-                    active.remove(a)
-                    if inf.get("converged", False):
-                        a["st"].clear()             # free the subspace
-                        found += [(float(w[i]), sp, v[:, i]) for i in range(w.shape[0])]
-                        record(a, inf, "converged")
-                    elif verify:
-                        set_aside.append(a)         # keeps its subspace for the verification
-                    else:
-                        a["st"].clear()
-                        record(a, inf, "dropped")
+                run(a)
+        settle(True)
         for a in set_aside:
             if bounded(a, upper(pending=False)):
                 continue
@@ -609,7 +722,7 @@ class XROperator:
                                 self.device, precond=sp.precondition, state=a["st"], info=inf)
                 a["iters"] += inf.get("iters", 0)
                 a["w"] = [float(x) for x in w]
-                found += [(float(w[i]), sp, v[:, i]) for i in range(w.shape[0])]
+                found.extend((float(w[i]), sp, v[:, i]) for i in range(w.shape[0]))
                 record(a, inf, "converged")
             else:
                 record(a, inf, "verified-above")
