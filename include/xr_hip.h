@@ -301,10 +301,10 @@ int xr_permute4(xr_ctx *ctx, const double *d_src, const int64_t *dims, const int
 /* Ingredients of the certified lower bound of a charge sector's lowest XR
  * eigenvalue (pair decomposition of the operator with a Lagrange multiplier
  * on the total charge):
- *   xr_pair_dress: out[r,c] = (B[r,c] + B[c,r])/2 + w [b_r == b_c] (hm[a_r,a_c]
- *     + hm[a_c,a_r])/2 + w [a_r == a_c] (hn[b_r,b_c] + hn[b_c,b_r])/2, the
+ *   xr_pair_dress: out[r,c] = (B[r,c] + B[c,r])/2 + wm [b_r == b_c] (hm[a_r,a_c]
+ *     + hm[a_c,a_r])/2 + wn [a_r == a_c] (hn[b_r,b_c] + hn[b_c,b_r])/2, the
  *     cnt x cnt fixed-charge pair block B over pair states (pa[r], pb[r])
- *     with the share w of both monomer matrices hm, hn;
+ *     with the shares wm, wn of the monomer matrices hm, hn;
  *   xr_lanczos: `steps` Lanczos steps (no reorthogonalisation) from a fixed
  *     pseudo-random start on each of nb <= 64 symmetric matrices (host
  *     arrays n, M of device pointers, ld): the tridiagonal entries
@@ -322,8 +322,8 @@ int xr_permute4(xr_ctx *ctx, const double *d_src, const int64_t *dims, const int
  * counterpart (the reference diagonalises the assembled XR matrix densely,
  * general-XRCC/excitonic.py:56-82, and has no charge-sector solver). */
 int xr_pair_dress(xr_ctx *ctx, int64_t cnt, const double *d_B, int64_t ldb, const int *d_pa, const int *d_pb,
-                  const double *d_hm, int64_t ldm, const double *d_hn, int64_t ldn, double w, double *d_out,
-                  int64_t ldo);
+                  const double *d_hm, int64_t ldm, const double *d_hn, int64_t ldn, double wm, double wn,
+                  double *d_out, int64_t ldo);
 int xr_lanczos(xr_ctx *ctx, int nb, const int64_t *n, const double *const *d_M, const int64_t *ld, int steps,
                double *d_alpha, double *d_beta, double *d_anorm);
 int xr_add_diagonal(xr_ctx *ctx, int nb, const int64_t *n, double *const *d_M, const int64_t *ld,

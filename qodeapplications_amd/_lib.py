@@ -69,7 +69,7 @@ SIGNATURES = {
     "xr_spin_blocks": (_I, [_P, _I, _P, _P, _P, _P, _P, _L]),
     "xr_spin_blocks_adjoint": (_I, [_P, _I, _P, _P, _P, _P, _P, _L]),
     "xr_permute4": (_I, [_P, _P, _P, _P, _P, _I]),
-    "xr_pair_dress": (_I, [_P, _L, _P, _L, _P, _P, _P, _L, _P, _L, _D, _P, _L]),
+    "xr_pair_dress": (_I, [_P, _L, _P, _L, _P, _P, _P, _L, _P, _L, _D, _D, _P, _L]),
     "xr_lanczos": (_I, [_P, _I, _P, _P, _P, _I, _P, _P, _P]),
     "xr_add_diagonal": (_I, [_P, _I, _P, _P, _P, _P]),
     "xr_cholesky_test": (_I, [_P, _I, _P, _P, _P, _P]),

@@ -9,16 +9,16 @@
 enum { WS_LZ_VEC = 91 };
 
 // out[r, c] = (B[r, c] + B[c, r]) / 2
-//           + w [b_r == b_c] (hm[a_r, a_c] + hm[a_c, a_r]) / 2
-//           + w [a_r == a_c] (hn[b_r, b_c] + hn[b_c, b_r]) / 2
+//           + wm [b_r == b_c] (hm[a_r, a_c] + hm[a_c, a_r]) / 2
+//           + wn [a_r == a_c] (hn[b_r, b_c] + hn[b_c, b_r]) / 2
 // for the cnt x cnt block B over pair states (a_r, b_r): the symmetric part of
-// the pair block plus the share w of both monomer terms.  One 32 x 32 output
-// tile per workgroup; the transposed input tile goes through LDS.
+// the pair block plus the shares wm, wn of the two monomer terms.  One 32 x 32
+// output tile per workgroup; the transposed input tile goes through LDS.
 __global__ __launch_bounds__(256) void k_pair_dress(int64_t cnt, const double* __restrict__ B, int64_t ldb,
                                                     const int* __restrict__ pa, const int* __restrict__ pb,
                                                     const double* __restrict__ hm, int64_t ldm,
-                                                    const double* __restrict__ hn, int64_t ldn, double w,
-                                                    double* __restrict__ out, int64_t ldo) {
+                                                    const double* __restrict__ hn, int64_t ldn, double wm,
+                                                    double wn, double* __restrict__ out, int64_t ldo) {
     __shared__ double T[32][33];
     const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -32,8 +32,8 @@ __global__ __launch_bounds__(256) void k_pair_dress(int64_t cnt, const double* _
         if (r < cnt && cc < cnt) {
             double v = 0.5 * (B[r * ldb + cc] + T[tx][i]);
             const int64_t ar = pa[r], br = pb[r], ac = pa[cc], bc = pb[cc];
-            if (br == bc) v += w * (0.5 * (hm[ar * ldm + ac] + hm[ac * ldm + ar]));
-            if (ar == ac) v += w * (0.5 * (hn[br * ldn + bc] + hn[bc * ldn + br]));
+            if (br == bc) v += wm * (0.5 * (hm[ar * ldm + ac] + hm[ac * ldm + ar]));
+            if (ar == ac) v += wn * (0.5 * (hn[br * ldn + bc] + hn[bc * ldn + br]));
             out[r * ldo + cc] = v;
         }
     }
@@ -186,8 +186,8 @@ __global__ __launch_bounds__(256) void k_add_diagonal(DiagBatch db) {
 }
 
 extern "C" int xr_pair_dress(xr_ctx* c, int64_t cnt, const double* B, int64_t ldb, const int* pa, const int* pb,
-                             const double* hm, int64_t ldm, const double* hn, int64_t ldn, double w, double* out,
-                             int64_t ldo) {
+                             const double* hm, int64_t ldm, const double* hn, int64_t ldn, double wm, double wn,
+                             double* out, int64_t ldo) {
     XR_ENTRY(c);
     if (cnt < 0) return xr_fail(c, XR_EINVAL, "negative size");
     if (cnt == 0) return XR_OK;
@@ -196,8 +196,8 @@ extern "C" int xr_pair_dress(xr_ctx* c, int64_t cnt, const double* B, int64_t ld
     const int64_t t = ceil_div(cnt, 32);
     if (t > 65535) return xr_fail(c, XR_EUNSUPPORTED, "block too large for the dress grid");
     ProfScope prof(c, XR_K_OTHER, 0.0, 24.0 * cnt * cnt);
-    k_pair_dress<<<dim3((unsigned)t, (unsigned)t), 256, 0, c->stream>>>(cnt, B, ldb, pa, pb, hm, ldm, hn, ldn, w,
-                                                                        out, ldo);
+    k_pair_dress<<<dim3((unsigned)t, (unsigned)t), 256, 0, c->stream>>>(cnt, B, ldb, pa, pb, hm, ldm, hn, ldn, wm,
+                                                                        wn, out, ldo);
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }

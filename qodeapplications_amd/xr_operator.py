@@ -140,8 +140,8 @@ class _SectorBoundJob:
     and step() only advances past stages that have finished, so the stages
     run beside the caller's own GPU work."""
 
-    def __init__(self, op, steps, max_block, side=False):
-        self.op, self.steps = op, steps
+    def __init__(self, op, steps, max_block, shares, side=False):
+        self.op, self.steps, self.shares = op, steps, shares
         self.done, self.out, self.lam = False, {}, {}
         op.block_bounds = {}
         nf = op.nf
@@ -212,7 +212,8 @@ class _SectorBoundJob:
         pa = self.dig.data_ptr() + 4 * (self.doff[(m, n)] + start)
         pb = pa + 4 * P["order_np"].size
         self.ctx.call("xr_pair_dress", cnt, blk.data_ptr(), blk.stride(0), pa, pb, op.H1[m].data_ptr(),
-                      op.dims[m], op.H1[n].data_ptr(), op.dims[n], self.wgt, int(self.ptrs[j]), cnt)
+                      op.dims[m], op.H1[n].data_ptr(), op.dims[n], self.shares[(m, (m, n))],
+                      self.shares[(n, (m, n))], int(self.ptrs[j]), cnt)
 
     def _cholesky(self, sel, shift):
         """Queues: shift[q] added to the diagonal of block sel[q], the
@@ -525,6 +526,18 @@ class XROperator:
         return out
 
     # -- eigenpairs -----------------------------------------------------------
+    def _sector_sizes(self):
+        """{Q: number of product states of total charge Q}."""
+        sizes = {0: 1}
+        for c in self.charges:
+            vals, cnts = np.unique(c, return_counts=True)
+            nxt = {}
+            for t, n in sizes.items():
+                for v, cnt in zip(vals, cnts):
+                    nxt[t + int(v)] = nxt.get(t + int(v), 0) + n * int(cnt)
+            sizes = nxt
+        return sizes
+
     def sector_space(self, Q):
         """The states of total charge Q in the compact sector layout."""
         return SectorSpace(self, Q)
@@ -542,9 +555,10 @@ class XROperator:
         there are at least 3 fragments and every pair has its term).
 
         Every fragment lies in the nf - 1 pairs (m, n), so
-        H = sum_(m<n) h_mn with h_mn = H2_mn + (H1_m + H1_n) / (nf - 1), and
-        the total charge N = sum_m n_m is Q on the sector.  For a unit
-        vector v of the sector and any mu,
+        H = sum_(m<n) h_mn with h_mn = H2_mn + w_m,mn H1_m + w_n,mn H1_n for
+        any shares w >= 0 with sum_n w_m,mn = 1 (monomer_shares), and the
+        total charge N = sum_m n_m is Q on the sector.  For a unit vector v
+        of the sector and any mu,
         <v|H|v> = -mu Q + sum_(m<n) <v| h_mn + mu (n_m + n_n) / (nf - 1) |v>.
         Term (m, n) conserves the pair charge s = n_m + n_n, and on the
         sector s takes only the values the other fragments' charges complete
@@ -563,9 +577,49 @@ class XROperator:
         |theta|)); then, and for blocks of more than max_block states, the
         block's bound is -inf.  The certified block bounds are kept in
         self.block_bounds ((m, n, s) -> bound)."""
-        job = _SectorBoundJob(self, steps, max_block)
+        sq = torch.empty(max(1, self._n_blocks()), dtype=torch.float64, device=self.device)
+        self._block_sqnorms(sq)
+        job = _SectorBoundJob(self, steps, max_block, self.monomer_shares_from(sq.cpu().numpy()))
         job.step(block=True)
         return job.out
+
+    def _n_blocks(self):
+        return sum(len(P["blocks"]) for P in self.pairs.values())
+
+    def _block_sqnorms(self, out):
+        """Queues out[j] = the squared Frobenius norm of pair block j (pairs
+        in order, blocks in charge order; xr_column_norms on the block as one
+        contiguous column)."""
+        j = 0
+        for P in self.pairs.values():
+            for blk in P["blocks"]:
+                n2 = blk.numel()
+                if not blk.is_contiguous():
+                    raise ValueError("pair blocks must be contiguous")
+                self.ctx.call("xr_column_norms", n2, 1, blk.data_ptr(), 1, out.data_ptr() + 8 * j, 1)
+                j += 1
+
+    def monomer_shares_from(self, sq):
+        """{(m, (m, n)): w}: fragment m's monomer term is shared over its
+        pairs in proportion to the pairs' Frobenius norms (sq: the squared
+        norms of the blocks, as _block_sqnorms orders them), uniform where
+        every pair of m is zero.  Any shares give a valid bound; weighting by
+        the coupling strength makes the strongly coupled pairs pay more of
+        the monomer excitation energy their coupling exploits (on the
+        4 x 13 chain it certifies one more sector than uniform shares)."""
+        norm, j = {}, 0
+        for p, P in self.pairs.items():
+            k = len(P["blocks"])
+            norm[p] = float(np.sqrt(max(0.0, float(np.sum(sq[j:j + k])))))
+            j += k
+        shares = {}
+        for m in range(self.nf):
+            ps = [p for p in self.pairs if m in p]
+            tot = sum(norm[p] for p in ps)
+            for p in ps:
+                shares[(m, p)] = norm[p] / tot if tot > 0 and np.isfinite(tot) else 1.0 / len(ps)
+        self.monomer_shares = shares
+        return shares
 
     def lowest(self, k=1, tol=1e-9, max_iter=400, block=None, charges=None, chunk=4, margin=4.0,
                screen=True, verify=True, verify_tol=1e-4, bound=True):
@@ -613,7 +667,14 @@ class XROperator:
         active = []
         iterative = []
         set_aside = []
-        for Q in (charges if charges is not None else self.total_charges()):
+        qs = charges if charges is not None else self.total_charges()
+        # the sector bounds are computed on a side stream while the sector of
+        # lowest diagonal energy (the likeliest home of the root) iterates
+        self.bounds = {}
+        job = None
+        sizes = self._sector_sizes()
+        bounding = screen and bound and sum(1 for Q in qs if sizes.get(Q, 0) > DENSE_SECTOR) > 1
+        for Q in qs:
             sp = self.sector_space(Q)
             kk = min(k, sp.size)
             if sp.size <= DENSE_SECTOR:
@@ -625,8 +686,17 @@ class XROperator:
                     self.log.append((sp.Q, sp.size, [float(w[i]) for i in range(kk)], 0.0, 0, "dense"))
             else:
                 iterative.append((sp, kk))
-        if iterative:                 # the sectors' lowest diagonal energies in one transfer
-            lams = torch.stack([sp.lam.min() for sp, _ in iterative]).cpu().numpy()
+        if iterative:                 # the sectors' lowest diagonal energies (and the pair blocks' norms) in one transfer
+            parts = [torch.stack([sp.lam.min() for sp, _ in iterative])]
+            if bounding:
+                sq = torch.empty(max(1, self._n_blocks()), dtype=torch.float64, device=self.device)
+                self._block_sqnorms(sq)
+                parts.append(sq)
+            h = torch.cat(parts).cpu().numpy()
+            lams = h[:len(iterative)]
+            if bounding:
+                job = _SectorBoundJob(self, BOUND_STEPS, BOUND_MAX_BLOCK,
+                                      self.monomer_shares_from(h[len(iterative):]), side=True)
             for (sp, kk), lam in zip(iterative, lams):
                 active.append({"sp": sp, "kk": kk, "b": block or _block_size(kk, sp.size), "st": {},
                                "w": None, "iters": 0, "lam": float(lam)})
@@ -643,13 +713,6 @@ class XROperator:
             if self.log is not None:
                 self.log.append((a["sp"].Q, a["sp"].size, a["w"], float(np.max(inf.get("rn", [0.0]))),
                                  a["iters"], how))
-
-        # the sector bounds are computed on a side stream while the sector of
-        # lowest diagonal energy (the likeliest home of the root) iterates
-        self.bounds = {}
-        job = None
-        if screen and bound and len(iterative) > 1:
-            job = _SectorBoundJob(self, BOUND_STEPS, BOUND_MAX_BLOCK, side=True)
 
         def settle(block):
             """Takes the bounds once the job is done (waiting for it when block)."""

@@ -55,13 +55,14 @@ def test_pair_dress_matches_torch():
     hm = torch.randn(dm, dm, dtype=torch.float64, generator=g).cuda()
     hn = torch.randn(dn, dn, dtype=torch.float64, generator=g).cuda()
     out = torch.full((cnt, cnt + 3), 9.0, dtype=torch.float64, device="cuda")
-    w = 0.37
-    ctx.call("xr_pair_dress", cnt, ptr(B), cnt + 5, ptr(pa), ptr(pb), ptr(hm), dm, ptr(hn), dn, w, ptr(out), cnt + 3)
+    wm, wn = 0.37, 0.81
+    ctx.call("xr_pair_dress", cnt, ptr(B), cnt + 5, ptr(pa), ptr(pb), ptr(hm), dm, ptr(hn), dn, wm, wn, ptr(out),
+             cnt + 3)
     Bs = B[:, :cnt]
     a, b = pa.long(), pb.long()
     hms, hns = 0.5 * (hm + hm.T), 0.5 * (hn + hn.T)
-    ref = 0.5 * (Bs + Bs.T) + w * hms[a][:, a] * (b[:, None] == b[None, :]) \
-        + w * hns[b][:, b] * (a[:, None] == a[None, :])
+    ref = 0.5 * (Bs + Bs.T) + wm * hms[a][:, a] * (b[:, None] == b[None, :]) \
+        + wn * hns[b][:, b] * (a[:, None] == a[None, :])
     assert float((out[:, :cnt] - ref).abs().max()) < 1e-14
     assert bool((out[:, cnt:] == 9.0).all())
 
@@ -196,7 +197,7 @@ def test_sector_bounds_below_exact(orbs, ns):
     bounds = op.sector_bounds()
     lows = _sector_lows(op)
     assert set(bounds) == set(lows)
-    wgt = 1.0 / (op.nf - 1)
+    shares = op.monomer_shares
     assert op.block_bounds
     for (m, n, s), lb in op.block_bounds.items():
         P = op.pairs[(m, n)]
@@ -206,8 +207,8 @@ def test_sector_bounds_below_exact(orbs, ns):
         a, b = idx // op.dims[n], idx % op.dims[n]
         B = P["blocks"][i]
         hm, hn = 0.5 * (op.H1[m] + op.H1[m].T), 0.5 * (op.H1[n] + op.H1[n].T)
-        A = 0.5 * (B + B.T) + wgt * hm[a][:, a] * (b[:, None] == b[None, :]) \
-            + wgt * hn[b][:, b] * (a[:, None] == a[None, :])
+        A = 0.5 * (B + B.T) + shares[(m, (m, n))] * hm[a][:, a] * (b[:, None] == b[None, :]) \
+            + shares[(n, (m, n))] * hn[b][:, b] * (a[:, None] == a[None, :])
         lam = float(torch.linalg.eigvalsh(A)[0])
         assert lb <= lam + 1e-12, (m, n, s, lb, lam)
         assert lam - lb < 0.05, (m, n, s, lb, lam)

@@ -12,6 +12,8 @@ s/(nf - 1) per pair block of charge sum s, must
   * equal it for two fragments (the one pair term is all of H);
   * beat the multiplier-free bound (mu = 0) on some sector, which shows the
     multiplier does work;
+  * stay below it for any other split of the monomer terms over the pairs
+    (random shares summing to 1 per fragment);
   * equal the optimum of the linear programme it is the dual of (scipy's
     linprog: convex weights x_ps >= 0 of each pair's blocks with
     sum_s x_ps = 1 and sum_ps x_ps c_ps = Q, minimising sum_ps x_ps lam_ps),
@@ -76,13 +78,16 @@ def _exact(ch, h, V):
     return out
 
 
-def _pair_lams(ch, h, V):
-    """Lowest eigenvalue of h_mn = V_mn + (h_m + h_n)/(nf - 1) per charge sum."""
+def _pair_lams(ch, h, V, shares=None):
+    """Lowest eigenvalue of h_mn = V_mn + w_m,mn h_m + w_n,mn h_n per charge
+    sum (shares w; uniform 1/(nf - 1) by default)."""
     nf = len(ch)
     lam = {}
     for (m, n), Vmn in V.items():
         dn = ch[n].size
-        A = Vmn + (np.kron(h[m], np.eye(dn)) + np.kron(np.eye(ch[m].size), h[n])) / (nf - 1)
+        wm = shares[(m, (m, n))] if shares else 1.0 / (nf - 1)
+        wn = shares[(n, (m, n))] if shares else 1.0 / (nf - 1)
+        A = Vmn + wm * np.kron(h[m], np.eye(dn)) + wn * np.kron(np.eye(ch[m].size), h[n])
         s = (ch[m][:, None] + ch[n][None, :]).ravel()
         for q in sorted(set(s.tolist())):
             sel = np.where(s == q)[0]
@@ -139,6 +144,23 @@ def test_sector_bound_against_brute_force(nf, seed):
     if nf > 2:
         b0 = {Q: charge_sector_bound(_lines(ch, lam, Q, lambda s: 0.0), Q) for Q in exact}
         assert any(b[Q] > b0[Q] + 1e-6 for Q in exact)
+
+
+@pytest.mark.parametrize("nf,seed", [(3, 6), (4, 7)])
+def test_sector_bound_with_any_monomer_shares(nf, seed):
+    rng = np.random.default_rng(seed)
+    ch, h, V = _system(rng, nf)
+    exact = _exact(ch, h, V)
+    for _ in range(3):
+        shares = {}
+        for m in range(nf):
+            ps = [p for p in V if m in p]
+            w = rng.dirichlet(np.ones(len(ps)))
+            shares.update({(m, p): float(x) for p, x in zip(ps, w)})
+        lam = _pair_lams(ch, h, V, shares)
+        for Q in exact:
+            b = charge_sector_bound(_lines(ch, lam, Q, lambda s: s / (nf - 1)), Q)
+            assert b <= exact[Q] + 1e-10, (Q, b, exact[Q])
 
 
 def test_planted_slips_exceed_the_exact_value():
