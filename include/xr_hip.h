@@ -331,6 +331,16 @@ int xr_add_diagonal(xr_ctx *ctx, int nb, const int64_t *n, double *const *d_M, c
 int xr_cholesky_test(xr_ctx *ctx, int nb, const int64_t *n, double *const *d_M, const int64_t *ld,
                      int *d_info);
 
+/* Symmetric product C = alpha A A^T (trans = 0, A n x K) or alpha A^T A
+ * (trans = 1, A K x n), row-major: the tiles on and above the diagonal on the
+ * FP64 matrix cores, then the lower triangle mirrored (C exactly symmetric,
+ * overwritten).  The pair densities of the XR gradient, sum over the other
+ * fragments of v v^T (the contractions of the eigenvector with itself over
+ * the other fragments' indices, StateSpaceOptimizer/state_gradients.py:193-199,
+ * for every charge sum of a pair at once). */
+int xr_gemm_syrk(xr_ctx *ctx, int trans, int64_t n, int64_t K, double alpha, const double *d_A, int64_t lda,
+                 double *d_C, int64_t ldc);
+
 /* The product as the two-electron contractions run it in precision `mode`
  * (XR_PREC_*): FP64; split-FP32; or in XR_PREC_SPLIT_I8 the split-int8
  * product where it pays (at least 2^33 flops, every dimension >= 128 and

@@ -61,6 +61,7 @@ SIGNATURES = {
     "xr_gemm_f64": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
     "xr_gemm_split_f32": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
     "xr_gemm_split_i8": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
+    "xr_gemm_syrk": (_I, [_P, _I, _L, _L, _D, _P, _L, _P, _L]),
     "xr_gemm_mode": (_I, [_P, _I, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
     "xr_precond_divide": (_I, [_P, _L, _L, _P, _L, _P, _L, _P, _P, _D, _D]),
     "xr_residual": (_I, [_P, _L, _I, _I, _P, _L, _P, _L, _P, _P, _L, _P]),

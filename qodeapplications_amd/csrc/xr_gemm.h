@@ -11,6 +11,8 @@ int xr_dgemm_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, 
 // C = alpha A^T A (A: K x n, C: n x n): upper tiles, then the mirror.
 int xr_dgemm_syrk_tn(xr_ctx* c, int64_t n, int64_t K, double alpha, const double* A, int64_t lda, double* C,
                      int64_t ldc);
+int xr_dgemm_syrk_nt(xr_ctx* c, int64_t n, int64_t K, double alpha, const double* A, int64_t lda, double* C,
+                     int64_t ldc);
 
 // Split-FP32 product (hi/lo FP32 MFMA, FP64 accumulation) and the
 // precision-switched GEMM used by the two-electron contractions.

@@ -925,6 +925,32 @@ int xr_dgemm_syrk_tn(xr_ctx* c, int64_t n, int64_t K, double alpha, const double
     return XR_OK;
 }
 
+// C = alpha A A^T for A (n x K, leading dimension lda): the same upper-tile
+// product and mirror.
+int xr_dgemm_syrk_nt(xr_ctx* c, int64_t n, int64_t K, double alpha, const double* A, int64_t lda, double* C,
+                     int64_t ldc) {
+    if (n <= 0) return XR_OK;
+    int rc = launch_tiles<128, 128>(c, 0, 1, n, n, K, alpha, A, lda, 0, A, lda, 0, 0.0, C, ldc, 0, 1, 0, true);
+    if (rc) return rc;
+    const int64_t t = ceil_div(n, 32);
+    if (t > 65535) return xr_fail(c, XR_EUNSUPPORTED, "matrix too large for the mirror grid");
+    ProfScope prof(c, XR_K_OTHER, 0.0, 8.0 * n * n);
+    k_mirror_upper<<<dim3((unsigned)t, (unsigned)t), dim3(32, 8), 0, c->stream>>>(C, n, ldc);
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}
+
+extern "C" int xr_gemm_syrk(xr_ctx* c, int trans, int64_t n, int64_t K, double alpha, const double* A, int64_t lda,
+                            double* C, int64_t ldc) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (n < 0 || K < 0) return xr_fail(c, XR_EINVAL, "negative size");
+    if (ldc < n) return xr_fail(c, XR_ESHAPE, "ldc < n");
+    if ((trans ? lda < n : lda < K) && n > 0 && K > 0) return xr_fail(c, XR_ESHAPE, "lda too small");
+    if (n == 0) return XR_OK;
+    return trans ? xr_dgemm_syrk_tn(c, n, K, alpha, A, lda, C, ldc) : xr_dgemm_syrk_nt(c, n, K, alpha, A, lda, C, ldc);
+}
+
 // One explicit configuration: tile 0 = 128x128, 1 = 128x64, 2 = 64x128,
 // 3 = 64x64; splits = K splits (0 = the heuristic).
 static int launch_cfg(xr_ctx* c, int tile, int splits, int ta, int tb, int64_t M, int64_t N, int64_t K,

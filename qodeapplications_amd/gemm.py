@@ -74,6 +74,24 @@ def mm(A, B, transA=0, transB=0, alpha=1.0, out=None, beta=0.0, split=False, mod
     return out
 
 
+def syrk(A, trans=0, alpha=1.0, out=None):
+    """alpha A A^T (trans = 0) or alpha A^T A (trans = 1) for a 2-D row-major
+    float64 CUDA tensor: the upper tiles on the matrix cores, the lower
+    triangle mirrored (xr_gemm_syrk; the result is exactly symmetric)."""
+    if A.dim() != 2 or A.dtype != torch.float64 or not A.is_cuda:
+        raise ValueError("syrk needs a 2-D float64 CUDA tensor")
+    if A.stride(1) != 1:
+        A = A.contiguous()
+    n, K = (A.shape[1], A.shape[0]) if trans else (A.shape[0], A.shape[1])
+    if out is None:
+        out = torch.empty(n, n, dtype=torch.float64, device=A.device)
+    if out.stride(1) != 1 or tuple(out.shape) != (n, n):
+        raise ValueError("out must be row-major (n, n)")
+    context(A.device.index).call("xr_gemm_syrk", int(trans), n, K, float(alpha), ptr(A), A.stride(0), ptr(out),
+                                 out.stride(0))
+    return out
+
+
 def permute4(src, perm, out=None, accumulate=False):
     """out = src.permute(perm) as a contiguous tensor (or out += it) for a
     contiguous 4-index float64 CUDA tensor, one library pass (xr_permute4)."""

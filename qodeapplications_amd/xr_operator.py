@@ -27,7 +27,7 @@ import torch
 
 from ._lib import context, ptr
 from .eigen import davidson, precond_divide, upload
-from .gemm import gemm_f64, mm
+from .gemm import gemm_f64, mm, syrk
 
 
 DENSE_SECTOR = 2048
@@ -821,7 +821,7 @@ class XROperator:
         blocks = []
         for start, cnt in P["segs"]:
             S = Vp[start:start + cnt]
-            blocks.append(mm(S, S, transB=1))
+            blocks.append(syrk(S))
         return blocks
 
     def pair_gather(self, G_blocks, m, n, pos, k1, k2, swapped):
@@ -1063,7 +1063,7 @@ class SectorSpace:
         G = {}
         for s, boff, R, W in pl["pair_sums"][pi]:
             Ps = P[boff // 8: boff // 8 + R * W].view(R, W)
-            G[s] = mm(Ps, Ps, transB=1)
+            G[s] = syrk(Ps)
         Pp = op.pairs[(m, n)]
         blocks = []
         for (start, cnt), q in zip(Pp["segs"], Pp["segq"]):

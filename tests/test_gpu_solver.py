@@ -91,3 +91,20 @@ def test_permute4_matches_torch(perm):
     ref = src.permute(*perm).contiguous()
     assert torch.equal(out, ref)
     assert torch.equal(acc, acc0 + ref)
+
+
+@pytest.mark.parametrize("trans,n,K", [(0, 1, 1), (0, 130, 77), (1, 130, 77), (0, 3072, 3072), (1, 257, 4000)])
+def test_syrk_matches_full_product(trans, n, K):
+    """The symmetric product equals the full product to rounding and is
+    exactly symmetric; rows of A given with a wider leading dimension."""
+    from qodeapplications_amd.gemm import mm, syrk
+    g = torch.Generator().manual_seed(n + K)
+    shape = (K, n) if trans else (n, K)
+    buf = torch.randn(shape[0], shape[1] + 3, dtype=torch.float64, generator=g).cuda()
+    A = buf[:, :shape[1]]
+    C = syrk(A, trans=trans, alpha=0.5)
+    ref = mm(A, A, transA=1, alpha=0.5) if trans else mm(A, A, transB=1, alpha=0.5)
+    torch.cuda.synchronize()
+    assert torch.equal(C, C.T)
+    scale = float(ref.abs().max())
+    assert float((C - ref).abs().max()) <= 1e-13 * max(1.0, scale) * max(1, K) ** 0.5
