@@ -276,6 +276,22 @@ int xr_column_norms(xr_ctx *ctx, int64_t n, int b, const double *d_A, int64_t ld
 int xr_transpose(xr_ctx *ctx, int64_t rows, int64_t cols, const double *d_A, int64_t lda, double *d_B,
                  int64_t ldb);
 
+/* The Davidson preconditioner's diagonal of a charge sector in the product
+ * eigenbasis, and the pair terms' contribution to it:
+ *   xr_sector_diag: for each of ntup tuples q (host arrays: dims[q*nf + m],
+ *     off[q], device pointers e[q*nf + m] to the monomer diagonals and
+ *     pd[q*np + p] to the pair diagonals, pairs (m < n) in lexicographic
+ *     order, np = nf(nf-1)/2), out[off[q] + i] = sum_m e_m[j_m] +
+ *     sum_(m<n) pd_mn[j_m * dims_n + j_n] for the row-major digits j of i,
+ *     summed in that order; nf <= 8;
+ *   xr_column_dots: out[j] = sum_i A[i,j] B[i,j] (rows x cols, fixed-order
+ *     sums), the diagonal of K^T H K from K and H K.
+ * No reference counterpart (the reference diagonalises densely). */
+int xr_sector_diag(xr_ctx *ctx, int nf, int ntup, const int64_t *dims, const int64_t *off,
+                   const double *const *d_e, const double *const *d_pd, double *d_out);
+int xr_column_dots(xr_ctx *ctx, int64_t rows, int64_t cols, const double *d_A, int64_t lda, const double *d_B,
+                   int64_t ldb, double *d_out);
+
 /* Spin-orbital integral blocks cut from spatial integrals (alpha spin
  * orbitals of a fragment first), one block per list entry: kinds[i] = 1, a
  * one-body block out[(s,p),(s',q)] = d(s,s') A[o0+p, o1+q] of the n x n

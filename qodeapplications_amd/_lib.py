@@ -67,6 +67,8 @@ SIGNATURES = {
     "xr_residual": (_I, [_P, _L, _I, _I, _P, _L, _P, _L, _P, _P, _L, _P]),
     "xr_column_norms": (_I, [_P, _L, _I, _P, _L, _P, _I]),
     "xr_transpose": (_I, [_P, _L, _L, _P, _L, _P, _L]),
+    "xr_sector_diag": (_I, [_P, _I, _I, _P, _P, _P, _P, _P]),
+    "xr_column_dots": (_I, [_P, _L, _L, _P, _L, _P, _L, _P]),
     "xr_spin_blocks": (_I, [_P, _I, _P, _P, _P, _P, _P, _L]),
     "xr_spin_blocks_adjoint": (_I, [_P, _I, _P, _P, _P, _P, _P, _L]),
     "xr_permute4": (_I, [_P, _P, _P, _P, _P, _I]),

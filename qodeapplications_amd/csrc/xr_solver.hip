@@ -204,3 +204,118 @@ extern "C" int xr_transpose(xr_ctx* c, int64_t rows, int64_t cols, const double*
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Diagonal of the XR operator in the product eigenbasis, one sector tuple at
+// a time: out[i] = sum_m e_m[j_m] + sum_(m<n) pd_mn[j_m, j_n] for the
+// row-major digits (j_0, ..., j_{nf-1}) of i, the terms summed in that order.
+// ---------------------------------------------------------------------------
+#define XR_DIAG_MAXF 8
+enum { WS_COLDOT = 93 };
+
+struct DiagTuple {
+    int nf;
+    int64_t size;
+    int64_t dims[XR_DIAG_MAXF];
+    const double* e[XR_DIAG_MAXF];
+    const double* pd[XR_DIAG_MAXF * (XR_DIAG_MAXF - 1) / 2];
+};
+
+__global__ __launch_bounds__(256) void k_sector_diag(DiagTuple t, double* __restrict__ out) {
+    XR_GRID_STRIDE(i, t.size) {
+        int64_t j[XR_DIAG_MAXF];
+        int64_t r = i;
+        for (int m = t.nf - 1; m >= 0; m--) {
+            const int64_t q = r / t.dims[m];
+            j[m] = r - q * t.dims[m];
+            r = q;
+        }
+        double v = 0.0;
+        for (int m = 0; m < t.nf; m++) v += t.e[m][j[m]];
+        int p = 0;
+        for (int m = 0; m < t.nf; m++)
+            for (int n = m + 1; n < t.nf; n++, p++) v += t.pd[p][j[m] * t.dims[n] + j[n]];
+        out[i] = v;
+    }
+}
+
+extern "C" int xr_sector_diag(xr_ctx* c, int nf, int ntup, const int64_t* dims, const int64_t* off,
+                              const double* const* e, const double* const* pd, double* out) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (nf < 1 || nf > XR_DIAG_MAXF) return xr_fail(c, XR_EUNSUPPORTED, "1 to 8 fragments");
+    if (ntup < 0) return xr_fail(c, XR_EINVAL, "negative tuple count");
+    if (ntup == 0) return XR_OK;
+    if (!dims || !off || !e || (nf > 1 && !pd) || !out) return xr_fail(c, XR_EINVAL, "null argument");
+    const int np = nf * (nf - 1) / 2;
+    for (int q = 0; q < ntup; q++) {
+        DiagTuple t{};
+        t.nf = nf;
+        t.size = 1;
+        for (int m = 0; m < nf; m++) {
+            const int64_t d = dims[(int64_t)q * nf + m];
+            if (d < 1) return xr_fail(c, XR_EINVAL, "empty tuple axis");
+            t.dims[m] = d;
+            t.size *= d;
+            t.e[m] = e[(int64_t)q * nf + m];
+            if (!t.e[m]) return xr_fail(c, XR_EINVAL, "null monomer diagonal");
+        }
+        for (int p = 0; p < np; p++) {
+            t.pd[p] = pd[(int64_t)q * np + p];
+            if (!t.pd[p]) return xr_fail(c, XR_EINVAL, "null pair diagonal");
+        }
+        ProfScope prof(c, XR_K_OTHER, 0.0, 8.0 * t.size);
+        k_sector_diag<<<xr_grid(t.size), 256, 0, c->stream>>>(t, out + off[q]);
+        XR_LAUNCH_CHECK(c);
+    }
+    return XR_OK;
+}
+
+// out[j] = sum_i A[i, j] B[i, j] for rows x cols blocks: partial sums over
+// chunks of 64 rows (one thread per column, in row order), then the chunks
+// in order (deterministic).
+__global__ __launch_bounds__(256) void k_coldot_part(int64_t rows, int64_t cols, const double* __restrict__ A,
+                                                     int64_t lda, const double* __restrict__ B, int64_t ldb,
+                                                     double* __restrict__ part) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.y * 64;
+    if (j >= cols) return;
+    const int64_t r1 = r0 + 64 < rows ? r0 + 64 : rows;
+    double s = 0.0;
+    for (int64_t i = r0; i < r1; i++) s += A[i * lda + j] * B[i * ldb + j];
+    part[(int64_t)blockIdx.y * cols + j] = s;
+}
+
+__global__ __launch_bounds__(256) void k_coldot_final(const double* __restrict__ part, int64_t nchunk, int64_t cols,
+                                                      double* __restrict__ out) {
+    XR_GRID_STRIDE(j, cols) {
+        double s = 0.0;
+        for (int64_t q = 0; q < nchunk; q++) s += part[q * cols + j];
+        out[j] = s;
+    }
+}
+
+extern "C" int xr_column_dots(xr_ctx* c, int64_t rows, int64_t cols, const double* A, int64_t lda, const double* B,
+                              int64_t ldb, double* out) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (rows < 0 || cols < 0) return xr_fail(c, XR_EINVAL, "negative size");
+    if (cols == 0) return XR_OK;
+    if (lda < cols || ldb < cols) return xr_fail(c, XR_ESHAPE, "leading dimension below the column count");
+    if (!A || !B || !out) return xr_fail(c, XR_EINVAL, "null operand");
+    const int64_t nchunk = std::max<int64_t>(1, ceil_div(rows, 64));
+    if (nchunk > 65535 || ceil_div(cols, 256) > 0x7fffffff) return xr_fail(c, XR_EUNSUPPORTED, "block too large");
+    double* part = (double*)xr_workspace(c, WS_COLDOT, sizeof(double) * (size_t)(nchunk * cols));
+    if (!part) return xr_fail(c, XR_ENOMEM, "workspace (column dots)");
+    ProfScope prof(c, XR_K_OTHER, 2.0 * rows * cols, 16.0 * rows * cols);
+    if (rows == 0) {
+        k_coldot_final<<<xr_grid(cols), 256, 0, c->stream>>>(part, 0, cols, out);
+    } else {
+        k_coldot_part<<<dim3((unsigned)ceil_div(cols, 256), (unsigned)nchunk), 256, 0, c->stream>>>(rows, cols, A, lda,
+                                                                                                   B, ldb, part);
+        XR_LAUNCH_CHECK(c);
+        k_coldot_final<<<xr_grid(cols), 256, 0, c->stream>>>(part, nchunk, cols, out);
+    }
+    XR_LAUNCH_CHECK(c);
+    return XR_OK;
+}

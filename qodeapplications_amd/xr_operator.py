@@ -486,7 +486,9 @@ class XROperator:
             off += n * n
             _, U = np.linalg.eigh(0.5 * (hh + hh.T))
             U = upload(U, self.device)
-            self._deig[m][c] = ((U * mm(h, U)).sum(0), U)
+            w = torch.empty(n, dtype=torch.float64, device=self.device)      # diag(U^T h U)
+            self.ctx.call("xr_column_dots", n, n, U.data_ptr(), n, mm(h, U).data_ptr(), n, w.data_ptr())
+            self._deig[m][c] = (w, U)
         return self._deig
 
     def _pack(self, dims, ncol, f0, f1, rowmap, nrows, full, packed, mode, alpha=1.0, ld=0):
@@ -916,34 +918,32 @@ class SectorSpace:
                 if (m, n, cm, cn) in self.pair_diag:
                     continue
                 Dm, Dn = self.blocks_of[m][cm][1], self.blocks_of[n][cn][1]
-                Hself = Hss[off:off + D, off:off + D].contiguous()
+                Hself = Hss[off:off + D, off:off + D]
                 K = torch.kron(self.eig[m][cm][1], self.eig[n][cn][1]).contiguous()
                 HK = mm(Hself, K)
-                self.pair_diag[(m, n, cm, cn)] = (K * HK).sum(0).reshape(Dm, Dn)
-        # lam over the tuples of one shape at once (stacked per fragment and
-        # pair: a few launches per shape instead of a few per tuple); the sums
-        # run in the same order as tuple by tuple
-        groups = {}
-        for i in range(len(self.sectors)):
-            groups.setdefault(tuple(self.shapes[i]), []).append(i)
-        self.lam = torch.empty(self.size, dtype=torch.float64, device=dev)
-        for sh, idx in groups.items():
-            ts = [self.sectors[i] for i in idx]
-            l = torch.zeros((len(idx),) + sh, dtype=torch.float64, device=dev)
+                pdv = torch.empty(D, dtype=torch.float64, device=dev)
+                op.ctx.call("xr_column_dots", D, D, K.data_ptr(), D, HK.data_ptr(), D, pdv.data_ptr())
+                self.pair_diag[(m, n, cm, cn)] = pdv.view(Dm, Dn)
+        # lam, tuple by tuple in the library: monomer then pair terms (all
+        # pairs m < n in order; a pair without a term adds zeros)
+        zero = {}
+        pd = []
+        for t, sh in zip(self.sectors, self.shapes):
             for m in range(nf):
-                vs = [len(idx)] + [1] * nf
-                vs[m + 1] = sh[m]
-                l = l + torch.stack([self.eig[m][t[m]][0] for t in ts]).reshape(vs)
-            for (m, n) in op.pairs:
-                vs = [len(idx)] + [1] * nf
-                vs[m + 1], vs[n + 1] = sh[m], sh[n]
-                l = l + torch.stack([self.pair_diag[(m, n, t[m], t[n])] for t in ts]).reshape(vs)
-            l = l.reshape(len(idx), -1)
-            if len(groups) == 1:
-                self.lam = l.reshape(-1)
-            else:
-                for j, i in enumerate(idx):
-                    self.lam[int(self.offsets[i]):int(self.offsets[i + 1])] = l[j]
+                for n in range(m + 1, nf):
+                    if (m, n) in op.pairs:
+                        pd.append(self.pair_diag[(m, n, t[m], t[n])].data_ptr())
+                    else:
+                        z = zero.setdefault((sh[m], sh[n]), torch.zeros(sh[m], sh[n], dtype=torch.float64,
+                                                                          device=dev))
+                        pd.append(z.data_ptr())
+        dims = _arr(self.shapes, np.int64).reshape(-1)
+        offs = _arr(self.offsets[:-1], np.int64)
+        e = _arr([self.eig[m][t[m]][0].data_ptr() for t in self.sectors for m in range(nf)], np.uint64)
+        pd = _arr(pd, np.uint64)
+        self.lam = torch.empty(self.size, dtype=torch.float64, device=dev)
+        op.ctx.call("xr_sector_diag", nf, len(self.sectors), _ptr(dims), _ptr(offs), _ptr(e), _ptr(pd),
+                    self.lam.data_ptr())
 
     def _pair_index(self, m, n, cm, cn):
         """Pair indices a * d_n + b of the states with charges (cm, cn)."""

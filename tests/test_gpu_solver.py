@@ -1,6 +1,7 @@
 """Vector kernels of the block-Davidson solver (xr_precond_divide,
 xr_residual, xr_column_norms, xr_transpose) against the torch expressions
 they replace, and the fixed-order column sums reproducing bit for bit."""
+import numpy as np
 import pytest
 import torch
 
@@ -108,3 +109,41 @@ def test_syrk_matches_full_product(trans, n, K):
     assert torch.equal(C, C.T)
     scale = float(ref.abs().max())
     assert float((C - ref).abs().max()) <= 1e-13 * max(1.0, scale) * max(1, K) ** 0.5
+
+
+def test_column_dots_and_sector_diag():
+    """xr_column_dots against torch (row counts across chunk boundaries,
+    leading dimensions above cols); xr_sector_diag against the broadcast sum
+    of the monomer and pair diagonals for three fragments, two tuples."""
+    ctx = context(0)
+    g = torch.Generator().manual_seed(31)
+    for rows, cols in [(1, 1), (64, 300), (65, 7), (1000, 1024)]:
+        A = torch.randn(rows, cols + 2, dtype=torch.float64, generator=g).cuda()
+        B = torch.randn(rows, cols + 5, dtype=torch.float64, generator=g).cuda()
+        out = torch.empty(cols, dtype=torch.float64, device="cuda")
+        ctx.call("xr_column_dots", rows, cols, ptr(A), cols + 2, ptr(B), cols + 5, ptr(out))
+        ref = (A[:, :cols] * B[:, :cols]).sum(0)
+        assert float((out - ref).abs().max()) <= 1e-13 * rows
+    shapes = [(3, 4, 5), (2, 1, 6)]
+    e = [[torch.randn(d, dtype=torch.float64, generator=g).cuda() for d in sh] for sh in shapes]
+    pairs = [(0, 1), (0, 2), (1, 2)]
+    pd = [[torch.randn(sh[m], sh[n], dtype=torch.float64, generator=g).cuda() for (m, n) in pairs]
+          for sh in shapes]
+    sizes = [int(np.prod(sh)) for sh in shapes]
+    offs = np.array([0, sizes[0]], dtype=np.int64)
+    dims = np.array(shapes, dtype=np.int64).reshape(-1)
+    ep = np.array([t.data_ptr() for row in e for t in row], dtype=np.uint64)
+    pp = np.array([t.data_ptr() for row in pd for t in row], dtype=np.uint64)
+    out = torch.empty(sum(sizes), dtype=torch.float64, device="cuda")
+    ctx.call("xr_sector_diag", 3, 2, dims.ctypes.data, offs.ctypes.data, ep.ctypes.data, pp.ctypes.data, ptr(out))
+    for q, sh in enumerate(shapes):
+        ref = torch.zeros(sh, dtype=torch.float64, device="cuda")
+        for m in range(3):
+            vs = [1, 1, 1]
+            vs[m] = sh[m]
+            ref = ref + e[q][m].view(vs)
+        for p, (m, n) in enumerate(pairs):
+            vs = [1, 1, 1]
+            vs[m], vs[n] = sh[m], sh[n]
+            ref = ref + pd[q][p].view(vs)
+        assert torch.equal(out[int(offs[q]):int(offs[q]) + sizes[q]], ref.reshape(-1)), q
