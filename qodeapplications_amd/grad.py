@@ -43,26 +43,28 @@ def mm(A, Bm, transA=0, transB=0, alpha=1.0, out=None, beta=0.0, split=False, mo
     return _mm(A.contiguous(), Bm.contiguous(), transA, transB, alpha, out, beta, split, mode)
 
 
-def bilinear_adjoint(G, A, W, Bm, s, Abar=None, Bbar=None, Wbar=None, split="f64", wbar_beta=1.0):
+def bilinear_adjoint(G, A, W, Bm, s, Abar=None, Bbar=None, Wbar=None, split="f64", wbar_beta=1.0,
+                     abar_beta=1.0, bbar_beta=1.0):
     """For H = s A W B^T (A: nA x p, W: p x q, B: nB x q) and G = dE/dH
     (nA x nB): Abar += s G B W^T, Bbar += s G^T A W, Wbar += s A^T G B.
     Each triple product is evaluated in the cheaper association; split is
     the precision mode the products run in ('f64', 'split_f32', 'split_i8':
-    two-electron terms follow the system's mode).  wbar_beta = 0 overwrites
-    Wbar instead of adding to it (a fresh accumulator needs no zero fill)."""
+    two-electron terms follow the system's mode).  A beta of 0 overwrites
+    its output instead of adding to it (a fresh accumulator needs no zero
+    fill)."""
     nA, p = A.shape
     nB, q = Bm.shape
     x = split
     if Abar is not None:
         if nA * nB * q + nA * q * p <= nB * q * p + nA * nB * p:
-            mm(mm(G, Bm, mode=x), W, transB=1, alpha=s, out=Abar, beta=1.0, mode=x)   # (G B) W^T
+            mm(mm(G, Bm, mode=x), W, transB=1, alpha=s, out=Abar, beta=abar_beta, mode=x)   # (G B) W^T
         else:
-            mm(G, mm(Bm, W, transB=1, mode=x), alpha=s, out=Abar, beta=1.0, mode=x)   # G (B W^T)
+            mm(G, mm(Bm, W, transB=1, mode=x), alpha=s, out=Abar, beta=abar_beta, mode=x)   # G (B W^T)
     if Bbar is not None:
         if nB * nA * p + nB * p * q <= nA * p * q + nB * nA * q:
-            mm(mm(G, A, transA=1, mode=x), W, alpha=s, out=Bbar, beta=1.0, mode=x)    # (G^T A) W
+            mm(mm(G, A, transA=1, mode=x), W, alpha=s, out=Bbar, beta=bbar_beta, mode=x)    # (G^T A) W
         else:
-            mm(G, mm(A, W, mode=x), transA=1, alpha=s, out=Bbar, beta=1.0, mode=x)    # G^T (A W)
+            mm(G, mm(A, W, mode=x), transA=1, alpha=s, out=Bbar, beta=bbar_beta, mode=x)    # G^T (A W)
     if Wbar is not None:
         if p * nA * nB + p * nB * q <= nA * nB * q + p * nA * q:
             mm(mm(A, G, transA=1, mode=x), Bm, alpha=s, out=Wbar, beta=wbar_beta, mode=x)   # (A^T G) B
@@ -71,16 +73,27 @@ def bilinear_adjoint(G, A, W, Bm, s, Abar=None, Bbar=None, Wbar=None, split="f64
 
 
 class _Acc:
-    """Zero-initialised accumulators keyed like their forward tensors."""
+    """Accumulators keyed like their forward tensors.  take() hands out a
+    tensor with the beta its writer uses: 0.0 for a new one, left
+    uninitialised since its first writer overwrites all of it, then 1.0."""
 
     def __init__(self, device):
         self.device = device
         self.d = {}
 
-    def get(self, key, shape):
+    def take(self, key, shape):
         t = self.d.get(key)
         if t is None:
-            t = self.d[key] = torch.zeros(shape, dtype=torch.float64, device=self.device)
+            t = self.d[key] = torch.empty(shape, dtype=torch.float64, device=self.device)
+            return t, 0.0
+        return t, 1.0
+
+    def zeroed(self, key, shape):
+        """The accumulator for a writer that can only add (a library call
+        accumulating in place): zero-filled when new."""
+        t, beta = self.take(key, shape)
+        if beta == 0.0:
+            t.zero_()
         return t
 
 
@@ -148,28 +161,35 @@ class XRAdjoint:
         ctx = context(dev.index)
 
         def ib(kind, *key):
+            """(accumulator of integral block (kind, key), the beta its writer uses)."""
             t = getattr(X.ints, kind)(*key)
-            return ibar.get((kind,) + key, t.shape)
+            return ibar.take((kind,) + key, t.shape)
 
         def h_bar(parts, g):
             for kind, key in parts:
-                ib(kind, *key).add_(g)
+                t, beta = ib(kind, *key)
+                if beta == 0.0:
+                    t.copy_(g)
+                else:
+                    t.add_(g)
 
         # ---- monomer blocks: e1 = Rca . h, E2 (sigma), nuclear constant
         for m, F in enumerate(X.frags):
             parts = [("T", (m, m)), ("U", (m, m, m))]
             h = X.ints.h_sum((m, m, "mono"), parts)
-            hb = torch.zeros_like(h)
+            hb = torch.empty_like(h)
+            hb_beta = 0.0
             for c in F.charges:
                 p = F.pos[c]
                 G1 = D[m][p][:, p].contiguous()
                 Rca = F.rho["ca"][(c, c)]
-                Rb = rbar[m].get(("ca", (c, c)), Rca.shape)
+                Rb, rb_beta = rbar[m].take(("ca", (c, c)), Rca.shape)
                 # e1[a] = sum Rca[a, pq] h[pq]
-                mm(G1.view(-1, 1), h.view(1, -1), out=_flat(Rb), beta=1.0)
-                mm(_flat(Rca), G1.view(-1, 1), transA=1, out=hb.view(-1, 1), beta=1.0)
+                mm(G1.view(-1, 1), h.view(1, -1), out=_flat(Rb), beta=rb_beta)
+                mm(_flat(Rca), G1.view(-1, 1), transA=1, out=hb.view(-1, 1), beta=hb_beta)
+                hb_beta = 1.0
                 blk = F.blocks[c]
-                Vb = ib("V", m, m, m, m)
+                Vb = ibar.zeroed(("V", m, m, m, m), X.ints.V(m, m, m, m).shape)
                 W = F.E2W.get(c)
                 args = (blk.n_orbs, blk.n_core, blk.n_elec, blk.n_states, blk.n_configs, ptr(blk.coeffs),
                         blk.n_states, blk.n_configs, ptr(blk.coeffs), ptr(X.ints.V(m, m, m, m)), ptr(G1),
@@ -198,8 +218,8 @@ class XRAdjoint:
             # back through the same (self-inverse) conjugation
             for (op, key) in [k for k in rbar[m].d if k in F.derived]:
                 cop, ckey = F.derived[(op, key)]
-                conjugate(rbar[m].d.pop((op, key)), out=rbar[m].get((cop, ckey), F.rho[cop][ckey].shape),
-                          accumulate=True)
+                out, beta = rbar[m].take((cop, ckey), F.rho[cop][ckey].shape)
+                conjugate(rbar[m].d.pop((op, key)), out=out, accumulate=beta == 1.0)
             for (op, key), Rb in rbar[m].d.items():
                 bb, kb = F.blocks[key[0]], F.blocks[key[1]]
                 ctx.call("xr_density_adjoint", op.encode(), kb.n_orbs, kb.n_core, kb.n_elec,
@@ -230,8 +250,17 @@ class XRAdjoint:
 
         R1 = lambda op: F1.rho[op][k1]
         R2 = lambda op: F2.rho[op][k2]
-        R1b = lambda op: _flat(rbar[m1].get((op, k1), F1.rho[op][k1].shape))
-        R2b = lambda op: _flat(rbar[m2].get((op, k2), F2.rho[op][k2].shape))
+        def R1b(op):
+            t, beta = rbar[m1].take((op, k1), F1.rho[op][k1].shape)
+            return _flat(t), beta
+
+        def R2b(op):
+            t, beta = rbar[m2].take((op, k2), F2.rho[op][k2].shape)
+            return _flat(t), beta
+
+        def adj(G_, A, W_, B_, s_, ab, bb, Wb_, **kw):
+            """bilinear_adjoint into the accumulators ab, bb ((tensor, beta) pairs)."""
+            bilinear_adjoint(G_, A, W_, B_, s_, ab[0], bb[0], Wb_, abar_beta=ab[1], bbar_beta=bb[1], **kw)
         n1, n2 = F1.dim, F2.dim
 
         def vperm(key, perm, shape2):
@@ -240,18 +269,21 @@ class XRAdjoint:
             Vt = X.ints.V(*key).contiguous()
             W = permute4(Vt, perm).view(*shape2)
             Wb = torch.empty_like(W)          # written whole by its one product (wbar_beta = 0)
-            return W, Wb, lambda: permute4(Wb.view([Vt.shape[i] for i in perm]), _inv(perm),
-                                           out=ib("V", *key), accumulate=True)
+
+            def fin():
+                out, beta = ib("V", *key)
+                permute4(Wb.view([Vt.shape[i] for i in perm]), _inv(perm), out=out, accumulate=beta == 1.0)
+            return W, Wb, fin
 
         if d in (-2, +2):
             if d == -2:
                 W, Wb, fin = vperm((m1, m1, m2, m2), (0, 1, 3, 2), (n1 * n1, n2 * n2))
-                bilinear_adjoint(G(False), _flat(R1("cc")), W, _flat(R2("aa")), 1.0,
-                                 R1b("cc"), R2b("aa"), Wb, split=sp, wbar_beta=0.0)
+                adj(G(False), _flat(R1("cc")), W, _flat(R2("aa")), 1.0, R1b("cc"), R2b("aa"), Wb, split=sp,
+                    wbar_beta=0.0)
             else:
                 W, Wb, fin = vperm((m2, m2, m1, m1), (0, 1, 3, 2), (n2 * n2, n1 * n1))
-                bilinear_adjoint(G(True), _flat(R2("cc")), W, _flat(R1("aa")), 1.0,
-                                 R2b("cc"), R1b("aa"), Wb, split=sp, wbar_beta=0.0)
+                adj(G(True), _flat(R2("cc")), W, _flat(R1("aa")), 1.0, R2b("cc"), R1b("aa"), Wb, split=sp,
+                    wbar_beta=0.0)
             fin()
             return
         if d in (-1, +1):
@@ -272,22 +304,21 @@ class XRAdjoint:
             Ra, Rab = ann[0], ann[1]
             h = X.ints.h_sum(hkey, parts)
             hb = torch.empty_like(h)
-            bilinear_adjoint(Gm, _flat(Rc("c")), h, _flat(Ra("a")), s, Rcb("c"), Rab("a"), hb, wbar_beta=0.0)
+            adj(Gm, _flat(Rc("c")), h, _flat(Ra("a")), s, Rcb("c"), Rab("a"), hb, wbar_beta=0.0)
             h_bar(parts, hb)
             W1, W1b, fin1 = vperm(k11, (1, 0, 2, 3), (nc ** 3, na))
-            bilinear_adjoint(Gm, _flat(Rc("cca")), W1, _flat(Ra("a")), 2.0 * s, Rcb("cca"), Rab("a"), W1b,
-                             split=sp, wbar_beta=0.0)
+            adj(Gm, _flat(Rc("cca")), W1, _flat(Ra("a")), 2.0 * s, Rcb("cca"), Rab("a"), W1b, split=sp,
+                wbar_beta=0.0)
             fin1()
             W2, W2b, fin2 = vperm(k22, (0, 1, 3, 2), (nc, na ** 3))
-            bilinear_adjoint(Gm, _flat(Rc("c")), W2, _flat(Ra("caa")), 2.0 * s, Rcb("c"), Rab("caa"), W2b,
-                             split=sp, wbar_beta=0.0)
+            adj(Gm, _flat(Rc("c")), W2, _flat(Ra("caa")), 2.0 * s, Rcb("c"), Rab("caa"), W2b, split=sp,
+                wbar_beta=0.0)
             fin2()
             return
         # d == 0
         Gm = G(False)
         W, Wb, fin = vperm((m1, m2, m1, m2), (0, 2, 1, 3), (n1 * n1, n2 * n2))
-        bilinear_adjoint(Gm, _flat(R1("ca")), W, _flat(R2("ca")), 4.0, R1b("ca"), R2b("ca"), Wb, split=sp,
-                         wbar_beta=0.0)
+        adj(Gm, _flat(R1("ca")), W, _flat(R2("ca")), 4.0, R1b("ca"), R2b("ca"), Wb, split=sp, wbar_beta=0.0)
         fin()
         # extPot terms: e2[b] on I1 == J1 rows, e1[a] on I2 == J2 columns
         dia1 = torch.zeros(nA, 1, dtype=torch.float64, device=X.device)
@@ -298,10 +329,11 @@ class XRAdjoint:
         e1b = mm(Gm, dia2)                     # (nA x 1)
         U12 = X.ints.U(m1, m2, m2)
         U21 = X.ints.U(m2, m1, m1)
-        mm(e2b, U12.view(1, -1), out=R2b("ca"), beta=1.0)
-        mm(_flat(R2("ca")), e2b, transA=1, out=ib("U", m1, m2, m2).view(-1, 1), beta=1.0)
-        mm(e1b, U21.view(1, -1), out=R1b("ca"), beta=1.0)
-        mm(_flat(R1("ca")), e1b, transA=1, out=ib("U", m2, m1, m1).view(-1, 1), beta=1.0)
+        for eb, U_, Rb_, R_, key in ((e2b, U12, R2b, R2, ("U", m1, m2, m2)), (e1b, U21, R1b, R1, ("U", m2, m1, m1))):
+            out, beta = Rb_("ca")
+            mm(eb, U_.view(1, -1), out=out, beta=beta)
+            out, beta = ib(*key)
+            mm(_flat(R_("ca")), eb, transA=1, out=out.view(-1, 1), beta=beta)
 
 
 def _inv(perm):
