@@ -451,11 +451,14 @@ class XROperator:
         ground = []                   # (charge, |0_n><0_n|) of each fragment's lowest monomer state
         for n in range(nf):
             c0 = min(eig0[n], key=lambda c: self._eig_low[(n, c)])
-            u = eig0[n][c0][1][:, 0]
-            ground.append((c0, torch.outer(u, u)))
+            ground.append((c0, eig0[n][c0][1][:, 0].contiguous()))
         # the dressed blocks of every (m, c) are formed on the device and
         # reach the host in one transfer
         dressed = []
+
+        def gemm(M, N, K, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch):
+            self.ctx.call("xr_gemm_f64", 0, 0, M, N, K, 1.0, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch)
+
         for m in range(nf):
             for c, h in sp.h1[m].items():
                 ht = h.clone()
@@ -463,19 +466,27 @@ class XROperator:
                     if m not in (a, b):
                         continue
                     n = b if a == m else a
-                    cn, rho = ground[n]
+                    cn, u = ground[n]
                     ca, cb = (c, cn) if a == m else (cn, c)
                     Hss, parts = sp.sq[(a, b, ca + cb)]
                     o, D = [(o, D) for c1, c2, o, D in parts if (c1, c2) == (ca, cb)][0]
                     da, db = sp.blocks_of[a][ca][1], sp.blocks_of[b][cb][1]
-                    blk = Hss[o:o + D, o:o + D].reshape(da, db, da, db)
-                    # sum over the partner's bra / ket states weighted by rho[ket, bra]
-                    if a == m:         # blk[i, b, j, d] rho[d, b]
-                        lhs, nd = blk.permute(0, 2, 1, 3), db
-                    else:              # blk[b, i, d, j] rho[d, b]
-                        lhs, nd = blk.permute(1, 3, 0, 2), da
-                    ht = ht + mm(lhs.contiguous().view(h.shape[0] * h.shape[0], nd * nd),
-                                 rho.T.contiguous().view(nd * nd, 1)).view(h.shape)
+                    # ht += (<u| on the partner) M (|u> on the partner), M the
+                    # pair block over (a, b) states: two batched products on
+                    # the strided block, the rows of M the batch
+                    blk, ld = Hss.data_ptr() + 8 * (o * Hss.shape[1] + o), Hss.shape[1]
+                    if a == m:         # rows (i, x): W[(i,x), j] = sum_y M[(i,x), (j,y)] u_y
+                        W = torch.empty(D, da, dtype=torch.float64, device=self.device)
+                        gemm(da, 1, db, blk, db, ld, u.data_ptr(), 1, 0, 0.0, W.data_ptr(), 1, da, D)
+                        # ht[i, :] += sum_x u_x W[(i,x), :]
+                        gemm(1, da, db, u.data_ptr(), db, 0, W.data_ptr(), da, db * da, 1.0, ht.data_ptr(), da,
+                             da, da)
+                    else:              # rows (x, i): W[(x,i), j] = sum_y u_y M[(x,i), (y,j)]
+                        W = torch.empty(D, db, dtype=torch.float64, device=self.device)
+                        gemm(1, db, da, u.data_ptr(), da, 0, blk, db, ld, 0.0, W.data_ptr(), db, db, D)
+                        # ht += sum_x u_x W[(x, :), :]
+                        gemm(1, db * db, da, u.data_ptr(), da, 0, W.data_ptr(), db * db, 0, 1.0, ht.data_ptr(),
+                             db * db, 0, 1)
                 dressed.append((m, c, h, ht))
         flat = torch.cat([ht.reshape(-1) for _, _, _, ht in dressed]).cpu().numpy()
         self._deig = [{} for _ in range(nf)]
