@@ -96,6 +96,10 @@ int xr_assume_hermitian_v(xr_ctx *ctx, int on);
  * No reference counterpart (the reference has no timers,
  * density_tensors.c:177-178). */
 int xr_profile(xr_ctx *ctx, int enable);
+/* Restricts the profiler to the kernel classes whose bits are set in mask
+ * (bit k = class k; default all), so that timing one class leaves the other
+ * launches without the two event records per launch. */
+int xr_profile_kinds(xr_ctx *ctx, uint32_t mask);
 int xr_profile_stats(xr_ctx *ctx, int kind, int64_t *launches, double *total_ms, double *flops,
                      double *bytes);
 /* Writes every profiled launch (kind, GEMM shape, ms, flops, bytes) as CSV. */

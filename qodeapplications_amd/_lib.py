@@ -78,6 +78,7 @@ SIGNATURES = {
     "xr_cholesky_test": (_I, [_P, _I, _P, _P, _P, _P]),
     "xr_gemm_f64_noalloc": (_I, [_P, _I, _I, _L, _L, _L, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _L]),
     "xr_set_precision": (_I, [_P, _I]),
+    "xr_profile_kinds": (_I, [_P, ctypes.c_uint32]),
     "xr_monomer_1e": (_I, [_P, _L, _L, _P, _P, _P]),
     "xr_dimer_2min2pls": (_I, [_P, _L, _L, _L, _P, _L, _P, _P, _P]),
     "xr_dimer_1min1pls_1e": (_I, [_P, _L, _L, _L, _P, _L, _P, _P, _P]),
@@ -188,6 +189,11 @@ class Context:
 
     def profile(self, enable):
         self.check(self.L.xr_profile(self.h, 1 if enable else 0))
+
+    def profile_kinds(self, kinds=None):
+        """Records only the listed kernel classes (None: all)."""
+        mask = 0xffffffff if kinds is None else sum(1 << int(k) for k in kinds)
+        self.check(self.L.xr_profile_kinds(self.h, ctypes.c_uint32(mask)))
 
     def profile_stats(self, kind):
         """(launches, total_ms, flops, bytes) of one kernel class since profile(True)."""

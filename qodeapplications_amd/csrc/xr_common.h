@@ -107,6 +107,7 @@ struct xr_ctx {
     int device = 0;
     int precision = XR_PREC_F64;  // two-electron contraction GEMMs
     bool prof = false;
+    uint32_t prof_kinds = 0xffffffffu;  // kernel classes the profiler records (xr_profile_kinds)
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> ev_pool;
     hipStream_t stream = nullptr;

@@ -118,7 +118,7 @@ static hipEvent_t ev_get(xr_ctx* c) {
 }
 
 ProfScope::ProfScope(xr_ctx* c_, int kind, double flops, double bytes) : c(c_) {
-    if (!c->prof) return;
+    if (!c->prof || kind < 0 || kind >= 32 || !((c->prof_kinds >> kind) & 1u)) return;
     ProfRec r;
     r.kind = kind;
     r.flops = flops;
@@ -158,6 +158,13 @@ extern "C" int xr_profile(xr_ctx* c, int enable) {
         c->recs.clear();
     }
     c->prof = enable != 0;
+    return XR_OK;
+}
+
+extern "C" int xr_profile_kinds(xr_ctx* c, uint32_t mask) {
+    if (!c) return XR_EINVAL;
+    std::lock_guard<std::mutex> g(c->mu);
+    c->prof_kinds = mask;
     return XR_OK;
 }
 
