@@ -90,7 +90,8 @@ int xr_assume_hermitian_v(xr_ctx *ctx, int on);
  * (0 FP64 MFMA tile GEMM, 1 gather, 2 expand, 3 excitation table, 4 block
  * scatter / axis packing, 5 braket assembly, 6 other (incl. split-K
  * reductions), 7 split-FP32 GEMM, 8 bandwidth-bound tall-skinny products,
- * 9 grouped FP64 tile GEMM, 10 the pair program's work-list GEMM):
+ * 9 grouped FP64 tile GEMM, 10 the pair program's work-list GEMM,
+ * 11 split-int8 GEMM, 12 its exponent and digit-plane passes):
  * launches, total ms, and the
  * algorithmic flops and bytes of those launches.  Synchronises the stream.
  * No reference counterpart (the reference has no timers,
