@@ -279,7 +279,7 @@ extern "C" int xr_add_diagonal(xr_ctx* c, int nb, const int64_t* n, double* cons
 //   trailing GEMM  A22 -= W W^T on the lower-triangle tiles (grouped,
 //                  accumulating).
 // info[b] = 0 when matrix b factored, else the 1-based column whose pivot was
-// not positive (NaN included).  Only the verdict is kept; the matrices are
+// not positive (NaN included).  Only the outcome is kept; the matrices are
 // left overwritten.
 // ---------------------------------------------------------------------------
 enum { WS_CHOL = 92 };

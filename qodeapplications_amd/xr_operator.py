@@ -134,7 +134,7 @@ def _side(device):
 class _SectorBoundJob:
     """The staged work behind XROperator.sector_bounds: the dressed pair
     blocks and their Lanczos passes; once those reach the host, the shifted
-    Cholesky tests; once their verdicts arrive (and any retry's), the block
+    Cholesky tests; once their outcomes arrive (and any retry's), the block
     and sector bounds.  With a side stream every stage is queued there with
     its own library context, after the caller's earlier work on its stream,
     and step() only advances past stages that have finished, so the stages
@@ -241,7 +241,7 @@ class _SectorBoundJob:
             if self.stage == 1:
                 self._estimates()
             elif self.stage == 2:
-                self._verdicts()
+                self._outcomes()
             else:
                 self.info[self.retry] = self.host.numpy()
                 self._finish(self.info)
@@ -278,8 +278,8 @@ class _SectorBoundJob:
         self.host = self._cholesky(self.sel, -self.t[self.sel])
         self.stage = 2
 
-    def _verdicts(self):
-        """First verdicts; a failed block is formed again (the factorisation
+    def _outcomes(self):
+        """First outcomes; a failed block is formed again (the factorisation
         overwrote it) and tested once more at a lower shift."""
         self.info[self.sel] = self.host.numpy()
         self.retry = np.array([j for j in self.sel if self.info[j] != 0], dtype=np.int64)

@@ -9,7 +9,7 @@
     of the spectrum; anorm is the largest absolute row sum;
   * xr_add_diagonal shifts exactly the diagonals;
   * xr_cholesky_test: on a batch of sizes 1..2000 (row strides above n
-    included) the verdict for A - t I at t = lambda_min -/+ 1e-3 is positive
+    included) the outcome for A - t I at t = lambda_min -/+ 1e-3 is positive
     definite / not, as LAPACK's dpotrf says, and a diagonally dominant matrix
     with one negative diagonal entry at column k fails exactly at k + 1
     (in the first, second and last panel); a NaN fails;
@@ -147,7 +147,7 @@ def _chol(mats, lds=None):
     return info.cpu().numpy()
 
 
-def test_cholesky_test_verdicts():
+def test_cholesky_test_outcomes():
     from scipy.linalg import lapack
     gen = torch.Generator().manual_seed(21)
     sizes = [1, 5, 64, 65, 130, 700, 2000]
