@@ -4,8 +4,9 @@
 // alpha * op(A) op(B) only (alpha is applied while A is staged in LDS) and
 // stores the accumulators untouched: any arithmetic on them in the epilogue
 // makes the compiler copy all 128 accumulator registers out of the AGPRs,
-// which halves occupancy.  beta != 0 and split-K go through a workspace and
-// a fixed-order reduction.  256 threads = 4 waves per
+// which halves occupancy.  Split-K goes through a workspace and a
+// fixed-order reduction, and so does beta != 0 unless the accumulating
+// variant (k_dgemm_acc) costs nothing extra: small grids or short K.  256 threads = 4 waves per
 // workgroup in a 2 x 2 arrangement; each wave owns a (BM/2) x (BN/2) block of
 // 16 x 16 MFMA tiles.  K advances 16 at a time through one LDS stage while the
 // next stage is prefetched into registers.  Workgroups are remapped so each
@@ -785,8 +786,10 @@ static int launch_nn_gram(xr_ctx* c, int64_t M, int64_t N, int64_t K, double alp
 // of resident workgroups each take K/s + 32 (per-workgroup start-up and
 // store, in k units); the reduction reads and writes (s + 1) M N doubles
 // against the product's 2 M N K flops (~46 flops per byte at the kernel's
-// rate).
-static double split_choice(int64_t units, int64_t slots, int64_t K, int64_t* splits) {
+// rate).  Each split keeps at least kmin of K: 256 in general, 64 for a
+// narrow output (one row or column of tiles), whose partials are few bytes
+// next to the operand it streams.
+static double split_choice(int64_t units, int64_t slots, int64_t K, int64_t* splits, int64_t kmin = 256) {
     auto score = [&](int64_t s) {
         const double rounds = std::ceil((double)(units * s) / (double)slots);
         const double ideal = (double)units * (double)K / (double)slots;
@@ -798,7 +801,7 @@ static double split_choice(int64_t units, int64_t slots, int64_t K, int64_t* spl
     if (K < 512 || units >= 4 * slots) return best;
     // up to two rounds of resident workgroups (at most 512 splits): a few
     // output tiles over a long K are bound by how many CUs stream operands
-    const int64_t smax = std::min<int64_t>(std::min<int64_t>(512, std::max<int64_t>(64, 2 * slots / units)), K / 256);
+    const int64_t smax = std::min<int64_t>(std::min<int64_t>(512, std::max<int64_t>(64, 2 * slots / units)), K / kmin);
     for (int64_t s = 2; s <= smax; s++) {
         const double e = score(s);
         if (e > best) { best = e; *splits = s; }
@@ -831,15 +834,17 @@ static int launch_tiles(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t
         splits = ceil_div(K, kchunk);
         if (splits < 1) { splits = 1; kchunk = K; }
     } else {
-        split_choice(tiles * batch, tile_slots<BM, BN>(), K, &splits);
+        split_choice(tiles * batch, tile_slots<BM, BN>(), K, &splits, (tm == 1 || tn == 1) ? 64 : 256);
         if (splits > 1) {
             kchunk = ceil_div(ceil_div(K, splits), GEMM_BK) * GEMM_BK;
             splits = ceil_div(K, kchunk);
         }
     }
     // accumulate in the epilogue when the tiles fit in one round of half the
-    // resident workgroups (the variant's lower occupancy then costs nothing)
-    if (!upper && splits == 1 && beta != 0.0 && 2 * tiles * batch <= tile_slots<BM, BN>()) {
+    // resident workgroups (the variant's lower occupancy then costs nothing),
+    // or when K is short: the product is then bound by the C traffic, and the
+    // staged path's workspace write and reduction pass would double it
+    if (!upper && splits == 1 && beta != 0.0 && (2 * tiles * batch <= tile_slots<BM, BN>() || K <= 256)) {
         dim3 grid((unsigned)tiles, 1, (unsigned)batch);
         ProfScope prof(c, XR_K_GEMM_F64, 2.0 * M * N * K * batch, 8.0 * (M * K + K * N + 2 * M * N) * batch);
         prof.shape(M, N, K, batch, ta, tb);

@@ -48,8 +48,10 @@ def test_noalloc_gemm_equals_allocating_gemm(ta, tb, M, N, K, beta):
 
 
 def test_noalloc_gemm_refuses_staging_beyond_the_scratch():
-    # beta != 0 over a 6000 x 6000 output stages 288 MB, more than the 64 MiB array
-    A, B, C = _inputs(0, 0, 6000, 6000, 64, 7)
+    # beta != 0 over a 6000 x 6000 output with K = 320 (too long for the
+    # accumulating epilogue, too short to split) stages 288 MB, more than
+    # the 64 MiB array
+    A, B, C = _inputs(0, 0, 6000, 6000, 320, 7)
     before = C.clone()
     with pytest.raises(XRError) as e:
         _gemm("xr_gemm_f64_noalloc", 0, 0, A, B, C, 1.0, 1.0)
