@@ -7,6 +7,8 @@
 #include "xr_common.h"
 #include "xr_gemm.h"
 #include <vector>
+#include <map>
+#include <mutex>
 
 // out[o0,o1,o2,o3] = in[...] with out axis j = in axis perm[j]
 __device__ __forceinline__ void k_permute4_elem(int64_t i, const double* __restrict__ in, int64_t d0, int64_t d1, int64_t d2,
@@ -585,7 +587,7 @@ extern "C" int xr_axes_pack_multi(xr_ctx* c, int n_items, const int* nf, const i
 // rows [w RB, w RB + RB) of the tile's output, lane = column.  Loads and
 // stores run along q (one contiguous run per p); for post == 1 the tile is
 // one contiguous 64 x n block, moved through LDS in both directions.
-#define XR_AXIS_ITEMS 16
+#define XR_AXIS_ITEMS 32
 struct AxisItem {
     const double* U;
     const double* src;
@@ -704,6 +706,26 @@ __global__ __launch_bounds__(256) void k_axis_apply_multi(AxisBatch bt) {
     }
 }
 
+// Workgroups of 256 threads of an axis-apply instance resident on the
+// device at once (CUs x occupancy), per instance.
+static int64_t axis_slots(const void* kernel) {
+    static std::mutex mu;
+    static std::map<const void*, int64_t> memo;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = memo.find(kernel);
+    if (it != memo.end()) return it->second;
+    int dev = 0, cus = 0, per = 0;
+    int64_t r = 1536;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0) == hipSuccess && cus > 0 && per > 0)
+        r = (int64_t)cus * per;
+    else
+        (void)hipGetLastError();
+    memo[kernel] = r;
+    return r;
+}
+
 extern "C" int xr_axis_apply_multi(xr_ctx* c, int n_items, const double* const* d_U, const int* n,
                                    const int* trans, const int64_t* pre, const int64_t* post,
                                    const double* const* d_src, double* const* d_dst, double beta) {
@@ -731,9 +753,16 @@ extern "C" int xr_axis_apply_multi(xr_ctx* c, int n_items, const double* const* 
         }
         if (most <= 0) continue;
         ProfScope prof(c, XR_K_GEMM_BW, 2.0 * flops, (beta != 0.0 ? 24.0 : 16.0) * elems);
-        // enough workgroups to fill the chip, each looping over at least two of
-        // its item's tiles (the second one's loads overlap the first's work)
-        int64_t gx = ceil_div(tiles, 2), cap = 4096 / bt.nitems;
+        // one round of resident workgroups, each looping over at least two of
+        // its item's tiles (the next one's loads overlap the current one's
+        // work, and op(U) is staged once per workgroup)
+        const void* kern = nmax <= 4 ? reinterpret_cast<const void*>(&k_axis_apply_multi<1>)
+                         : nmax <= 8 ? reinterpret_cast<const void*>(&k_axis_apply_multi<2>)
+                         : nmax <= 16 ? reinterpret_cast<const void*>(&k_axis_apply_multi<4>)
+                         : nmax <= 32 ? reinterpret_cast<const void*>(&k_axis_apply_multi<8>)
+                         : reinterpret_cast<const void*>(&k_axis_apply_multi<16>);
+        int64_t gx = ceil_div(tiles, 2), cap = axis_slots(kern) / bt.nitems;
+        if (cap < 1) cap = 1;
         gx = gx < cap ? gx : cap;
         dim3 grid((unsigned)(gx > 0 ? gx : 1), (unsigned)bt.nitems);
         if (nmax <= 4) k_axis_apply_multi<1><<<grid, 256, 0, c->stream>>>(bt);
