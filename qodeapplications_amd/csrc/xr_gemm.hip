@@ -1043,6 +1043,10 @@ int xr_dgemm_launch(xr_ctx* c, int ta, int tb, int64_t M, int64_t N, int64_t K, 
     if (it != c->tune.end())
         return launch_cfg(c, it->second.first, it->second.second, ta, tb, M, N, K, alpha, A, lda, sA,
                           B, ldb, sB, beta, C, ldc, sC, batch);
+    // a short-K update of C is a read-modify-write of C: the accumulating
+    // 64 x 64 tiles keep more workgroups (and C loads) in flight
+    if (beta != 0.0 && K <= 64)
+        return launch_tiles<64, 64>(c, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
     // 128 x 128 tiles when they (with a K split where K allows one) fill the
     // rounds of resident workgroups well; smaller tiles for skinny or
     // small problems
