@@ -340,33 +340,75 @@ struct AxesDesc {
     FastDiv fx, fc, fd1, frest[XR_MAX_FRAG_AX];   // by xsize, ncol, d1, rest_dim[k]
 };
 
-__device__ __forceinline__ void axes_pack_elem(const AxesDesc& ad, const int32_t* __restrict__ rowmap,
-                                               double* __restrict__ full, double* __restrict__ packed,
-                                               int64_t ldp, int mode, double alpha, int64_t i) {
-    {
-        const bool sm = ad.small;
-        const int64_t r = sm ? fdiv(i, ad.fx) : i / ad.xsize, x = i - r * ad.xsize;
-        const int64_t pi = r * ldp + x;
-        const int64_t f = rowmap ? (int64_t)rowmap[r] : r;
-        if (f < 0 || f >= ad.nfront) return;
-        const int64_t a = sm ? fdiv(f, ad.fd1) : f / ad.d1, b = f - a * ad.d1;
-        int64_t off = a * ad.s0 + b * ad.s1;
-        int64_t y = sm ? fdiv(x, ad.fc) : x / ad.ncol;
-        off += x - y * ad.ncol;
-        for (int k = ad.nrest - 1; k >= 0; k--) {
-            const int64_t q = sm ? fdiv(y, ad.frest[k]) : y / ad.rest_dim[k];
-            off += (y - q * ad.rest_dim[k]) * ad.rest_stride[k];
-            y = q;
+// offset in the full tensor of packed column x (rest axes and column)
+__device__ __forceinline__ int64_t axes_rest_offset(const AxesDesc& ad, int64_t x) {
+    const bool sm = ad.small;
+    int64_t y = sm ? fdiv(x, ad.fc) : x / ad.ncol;
+    int64_t off = x - y * ad.ncol;
+    for (int k = ad.nrest - 1; k >= 0; k--) {
+        const int64_t q = sm ? fdiv(y, ad.frest[k]) : y / ad.rest_dim[k];
+        off += (y - q * ad.rest_dim[k]) * ad.rest_stride[k];
+        y = q;
+    }
+    return off;
+}
+
+// Packed element i: its offset in the packed matrix (pi) and in the full
+// tensor (off); false when its row maps outside the front index space.
+__device__ __forceinline__ bool axes_pack_index(const AxesDesc& ad, const int32_t* __restrict__ rowmap,
+                                                int64_t ldp, int64_t i, int64_t* pi, int64_t* off) {
+    const bool sm = ad.small;
+    const int64_t r = sm ? fdiv(i, ad.fx) : i / ad.xsize, x = i - r * ad.xsize;
+    *pi = r * ldp + x;
+    const int64_t f = rowmap ? (int64_t)rowmap[r] : r;
+    if (f < 0 || f >= ad.nfront) return false;
+    const int64_t a = sm ? fdiv(f, ad.fd1) : f / ad.d1, b = f - a * ad.d1;
+    *off = a * ad.s0 + b * ad.s1 + axes_rest_offset(ad, x);
+    return true;
+}
+
+// Grid-stride pass over the packed elements [0, total), XR_PACK_UNROLL per
+// thread and round: every element's address is computed and every load
+// issued before any store, so a wave keeps that many loads in flight.
+#define XR_PACK_UNROLL 4
+__device__ __forceinline__ void axes_pack_run(const AxesDesc& ad, const int32_t* __restrict__ rowmap,
+                                              double* full, double* packed, int64_t ldp, int mode,
+                                              double alpha, int64_t total) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < total; i0 += XR_PACK_UNROLL * stride) {
+        int64_t pi[XR_PACK_UNROLL], off[XR_PACK_UNROLL];
+        bool ok[XR_PACK_UNROLL];
+#pragma unroll
+        for (int u = 0; u < XR_PACK_UNROLL; u++) {
+            const int64_t i = i0 + u * stride;
+            pi[u] = 0;
+            off[u] = 0;
+            ok[u] = i < total && axes_pack_index(ad, rowmap, ldp, i, &pi[u], &off[u]);
         }
-        if (mode == 0) packed[pi] = full[off];
-        else full[off] += alpha * packed[pi];
+        if (mode == 0) {
+            double v[XR_PACK_UNROLL];
+#pragma unroll
+            for (int u = 0; u < XR_PACK_UNROLL; u++) v[u] = ok[u] ? full[off[u]] : 0.0;
+#pragma unroll
+            for (int u = 0; u < XR_PACK_UNROLL; u++)
+                if (ok[u]) packed[pi[u]] = v[u];
+        } else {
+            double p[XR_PACK_UNROLL], q[XR_PACK_UNROLL];
+#pragma unroll
+            for (int u = 0; u < XR_PACK_UNROLL; u++) {
+                p[u] = ok[u] ? packed[pi[u]] : 0.0;
+                q[u] = ok[u] ? full[off[u]] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < XR_PACK_UNROLL; u++)
+                if (ok[u]) full[off[u]] = q[u] + alpha * p[u];
+        }
     }
 }
 
 __global__ void k_axes_pack(AxesDesc ad, const int32_t* __restrict__ rowmap, double* __restrict__ full,
                             double* __restrict__ packed, int64_t ldp, int mode, double alpha) {
-    const int64_t total = ad.nrows * ad.xsize;
-    XR_GRID_STRIDE(i, total) axes_pack_elem(ad, rowmap, full, packed, ldp, mode, alpha, i);
+    axes_pack_run(ad, rowmap, full, packed, ldp, mode, alpha, ad.nrows * ad.xsize);
 }
 
 static int axes_desc(xr_ctx* c, int nf, const int64_t* dims, int64_t ncol, int front0, int front1,
@@ -427,7 +469,7 @@ extern "C" int xr_axes_pack(xr_ctx* c, int nf, const int64_t* dims, int64_t ncol
     if (inner == ncol && ad.nrest > 0)          // transposing: LDS tiles (pack_launch below)
         return axes_pack_tiled_one(c, ad, d_rowmap, d_full, d_packed, ld_packed, mode, alpha);
     ProfScope prof(c, XR_K_SCATTER, 0.0, (mode ? 24.0 : 16.0) * total);
-    k_axes_pack<<<xr_grid(total), 256, 0, c->stream>>>(ad, d_rowmap, d_full, d_packed, ld_packed, mode, alpha);
+    k_axes_pack<<<xr_grid(ceil_div(total, XR_PACK_UNROLL)), 256, 0, c->stream>>>(ad, d_rowmap, d_full, d_packed, ld_packed, mode, alpha);
     XR_LAUNCH_CHECK(c);
     return XR_OK;
 }
@@ -455,23 +497,9 @@ struct AxesBatch {
     AxesItem it[XR_PACK_ITEMS];
 };
 
-// offset in the full tensor of packed column x (rest axes and column)
-__device__ __forceinline__ int64_t axes_rest_offset(const AxesDesc& ad, int64_t x) {
-    const bool sm = ad.small;
-    int64_t y = sm ? fdiv(x, ad.fc) : x / ad.ncol;
-    int64_t off = x - y * ad.ncol;
-    for (int k = ad.nrest - 1; k >= 0; k--) {
-        const int64_t q = sm ? fdiv(y, ad.frest[k]) : y / ad.rest_dim[k];
-        off += (y - q * ad.rest_dim[k]) * ad.rest_stride[k];
-        y = q;
-    }
-    return off;
-}
-
 __global__ __launch_bounds__(256) void k_axes_pack_multi(AxesBatch bt) {
     const AxesItem& I = bt.it[blockIdx.y];
-    const int64_t total = I.ad.nrows * I.ad.xsize;
-    XR_GRID_STRIDE(i, total) axes_pack_elem(I.ad, I.rowmap, I.full, I.packed, I.ldp, bt.mode, bt.alpha, i);
+    axes_pack_run(I.ad, I.rowmap, I.full, I.packed, I.ldp, bt.mode, bt.alpha, I.ad.nrows * I.ad.xsize);
 }
 
 __global__ __launch_bounds__(256) void k_axes_pack_tiled(AxesBatch bt) {
@@ -524,7 +552,8 @@ static int pack_launch(xr_ctx* c, const std::vector<AxesItem>& items, int tiled,
         for (int t = 0; t < bt.n; t++) {
             const AxesItem& I = bt.it[t] = items[base + t];
             const int64_t tot = I.ad.nrows * I.ad.xsize;
-            const int64_t need = tiled ? ceil_div(I.ad.nrows, 64) * ceil_div(I.ad.xsize, 64) : (int64_t)xr_grid(tot);
+            const int64_t need = tiled ? ceil_div(I.ad.nrows, 64) * ceil_div(I.ad.xsize, 64)
+                                       : (int64_t)xr_grid(ceil_div(tot, XR_PACK_UNROLL));
             most = need > most ? need : most;
             bytes += tot;
         }
