@@ -206,10 +206,19 @@ class XRAdjoint:
             # the pair matrix is built as S + S^T from the 'source' blocks and
             # half of the 'self' ones; Gb is symmetric, so a source block's
             # adjoint counts twice and the transposed combinations none
+            same = []            # charge-conserving combinations, taken together
             for k1, k2 in X.pair_charge_combos(m1, m2):
                 role = X.combo_role(k1, k2)
-                if role != "derived":
-                    self._pair_adjoint(m1, m2, k1, k2, Gb, rbar, ib, h_bar, 1.0 if role == "self" else 2.0)
+                if role == "derived":
+                    continue
+                f = 1.0 if role == "self" else 2.0
+                if k1[0] == k1[1]:
+                    same.append((k1, k2, f))
+                else:
+                    self._pair_adjoint(m1, m2, k1, k2, Gb, rbar, ib, h_bar, f)
+            if same and not self._pair_adjoint_same(m1, m2, same, Gb, rbar, ib):
+                for k1, k2, f in same:
+                    self._pair_adjoint(m1, m2, k1, k2, Gb, rbar, ib, h_bar, f)
             del Gb
 
         # ---- densities -> coefficients
@@ -239,8 +248,6 @@ class XRAdjoint:
         F1, F2 = X.frags[m1], X.frags[m2]
         (i1c, j1c), (i2c, j2c) = k1, k2
         sp = X.precision
-        nI1, nJ1, nI2, nJ2 = F1.n_states[i1c], F1.n_states[j1c], F2.n_states[i2c], F2.n_states[j2c]
-        nA, nB = nI1 * nJ1, nI2 * nJ2
         d = i1c - j1c
         sign2 = float((-1) ** ((F2.n_elec_ref - i2c) % 2))
 
@@ -320,20 +327,83 @@ class XRAdjoint:
         W, Wb, fin = vperm((m1, m2, m1, m2), (0, 2, 1, 3), (n1 * n1, n2 * n2))
         adj(Gm, _flat(R1("ca")), W, _flat(R2("ca")), 4.0, R1b("ca"), R2b("ca"), Wb, split=sp, wbar_beta=0.0)
         fin()
-        # extPot terms: e2[b] on I1 == J1 rows, e1[a] on I2 == J2 columns
-        dia1 = torch.zeros(nA, 1, dtype=torch.float64, device=X.device)
+        self._ext_pot(m1, m2, k1, k2, Gm, rbar, ib)
+
+    def _ext_pot(self, m1, m2, k1, k2, Gm, rbar, ib):
+        """extPot terms of a charge-conserving combination (build_H.py:85-86):
+        e2[b] on the I1 == J1 rows of the pair density, e1[a] on its I2 == J2
+        columns."""
+        X = self.X
+        F1, F2 = X.frags[m1], X.frags[m2]
+        nI1, nJ1 = F1.n_states[k1[0]], F1.n_states[k1[1]]
+        nI2, nJ2 = F2.n_states[k2[0]], F2.n_states[k2[1]]
+        dia1 = torch.zeros(nI1 * nJ1, 1, dtype=torch.float64, device=X.device)
         dia1.view(nI1, nJ1).diagonal().fill_(1.0)
-        dia2 = torch.zeros(nB, 1, dtype=torch.float64, device=X.device)
+        dia2 = torch.zeros(nI2 * nJ2, 1, dtype=torch.float64, device=X.device)
         dia2.view(nI2, nJ2).diagonal().fill_(1.0)
         e2b = mm(Gm, dia1, transA=1)           # (nB x 1)
         e1b = mm(Gm, dia2)                     # (nA x 1)
         U12 = X.ints.U(m1, m2, m2)
         U21 = X.ints.U(m2, m1, m1)
-        for eb, U_, Rb_, R_, key in ((e2b, U12, R2b, R2, ("U", m1, m2, m2)), (e1b, U21, R1b, R1, ("U", m2, m1, m1))):
-            out, beta = Rb_("ca")
-            mm(eb, U_.view(1, -1), out=out, beta=beta)
+        for eb, U_, F, m, k, key in ((e2b, U12, F2, m2, k2, ("U", m1, m2, m2)),
+                                     (e1b, U21, F1, m1, k1, ("U", m2, m1, m1))):
+            R = F.rho["ca"][k]
+            out, beta = rbar[m].take(("ca", k), R.shape)
+            mm(eb, U_.view(1, -1), out=_flat(out), beta=beta)
             out, beta = ib(*key)
-            mm(_flat(R_("ca")), eb, transA=1, out=out.view(-1, 1), beta=beta)
+            mm(_flat(R), eb, transA=1, out=out.view(-1, 1), beta=beta)
+
+    def _pair_adjoint_same(self, m1, m2, combos, Gb, rbar, ib):
+        """The charge-conserving combinations (k1 = (a, a), k2 = (b, b)) of
+        one pair at once.  H_c = 4 A_k1 W B_k2^T with one integral block W for
+        all of them, so with T_k1 = sum_k2 G_c B_k2 and U_k2 = sum_k1 G_c^T A_k1
+        the adjoints of bilinear_adjoint summed over the combinations are
+            Abar_k1 += 4 T_k1 W^T,  Bbar_k2 += 4 U_k2 W,  Wbar = 4 sum_k1 A_k1^T T_k1:
+        three products with W per charge instead of one per combination, and
+        no A^T G products.  Returns False, doing nothing, when taking each
+        combination on its own (bilinear_adjoint's cheaper association per
+        product) needs fewer flops."""
+        X = self.X
+        F1, F2 = X.frags[m1], X.frags[m2]
+        sp = X.precision
+        p, q = F1.dim * F1.dim, F2.dim * F2.dim
+        k1s = list(dict.fromkeys(k1 for k1, _, _ in combos))
+        k2s = list(dict.fromkeys(k2 for _, k2, _ in combos))
+        nA = {k: F1.n_states[k[0]] * F1.n_states[k[1]] for k in k1s}
+        nB = {k: F2.n_states[k[0]] * F2.n_states[k[1]] for k in k2s}
+        alone, joint = 0, 0
+        for k1, k2, _ in combos:
+            a, b = nA[k1], nB[k2]
+            alone += (min(a * b * q + a * q * p, b * q * p + a * b * p) + min(b * a * p + b * p * q, a * p * q + b * a * q)
+                      + min(p * a * b + p * b * q, a * b * q + p * a * q))
+            joint += a * b * (p + q)
+        joint += sum(2 * nA[k] * p * q for k in k1s) + sum(nB[k] * p * q for k in k2s)
+        if joint > alone:
+            return False
+        key, perm = (m1, m2, m1, m2), (0, 2, 1, 3)
+        Vt = X.ints.V(*key).contiguous()
+        W = permute4(Vt, perm).view(p, q)
+        T, U = {}, {}
+        for k1, k2, f in combos:
+            g = self.op.pair_gather(Gb, m1, m2, (F1.pos, F2.pos), k1, k2, False)
+            if f != 1.0:
+                g.mul_(f)
+            T[k1] = mm(g, _flat(F2.rho["ca"][k2]), out=T.get(k1), beta=1.0 if k1 in T else 0.0, mode=sp)
+            U[k2] = mm(g, _flat(F1.rho["ca"][k1]), transA=1, out=U.get(k2), beta=1.0 if k2 in U else 0.0,
+                       mode=sp)
+            self._ext_pot(m1, m2, k1, k2, g, rbar, ib)
+        for k1 in k1s:
+            t, beta = rbar[m1].take(("ca", k1), F1.rho["ca"][k1].shape)
+            mm(T[k1], W, transB=1, alpha=4.0, out=_flat(t), beta=beta, mode=sp)
+        for k2 in k2s:
+            t, beta = rbar[m2].take(("ca", k2), F2.rho["ca"][k2].shape)
+            mm(U[k2], W, alpha=4.0, out=_flat(t), beta=beta, mode=sp)
+        Wb = torch.empty_like(W)               # written whole by the first product
+        for i, k1 in enumerate(k1s):
+            mm(_flat(F1.rho["ca"][k1]), T[k1], transA=1, alpha=4.0, out=Wb, beta=0.0 if i == 0 else 1.0, mode=sp)
+        out, beta = ib("V", *key)
+        permute4(Wb.view([Vt.shape[i] for i in perm]), _inv(perm), out=out, accumulate=beta == 1.0)
+        return True
 
 
 def _inv(perm):
