@@ -206,19 +206,26 @@ class XRAdjoint:
             # the pair matrix is built as S + S^T from the 'source' blocks and
             # half of the 'self' ones; Gb is symmetric, so a source block's
             # adjoint counts twice and the transposed combinations none
-            same = []            # charge-conserving combinations, taken together
+            # the charge-conserving (d = 0) and the one-electron-transfer
+            # d = -1 combinations share their integral blocks: each set is
+            # taken together; the others one by one
+            same, transfer = [], []
             for k1, k2 in X.pair_charge_combos(m1, m2):
                 role = X.combo_role(k1, k2)
                 if role == "derived":
                     continue
                 f = 1.0 if role == "self" else 2.0
-                if k1[0] == k1[1]:
+                d = k1[0] - k1[1]
+                if d == 0:
                     same.append((k1, k2, f))
+                elif d == -1:
+                    transfer.append((k1, k2, f))
                 else:
                     self._pair_adjoint(m1, m2, k1, k2, Gb, rbar, ib, h_bar, f)
-            if same and not self._pair_adjoint_same(m1, m2, same, Gb, rbar, ib):
-                for k1, k2, f in same:
-                    self._pair_adjoint(m1, m2, k1, k2, Gb, rbar, ib, h_bar, f)
+            if same:
+                self._pair_adjoint_same(m1, m2, same, Gb, rbar, ib)
+            if transfer:
+                self._pair_adjoint_transfer(m1, m2, transfer, Gb, rbar, ib, h_bar)
             del Gb
 
         # ---- densities -> coefficients
@@ -353,57 +360,163 @@ class XRAdjoint:
             out, beta = ib(*key)
             mm(_flat(R), eb, transA=1, out=out.view(-1, 1), beta=beta)
 
-    def _pair_adjoint_same(self, m1, m2, combos, Gb, rbar, ib):
-        """The charge-conserving combinations (k1 = (a, a), k2 = (b, b)) of
-        one pair at once.  H_c = 4 A_k1 W B_k2^T with one integral block W for
-        all of them, so with T_k1 = sum_k2 G_c B_k2 and U_k2 = sum_k1 G_c^T A_k1
-        the adjoints of bilinear_adjoint summed over the combinations are
-            Abar_k1 += 4 T_k1 W^T,  Bbar_k2 += 4 U_k2 W,  Wbar = 4 sum_k1 A_k1^T T_k1:
-        three products with W per charge instead of one per combination, and
-        no A^T G products.  Returns False, doing nothing, when taking each
-        combination on its own (bilinear_adjoint's cheaper association per
-        product) needs fewer flops."""
-        X = self.X
-        F1, F2 = X.frags[m1], X.frags[m2]
-        sp = X.precision
-        p, q = F1.dim * F1.dim, F2.dim * F2.dim
-        k1s = list(dict.fromkeys(k1 for k1, _, _ in combos))
-        k2s = list(dict.fromkeys(k2 for _, k2, _ in combos))
-        nA = {k: F1.n_states[k[0]] * F1.n_states[k[1]] for k in k1s}
-        nB = {k: F2.n_states[k[0]] * F2.n_states[k[1]] for k in k2s}
-        alone, joint = 0, 0
-        for k1, k2, _ in combos:
-            a, b = nA[k1], nB[k2]
-            alone += (min(a * b * q + a * q * p, b * q * p + a * b * p) + min(b * a * p + b * p * q, a * p * q + b * a * q)
-                      + min(p * a * b + p * b * q, a * b * q + p * a * q))
-            joint += a * b * (p + q)
-        joint += sum(2 * nA[k] * p * q for k in k1s) + sum(nB[k] * p * q for k in k2s)
-        if joint > alone:
-            return False
-        key, perm = (m1, m2, m1, m2), (0, 2, 1, 3)
-        Vt = X.ints.V(*key).contiguous()
-        W = permute4(Vt, perm).view(p, q)
-        T, U = {}, {}
+    def _vperm(self, key, perm, shape2, ib):
+        """W = V[key] with axes permuted and flattened, an uninitialised
+        accumulator Wb of its shape, and fin() mapping Wb back through the
+        same permutation into the accumulator of V[key] (library passes)."""
+        Vt = self.X.ints.V(*key).contiguous()
+        W = permute4(Vt, perm).view(*shape2)
+        Wb = torch.empty_like(W)
+
+        def fin():
+            out, beta = ib("V", *key)
+            permute4(Wb.view([Vt.shape[i] for i in perm]), _inv(perm), out=out, accumulate=beta == 1.0)
+        return W, Wb, fin
+
+    @staticmethod
+    def _group_adjoint(items, A, W, B, s, a_take, b_take, Wb, cache, akey, bkey, mode):
+        """bilinear_adjoint summed over charge combinations that share W.
+        items: [(k1, k2, g)] with g = dE/dH_c (its combination's factor and
+        sign included) for H_c = s A[k1] W B[k2]^T.  Adds
+            Abar[k1] += s sum_k2 g B[k2] W^T,   Bbar[k2] += s sum_k1 g^T A[k1] W
+        into a_take(k1) / b_take(k2) ((tensor, beta) of the accumulator) and
+        writes Wb = s sum A[k1]^T g B[k2].  Each output takes the association
+        with fewer flops; the partner sums T[k1] = sum_k2 g B[k2] and
+        U[k2] = sum_k1 g^T A[k1] are kept in cache (keyed by the operand's name
+        akey / bkey and the precision mode, an FP64 sum serving any mode) for
+        the other terms of the same combinations.  Either association never
+        needs more flops than taking the combinations one by one."""
+        k1s = list(dict.fromkeys(k1 for k1, _, _ in items))
+        k2s = list(dict.fromkeys(k2 for _, k2, _ in items))
+        p, q = W.shape
+        nA = {k: A[k].shape[0] for k in k1s}
+        nB = {k: B[k].shape[0] for k in k2s}
+
+        def have(kind, name, ks):
+            return all((kind, name, k, "f64") in cache or (kind, name, k, mode) in cache for k in ks)
+
+        def part(kind, name, k):
+            """T[k] (kind 'T', over the items with k1 = k) or U[k] (kind 'U')."""
+            for m in ("f64", mode):
+                if (kind, name, k, m) in cache:
+                    return cache[(kind, name, k, m)]
+            t = None
+            for a, b, g in items:
+                if kind == "T" and a == k:
+                    t = mm(g, B[b], out=t, beta=0.0 if t is None else 1.0, mode=mode)
+                elif kind == "U" and b == k:
+                    t = mm(g, A[a], transA=1, out=t, beta=0.0 if t is None else 1.0, mode=mode)
+            cache[(kind, name, k, mode)] = t
+            return t
+
+        cT = 0 if have("T", bkey, k1s) else sum(nA[a] * nB[b] * q for a, b, _ in items)
+        cU = 0 if have("U", akey, k2s) else sum(nA[a] * nB[b] * p for a, b, _ in items)
+        # Abar: s T[k1] W^T, or s sum_k2 g (B[k2] W^T)
+        if cT + sum(nA[k] * q * p for k in k1s) <= (sum(nB[k] * q * p for k in k2s)
+                                                     + sum(nA[a] * nB[b] * p for a, b, _ in items)):
+            for k1 in k1s:
+                t, beta = a_take(k1)
+                mm(part("T", bkey, k1), W, transB=1, alpha=s, out=t, beta=beta, mode=mode)
+            cT = 0
+        else:
+            BW = {k2: mm(B[k2], W, transB=1, mode=mode) for k2 in k2s}
+            for k1 in k1s:
+                t, beta = a_take(k1)
+                for a, b, g in items:
+                    if a == k1:
+                        mm(g, BW[b], alpha=s, out=t, beta=beta, mode=mode)
+                        beta = 1.0
+        # Bbar: s U[k2] W, or s sum_k1 g^T (A[k1] W)
+        if cU + sum(nB[k] * p * q for k in k2s) <= (sum(nA[k] * p * q for k in k1s)
+                                                     + sum(nA[a] * nB[b] * q for a, b, _ in items)):
+            for k2 in k2s:
+                t, beta = b_take(k2)
+                mm(part("U", akey, k2), W, alpha=s, out=t, beta=beta, mode=mode)
+            cU = 0
+        else:
+            AW = {k1: mm(A[k1], W, mode=mode) for k1 in k1s}
+            for k2 in k2s:
+                t, beta = b_take(k2)
+                for a, b, g in items:
+                    if b == k2:
+                        mm(g, AW[a], transA=1, alpha=s, out=t, beta=beta, mode=mode)
+                        beta = 1.0
+        # Wbar: s sum_k1 A[k1]^T T[k1], or s sum_k2 U[k2]^T B[k2]
+        if cT + sum(p * nA[k] * q for k in k1s) <= cU + sum(p * nB[k] * q for k in k2s):
+            for i, k1 in enumerate(k1s):
+                mm(A[k1], part("T", bkey, k1), transA=1, alpha=s, out=Wb, beta=0.0 if i == 0 else 1.0, mode=mode)
+        else:
+            for i, k2 in enumerate(k2s):
+                mm(part("U", akey, k2), B[k2], transA=1, alpha=s, out=Wb, beta=0.0 if i == 0 else 1.0, mode=mode)
+
+    def _gathered(self, m1, m2, combos, Gb, sign=None):
+        """[(k1, k2, g)]: each combination's block of the pair density times
+        its factor (and sign(k2) when given)."""
+        F1, F2 = self.X.frags[m1], self.X.frags[m2]
+        items = []
         for k1, k2, f in combos:
             g = self.op.pair_gather(Gb, m1, m2, (F1.pos, F2.pos), k1, k2, False)
-            if f != 1.0:
-                g.mul_(f)
-            T[k1] = mm(g, _flat(F2.rho["ca"][k2]), out=T.get(k1), beta=1.0 if k1 in T else 0.0, mode=sp)
-            U[k2] = mm(g, _flat(F1.rho["ca"][k1]), transA=1, out=U.get(k2), beta=1.0 if k2 in U else 0.0,
-                       mode=sp)
+            c = f * (sign(k2) if sign is not None else 1.0)
+            if c != 1.0:
+                g.mul_(c)
+            items.append((k1, k2, g))
+        return items
+
+    def _takers(self, m, F, rbar):
+        """op -> (k -> (flattened accumulator of F.rho[op][k], beta))."""
+        def for_op(op):
+            def take(k):
+                t, beta = rbar[m].take((op, k), F.rho[op][k].shape)
+                return _flat(t), beta
+            return take
+        return for_op
+
+    def _pair_adjoint_same(self, m1, m2, combos, Gb, rbar, ib):
+        """The charge-conserving combinations (k1 = (a, a), k2 = (b, b)) of
+        one pair together: H_c = 4 Rca1[k1] W Rca2[k2]^T with one permuted
+        V(m1, m2, m1, m2) for all of them (the d == 0 case of _pair_adjoint),
+        plus each combination's extPot terms."""
+        X = self.X
+        F1, F2 = X.frags[m1], X.frags[m2]
+        n1, n2 = F1.dim, F2.dim
+        items = self._gathered(m1, m2, combos, Gb)
+        W, Wb, fin = self._vperm((m1, m2, m1, m2), (0, 2, 1, 3), (n1 * n1, n2 * n2), ib)
+        A = {k1: _flat(F1.rho["ca"][k1]) for k1, _, _ in items}
+        B = {k2: _flat(F2.rho["ca"][k2]) for _, k2, _ in items}
+        self._group_adjoint(items, A, W, B, 4.0, self._takers(m1, F1, rbar)("ca"),
+                            self._takers(m2, F2, rbar)("ca"), Wb, {}, "ca", "ca", X.precision)
+        fin()
+        for k1, k2, g in items:
             self._ext_pot(m1, m2, k1, k2, g, rbar, ib)
-        for k1 in k1s:
-            t, beta = rbar[m1].take(("ca", k1), F1.rho["ca"][k1].shape)
-            mm(T[k1], W, transB=1, alpha=4.0, out=_flat(t), beta=beta, mode=sp)
-        for k2 in k2s:
-            t, beta = rbar[m2].take(("ca", k2), F2.rho["ca"][k2].shape)
-            mm(U[k2], W, alpha=4.0, out=_flat(t), beta=beta, mode=sp)
-        Wb = torch.empty_like(W)               # written whole by the first product
-        for i, k1 in enumerate(k1s):
-            mm(_flat(F1.rho["ca"][k1]), T[k1], transA=1, alpha=4.0, out=Wb, beta=0.0 if i == 0 else 1.0, mode=sp)
-        out, beta = ib("V", *key)
-        permute4(Wb.view([Vt.shape[i] for i in perm]), _inv(perm), out=out, accumulate=beta == 1.0)
-        return True
+
+    def _pair_adjoint_transfer(self, m1, m2, combos, Gb, rbar, ib, h_bar):
+        """The d = -1 combinations (fragment 1 gains the electron fragment 2
+        loses) of one pair together: the one-electron term s c1 h a2^T and
+        the two-electron terms 2 s cca1 W1 a2^T and 2 s c1 W2 caa2^T (the
+        d == -1 case of _pair_adjoint), with s the sign of k2 folded into
+        each combination's block.  h, W1 and W2 are shared by all of them,
+        and so are the partner sums over a2 and c1."""
+        X = self.X
+        F1, F2 = X.frags[m1], X.frags[m2]
+        n1, n2 = F1.dim, F2.dim
+        sp = X.precision
+        items = self._gathered(m1, m2, combos, Gb,
+                               sign=lambda k2: float((-1) ** ((F2.n_elec_ref - k2[0]) % 2)))
+        R1 = lambda op: {k1: _flat(F1.rho[op][k1]) for k1, _, _ in items}
+        R2 = lambda op: {k2: _flat(F2.rho[op][k2]) for _, k2, _ in items}
+        t1, t2 = self._takers(m1, F1, rbar), self._takers(m2, F2, rbar)
+        cache = {}
+        parts = [("T", (m1, m2)), ("U", (m1, m1, m2)), ("U", (m2, m1, m2))]
+        h = X.ints.h_sum((m1, m2, "1min"), parts)
+        hb = torch.empty_like(h)
+        self._group_adjoint(items, R1("c"), h, R2("a"), 1.0, t1("c"), t2("a"), hb, cache, "c", "a", "f64")
+        h_bar(parts, hb)
+        W1, W1b, fin1 = self._vperm((m1, m1, m1, m2), (1, 0, 2, 3), (n1 ** 3, n2), ib)
+        self._group_adjoint(items, R1("cca"), W1, R2("a"), 2.0, t1("cca"), t2("a"), W1b, cache, "cca", "a", sp)
+        fin1()
+        W2, W2b, fin2 = self._vperm((m1, m2, m2, m2), (0, 1, 3, 2), (n1, n2 ** 3), ib)
+        self._group_adjoint(items, R1("c"), W2, R2("caa"), 2.0, t1("c"), t2("caa"), W2b, cache, "c", "caa", sp)
+        fin2()
 
 
 def _inv(perm):
