@@ -113,6 +113,7 @@ class XRAdjoint:
         if len(system.frags) < 2:
             raise ValueError("the XR gradient needs at least two fragments")
         self.X = system
+        self._diag_vecs = {}
 
     def forward(self):
         X = self.X
@@ -334,31 +335,42 @@ class XRAdjoint:
         W, Wb, fin = vperm((m1, m2, m1, m2), (0, 2, 1, 3), (n1 * n1, n2 * n2))
         adj(Gm, _flat(R1("ca")), W, _flat(R2("ca")), 4.0, R1b("ca"), R2b("ca"), Wb, split=sp, wbar_beta=0.0)
         fin()
-        self._ext_pot(m1, m2, k1, k2, Gm, rbar, ib)
+        self._ext_pot(m1, m2, [(k1, k2, Gm)], rbar, ib)
 
-    def _ext_pot(self, m1, m2, k1, k2, Gm, rbar, ib):
-        """extPot terms of a charge-conserving combination (build_H.py:85-86):
-        e2[b] on the I1 == J1 rows of the pair density, e1[a] on its I2 == J2
-        columns."""
+    def _diag_vec(self, nI, nJ):
+        """(nI nJ x 1) indicator of the I == J entries of a state-pair index
+        (kept: the same few shapes recur in every backward)."""
+        cache = self._diag_vecs
+        t = cache.get((nI, nJ))
+        if t is None:
+            t = torch.zeros(nI * nJ, 1, dtype=torch.float64, device=self.X.device)
+            t.view(nI, nJ).diagonal().fill_(1.0)
+            cache[(nI, nJ)] = t
+        return t
+
+    def _ext_pot(self, m1, m2, items, rbar, ib):
+        """extPot terms of charge-conserving combinations [(k1, k2, G)]
+        (build_H.py:85-86): e2[b] from the I1 == J1 rows of each pair density
+        block, e1[a] from its I2 == J2 columns.  Both enter linearly, so they
+        are summed over the combinations first: e2 per k2, e1 per k1, then one
+        outer product with U and one product with the density per charge."""
         X = self.X
         F1, F2 = X.frags[m1], X.frags[m2]
-        nI1, nJ1 = F1.n_states[k1[0]], F1.n_states[k1[1]]
-        nI2, nJ2 = F2.n_states[k2[0]], F2.n_states[k2[1]]
-        dia1 = torch.zeros(nI1 * nJ1, 1, dtype=torch.float64, device=X.device)
-        dia1.view(nI1, nJ1).diagonal().fill_(1.0)
-        dia2 = torch.zeros(nI2 * nJ2, 1, dtype=torch.float64, device=X.device)
-        dia2.view(nI2, nJ2).diagonal().fill_(1.0)
-        e2b = mm(Gm, dia1, transA=1)           # (nB x 1)
-        e1b = mm(Gm, dia2)                     # (nA x 1)
+        E2, E1 = {}, {}
+        for k1, k2, Gm in items:
+            d1 = self._diag_vec(F1.n_states[k1[0]], F1.n_states[k1[1]])
+            d2 = self._diag_vec(F2.n_states[k2[0]], F2.n_states[k2[1]])
+            E2[k2] = mm(Gm, d1, transA=1, out=E2.get(k2), beta=1.0 if k2 in E2 else 0.0)   # (nB x 1)
+            E1[k1] = mm(Gm, d2, out=E1.get(k1), beta=1.0 if k1 in E1 else 0.0)             # (nA x 1)
         U12 = X.ints.U(m1, m2, m2)
         U21 = X.ints.U(m2, m1, m1)
-        for eb, U_, F, m, k, key in ((e2b, U12, F2, m2, k2, ("U", m1, m2, m2)),
-                                     (e1b, U21, F1, m1, k1, ("U", m2, m1, m1))):
-            R = F.rho["ca"][k]
-            out, beta = rbar[m].take(("ca", k), R.shape)
-            mm(eb, U_.view(1, -1), out=_flat(out), beta=beta)
-            out, beta = ib(*key)
-            mm(_flat(R), eb, transA=1, out=out.view(-1, 1), beta=beta)
+        for E, U_, F, m, key in ((E2, U12, F2, m2, ("U", m1, m2, m2)), (E1, U21, F1, m1, ("U", m2, m1, m1))):
+            for k, eb in E.items():
+                R = F.rho["ca"][k]
+                out, beta = rbar[m].take(("ca", k), R.shape)
+                mm(eb, U_.view(1, -1), out=_flat(out), beta=beta)
+                out, beta = ib(*key)
+                mm(_flat(R), eb, transA=1, out=out.view(-1, 1), beta=beta)
 
     def _vperm(self, key, perm, shape2, ib):
         """W = V[key] with axes permuted and flattened, an uninitialised
@@ -486,8 +498,7 @@ class XRAdjoint:
         self._group_adjoint(items, A, W, B, 4.0, self._takers(m1, F1, rbar)("ca"),
                             self._takers(m2, F2, rbar)("ca"), Wb, {}, "ca", "ca", X.precision)
         fin()
-        for k1, k2, g in items:
-            self._ext_pot(m1, m2, k1, k2, g, rbar, ib)
+        self._ext_pot(m1, m2, items, rbar, ib)
 
     def _pair_adjoint_transfer(self, m1, m2, combos, Gb, rbar, ib, h_bar):
         """The d = -1 combinations (fragment 1 gains the electron fragment 2
