@@ -109,10 +109,14 @@ class XRAdjoint:
     fragments (monomer and pair terms).  The lowest root comes from the
     matrix-free operator (dense eigendecomposition for small spaces)."""
 
-    def __init__(self, system):
+    def __init__(self, system, group_combos=True):
         if len(system.frags) < 2:
             raise ValueError("the XR gradient needs at least two fragments")
         self.X = system
+        # take the charge combinations of a pair that share an integral block
+        # together (False: every combination on its own, the same sums in
+        # another order)
+        self.group_combos = group_combos
         self._diag_vecs = {}
 
     def forward(self):
@@ -217,9 +221,9 @@ class XRAdjoint:
                     continue
                 f = 1.0 if role == "self" else 2.0
                 d = k1[0] - k1[1]
-                if d == 0:
+                if self.group_combos and d == 0:
                     same.append((k1, k2, f))
-                elif d == -1:
+                elif self.group_combos and d == -1:
                     transfer.append((k1, k2, f))
                 else:
                     self._pair_adjoint(m1, m2, k1, k2, Gb, rbar, ib, h_bar, f)

@@ -14,7 +14,7 @@ import torch
 from qodeapplications_amd import density as D
 from qodeapplications_amd import synthetic as syn
 from qodeapplications_amd._lib import context, ptr
-from qodeapplications_amd.grad import DimerAdjoint, energy_and_gradient
+from qodeapplications_amd.grad import DimerAdjoint, XRAdjoint, energy_and_gradient
 from qodeapplications_amd.orbitals import OrbitalRotation, dimer_integral_keys, energy_and_gradients
 from qodeapplications_amd.xr import XRSystem
 
@@ -184,6 +184,29 @@ def _energy(X, coeffs, generalised):
     Li = torch.linalg.inv(L)
     Ht = Li @ (0.5 * (M + M.T)) @ Li.T
     return float(torch.linalg.eigvalsh(Ht)[0])
+
+
+@pytest.mark.parametrize("orbs", [(3, 3), (2, 4)])
+def test_grouped_pair_adjoint_matches_per_combination(orbs):
+    """The charge-conserving and one-electron-transfer combinations of a
+    pair, taken together (partner sums first, each output in its cheaper
+    association; uneven fragments pick different associations per term),
+    give the adjoint of taking every combination on its own."""
+    frags, (orbs_, h, V, U), nuc = _dimer(41, orbs=orbs)
+    X = XRSystem(frags, syn.FragmentBlocked(orbs_, h, V, U), nuc)
+    got = []
+    for group in (True, False):
+        adj = XRAdjoint(X, group_combos=group)
+        adj.forward()
+        got.append(adj.backward())
+    (z1, i1), (z2, i2) = got
+    for m in range(2):
+        for c in z1[m]:
+            ref = z2[m][c]
+            assert float((z1[m][c] - ref).abs().max()) <= 1e-11 * max(1.0, float(ref.abs().max())), (m, c)
+    assert set(i1) == set(i2)
+    for k, ref in i2.items():
+        assert float((i1[k] - ref).abs().max()) <= 1e-11 * max(1.0, float(ref.abs().max())), k
 
 
 @pytest.mark.parametrize("lagrangian", [False, True])
